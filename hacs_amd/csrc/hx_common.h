@@ -1,0 +1,107 @@
+// hx_common.h -- internal helpers shared by the HIP sources of libhacs_mi355x.so:
+// error plumbing for the C-ABI and the device-side exact modular arithmetic.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/hacs_mi355x.h"
+
+namespace hx {
+
+// ---- host: thread-local last error (the C form of CudaError, Cuda/CudaCtx.h:40-54) ----
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+#define HX_HIP(call)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (call);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            throw ::hx::Error{HX_E_RUNTIME, std::string(#call) + ": " + hipGetErrorString(e_) + " (" + \
+                                                __FILE__ + ":" + std::to_string(__LINE__) + ")"};      \
+    } while (0)
+
+#define HX_CHECK(cond, msg)                                                                   \
+    do {                                                                                      \
+        if (!(cond)) throw ::hx::Error{HX_E_ARGUMENT, std::string(msg)};                      \
+    } while (0)
+
+// Runs body, converting hx::Error / std::exception into a status code + last error.
+template <class F>
+int guard(const char* where, F&& body) {
+    try {
+        body();
+        return HX_OK;
+    } catch (const Error& e) {
+        return fail(e.code, std::string(where) + ": " + e.msg);
+    } catch (const std::exception& e) {
+        return fail(HX_E_RUNTIME, std::string(where) + ": " + e.what());
+    }
+}
+
+// Checks the launch just issued.
+inline void after_launch(const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw Error{HX_E_RUNTIME, std::string(what) + ": " + hipGetErrorString(e)};
+}
+
+// Library-owned device scratch (host side): grows, never shrinks, per device.
+void* scratch(size_t bytes, int slot);
+
+// ---- device: exact arithmetic mod p, p < 2^32 (ModCtx.h:202-333) ----
+struct DevMod {
+    uint32_t p;
+    uint64_t bm;        // floor(2^64 / p)
+    uint64_t t64;       // 2^64 mod p
+};
+
+// x mod p for every x < 2^64 (ModCtx.h:323-329: q = mulhi(x, m), one correction).
+__device__ __forceinline__ uint32_t red64(uint64_t x, const DevMod& M) {
+    const uint64_t q = __umul64hi(x, M.bm);
+    uint64_t r = x - q * (uint64_t)M.p;
+    if (r >= M.p) r -= M.p;
+    return (uint32_t)r;
+}
+__device__ __forceinline__ uint32_t addm(uint32_t a, uint32_t b, uint32_t p) {
+    uint32_t r = a + b;
+    if (r < a || r >= p) r -= p;
+    return r;
+}
+__device__ __forceinline__ uint32_t subm(uint32_t a, uint32_t b, uint32_t p) { return a >= b ? a - b : a + (p - b); }
+__device__ __forceinline__ uint32_t mulm(uint32_t a, uint32_t b, const DevMod& M) {
+    return red64((uint64_t)a * b, M);
+}
+// (hi * 2^64 + lo) mod p for hi < 2^32: the exact 96-bit accumulator of the plain kernels.
+__device__ __forceinline__ uint32_t red96(uint32_t hi, uint64_t lo, const DevMod& M) {
+    const uint32_t r = red64(lo, M);
+    if (hi == 0) return r;
+    return addm(r, red64((uint64_t)hi * M.t64, M), M.p);
+}
+// a^{-1} mod p by the extended Euclidean algorithm (ModArithmetic.cpp:54-78); 0 if gcd(a, p) != 1.
+__device__ __forceinline__ uint32_t invm(uint32_t a, uint32_t p) {
+    int64_t r0 = p, r1 = a % p, s0 = 0, s1 = 1;
+    while (r1 != 0) {
+        const uint32_t q = (uint32_t)r0 / (uint32_t)r1;
+        const int64_t r2 = r0 - (int64_t)q * r1, s2 = s0 - (int64_t)q * s1;
+        r0 = r1; r1 = r2; s0 = s1; s1 = s2;
+    }
+    if (r0 != 1) return 0;
+    int64_t s = s0 % (int64_t)p;
+    if (s < 0) s += p;
+    return (uint32_t)s;
+}
+__device__ __forceinline__ uint32_t apply_op(uint32_t c, uint32_t r, int op, uint32_t p) {
+    return op == HX_OP_SET ? r : op == HX_OP_ADD ? addm(c, r, p) : subm(c, r, p);
+}
+
+inline DevMod devmod(const hx_modctx& c) { return DevMod{c.p, c.barrett_m, c.two64_mod}; }
+
+}  // namespace hx

@@ -1,0 +1,586 @@
+// hx_gemm.hip -- exact GEMM mod p on gfx950: C op= A * B mod p for uint32 residues.
+//
+// Two device kernels compute the same bytes (the product is the canonical residue of an
+// exact integer sum, so every correct kernel agrees bit for bit; ENGINE_API.md:93-102):
+//
+//  * plain_kernel: u32 x u32 products accumulated EXACTLY on the VALU in a 96-bit
+//    accumulator (u64 + a u32 carry count), one Barrett reduction per output cell at the
+//    end.  The device twin of the reference's u32 GEMM (Cuda/cuda_kernels.cu:195-300,
+//    which instead flushes a u64 sum by Barrett every k_max products, k_max = 4 for 31-bit p).
+//
+//  * limb route (limb_prep_kernel + limb_kernel<L>): every residue x is replaced by its
+//    balanced representative (|x| <= (p-1)/2, the reference's `bal` trick,
+//    Cuda/ImmaKernels.cuh:15-18) written in L signed base-256 digits in [-128, 127]:
+//        x == sum_s d_s 256^s (mod p).
+//    Then A*B = sum_{s,t} 256^{s+t} A_s B_t, and each limb product A_s B_t is an int8 GEMM
+//    on the MFMA cores (v_mfma_i32_32x32x32_i8, exact int32 accumulation).  Products on one
+//    anti-diagonal d = s + t share one int32 accumulator D_d: |D_d| <= K * L * 128 * 128
+//    < 2^31 for K <= limb_k_chunk(L), so nothing wraps.  The epilogue recombines
+//    sum_d 256^d D_d exactly in int64 (Horner, reduced mod p every three steps) and applies
+//    the op.  L = 4 covers every p <= 4278124287 (all 31-bit moduli); L = 3 covers
+//    p <= 16711423, L = 2 p <= 65279, L = 1 p <= 255.
+//    The operands are split into limb planes once per call (limb_prep_kernel, HBM-bound:
+//    4 bytes in, L bytes out per element), so the MFMA loop streams int8 planes only.
+//
+// No floating point anywhere; no atomics on results.
+#include "hx_gemm.h"
+
+#include <algorithm>
+#include <mutex>
+
+namespace hx {
+namespace {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ uint32_t ld_a(const GemmArgs& g, uint32_t i, uint32_t t) {
+    const int64_t r = g.a_rows ? (int64_t)g.a_rows[i] : (int64_t)i;
+    const int64_t c = g.a_cols ? (int64_t)g.a_cols[t] : (int64_t)t;
+    return g.A[r * g.ars + c * g.acs];
+}
+__device__ __forceinline__ uint32_t ld_cin(const GemmArgs& g, uint32_t i, uint32_t j) {
+    const int64_t r = g.c_rows ? (int64_t)g.c_rows[i] : (int64_t)i;
+    return g.Cin[r * g.cirs + (int64_t)j * g.cics];
+}
+
+// XCD-aware tile order: consecutive workgroups are dealt round-robin over the 8 XCDs, so
+// workgroup b is remapped to let every XCD own one contiguous run of tiles; inside a run,
+// tiles go down groups of 8 tile-rows first, so the A rows and B columns a run touches stay
+// in that XCD's L2.
+__device__ __forceinline__ void tile_of(uint32_t b, uint32_t ntm, uint32_t ntn, uint32_t& tm, uint32_t& tn) {
+    const uint32_t T = ntm * ntn, per = T / 8;
+    uint32_t t = b;
+    if (b < per * 8) t = (b % 8) * per + b / 8;
+    const uint32_t G = 8, width = G * ntn;
+    const uint32_t group = t / width, first = group * G;
+    const uint32_t gm = min(ntm - first, G);
+    const uint32_t in = t % width;
+    tm = first + in % gm;
+    tn = in / gm;
+}
+
+// ------------------------------------------------------------------------------------------
+// plain kernel: 64 x 64 tile, 256 threads (4 x 4 outputs each), BK = 32, register prefetch.
+// ------------------------------------------------------------------------------------------
+constexpr int PM = 64, PN = 64, PK = 32;
+
+__global__ __launch_bounds__(256) void plain_kernel(GemmArgs g, DevMod M) {
+    __shared__ uint32_t As[2][PK][PM + 1];
+    __shared__ uint32_t Bs[2][PK][PN + 1];
+    const uint32_t m = g.m_dev ? min(*g.m_dev, g.m) : g.m;
+    const uint32_t k = g.k_dev ? min(*g.k_dev, g.k) : g.k;
+    const uint32_t n = g.n;
+    const uint32_t ntm = (g.m + PM - 1) / PM, ntn = (n + PN - 1) / PN;
+    uint32_t tm, tn;
+    tile_of(blockIdx.x, ntm, ntn, tm, tn);
+    const uint32_t m0 = tm * PM, n0 = tn * PN;
+    if (m0 >= m) return;
+    const int tid = threadIdx.x, tx = tid % 16, ty = tid / 16;
+
+    uint64_t lo[4][4];
+    uint32_t hi[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) { lo[i][j] = 0; hi[i][j] = 0; }
+
+    uint32_t ra[8], rb[8];
+    auto fetch = [&](uint32_t k0) {
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int e = tid + 256 * t;
+            const uint32_t r = e / PK, c = e % PK;
+            ra[t] = (m0 + r < m && k0 + c < k) ? ld_a(g, m0 + r, k0 + c) : 0u;
+            const uint32_t br = e / PN, bc = e % PN;
+            rb[t] = (k0 + br < k && n0 + bc < n) ? g.B[(int64_t)(k0 + br) * g.brs + (int64_t)(n0 + bc) * g.bcs] : 0u;
+        }
+    };
+    auto stash = [&](int buf) {
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int e = tid + 256 * t;
+            As[buf][e % PK][e / PK] = ra[t];
+            Bs[buf][e / PN][e % PN] = rb[t];
+        }
+    };
+    const uint32_t KT = (k + PK - 1) / PK;
+    if (KT > 0) {
+        fetch(0);
+        stash(0);
+    }
+    __syncthreads();
+    for (uint32_t kt = 0; kt < KT; ++kt) {
+        const int buf = kt & 1;
+        if (kt + 1 < KT) fetch((kt + 1) * PK);
+#pragma unroll 4
+        for (int kk = 0; kk < PK; ++kk) {
+            uint32_t a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = As[buf][kk][ty + 16 * i];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[j] = Bs[buf][kk][tx + 16 * j];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const uint64_t prod = (uint64_t)a[i] * b[j];
+                    const uint64_t s = lo[i][j] + prod;
+                    hi[i][j] += (s < prod) ? 1u : 0u;
+                    lo[i][j] = s;
+                }
+        }
+        if (kt + 1 < KT) stash(buf ^ 1);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const uint32_t row = m0 + ty + 16 * i;
+        if (row >= m) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t col = n0 + tx + 16 * j;
+            if (col >= n) continue;
+            const uint32_t r = red96(hi[i][j], lo[i][j], M);
+            const uint32_t c = g.op == HX_OP_SET ? 0u : ld_cin(g, row, col);
+            g.Cout[(int64_t)row * g.cors + (int64_t)col * g.cocs] = apply_op(c, r, g.op, M.p);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// limb route
+// ------------------------------------------------------------------------------------------
+// Balanced signed base-256 digits of a canonical residue x (x < p): L digits in [-128, 127].
+__device__ __forceinline__ void limbs_of(uint32_t x, uint32_t p, uint32_t half, int L, int32_t* d) {
+    int32_t v = x > half ? (int32_t)((int64_t)x - (int64_t)p) : (int32_t)x;
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        if (s >= L - 1) break;
+        const int32_t t = ((v + 128) & 255) - 128;
+        d[s] = t;
+        v = (v - t) >> 8;
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+        if (s == L - 1) d[s] = v;
+}
+
+struct PrepArgs {
+    const uint32_t* src;
+    int64_t s_r, s_c;             // source strides along output rows / columns
+    const uint32_t* rlist;        // optional gathers (output row r reads source row rlist[r])
+    const uint32_t* clist;
+    uint32_t R, K;                // true output rows / columns (upper bounds if r_dev / k_dev)
+    const uint32_t* r_dev;
+    const uint32_t* k_dev;
+    uint32_t Kp;                  // padded row length of a plane (multiple of 64)
+    int8_t* out;
+    size_t plane;                 // bytes per plane
+    uint32_t p, half;
+    int L;
+};
+
+// One 64 x 64 tile of the output (rows x k) per workgroup: coalesced loads along the source's
+// contiguous direction into LDS, then every thread writes 16 k-bytes of one row per plane.
+__global__ __launch_bounds__(256) void limb_prep_kernel(PrepArgs a) {
+    __shared__ uint32_t T[64][65];
+    const uint32_t r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+    const uint32_t R = a.r_dev ? min(*a.r_dev, a.R) : a.R;
+    const uint32_t K = a.k_dev ? min(*a.k_dev, a.K) : a.K;
+    const int tid = threadIdx.x;
+    const bool ccontig = a.s_c == 1 && a.clist == nullptr;
+#pragma unroll 4
+    for (int t = 0; t < 16; ++t) {
+        const int e = tid + 256 * t;
+        const uint32_t r = ccontig ? e / 64 : e % 64, c = ccontig ? e % 64 : e / 64;
+        uint32_t v = 0;
+        if (r0 + r < R && c0 + c < K) {
+            const int64_t sr = a.rlist ? (int64_t)a.rlist[r0 + r] : (int64_t)(r0 + r);
+            const int64_t sc = a.clist ? (int64_t)a.clist[c0 + c] : (int64_t)(c0 + c);
+            v = a.src[sr * a.s_r + sc * a.s_c];
+        }
+        T[r][c] = v;
+    }
+    __syncthreads();
+    const int r = tid / 4, q = tid % 4;
+    uint32_t w[4][4];   // [plane][word]: 16 k-bytes of this row per plane
+#pragma unroll
+    for (int wd = 0; wd < 4; ++wd) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) w[s][wd] = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            int32_t d[4] = {0, 0, 0, 0};
+            limbs_of(T[r][16 * q + 4 * wd + b], a.p, a.half, a.L, d);
+#pragma unroll
+            for (int s = 0; s < 4; ++s) w[s][wd] |= ((uint32_t)d[s] & 0xffu) << (8 * b);
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        if (s >= a.L) break;
+        v4i vw;
+#pragma unroll
+        for (int wd = 0; wd < 4; ++wd) vw[wd] = (int)w[s][wd];
+        *(v4i*)(a.out + (size_t)s * a.plane + (size_t)(r0 + r) * a.Kp + c0 + 16 * q) = vw;
+    }
+}
+
+constexpr int LM = 64, LN = 128, LK = 64;      // workgroup tile; k-depth of one LDS stage
+constexpr int LROW = 80;                        // LDS bytes per (plane, row): 64 k-bytes + 16 pad
+
+struct LimbArgs {
+    const int8_t* A8;
+    const int8_t* B8;
+    size_t a_plane, b_plane;
+    uint32_t Kp;                 // k-bytes per plane row (multiple of 64)
+    uint32_t m, n;               // true output shape (m an upper bound if m_dev)
+    const uint32_t* m_dev;
+    uint32_t ntm, ntn;
+    const uint32_t* Cin;
+    int64_t cirs, cics;
+    const uint32_t* c_rows;
+    uint32_t* Cout;
+    int64_t cors, cocs;
+    int op;
+    int64_t off;                 // p * ceil(2^57 / p): makes |x| < 2^57 non-negative before Barrett
+    DevMod M;
+};
+
+template <int L>
+__global__ __launch_bounds__(256, 1) void limb_kernel(LimbArgs a) {
+    extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
+    constexpr int A_BYTES = L * LM * LROW, B_BYTES = L * LN * LROW, STAGE = A_BYTES + B_BYTES;
+    constexpr int ND = 2 * L - 1;
+    uint32_t tm, tn;
+    tile_of(blockIdx.x, a.ntm, a.ntn, tm, tn);
+    const uint32_t m = a.m_dev ? min(*a.m_dev, a.m) : a.m;
+    const uint32_t m0 = tm * LM, n0 = tn * LN;
+    if (m0 >= m) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+
+    v16i acc[2][ND];
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+        for (int d = 0; d < ND; ++d)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[cb][d][r] = 0;
+
+    // global -> register staging: L A-chunks and 2L B-chunks of 16 bytes per thread
+    v4i ga[L], gb[2 * L];
+    const int8_t* Abase = a.A8 + (size_t)m0 * a.Kp;
+    const int8_t* Bbase = a.B8 + (size_t)n0 * a.Kp;
+    auto fetch = [&](uint32_t k0) {
+#pragma unroll
+        for (int i = 0; i < L; ++i) {
+            const int c = tid + 256 * i;
+            const int s = c / 256, row = (c % 256) / 4, q = c % 4;
+            ga[i] = *(const v4i*)(Abase + (size_t)s * a.a_plane + (size_t)row * a.Kp + k0 + 16 * q);
+        }
+#pragma unroll
+        for (int i = 0; i < 2 * L; ++i) {
+            const int c = tid + 256 * i;
+            const int s = c / 512, col = (c % 512) / 4, q = c % 4;
+            gb[i] = *(const v4i*)(Bbase + (size_t)s * a.b_plane + (size_t)col * a.Kp + k0 + 16 * q);
+        }
+    };
+    auto stash = [&](int buf) {
+        uint8_t* sa = smem + buf * STAGE;
+        uint8_t* sb = sa + A_BYTES;
+#pragma unroll
+        for (int i = 0; i < L; ++i) {
+            const int c = tid + 256 * i;
+            const int s = c / 256, row = (c % 256) / 4, q = c % 4;
+            *(v4i*)(sa + (s * LM + row) * LROW + 16 * q) = ga[i];
+        }
+#pragma unroll
+        for (int i = 0; i < 2 * L; ++i) {
+            const int c = tid + 256 * i;
+            const int s = c / 512, col = (c % 512) / 4, q = c % 4;
+            *(v4i*)(sb + (s * LN + col) * LROW + 16 * q) = gb[i];
+        }
+    };
+
+    const uint32_t KT = a.Kp / LK;
+    if (KT > 0) {
+        fetch(0);
+        stash(0);
+    }
+    __syncthreads();
+    const int lr = lane & 31, lg = lane >> 5;
+    for (uint32_t kt = 0; kt < KT; ++kt) {
+        const int buf = kt & 1;
+        if (kt + 1 < KT) fetch((kt + 1) * LK);
+        const uint8_t* sa = smem + buf * STAGE;
+        const uint8_t* sb = sa + A_BYTES;
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+            v4i fa[L], fb[2][L];
+#pragma unroll
+            for (int s = 0; s < L; ++s)
+                fa[s] = *(const v4i*)(sa + (s * LM + wm * 32 + lr) * LROW + ks * 32 + lg * 16);
+#pragma unroll
+            for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+                for (int s = 0; s < L; ++s)
+                    fb[cb][s] = *(const v4i*)(sb + (s * LN + wn * 64 + cb * 32 + lr) * LROW + ks * 32 + lg * 16);
+#pragma unroll
+            for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+                for (int s = 0; s < L; ++s)
+#pragma unroll
+                    for (int t = 0; t < L; ++t)
+                        acc[cb][s + t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[s], fb[cb][t], acc[cb][s + t], 0, 0, 0);
+        }
+        if (kt + 1 < KT) stash(buf ^ 1);
+        __syncthreads();
+    }
+
+    // epilogue: exact recombination sum_d 256^d D_d, op into C
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) {
+        const uint32_t col = n0 + wn * 64 + cb * 32 + lr;
+        if (col >= a.n) continue;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const uint32_t row = m0 + wm * 32 + (r / 4) * 8 + lg * 4 + (r % 4);
+            if (row >= m) continue;
+            int64_t x = acc[cb][ND - 1][r];
+            int steps = 0;
+#pragma unroll
+            for (int d = ND - 2; d >= 0; --d) {
+                x = x * 256 + (int64_t)acc[cb][d][r];
+                if (++steps == 3 && d > 0) {
+                    x = (int64_t)red64((uint64_t)(x + a.off), a.M);
+                    steps = 0;
+                }
+            }
+            const uint32_t res = red64((uint64_t)(x + a.off), a.M);
+            uint32_t c = 0;
+            if (a.op != HX_OP_SET) {
+                const int64_t rr = a.c_rows ? (int64_t)a.c_rows[row] : (int64_t)row;
+                c = a.Cin[rr * a.cirs + (int64_t)col * a.cics];
+            }
+            a.Cout[(int64_t)row * a.cors + (int64_t)col * a.cocs] = apply_op(c, res, a.op, a.M.p);
+        }
+    }
+}
+
+template <int L>
+void launch_limb(const LimbArgs& la, hipStream_t s) {
+    constexpr int STAGE = L * (LM + LN) * LROW;
+    const size_t lds = 2 * (size_t)STAGE;
+    static std::once_flag once;
+    std::call_once(once, [&] {
+        (void)hipFuncSetAttribute((const void*)limb_kernel<L>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    });
+    const uint32_t blocks = la.ntm * la.ntn;
+    limb_kernel<L><<<blocks, 256, lds, s>>>(la);
+    after_launch("limb_kernel");
+}
+
+inline uint32_t round_up(uint32_t x, uint32_t a) { return (x + a - 1) / a * a; }
+
+}  // namespace
+
+uint32_t limb_k_chunk(uint32_t limbs) {
+    const uint64_t kmax = 2147483647ull / ((uint64_t)limbs * 16384ull);
+    return (uint32_t)(kmax / 64 * 64);
+}
+
+size_t limb_workspace_bytes(const hx_modctx& ctx, uint32_t m, uint32_t n, uint32_t k) {
+    if (ctx.limbs == 0) return 0;
+    const uint32_t kc = round_up(std::max<uint32_t>(1, std::min(k, limb_k_chunk(ctx.limbs))), 64);
+    return (size_t)ctx.limbs * ((size_t)round_up(std::max<uint32_t>(m, 1), LM) + round_up(std::max<uint32_t>(n, 1), LN)) * kc;
+}
+
+void gemm_plain(const hx_modctx& ctx, const GemmArgs& g, hipStream_t s, int /*tile*/) {
+    if (g.m == 0 || g.n == 0) return;
+    const uint32_t ntm = (g.m + PM - 1) / PM, ntn = (g.n + PN - 1) / PN;
+    plain_kernel<<<ntm * ntn, 256, 0, s>>>(g, devmod(ctx));
+    after_launch("plain_kernel");
+}
+
+void gemm_limb(const hx_modctx& ctx, const GemmArgs& g0, hipStream_t s, void* ws, size_t ws_bytes, int /*tile*/) {
+    const int L = (int)ctx.limbs;
+    HX_CHECK(L >= 1 && L <= 4, "the limb route needs p <= 4278124287 (ctx.limbs > 0)");
+    if (g0.m == 0 || g0.n == 0) return;
+    const uint32_t kchunk = limb_k_chunk(L);
+    HX_CHECK(g0.k_dev == nullptr || g0.k <= kchunk, "device-side k needs k <= one limb chunk");
+    const size_t need = limb_workspace_bytes(ctx, g0.m, g0.n, g0.k);
+    if (ws == nullptr) {
+        ws = scratch(need, 0);
+    } else {
+        HX_CHECK(ws_bytes >= need, "limb workspace too small");
+    }
+    const uint32_t Mp = round_up(g0.m, LM), Np = round_up(g0.n, LN);
+    const uint32_t p = ctx.p, half = (p - 1) / 2;
+    const uint64_t two57 = 1ull << 57;
+    const int64_t off = (int64_t)(((two57 + p - 1) / p) * (uint64_t)p);
+    GemmArgs g = g0;
+    uint32_t k0 = 0;
+    do {
+        const uint32_t kc = std::min(kchunk, g.k - k0);
+        const uint32_t Kp = round_up(kc, 64);
+        int8_t* A8 = (int8_t*)ws;
+        int8_t* B8 = A8 + (size_t)L * Mp * Kp;
+        if (Kp > 0) {
+            PrepArgs pa{};
+            pa.src = g.a_cols ? g.A : g.A + (int64_t)k0 * g.acs;
+            pa.s_r = g.ars;
+            pa.s_c = g.acs;
+            pa.rlist = g.a_rows;
+            pa.clist = g.a_cols ? g.a_cols + k0 : nullptr;
+            pa.R = g.m;
+            pa.K = kc;
+            pa.r_dev = g.m_dev;
+            pa.k_dev = g.k_dev;
+            pa.Kp = Kp;
+            pa.out = A8;
+            pa.plane = (size_t)Mp * Kp;
+            pa.p = p;
+            pa.half = half;
+            pa.L = L;
+            limb_prep_kernel<<<dim3(Kp / 64, Mp / 64), 256, 0, s>>>(pa);
+            after_launch("limb_prep_kernel(A)");
+            PrepArgs pb = pa;
+            pb.src = g.B + (int64_t)k0 * g.brs;
+            pb.s_r = g.bcs;
+            pb.s_c = g.brs;
+            pb.rlist = nullptr;
+            pb.clist = nullptr;
+            pb.R = g.n;
+            pb.r_dev = nullptr;
+            pb.out = B8;
+            pb.plane = (size_t)Np * Kp;
+            limb_prep_kernel<<<dim3(Kp / 64, Np / 64), 256, 0, s>>>(pb);
+            after_launch("limb_prep_kernel(B)");
+        }
+        LimbArgs la{};
+        la.A8 = A8;
+        la.B8 = B8;
+        la.a_plane = (size_t)Mp * Kp;
+        la.b_plane = (size_t)Np * Kp;
+        la.Kp = Kp;
+        la.m = g.m;
+        la.n = g.n;
+        la.m_dev = g.m_dev;
+        la.ntm = Mp / LM;
+        la.ntn = Np / LN;
+        la.Cin = g.Cin;
+        la.cirs = g.cirs;
+        la.cics = g.cics;
+        la.c_rows = g.c_rows;
+        la.Cout = g.Cout;
+        la.cors = g.cors;
+        la.cocs = g.cocs;
+        la.op = g.op;
+        la.off = off;
+        la.M = devmod(ctx);
+        switch (L) {
+            case 1: launch_limb<1>(la, s); break;
+            case 2: launch_limb<2>(la, s); break;
+            case 3: launch_limb<3>(la, s); break;
+            default: launch_limb<4>(la, s); break;
+        }
+        // later chunks accumulate into the C just written
+        k0 += kc;
+        g.op = g.op == HX_OP_SUB ? HX_OP_SUB : HX_OP_ADD;
+        g.Cin = g.Cout;
+        g.cirs = g.cors;
+        g.cics = g.cocs;
+        g.c_rows = nullptr;
+    } while (k0 < g.k);
+}
+
+int gemm_auto_variant(const hx_modctx& ctx, uint32_t m, uint32_t n, uint32_t k) {
+    if (ctx.limbs == 0) return HX_GEMM_PLAIN;
+    const double work = (double)m * n * k;
+    return work >= 4.0e6 ? HX_GEMM_LIMB : HX_GEMM_PLAIN;
+}
+
+void gemm_dispatch(const hx_modctx& ctx, const GemmArgs& g, hipStream_t s, int variant, void* ws, size_t ws_bytes,
+                   int tile) {
+    if (variant == HX_GEMM_AUTO) variant = gemm_auto_variant(ctx, g.m, g.n, g.k);
+    if (variant == HX_GEMM_LIMB)
+        gemm_limb(ctx, g, s, ws, ws_bytes, tile);
+    else
+        gemm_plain(ctx, g, s, tile);
+}
+
+}  // namespace hx
+
+using namespace hx;
+
+namespace {
+
+bool overlap(const hx_view& a, const hx_view& b) {
+    if (a.rows == 0 || a.cols == 0 || b.rows == 0 || b.cols == 0) return false;
+    auto span = [](const hx_view& v, uintptr_t& lo, uintptr_t& hi) {
+        const int64_t i0 = 0, i1 = (int64_t)(v.rows - 1) * v.rs, j1 = (int64_t)(v.cols - 1) * v.cs;
+        const int64_t mn = std::min<int64_t>(i0, std::min<int64_t>(i1, std::min<int64_t>(j1, i1 + j1)));
+        const int64_t mx = std::max<int64_t>(i0, std::max<int64_t>(i1, std::max<int64_t>(j1, i1 + j1)));
+        lo = (uintptr_t)v.base + mn * 4;
+        hi = (uintptr_t)v.base + mx * 4 + 4;
+    };
+    uintptr_t alo, ahi, blo, bhi;
+    span(a, alo, ahi);
+    span(b, blo, bhi);
+    return !(ahi <= blo || bhi <= alo);
+}
+
+GemmArgs args_of(hx_view A, hx_view B, hx_view C, int op) {
+    GemmArgs g;
+    g.m = A.rows;
+    g.k = A.cols;
+    g.n = B.cols;
+    g.A = (const uint32_t*)A.base;
+    g.ars = A.rs;
+    g.acs = A.cs;
+    g.B = (const uint32_t*)B.base;
+    g.brs = B.rs;
+    g.bcs = B.cs;
+    g.Cin = (const uint32_t*)C.base;
+    g.cirs = C.rs;
+    g.cics = C.cs;
+    g.Cout = (uint32_t*)C.base;
+    g.cors = C.rs;
+    g.cocs = C.cs;
+    g.op = op;
+    return g;
+}
+
+}  // namespace
+
+extern "C" {
+
+HX_API size_t hx_gemm_workspace_bytes(const hx_modctx* ctx, uint32_t m, uint32_t n, uint32_t k, int variant) {
+    if (ctx == nullptr) return 0;
+    if (variant == HX_GEMM_AUTO) variant = gemm_auto_variant(*ctx, m, n, k);
+    return variant == HX_GEMM_LIMB ? limb_workspace_bytes(*ctx, m, n, k) : 0;
+}
+
+HX_API int hx_gemm_u32(const hx_modctx* ctx, hx_view A, hx_view B, hx_view C, int op, hx_stream s,
+                       const hx_gemm_opts* opts) {
+    return guard("hx_gemm_u32", [&] {
+        HX_CHECK(ctx != nullptr && ctx->p >= 2, "bad modulus context");
+        HX_CHECK(A.cols == B.rows && C.rows == A.rows && C.cols == B.cols, "gemm: shape mismatch");
+        HX_CHECK(op == HX_OP_SET || op == HX_OP_ADD || op == HX_OP_SUB, "gemm: bad op");
+        HX_CHECK(!overlap(C, A) && !overlap(C, B), "gemm: C aliases an operand");
+        HX_CHECK(C.rows == 0 || C.cols == 0 || C.base != nullptr, "gemm: C is NULL");
+        const int variant = opts ? opts->variant : HX_GEMM_AUTO;
+        HX_CHECK(variant != HX_GEMM_LIMB || ctx->limbs > 0, "gemm: the limb route needs p <= 4278124287");
+        gemm_dispatch(*ctx, args_of(A, B, C, op), (hipStream_t)s, variant, opts ? opts->workspace : nullptr,
+                      opts ? opts->workspace_bytes : 0, opts ? opts->tile : 0);
+    });
+}
+
+HX_API int hx_schur_update_u32(const hx_modctx* ctx, hx_view L, hx_view U, hx_view C, hx_stream s,
+                               const hx_gemm_opts* opts) {
+    return hx_gemm_u32(ctx, L, U, C, HX_OP_SUB, s, opts);
+}
+
+}  // extern "C"

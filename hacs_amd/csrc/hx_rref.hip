@@ -1,0 +1,127 @@
+// hx_rref.hip -- the single-matrix RREF entry points (imma_rref's role, Cuda/CudaKernels.h:249-298,
+// generalised from p <= 127 to every prime p < 2^31).
+#include <vector>
+
+#include "hx_common.h"
+
+namespace hx {
+int rref_batch_create_impl(const hx_modctx& ctx, const hx_rref_job* jobs, uint32_t njobs, void** out);
+int rref_batch_run_impl(void* b, hipStream_t s);
+int rref_batch_status_impl(void* b, hipStream_t s);
+void rref_batch_destroy_impl(void* b);
+// The multi-kernel path for large matrices (hx_rref_large.hip); false when it does not take the shape.
+bool rref_large(const hx_modctx& ctx, uint32_t m, uint32_t n, const uint32_t* A, int64_t lda, uint32_t* R,
+                int64_t ldr, uint32_t* rank, uint32_t* pivots, uint32_t* pivot_rows, hipStream_t s);
+}  // namespace hx
+
+using namespace hx;
+
+namespace {
+
+// One job through the persistent workgroup kernel: R <- A, then R is eliminated in place.
+void rref_via_batch(const hx_modctx& ctx, uint32_t m, uint32_t n, const uint32_t* A, int64_t lda, uint32_t* R,
+                    int64_t ldr, uint32_t* rank, uint32_t* pivots, uint32_t* pivot_rows, hipStream_t s) {
+    uint32_t* work = nullptr;
+    uint32_t* dst = R;
+    int64_t ldd = ldr;
+    const uint32_t mn = std::min(m, n);
+    const size_t small = (size_t)2 * std::max<uint32_t>(mn, 1) + 1;
+    if (dst == nullptr) {
+        HX_HIP(hipMalloc(&work, ((size_t)m * n + small) * 4));
+        dst = work;
+        ldd = n;
+    } else {
+        HX_HIP(hipMalloc(&work, small * 4));
+    }
+    void* b = nullptr;
+    try {
+        HX_HIP(hipMemcpy2DAsync(dst, (size_t)ldd * 4, A, (size_t)lda * 4, (size_t)n * 4, m, hipMemcpyDeviceToDevice, s));
+        uint32_t* d_piv = dst == work ? work + (size_t)m * n : work;
+        uint32_t* d_prow = d_piv + std::max<uint32_t>(mn, 1);
+        uint32_t* d_rank = d_prow + std::max<uint32_t>(mn, 1);
+        hx_rref_job job{m, n, dst, ldd, d_piv, d_prow, d_rank};
+        rref_batch_create_impl(ctx, &job, 1, &b);
+        rref_batch_run_impl(b, s);
+        rref_batch_status_impl(b, s);
+        HX_HIP(hipMemcpyAsync(rank, d_rank, 4, hipMemcpyDeviceToHost, s));
+        HX_HIP(hipStreamSynchronize(s));
+        if (*rank) {
+            if (pivots) HX_HIP(hipMemcpyAsync(pivots, d_piv, (size_t)*rank * 4, hipMemcpyDeviceToHost, s));
+            if (pivot_rows) HX_HIP(hipMemcpyAsync(pivot_rows, d_prow, (size_t)*rank * 4, hipMemcpyDeviceToHost, s));
+            HX_HIP(hipStreamSynchronize(s));
+        }
+    } catch (...) {
+        if (b) rref_batch_destroy_impl(b);
+        (void)hipFree(work);
+        throw;
+    }
+    rref_batch_destroy_impl(b);
+    HX_HIP(hipFree(work));
+}
+
+}  // namespace
+
+extern "C" {
+
+HX_API int hx_rref_u32(const hx_modctx* ctx, uint32_t m, uint32_t n, const uint32_t* A, int64_t lda, uint32_t* R,
+                       int64_t ldr, uint32_t* rank, uint32_t* pivots, uint32_t* pivot_rows, hx_stream s) {
+    return guard("hx_rref_u32", [&] {
+        HX_CHECK(ctx != nullptr && rank != nullptr, "NULL argument");
+        HX_CHECK(ctx->p >= 2 && ctx->p < 0x80000000u, "rref: needs a prime 2 <= p < 2^31");
+        HX_CHECK(lda >= (int64_t)n && (R == nullptr || ldr >= (int64_t)n), "rref: leading dimension < n");
+        *rank = 0;
+        if (m == 0 || n == 0) return;
+        HX_CHECK(A != nullptr, "rref: A is NULL");
+        if (rref_large(*ctx, m, n, A, lda, R, ldr, rank, pivots, pivot_rows, (hipStream_t)s)) return;
+        rref_via_batch(*ctx, m, n, A, lda, R, ldr, rank, pivots, pivot_rows, (hipStream_t)s);
+    });
+}
+
+HX_API int hx_rref_u32_host(const hx_modctx* ctx, uint32_t m, uint32_t n, hx_row_source src, hx_row_sink sink,
+                            void* user, uint32_t slab_rows, uint32_t* rank, uint32_t* pivots, uint32_t* pivot_rows) {
+    return guard("hx_rref_u32_host", [&] {
+        HX_CHECK(ctx != nullptr && src != nullptr && rank != nullptr, "NULL argument");
+        *rank = 0;
+        if (m == 0 || n == 0) return;
+        if (slab_rows == 0) slab_rows = (uint32_t)std::max<uint64_t>(1, (64ull << 20) / (4ull * n));
+        uint32_t* dA = nullptr;
+        uint32_t* dR = nullptr;
+        hipStream_t s;
+        HX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        try {
+            HX_HIP(hipMalloc(&dA, (size_t)m * n * 4));
+            if (sink) HX_HIP(hipMalloc(&dR, (size_t)m * n * 4));
+            for (uint32_t r0 = 0; r0 < m; r0 += slab_rows) {
+                const uint32_t nr = std::min(slab_rows, m - r0);
+                int64_t ld = n;
+                const uint32_t* rows = src(user, r0, nr, &ld);
+                HX_CHECK(rows != nullptr && ld >= (int64_t)n, "row source returned no rows");
+                HX_HIP(hipMemcpy2DAsync(dA + (size_t)r0 * n, (size_t)n * 4, rows, (size_t)ld * 4, (size_t)n * 4, nr,
+                                        hipMemcpyHostToDevice, s));
+                HX_HIP(hipStreamSynchronize(s));   // the source view is valid until the next call
+            }
+            const int rc = hx_rref_u32(ctx, m, n, dA, n, dR, n, rank, pivots, pivot_rows, (hx_stream)s);
+            if (rc != HX_OK) throw Error{rc, hx_last_error()};
+            if (sink && *rank > 0) {
+                int64_t ld = n;
+                uint32_t* out = sink(user, *rank, &ld);
+                if (out) {
+                    HX_CHECK(ld >= (int64_t)n, "row sink: ld < n");
+                    HX_HIP(hipMemcpy2DAsync(out, (size_t)ld * 4, dR, (size_t)n * 4, (size_t)n * 4, *rank,
+                                            hipMemcpyDeviceToHost, s));
+                    HX_HIP(hipStreamSynchronize(s));
+                }
+            }
+        } catch (...) {
+            (void)hipFree(dA);
+            (void)hipFree(dR);
+            (void)hipStreamDestroy(s);
+            throw;
+        }
+        HX_HIP(hipFree(dA));
+        if (dR) HX_HIP(hipFree(dR));
+        HX_HIP(hipStreamDestroy(s));
+    });
+}
+
+}  // extern "C"

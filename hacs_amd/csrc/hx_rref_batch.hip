@@ -1,0 +1,480 @@
+// hx_rref_batch.hip -- RREF over F_p for many independent, uneven matrices in ONE persistent launch.
+//
+// One 256-thread workgroup eliminates one job at a time and pulls the next job index from an atomic
+// counter (jobs pre-sorted by decreasing work), so thousands of jobs of shapes 16..1024 share the
+// chip without a launch per job.  Per job (in place on its buffer, like elim::rref, DenseElim.h:149-151):
+//
+//   forward, in column panels of w <= 32 columns:
+//     (a) the panel of every still-unused row goes to LDS; per column the pivot is the unused row of
+//         SMALLEST ORIGINAL INDEX with a non-zero Schur entry (the reference's pivot_rows rule,
+//         Cuda/CudaKernels.h:255-265; rows never move, so list order == original order), found by a
+//         wave-min + one barrier; the other rows are eliminated division-free inside the panel
+//         (x <- (v x - f y) R^-1, Montgomery): only zero tests are read from this copy.
+//     (c) the k pivot rows of the panel (their pre-panel values, columns c..n) go to LDS and are reduced
+//         to X = S[P,J]^{-1} S[P, c:n]: division-free forward over the pivot order, one inverse per
+//         pivot (all in parallel), back substitution -- X has the identity at the panel's pivot columns.
+//     (d) trailing update of every other unused row: S[i, j] -= sum_t S[i, J_t] X[t, j] for j past the
+//         panel, exact u64+carry dot products, one Barrett reduction per entry (the Schur complement
+//         w.r.t. the panel pivots; the panel's own columns of those rows become zero and are dropped).
+//     (e) X is stored as the next k rows of the U workspace (pivot order).
+//   back substitution: U's rows have the identity at their own panel's pivot columns; panels are
+//   cleared against all later rows from the bottom (non-pivot columns only: pivot columns are 0 / 1).
+//   The rank rows of U are written over the job's matrix, then zero rows (RREF with zero rows last).
+//
+// Every value is exact; the RREF, its pivot columns and the smallest-row pivot_rows are unique
+// functions of the input, so the result is bit-identical to the CPU engine's.
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+#include "hx_elim.h"
+
+namespace hx {
+namespace {
+
+constexpr int RT = 256;   // threads per job workgroup
+constexpr int RW = 32;    // widest panel
+
+struct RJob {
+    uint32_t m, n;
+    uint32_t* A;
+    int64_t lda;
+    uint32_t* pivots;
+    uint32_t* pivot_rows;
+    uint32_t* rank;
+    uint32_t* U;          // min(m, n) x n, row stride n
+};
+
+struct RArgs {
+    const RJob* jobs;
+    const uint32_t* order;
+    uint32_t njobs;
+    uint32_t* next;
+    int* status;
+    uint32_t region;      // LDS words for the panel / pivot-row block
+    uint32_t mmax, nmax;
+    DevMod M;
+};
+
+__device__ __forceinline__ uint32_t wave_min(uint32_t x) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x = min(x, (uint32_t)__shfl_xor((int)x, off));
+    return x;
+}
+
+template <class Ar>
+__device__ void rref_one(const RJob& job, const Ar& ar, const RArgs& a, uint32_t* sm) {
+    __shared__ uint32_t s_cand[2][RT / 64];
+    __shared__ uint32_t s_P[RW], s_q[RW], s_inv[RW], s_fp[RW];
+    __shared__ uint32_t s_chunk[RT];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t m = job.m, n = job.n, p = ar.p;
+    const int64_t lda = job.lda;
+    uint32_t* pan = sm;
+    uint32_t* list = sm + a.region;
+    uint8_t* chosen = (uint8_t*)(list + a.mmax);
+    uint8_t* ispiv = chosen + a.mmax;
+    if (m == 0 || n == 0) {
+        if (tid == 0) *job.rank = 0;
+        return;
+    }
+    uint32_t w = RW;
+    while (w > 1 && ((uint64_t)m * (w | 1) > a.region || (uint64_t)w * n > a.region)) w >>= 1;
+    for (uint32_t i = tid; i < m; i += RT) list[i] = i;
+    for (uint32_t j = tid; j < n; j += RT) ispiv[j] = 0;
+    __syncthreads();
+
+    uint32_t mu = m, r = 0;
+    for (uint32_t c = 0; c < n && mu > 0; c += w) {
+        const uint32_t wc = min(w, n - c), ws = wc | 1u, nb = n - c;
+        // (a) panel of the unused rows, pivot search by the smallest-row rule
+        for (uint32_t idx = tid; idx < mu * wc; idx += RT) {
+            const uint32_t u = idx / wc, q = idx % wc;
+            pan[u * ws + q] = job.A[(int64_t)list[u] * lda + c + q];
+        }
+        for (uint32_t u = tid; u < mu; u += RT) chosen[u] = 0;
+        __syncthreads();
+        uint32_t k = 0;
+        for (uint32_t q = 0; q < wc; ++q) {
+            uint32_t best = 0xffffffffu;
+            for (uint32_t u = tid; u < mu; u += RT)
+                if (!chosen[u] && pan[u * ws + q] != 0) { best = u; break; }
+            best = wave_min(best);
+            if (lane == 0) s_cand[q & 1][wave] = best;
+            __syncthreads();
+            const uint32_t bu = min(min(s_cand[q & 1][0], s_cand[q & 1][1]), min(s_cand[q & 1][2], s_cand[q & 1][3]));
+            if (bu == 0xffffffffu) continue;
+            const uint32_t* prow = pan + bu * ws;
+            const uint32_t v = prow[q];
+            for (uint32_t u = tid; u < mu; u += RT) {
+                if (u == bu) { chosen[u] = 1; continue; }
+                if (chosen[u]) continue;
+                uint32_t* row = pan + u * ws;
+                const uint32_t f = row[q];
+                if (f == 0) continue;
+                const uint32_t fneg = p - f;
+                for (uint32_t q2 = q + 1; q2 < wc; ++q2) row[q2] = ar.dfree(v, row[q2], fneg, prow[q2]);
+            }
+            if (tid == 0) { s_P[k] = list[bu]; s_q[k] = q; }
+            ++k;
+        }
+        __syncthreads();
+        if (k == 0) continue;
+
+        // (c) the pivot rows, reduced to X (identity at the panel's pivot columns)
+        uint32_t* B = pan;
+        for (uint32_t idx = tid; idx < k * nb; idx += RT) {
+            const uint32_t t = idx / nb, j = idx % nb;
+            B[t * nb + j] = job.A[(int64_t)s_P[t] * lda + c + j];
+        }
+        if (tid < k) ispiv[c + s_q[tid]] = 1;
+        __syncthreads();
+        for (uint32_t t = 0; t + 1 < k; ++t) {       // division-free forward over the pivot order
+            const uint32_t qt = s_q[t], wdt = nb - qt - 1, cnt = (k - t - 1) * wdt;
+            const uint32_t v = B[t * nb + qt];
+            for (uint32_t it = tid; it < cnt; it += RT) {
+                const uint32_t row = t + 1 + it / wdt, j = qt + 1 + it % wdt;
+                const uint32_t f = B[row * nb + qt];
+                if (f != 0) B[row * nb + j] = ar.dfree(v, B[row * nb + j], p - f, B[t * nb + j]);
+            }
+            __syncthreads();
+        }
+        if (tid < k) {                                // one inverse per pivot, all at once
+            const uint32_t d = B[tid * nb + s_q[tid]];
+            const uint32_t iv = invm(d, p);
+            if (iv == 0) atomicExch(a.status, 1);
+            s_inv[tid] = ar.prep(iv);
+        }
+        __syncthreads();
+        for (uint32_t idx = tid; idx < k * nb; idx += RT) {
+            const uint32_t t = idx / nb, j = idx % nb;
+            if (j > s_q[t]) B[idx] = ar.mul(B[idx], s_inv[t]);
+        }
+        __syncthreads();
+        for (uint32_t t = k - 1; t >= 1; --t) {       // back substitution, non-pivot columns only
+            const uint32_t qt = s_q[t], wdt = nb - qt - 1, cnt = t * wdt;
+            if (tid < t) s_fp[tid] = ar.prep(B[tid * nb + qt]);
+            __syncthreads();
+            for (uint32_t it = tid; it < cnt; it += RT) {
+                const uint32_t row = it / wdt, j = qt + 1 + it % wdt;
+                if (ispiv[c + j]) continue;
+                const uint32_t fp = s_fp[row];
+                if (fp != 0) B[row * nb + j] = subm(B[row * nb + j], ar.mul(B[t * nb + j], fp), p);
+            }
+            __syncthreads();
+        }
+        for (uint32_t idx = tid; idx < k * k; idx += RT) {
+            const uint32_t t2 = idx / k, t = idx % k;
+            B[t2 * nb + s_q[t]] = t2 == t ? 1u : 0u;
+        }
+        __syncthreads();
+
+        // (e) U rows, pivots, pivot rows
+        for (uint32_t idx = tid; idx < k * n; idx += RT) {
+            const uint32_t t = idx / n, j = idx % n;
+            job.U[(size_t)(r + t) * n + j] = j < c ? 0u : B[t * nb + (j - c)];
+        }
+        if (tid < k) {
+            job.pivots[r + tid] = c + s_q[tid];
+            job.pivot_rows[r + tid] = s_P[tid];
+        }
+        // (d) trailing update of the other unused rows: S[i, j] -= sum_t S[i, J_t] X[t, j]
+        const uint32_t tc = nb - wc;
+        if (tc > 0) {
+            for (uint32_t u = wave; u < mu; u += RT / 64) {
+                if (chosen[u]) continue;
+                const int64_t row = (int64_t)list[u] * lda;
+                uint32_t L[RW];
+                bool any = false;
+#pragma unroll
+                for (int t = 0; t < RW; ++t) {
+                    L[t] = (uint32_t)t < k ? job.A[row + c + s_q[t]] : 0u;
+                    any |= L[t] != 0;
+                }
+                if (!any) continue;
+                for (uint32_t jj = lane; jj < tc; jj += 64) {
+                    const uint32_t j = wc + jj;
+                    uint64_t lo = 0;
+                    uint32_t hi = 0;
+#pragma unroll
+                    for (int t = 0; t < RW; ++t) {
+                        if ((uint32_t)t >= k) break;
+                        const uint64_t prod = (uint64_t)L[t] * B[t * nb + j];
+                        lo += prod;
+                        hi += lo < prod ? 1u : 0u;
+                    }
+                    uint32_t* x = job.A + row + c + j;
+                    *x = subm(*x, red96(hi, lo, a.M), p);
+                }
+            }
+        }
+        __syncthreads();
+        // stable compaction of the unused list (chosen rows leave)
+        {
+            const uint32_t per = (mu + RT - 1) / RT, s0 = min(tid * per, mu), s1 = min(s0 + per, mu);
+            uint32_t keep = 0;
+            for (uint32_t u = s0; u < s1; ++u) keep += chosen[u] ? 0u : 1u;
+            s_chunk[tid] = keep;
+            __syncthreads();
+            if (tid == 0) {
+                uint32_t acc = 0;
+                for (int t = 0; t < RT; ++t) { const uint32_t x = s_chunk[t]; s_chunk[t] = acc; acc += x; }
+            }
+            __syncthreads();
+            uint32_t vals[16];
+            uint32_t nv = 0;
+            for (uint32_t u = s0; u < s1; ++u)
+                if (!chosen[u]) vals[nv++] = list[u];
+            __syncthreads();
+            const uint32_t base = s_chunk[tid];
+            for (uint32_t t = 0; t < nv; ++t) list[base + t] = vals[t];
+            __syncthreads();
+        }
+        mu -= k;
+        r += k;
+    }
+
+    // back substitution across panels (U rows in pivot order; panel of pivot t = pivots[t] / w)
+    uint32_t* piv = list;
+    for (uint32_t t = tid; t < r; t += RT) piv[t] = job.pivots[t];
+    __syncthreads();
+    uint32_t t1 = r;
+    while (t1 > 0) {
+        const uint32_t b = piv[t1 - 1] / w;
+        uint32_t t0 = t1 - 1;
+        while (t0 > 0 && piv[t0 - 1] / w == b) --t0;
+        if (t1 < r) {
+            const uint32_t jstart = (b + 1) * w;
+            for (uint32_t t = t0 + wave; t < t1; t += RT / 64) {
+                uint32_t* Ut = job.U + (size_t)t * n;
+                for (uint32_t j0 = jstart; j0 < n; j0 += 64 * 8) {
+                    uint64_t lo[8];
+                    uint32_t hi[8];
+#pragma unroll
+                    for (int s = 0; s < 8; ++s) { lo[s] = 0; hi[s] = 0; }
+                    for (uint32_t t2 = t1; t2 < r; ++t2) {
+                        const uint32_t coef = Ut[piv[t2]];
+                        if (coef == 0) continue;
+                        const uint32_t* U2 = job.U + (size_t)t2 * n;
+#pragma unroll
+                        for (int s = 0; s < 8; ++s) {
+                            const uint32_t j = j0 + lane + 64 * s;
+                            if (j < n) {
+                                const uint64_t prod = (uint64_t)coef * U2[j];
+                                lo[s] += prod;
+                                hi[s] += lo[s] < prod ? 1u : 0u;
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int s = 0; s < 8; ++s) {
+                        const uint32_t j = j0 + lane + 64 * s;
+                        if (j < n && !ispiv[j]) Ut[j] = subm(Ut[j], red96(hi[s], lo[s], a.M), p);
+                    }
+                }
+            }
+            __syncthreads();
+            for (uint32_t idx = tid; idx < (t1 - t0) * (r - t1); idx += RT) {
+                const uint32_t t = t0 + idx / (r - t1), t2 = t1 + idx % (r - t1);
+                job.U[(size_t)t * n + piv[t2]] = 0;
+            }
+            __syncthreads();
+        }
+        t1 = t0;
+    }
+    // the RREF over the job's matrix: rank rows of U, then zero rows
+    for (uint64_t idx = tid; idx < (uint64_t)m * n; idx += RT) {
+        const uint32_t i = (uint32_t)(idx / n), j = (uint32_t)(idx % n);
+        job.A[(int64_t)i * lda + j] = i < r ? job.U[(size_t)i * n + j] : 0u;
+    }
+    if (tid == 0) *job.rank = r;
+    __syncthreads();
+}
+
+template <class Ar>
+__global__ __launch_bounds__(RT) void rref_batch_kernel(RArgs a, Ar ar) {
+    extern __shared__ __attribute__((aligned(16))) uint32_t sm[];
+    __shared__ uint32_t s_job;
+    for (;;) {
+        if (threadIdx.x == 0) s_job = atomicAdd(a.next, 1u);
+        __syncthreads();
+        const uint32_t jj = s_job;
+        __syncthreads();
+        if (jj >= a.njobs) break;
+        rref_one(a.jobs[a.order[jj]], ar, a, sm);
+    }
+}
+
+__global__ void zero_u32_kernel(uint32_t* x, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = 0;
+}
+
+struct RBatch {
+    hx_modctx ctx;
+    uint32_t njobs = 0;
+    RJob* d_jobs = nullptr;
+    uint32_t* d_order = nullptr;
+    uint32_t* d_counters = nullptr;     // [0] next job, [1] status
+    uint32_t* d_U = nullptr;
+    uint32_t region = 0, mmax = 0, nmax = 0;
+    size_t lds = 0;
+    uint32_t grid = 0;
+};
+
+constexpr size_t kLdsCap = 156 * 1024;
+
+}  // namespace
+
+// Shared with the single-matrix entry point.
+int rref_batch_create_impl(const hx_modctx& ctx, const hx_rref_job* jobs, uint32_t njobs, void** out);
+int rref_batch_run_impl(void* b, hipStream_t s);
+void rref_batch_destroy_impl(void* b);
+
+int rref_batch_create_impl(const hx_modctx& ctx, const hx_rref_job* jobs, uint32_t njobs, void** out) {
+    HX_CHECK(ctx.p >= 2 && ctx.p < 0x80000000u, "rref: needs a prime 2 <= p < 2^31");
+    HX_CHECK(jobs != nullptr || njobs == 0, "jobs is NULL");
+    RBatch* b = new RBatch();
+    try {
+        b->ctx = ctx;
+        b->njobs = njobs;
+        std::vector<RJob> hj(njobs);
+        size_t ucount = 0;
+        uint32_t mmax = 1, nmax = 1;
+        for (uint32_t i = 0; i < njobs; ++i) {
+            const hx_rref_job& j = jobs[i];
+            HX_CHECK(j.m == 0 || j.n == 0 || (j.A && j.pivots && j.pivot_rows && j.rank), "rref job: NULL pointer");
+            HX_CHECK(j.lda >= (int64_t)j.n, "rref job: lda < n");
+            HX_CHECK(j.m <= 4096, "rref job: m > 4096 (use hx_rref_u32)");
+            HX_CHECK(j.rank != nullptr, "rref job: rank is NULL");
+            hj[i] = RJob{j.m, j.n, j.A, j.lda, j.pivots, j.pivot_rows, j.rank, nullptr};
+            ucount += (size_t)std::min(j.m, j.n) * j.n;
+            mmax = std::max(mmax, j.m);
+            nmax = std::max(nmax, j.n);
+        }
+        b->mmax = (mmax + 3) / 4 * 4;
+        b->nmax = (nmax + 3) / 4 * 4;
+        const size_t fixed = (size_t)b->mmax * 4 + b->mmax + b->nmax + 64;
+        HX_CHECK(fixed + 4 * 64 < kLdsCap, "rref batch: job too tall for the LDS row lists");
+        size_t want = std::max<size_t>((size_t)mmax * (RW | 1), (size_t)RW * nmax);
+        const size_t cap_words = (kLdsCap - fixed) / 4;
+        b->region = (uint32_t)std::min(want, cap_words);
+        HX_CHECK(b->region >= std::max<size_t>(mmax + 1, nmax), "rref batch: shape exceeds the LDS panel");
+        b->lds = (size_t)b->region * 4 + fixed;
+        HX_HIP(hipMalloc(&b->d_U, std::max<size_t>(ucount, 1) * 4));
+        size_t off = 0;
+        for (uint32_t i = 0; i < njobs; ++i) {
+            hj[i].U = b->d_U + off;
+            off += (size_t)std::min(hj[i].m, hj[i].n) * hj[i].n;
+        }
+        std::vector<uint32_t> order(njobs);
+        std::iota(order.begin(), order.end(), 0u);
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+            const double wx = (double)hj[x].m * hj[x].n * std::min(hj[x].m, hj[x].n);
+            const double wy = (double)hj[y].m * hj[y].n * std::min(hj[y].m, hj[y].n);
+            return wx > wy;
+        });
+        HX_HIP(hipMalloc(&b->d_jobs, std::max<size_t>(njobs, 1) * sizeof(RJob)));
+        HX_HIP(hipMalloc(&b->d_order, std::max<size_t>(njobs, 1) * sizeof(uint32_t)));
+        HX_HIP(hipMalloc(&b->d_counters, 2 * sizeof(uint32_t)));
+        if (njobs) {
+            HX_HIP(hipMemcpy(b->d_jobs, hj.data(), njobs * sizeof(RJob), hipMemcpyHostToDevice));
+            HX_HIP(hipMemcpy(b->d_order, order.data(), njobs * sizeof(uint32_t), hipMemcpyHostToDevice));
+        }
+        HX_HIP(hipMemset(b->d_counters, 0, 2 * sizeof(uint32_t)));
+        int cus = 256;
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, 0) == hipSuccess) cus = prop.multiProcessorCount;
+        const uint32_t per_cu = std::max<uint32_t>(1, (uint32_t)(160 * 1024 / std::max<size_t>(b->lds, 1)));
+        b->grid = std::max<uint32_t>(1, std::min<uint32_t>(njobs, (uint32_t)cus * std::min<uint32_t>(per_cu, 4)));
+        static bool attr_done = false;
+        if (!attr_done) {
+            (void)hipFuncSetAttribute((const void*)rref_batch_kernel<ArMont>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)kLdsCap);
+            (void)hipFuncSetAttribute((const void*)rref_batch_kernel<ArBarrett>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsCap);
+            attr_done = true;
+        }
+    } catch (...) {
+        rref_batch_destroy_impl(b);
+        throw;
+    }
+    *out = b;
+    return HX_OK;
+}
+
+int rref_batch_run_impl(void* vb, hipStream_t s) {
+    RBatch* b = (RBatch*)vb;
+    if (b->njobs == 0) return HX_OK;
+    zero_u32_kernel<<<1, 64, 0, s>>>(b->d_counters, 2);   // job counter and status (a kernel, not a memset: capturable)
+    after_launch("zero_u32_kernel");
+    RArgs a;
+    a.jobs = b->d_jobs;
+    a.order = b->d_order;
+    a.njobs = b->njobs;
+    a.next = b->d_counters;
+    a.status = (int*)(b->d_counters + 1);
+    a.region = b->region;
+    a.mmax = b->mmax;
+    a.nmax = b->nmax;
+    a.M = devmod(b->ctx);
+    if (b->ctx.p & 1u) {
+        rref_batch_kernel<ArMont><<<b->grid, RT, b->lds, s>>>(a, make_mont(b->ctx.p));
+    } else {
+        rref_batch_kernel<ArBarrett><<<b->grid, RT, b->lds, s>>>(a, make_barrett(b->ctx.p));
+    }
+    after_launch("rref_batch_kernel");
+    return HX_OK;
+}
+
+int rref_batch_status_impl(void* vb, hipStream_t s) {
+    RBatch* b = (RBatch*)vb;
+    uint32_t st = 0;
+    HX_HIP(hipMemcpyAsync(&st, b->d_counters + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HX_HIP(hipStreamSynchronize(s));
+    if (st != 0) throw Error{HX_E_NOT_INVERTIBLE, "a pivot had no inverse mod p (is p prime?)"};
+    return HX_OK;
+}
+
+void rref_batch_destroy_impl(void* vb) {
+    RBatch* b = (RBatch*)vb;
+    if (!b) return;
+    if (b->d_jobs) (void)hipFree(b->d_jobs);
+    if (b->d_order) (void)hipFree(b->d_order);
+    if (b->d_counters) (void)hipFree(b->d_counters);
+    if (b->d_U) (void)hipFree(b->d_U);
+    delete b;
+}
+
+}  // namespace hx
+
+using namespace hx;
+
+extern "C" {
+
+HX_API int hx_rref_batch_create(const hx_modctx* ctx, const hx_rref_job* jobs, uint32_t njobs, int /*variant*/,
+                                hx_rref_batch* out) {
+    return guard("hx_rref_batch_create", [&] {
+        HX_CHECK(ctx != nullptr && out != nullptr, "NULL argument");
+        void* b = nullptr;
+        rref_batch_create_impl(*ctx, jobs, njobs, &b);
+        *out = b;
+    });
+}
+HX_API int hx_rref_batch_run(hx_rref_batch b, hx_stream s) {
+    return guard("hx_rref_batch_run", [&] {
+        HX_CHECK(b != nullptr, "NULL batch");
+        rref_batch_run_impl(b, (hipStream_t)s);
+    });
+}
+HX_API int hx_rref_batch_status(hx_rref_batch b, hx_stream s) {
+    return guard("hx_rref_batch_status", [&] {
+        HX_CHECK(b != nullptr, "NULL batch");
+        rref_batch_status_impl(b, (hipStream_t)s);
+    });
+}
+HX_API int hx_rref_batch_destroy(hx_rref_batch b) {
+    return guard("hx_rref_batch_destroy", [&] { rref_batch_destroy_impl(b); });
+}
+
+}  // extern "C"

@@ -1,0 +1,274 @@
+// hx_runtime.hip -- the runtime part of the C-ABI: availability, errors, the modulus,
+// streams, events, memory, transfers, graph capture and device input generation.
+// Replaces engine::cuda::CudaCtx / GraphCapture (Cuda/CudaCtx.h:51-253) for one
+// process on one GPU.  Transfers go straight through hipMemcpyAsync: on MI355X the
+// runtime stages pageable copies itself, and pinned buffers are the caller's choice.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "hx_common.h"
+
+namespace hx {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+namespace {
+
+struct Avail {
+    bool ok = false;
+    std::string reason;
+};
+
+const Avail& avail() {
+    static Avail a = [] {
+        Avail r;
+        int n = 0;
+        hipError_t e = hipGetDeviceCount(&n);
+        if (e != hipSuccess) {
+            r.reason = std::string("hipGetDeviceCount: ") + hipGetErrorString(e);
+        } else if (n <= 0) {
+            r.reason = "no HIP device";
+        } else {
+            hipDeviceProp_t prop;
+            e = hipGetDeviceProperties(&prop, 0);
+            if (e != hipSuccess) {
+                r.reason = std::string("hipGetDeviceProperties: ") + hipGetErrorString(e);
+            } else if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+                r.reason = std::string("device 0 is ") + prop.gcnArchName + ", this build targets gfx950 only";
+            } else {
+                r.ok = true;
+            }
+        }
+        return r;
+    }();
+    return a;
+}
+
+struct ScratchSlot {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+std::mutex g_scratch_mu;
+std::vector<ScratchSlot> g_scratch(16);
+
+}  // namespace
+
+void* scratch(size_t bytes, int slot) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    ScratchSlot& s = g_scratch.at(slot);
+    if (s.bytes < bytes) {
+        if (s.ptr) HX_HIP(hipFree(s.ptr));
+        s.ptr = nullptr;
+        s.bytes = 0;
+        const size_t want = bytes + bytes / 4 + 4096;
+        if (hipMalloc(&s.ptr, want) != hipSuccess) {
+            (void)hipGetLastError();
+            throw Error{HX_E_NOMEM, "library scratch allocation of " + std::to_string(want) + " bytes failed"};
+        }
+        s.bytes = want;
+    }
+    return s.ptr;
+}
+
+}  // namespace hx
+
+using namespace hx;
+
+namespace {
+
+__global__ void random_entries_kernel(uint32_t* out, uint32_t rows, uint32_t cols, int64_t ld, uint32_t ell,
+                                      uint64_t seed) {
+    const uint64_t total = (uint64_t)rows * cols;
+    for (uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (uint64_t)gridDim.x * blockDim.x) {
+        // SplitMix64's state after idx + 1 draws is seed + (idx + 1) * gamma: every draw is independent.
+        uint64_t z = seed + (idx + 1) * 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        const uint32_t i = (uint32_t)(idx / cols), j = (uint32_t)(idx % cols);
+        out[(int64_t)i * ld + j] = (uint32_t)(z % ell);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+HX_API int hx_available(void) { return avail().ok ? 1 : 0; }
+HX_API const char* hx_unavailable_reason(void) { return avail().ok ? "" : avail().reason.c_str(); }
+HX_API const char* hx_last_error(void) { return g_last_error.c_str(); }
+HX_API const char* hx_version(void) { return "hacs_mi355x 0.1 (gfx950)"; }
+
+HX_API int hx_device_info_get(hx_device_info* out) {
+    return guard("hx_device_info_get", [&] {
+        HX_CHECK(out != nullptr, "out is NULL");
+        if (!avail().ok) throw Error{HX_E_UNAVAILABLE, avail().reason};
+        hipDeviceProp_t prop;
+        HX_HIP(hipGetDeviceProperties(&prop, 0));
+        std::memset(out, 0, sizeof(*out));
+        std::strncpy(out->name, prop.name, sizeof(out->name) - 1);
+        std::strncpy(out->arch, prop.gcnArchName, sizeof(out->arch) - 1);
+        out->index = 0;
+        out->compute_units = prop.multiProcessorCount;
+        out->clock_khz = prop.clockRate;
+        size_t fr = 0, tot = 0;
+        HX_HIP(hipMemGetInfo(&fr, &tot));
+        out->total_mem = tot;
+        out->free_mem = fr;
+        HX_HIP(hipRuntimeGetVersion(&out->runtime_version));
+        HX_HIP(hipDriverGetVersion(&out->driver_version));
+    });
+}
+
+HX_API int hx_modctx_init(uint32_t p, hx_modctx* out) {
+    return guard("hx_modctx_init", [&] {
+        HX_CHECK(out != nullptr, "out is NULL");
+        HX_CHECK(p >= 2, "modulus must be >= 2");
+        out->p = p;
+        out->barrett_m = (uint64_t)(((unsigned __int128)1 << 64) / p);
+        const uint64_t d = (uint64_t)(p - 1) * (p - 1);
+        out->k_max = d == 0 ? UINT64_MAX : UINT64_MAX / d;
+        out->two64_mod = (uint64_t)(((unsigned __int128)1 << 64) % p);
+        // A balanced residue |x| <= (p - 1) / 2 fits L signed 8-bit digits when it is at most
+        // 127 * (256^L - 1) / 255 (the positive end of the digit range).
+        const uint64_t half = (uint64_t)(p - 1) / 2;
+        uint32_t L = 0;
+        for (uint32_t l = 1; l <= 4; ++l) {
+            uint64_t cap = 0;
+            for (uint32_t t = 0; t < l; ++t) cap = cap * 256 + 127;
+            if (half <= cap) { L = l; break; }
+        }
+        out->limbs = L;
+    });
+}
+
+HX_API int hx_stream_create(hx_stream* s) {
+    return guard("hx_stream_create", [&] {
+        hipStream_t h;
+        HX_HIP(hipStreamCreateWithFlags(&h, hipStreamNonBlocking));
+        *s = (hx_stream)h;
+    });
+}
+HX_API int hx_stream_destroy(hx_stream s) {
+    return guard("hx_stream_destroy", [&] { HX_HIP(hipStreamDestroy((hipStream_t)s)); });
+}
+HX_API int hx_stream_sync(hx_stream s) {
+    return guard("hx_stream_sync", [&] { HX_HIP(hipStreamSynchronize((hipStream_t)s)); });
+}
+HX_API int hx_device_sync(void) {
+    return guard("hx_device_sync", [&] { HX_HIP(hipDeviceSynchronize()); });
+}
+HX_API int hx_event_create(hx_event* e, int timing) {
+    return guard("hx_event_create", [&] {
+        hipEvent_t h;
+        HX_HIP(hipEventCreateWithFlags(&h, timing ? hipEventDefault : hipEventDisableTiming));
+        *e = (hx_event)h;
+    });
+}
+HX_API int hx_event_record(hx_event e, hx_stream s) {
+    return guard("hx_event_record", [&] { HX_HIP(hipEventRecord((hipEvent_t)e, (hipStream_t)s)); });
+}
+HX_API int hx_event_wait(hx_stream s, hx_event e) {
+    return guard("hx_event_wait", [&] { HX_HIP(hipStreamWaitEvent((hipStream_t)s, (hipEvent_t)e, 0)); });
+}
+HX_API int hx_event_sync(hx_event e) {
+    return guard("hx_event_sync", [&] { HX_HIP(hipEventSynchronize((hipEvent_t)e)); });
+}
+HX_API int hx_event_elapsed_ms(hx_event a, hx_event b, float* ms) {
+    return guard("hx_event_elapsed_ms", [&] { HX_HIP(hipEventElapsedTime(ms, (hipEvent_t)a, (hipEvent_t)b)); });
+}
+HX_API int hx_event_destroy(hx_event e) {
+    return guard("hx_event_destroy", [&] { HX_HIP(hipEventDestroy((hipEvent_t)e)); });
+}
+
+HX_API int hx_malloc(void** ptr, size_t bytes) {
+    return guard("hx_malloc", [&] {
+        HX_CHECK(ptr != nullptr, "ptr is NULL");
+        *ptr = nullptr;
+        if (bytes == 0) return;
+        if (hipMalloc(ptr, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            throw Error{HX_E_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed"};
+        }
+    });
+}
+HX_API int hx_free(void* ptr) {
+    return guard("hx_free", [&] {
+        if (ptr) HX_HIP(hipFree(ptr));
+    });
+}
+HX_API int hx_upload(void* dev, const void* host, size_t bytes, hx_stream s) {
+    return guard("hx_upload", [&] {
+        if (bytes == 0) return;
+        HX_HIP(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, (hipStream_t)s));
+        HX_HIP(hipStreamSynchronize((hipStream_t)s));
+    });
+}
+HX_API int hx_download(void* host, const void* dev, size_t bytes, hx_stream s) {
+    return guard("hx_download", [&] {
+        if (bytes == 0) return;
+        HX_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, (hipStream_t)s));
+        HX_HIP(hipStreamSynchronize((hipStream_t)s));
+    });
+}
+HX_API int hx_copy_d2d(void* dst, const void* src, size_t bytes, hx_stream s) {
+    return guard("hx_copy_d2d", [&] {
+        if (bytes == 0) return;
+        HX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)s));
+    });
+}
+HX_API int hx_memset(void* dst, int value, size_t bytes, hx_stream s) {
+    return guard("hx_memset", [&] {
+        if (bytes == 0) return;
+        HX_HIP(hipMemsetAsync(dst, value, bytes, (hipStream_t)s));
+    });
+}
+
+HX_API int hx_graph_begin(hx_stream s) {
+    return guard("hx_graph_begin", [&] {
+        HX_HIP(hipStreamBeginCapture((hipStream_t)s, hipStreamCaptureModeThreadLocal));
+    });
+}
+HX_API int hx_graph_end(hx_stream s, hx_graph* g) {
+    return guard("hx_graph_end", [&] {
+        hipGraph_t graph;
+        HX_HIP(hipStreamEndCapture((hipStream_t)s, &graph));
+        hipGraphExec_t exec;
+        hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) throw Error{HX_E_RUNTIME, std::string("hipGraphInstantiate: ") + hipGetErrorString(e)};
+        *g = (hx_graph)exec;
+    });
+}
+HX_API int hx_graph_launch(hx_graph g, hx_stream s) {
+    return guard("hx_graph_launch", [&] { HX_HIP(hipGraphLaunch((hipGraphExec_t)g, (hipStream_t)s)); });
+}
+HX_API int hx_graph_destroy(hx_graph g) {
+    return guard("hx_graph_destroy", [&] { HX_HIP(hipGraphExecDestroy((hipGraphExec_t)g)); });
+}
+
+HX_API int hx_random_entries(uint32_t* out, uint32_t rows, uint32_t cols, int64_t ld, uint32_t ell, uint64_t seed,
+                             hx_stream s) {
+    return guard("hx_random_entries", [&] {
+        HX_CHECK(ell >= 1, "ell must be >= 1");
+        HX_CHECK(ld >= (int64_t)cols, "ld < cols");
+        if ((uint64_t)rows * cols == 0) return;
+        const uint64_t total = (uint64_t)rows * cols;
+        const uint32_t blocks = (uint32_t)std::min<uint64_t>((total + 255) / 256, 65536);
+        random_entries_kernel<<<blocks, 256, 0, (hipStream_t)s>>>(out, rows, cols, ld, ell, seed);
+        after_launch("random_entries_kernel");
+    });
+}
+
+}  // extern "C"

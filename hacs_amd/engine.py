@@ -1,0 +1,290 @@
+"""Python mirror of the reference's device-kernel surface (engine::cuda, Cuda/CudaKernels.h)
+over the C-ABI of libhacs_mi355x.so.
+
+Matrices are 2-D torch tensors on the GPU holding uint32 residues (dtype int32 or
+uint32: the bits are what matter), addressed through their strides exactly like the
+reference's DeviceView {base, rs, cs, rows, cols} (Cuda/DeviceBuf.h:50-91).  PyTorch
+only provides device memory and the stream; every value is computed by the library.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import (HX_GEMM_AUTO, HX_GEMM_LIMB, HX_GEMM_PLAIN, HX_OP_ADD, HX_OP_SET, HX_OP_SUB, HxError,
+                   check, lib)
+
+__all__ = ["ModCtx", "GemmOp", "gemm", "gemm_nt", "schur_update", "to_device", "from_device", "HxError",
+           "HX_GEMM_AUTO", "HX_GEMM_PLAIN", "HX_GEMM_LIMB", "current_stream", "view"]
+
+
+class GemmOp:
+    """engine::kernels::GemmOp (Set / Add / Sub)."""
+    Set = HX_OP_SET
+    Add = HX_OP_ADD
+    Sub = HX_OP_SUB
+
+
+class ModCtx:
+    """The device-relevant part of engine::ModCtx<uint32_t> (ModCtx.h:202-250)."""
+
+    def __init__(self, p: int):
+        self.c = _lib.hx_modctx()
+        check(lib().hx_modctx_init(p, C.byref(self.c)))
+
+    @property
+    def p(self) -> int:
+        return self.c.p
+
+    @property
+    def limbs(self) -> int:
+        return self.c.limbs
+
+    @property
+    def k_max(self) -> int:
+        return self.c.k_max
+
+    @property
+    def barrett_m(self) -> int:
+        return self.c.barrett_m
+
+
+def current_stream() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _u32_tensor(t: torch.Tensor) -> None:
+    if t.dtype not in (torch.int32, torch.uint32):
+        raise TypeError(f"expected an int32/uint32 tensor of residues, got {t.dtype}")
+    if not t.is_cuda:
+        raise TypeError("expected a device tensor")
+
+
+def view(t: torch.Tensor) -> _lib.hx_view:
+    """hx_view of a 2-D device tensor (strides in elements, like DeviceView)."""
+    _u32_tensor(t)
+    if t.dim() != 2:
+        raise ValueError("expected a 2-D tensor")
+    return _lib.hx_view(C.c_void_p(t.data_ptr()), t.stride(0), t.stride(1), t.shape[0], t.shape[1])
+
+
+def to_device(a: np.ndarray, device: str = "cuda") -> torch.Tensor:
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    return torch.from_numpy(a.view(np.int32)).to(device)
+
+
+def from_device(t: torch.Tensor) -> np.ndarray:
+    return t.detach().cpu().contiguous().numpy().view(np.uint32)
+
+
+def _opts(variant: int, tile: int = 0, workspace: torch.Tensor | None = None) -> _lib.hx_gemm_opts:
+    o = _lib.hx_gemm_opts()
+    o.variant = variant
+    o.tile = tile
+    if workspace is not None:
+        o.workspace = C.c_void_p(workspace.data_ptr())
+        o.workspace_bytes = workspace.numel() * workspace.element_size()
+    return o
+
+
+def gemm(ctx: ModCtx, A: torch.Tensor, B: torch.Tensor, C_: torch.Tensor, op: int = GemmOp.Set, stream=None,
+         variant: int = HX_GEMM_AUTO, tile: int = 0, workspace: torch.Tensor | None = None) -> None:
+    """C op= A * B mod p  (engine::cuda::gemm, Cuda/CudaKernels.h:120-123)."""
+    o = _opts(variant, tile, workspace)
+    check(lib().hx_gemm_u32(C.byref(ctx.c), view(A), view(B), view(C_), op,
+                            current_stream() if stream is None else stream, C.byref(o)))
+
+
+def gemm_nt(ctx: ModCtx, A, B, C_, op: int = GemmOp.Set, stream=None, variant: int = HX_GEMM_AUTO) -> None:
+    """C op= A * B^T with B given n x k (engine::cuda::gemm_nt, CudaKernels.h:125-128)."""
+    gemm(ctx, A, B.t(), C_, op, stream, variant)
+
+
+def schur_update(ctx: ModCtx, L, U, C_, stream=None, variant: int = HX_GEMM_AUTO) -> None:
+    """C -= L * U (engine::cuda::schur_update, CudaKernels.h:130-133)."""
+    gemm(ctx, L, U, C_, GemmOp.Sub, stream, variant)
+
+
+def gemm_workspace_bytes(ctx: ModCtx, m: int, n: int, k: int, variant: int = HX_GEMM_LIMB) -> int:
+    return lib().hx_gemm_workspace_bytes(C.byref(ctx.c), m, n, k, variant)
+
+
+def random_entries(rows: int, cols: int, ell: int, seed: int, stream=None) -> torch.Tensor:
+    """RandomEntries(rows * cols, ell, seed) generated on the device (PerformanceSmokeValidation.cpp:82-89)."""
+    t = torch.empty((rows, cols), dtype=torch.int32, device="cuda")
+    check(lib().hx_random_entries(C.c_void_p(t.data_ptr()), rows, cols, cols, ell, seed,
+                                  current_stream() if stream is None else stream))
+    return t
+
+
+@dataclass
+class DeviceInfo:
+    name: str
+    arch: str
+    compute_units: int
+    clock_khz: int
+    total_mem: int
+
+
+def device_info() -> DeviceInfo:
+    d = _lib.hx_device_info()
+    check(lib().hx_device_info_get(C.byref(d)))
+    return DeviceInfo(d.name.decode(), d.arch.decode(), d.compute_units, d.clock_khz, d.total_mem)
+
+
+# ---------------------------------------------------------------------------
+# elimination
+# ---------------------------------------------------------------------------
+@dataclass
+class RrefResult:
+    """rank, pivot columns (ascending) and pivot_rows (ImmaRrefResult, Cuda/CudaKernels.h:269-273)."""
+    rank: int
+    pivots: np.ndarray
+    pivot_rows: np.ndarray
+
+
+def rref(ctx: ModCtx, A: torch.Tensor, R: torch.Tensor | None = None, stream=None) -> RrefResult:
+    """Device RREF of A (m x n, not modified); R (>= m rows) receives RREF(A), zero rows last.
+
+    The device replacement of elim::rref / imma_rref (DenseElim.h:149-151, CudaKernels.h:276-288)."""
+    _u32_tensor(A)
+    m, n = A.shape
+    if A.stride(1) != 1:
+        raise ValueError("A must be row-contiguous")
+    mn = max(1, min(m, n))
+    piv = np.zeros(mn, np.uint32)
+    prow = np.zeros(mn, np.uint32)
+    rank = C.c_uint32(0)
+    rp, ldr = (None, 0)
+    if R is not None:
+        _u32_tensor(R)
+        if R.stride(1) != 1 or R.shape[0] < m or R.shape[1] < n:
+            raise ValueError("R must be row-contiguous with at least m rows and n columns")
+        rp, ldr = C.c_void_p(R.data_ptr()), R.stride(0)
+    check(lib().hx_rref_u32(C.byref(ctx.c), m, n, C.c_void_p(A.data_ptr()), A.stride(0), rp, ldr, C.byref(rank),
+                            piv.ctypes.data_as(_lib._U32P), prow.ctypes.data_as(_lib._U32P),
+                            current_stream() if stream is None else stream))
+    r = rank.value
+    return RrefResult(r, piv[:r].copy(), prow[:r].copy())
+
+
+class RrefBatch:
+    """Many uneven RREF jobs in one persistent launch; each job's matrix is overwritten by its RREF.
+
+    The job table is uploaded once (not capturable); run() is capturable."""
+
+    def __init__(self, ctx: ModCtx, mats: list[torch.Tensor]):
+        self.ctx = ctx
+        self.mats = mats
+        n = len(mats)
+        self.meta = torch.zeros(max(1, sum(2 * max(1, min(M.shape)) + 1 for M in mats)), dtype=torch.int32,
+                                device="cuda")
+        jobs = (_lib.hx_rref_job * max(1, n))()
+        off = 0
+        base = self.meta.data_ptr()
+        self.offsets = []
+        for i, M in enumerate(mats):
+            _u32_tensor(M)
+            m_, n_ = M.shape
+            mn = max(1, min(m_, n_))
+            jobs[i].m, jobs[i].n = m_, n_
+            jobs[i].A = C.c_void_p(M.data_ptr())
+            jobs[i].lda = M.stride(0)
+            jobs[i].pivots = C.c_void_p(base + 4 * off)
+            jobs[i].pivot_rows = C.c_void_p(base + 4 * (off + mn))
+            jobs[i].rank = C.c_void_p(base + 4 * (off + 2 * mn))
+            self.offsets.append((off, mn))
+            off += 2 * mn + 1
+        self.h = C.c_void_p()
+        check(lib().hx_rref_batch_create(C.byref(ctx.c), jobs, n, 0, C.byref(self.h)))
+
+    def run(self, stream=None) -> None:
+        check(lib().hx_rref_batch_run(self.h, current_stream() if stream is None else stream))
+
+    def status(self, stream=None) -> None:
+        check(lib().hx_rref_batch_status(self.h, current_stream() if stream is None else stream))
+
+    def results(self) -> list[RrefResult]:
+        meta = from_device(self.meta)
+        out = []
+        for off, mn in self.offsets:
+            r = int(meta[off + 2 * mn])
+            out.append(RrefResult(r, meta[off:off + r].copy(), meta[off + mn:off + mn + r].copy()))
+        return out
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().hx_rref_batch_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def trsm_unit(ctx: ModCtx, T: torch.Tensor, B: torch.Tensor, upper: bool, stream=None) -> None:
+    """B <- T^{-1} B, T unit triangular (engine::cuda::trsm_unit, Cuda/CudaKernels.h:240-247)."""
+    check(lib().hx_trsm_unit_u32(C.byref(ctx.c), view(T), view(B), 1 if upper else 0,
+                                 current_stream() if stream is None else stream))
+
+
+# ---------------------------------------------------------------------------
+# Howell form over Z/2^k
+# ---------------------------------------------------------------------------
+@dataclass
+class HowellResult:
+    """H (h x n, masked to k bits), strictly increasing pivots, pivot_v (HowellForm, Ring2k.h:415-422)."""
+    H: np.ndarray
+    pivots: np.ndarray
+    pivot_v: np.ndarray
+
+
+class HowellBatch:
+    """Howell forms over Z/2^k of many uneven matrices in one persistent launch (ring2k::howell)."""
+
+    def __init__(self, k: int, mats: list[torch.Tensor]):
+        self.k = k
+        self.mats = mats
+        n_jobs = len(mats)
+        jobs = (_lib.hx_howell_job * max(1, n_jobs))()
+        self.outs = []
+        for i, M in enumerate(mats):
+            _u32_tensor(M)
+            m_, n_ = M.shape
+            H = torch.zeros((max(1, n_), max(1, n_)), dtype=torch.int32, device="cuda")
+            meta = torch.zeros(2 * max(1, n_) + 1, dtype=torch.int32, device="cuda")
+            work = torch.empty(max(1, lib().hx_howell_work_words(m_, n_)), dtype=torch.int32, device="cuda")
+            jobs[i].m, jobs[i].n = m_, n_
+            jobs[i].A = C.c_void_p(M.data_ptr())
+            jobs[i].lda = M.stride(0) if m_ else n_
+            jobs[i].H = C.c_void_p(H.data_ptr())
+            jobs[i].pivots = C.c_void_p(meta.data_ptr())
+            jobs[i].pivot_v = C.c_void_p(meta.data_ptr() + 4 * max(1, n_))
+            jobs[i].h = C.c_void_p(meta.data_ptr() + 8 * max(1, n_))
+            jobs[i].work = C.c_void_p(work.data_ptr())
+            self.outs.append((H, meta, work, n_))
+        self.h = C.c_void_p()
+        check(lib().hx_howell_batch_create(k, jobs, n_jobs, C.byref(self.h)))
+
+    def run(self, stream=None) -> None:
+        check(lib().hx_howell_batch_run(self.h, current_stream() if stream is None else stream))
+
+    def results(self) -> list[HowellResult]:
+        res = []
+        for H, meta, _, n_ in self.outs:
+            mt = from_device(meta)
+            nn = max(1, n_)
+            h = int(mt[2 * nn])
+            res.append(HowellResult(from_device(H)[:h, :n_].copy(), mt[:h].copy(), mt[nn:nn + h].copy()))
+        return res
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().hx_howell_batch_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass

@@ -1,0 +1,243 @@
+/*
+ * hacs_mi355x.h -- C-ABI of the MI355X (gfx950) device backend of HACS's
+ * exact matrix engine.
+ *
+ * The library (libhacs_mi355x.so) replaces the device side that the reference
+ * reaches through src/Cuda (namespace engine::cuda): the runtime context and
+ * buffers (Cuda/CudaCtx.h, Cuda/DeviceBuf.h), the dense kernels
+ * (Cuda/CudaKernels.h), the device elimination (Cuda/ImmaElim.cuh through
+ * CudaKernels.h:249-298) and the Z/2^k device work (Cuda/CudaRing2k.h).  Each
+ * declaration below cites the reference interface it replaces.
+ *
+ * Conventions
+ *   - Every function returns 0 on success or a negative HX_E* code; the text
+ *     of the last error of the calling thread is hx_last_error().  This is the
+ *     C form of the reference's engine::cuda::CudaError / ENGINE_CHECK
+ *     (CudaCtx.h:40-54, cuda_kernels.cu:384-394); a C++ facade rethrows it.
+ *   - Device pointers are plain pointers into HBM; hx_stream is a hipStream_t
+ *     (NULL = the null stream).  Launches are asynchronous on the stream
+ *     unless a function says it synchronises.
+ *   - All arithmetic is exact integer arithmetic on canonical residues in
+ *     [0, p); results are bit-identical to the reference CPU engine.
+ *   - No torch type appears in any signature.
+ */
+#ifndef HACS_MI355X_H
+#define HACS_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HX_API __attribute__((visibility("default")))
+
+/* ---- status codes -------------------------------------------------------- */
+enum {
+    HX_OK = 0,
+    HX_E_RUNTIME = -1,     /* a HIP runtime / launch error (CudaError) */
+    HX_E_ARGUMENT = -2,    /* a contract breach: shape, stride, range (ENGINE_CHECK) */
+    HX_E_UNAVAILABLE = -3, /* no usable device (available() == false) */
+    HX_E_NOMEM = -4,       /* device allocation failed */
+    HX_E_NOT_INVERTIBLE = -5, /* a pivot had no inverse mod p (p not prime) */
+    HX_E_CAPTURE = -6      /* the call is not legal inside a stream capture */
+};
+
+/* ---- opaque handles (CudaCtx.h:63-64 Stream / Event) --------------------- */
+typedef void *hx_stream;   /* hipStream_t */
+typedef void *hx_event;    /* hipEvent_t */
+typedef void *hx_graph;    /* an instantiated hipGraphExec_t */
+
+/* ---- the modulus (ModCtx.h:123-250, the u32 fields the kernels read) ----- */
+typedef struct hx_modctx {
+    uint32_t p;          /* modulus, 2 <= p < 2^32 */
+    uint32_t limbs;      /* signed 8-bit limbs of a balanced residue (1..4); 0 = too wide for the limb route */
+    uint64_t barrett_m;  /* floor(2^64 / p)                          (ModCtx.h:219) */
+    uint64_t k_max;      /* floor((2^64 - 1) / (p - 1)^2)            (ModCtx.h:220-222) */
+    uint64_t two64_mod;  /* 2^64 mod p */
+} hx_modctx;
+
+/* ---- a strided device matrix view (DeviceBuf.h:50-91 DeviceView) -------- */
+/* element (i, j) lives at base[i * rs + j * cs]; rs, cs in elements. */
+typedef struct hx_view {
+    void *base;
+    int64_t rs, cs;
+    uint32_t rows, cols;
+} hx_view;
+
+/* GemmOp (ModArithmetic/Kernels.h, reference ref::gemm encoding RefLinAlg.h:37-39) */
+enum { HX_OP_SET = 0, HX_OP_ADD = 1, HX_OP_SUB = 2 };
+
+/* GEMM kernel variants (no counterpart: the reference has one u32 kernel and the p <= 127 IMMA route) */
+enum {
+    HX_GEMM_AUTO = 0,   /* the selector picks (never changes a value) */
+    HX_GEMM_PLAIN = 1,  /* u32 x u32 -> exact u64 (+carry word) accumulation on the VALU, Barrett epilogue */
+    HX_GEMM_LIMB = 2    /* balanced residues as signed 8-bit limbs on the INT8 MFMA cores, exact recombination */
+};
+
+typedef struct hx_gemm_opts {   /* GemmOpts, CudaKernels.h:64-79 */
+    int variant;            /* HX_GEMM_* */
+    int tile;               /* 0 = automatic; a variant-specific tile index (tuning only) */
+    void *workspace;        /* limb route: >= hx_gemm_workspace_bytes(); NULL = library-owned (not capturable) */
+    size_t workspace_bytes;
+} hx_gemm_opts;
+
+typedef struct hx_device_info {   /* CudaCtx.h:55-64 DeviceInfo */
+    char name[128];
+    char arch[64];
+    int index;
+    int compute_units;
+    int clock_khz;
+    size_t total_mem, free_mem;
+    int runtime_version, driver_version;
+} hx_device_info;
+
+/* ---- availability, errors (CudaCtx.h:56-62) ------------------------------ */
+HX_API int hx_available(void);
+HX_API const char *hx_unavailable_reason(void);
+HX_API int hx_device_info_get(hx_device_info *out);
+HX_API const char *hx_last_error(void);
+HX_API const char *hx_version(void);   /* kernel_layer_version(), CudaKernels.h:307 */
+
+/* ModCtx<uint32_t>::init (ModCtx.h:202-250), device-relevant fields only. */
+HX_API int hx_modctx_init(uint32_t p, hx_modctx *out);
+
+/* ---- streams, events, memory, transfers (CudaCtx.h:120-214) -------------- */
+HX_API int hx_stream_create(hx_stream *s);
+HX_API int hx_stream_destroy(hx_stream s);
+HX_API int hx_stream_sync(hx_stream s);
+HX_API int hx_device_sync(void);
+HX_API int hx_event_create(hx_event *e, int timing);
+HX_API int hx_event_record(hx_event e, hx_stream s);
+HX_API int hx_event_wait(hx_stream s, hx_event e);
+HX_API int hx_event_sync(hx_event e);
+HX_API int hx_event_elapsed_ms(hx_event a, hx_event b, float *ms);
+HX_API int hx_event_destroy(hx_event e);
+HX_API int hx_malloc(void **ptr, size_t bytes);        /* dalloc */
+HX_API int hx_free(void *ptr);                          /* dfree */
+HX_API int hx_upload(void *dev, const void *host, size_t bytes, hx_stream s);     /* synchronous w.r.t. host memory */
+HX_API int hx_download(void *host, const void *dev, size_t bytes, hx_stream s);   /* returns when the bytes are in host memory */
+HX_API int hx_copy_d2d(void *dst, const void *src, size_t bytes, hx_stream s);
+HX_API int hx_memset(void *dst, int value, size_t bytes, hx_stream s);
+
+/* ---- graph capture (CudaCtx.h:223-253 GraphCapture) ---------------------- */
+HX_API int hx_graph_begin(hx_stream s);
+HX_API int hx_graph_end(hx_stream s, hx_graph *g);
+HX_API int hx_graph_launch(hx_graph g, hx_stream s);
+HX_API int hx_graph_destroy(hx_graph g);
+
+/* ---- GEMM (CudaKernels.h:120-133 gemm / gemm_nt / schur_update) --------- */
+/* C op= A * B mod p; A m x k, B k x n, C m x n, any strides; C must not overlap A or B.
+ * variant PLAIN: any p; LIMB: p with ctx->limbs > 0 (p <= 4278124287), else HX_E_ARGUMENT. */
+HX_API int hx_gemm_u32(const hx_modctx *ctx, hx_view A, hx_view B, hx_view C, int op, hx_stream s,
+                       const hx_gemm_opts *opts);
+/* Device workspace bytes the limb route needs for these shapes (0 for PLAIN). */
+HX_API size_t hx_gemm_workspace_bytes(const hx_modctx *ctx, uint32_t m, uint32_t n, uint32_t k, int variant);
+/* C -= L * U (schur_update, CudaKernels.h:130-133). */
+HX_API int hx_schur_update_u32(const hx_modctx *ctx, hx_view L, hx_view U, hx_view C, hx_stream s,
+                               const hx_gemm_opts *opts);
+
+/* ---- batched GEMM of uneven shapes (generalises GemmBatch<Elt>, CudaKernels.h:135-185, whose
+ *      instances must share one shape) ------------------------------------------------------ */
+typedef struct hx_gemm_job {
+    uint32_t m, n, k;
+    int32_t op;                 /* HX_OP_* */
+    const uint32_t *A;          /* row-major, leading dimension lda (device) */
+    const uint32_t *B;
+    uint32_t *C;
+    int64_t lda, ldb, ldc;
+} hx_gemm_job;
+typedef void *hx_gemm_batch;
+/* set_shape / push / upload (NOT capturable): the job table goes to device memory. */
+HX_API int hx_gemm_batch_create(const hx_modctx *ctx, const hx_gemm_job *jobs, uint32_t njobs, int variant,
+                                hx_gemm_batch *out);
+/* run (capturable once created): every job in one launch. */
+HX_API int hx_gemm_batch_run(hx_gemm_batch b, hx_stream s);
+HX_API int hx_gemm_batch_destroy(hx_gemm_batch b);
+
+/* ---- triangular solve (CudaKernels.h:240-247 trsm_unit) ------------------ */
+/* B <- T^{-1} B in place, T square unit triangular (only the strict triangle is read). */
+HX_API int hx_trsm_unit_u32(const hx_modctx *ctx, hx_view T, hx_view B, int upper, hx_stream s);
+
+/* ---- device RREF over F_p (CudaKernels.h:249-298 imma_rref; DenseElim.h:111-123 hook) ----
+ * p prime, 2 <= p < 2^31.  A (m x n, row stride lda, device) is not modified.  R (device, row
+ * stride ldr, at least m rows) receives RREF(A) with its zero rows at the bottom (the in-place
+ * convention of elim::rref); pass R == NULL for rank / pivots / pivot_rows only.
+ * pivots[t]: the pivot columns, ascending (the column rank profile).  pivot_rows[t]: the original
+ * row that supplied pivot t under the smallest-original-row rule (CudaKernels.h:255-265), i.e. the
+ * row rank profile in pivot order.  pivots / pivot_rows are HOST arrays of min(m, n) entries.
+ * Synchronous: returns when rank and pivots are on the host. */
+HX_API int hx_rref_u32(const hx_modctx *ctx, uint32_t m, uint32_t n, const uint32_t *A, int64_t lda, uint32_t *R,
+                       int64_t ldr, uint32_t *rank, uint32_t *pivots, uint32_t *pivot_rows, hx_stream s);
+/* The reference's row-slab form (imma_rref, CudaKernels.h:276-288): src(user, r0, nr) returns a
+ * host pointer to rows [r0, r0 + nr) (row-contiguous, row stride = the returned ld); sink(user, rank,
+ * &ld) returns where the rank x n RREF rows go (NULL: no rows wanted). */
+typedef const uint32_t *(*hx_row_source)(void *user, uint32_t r0, uint32_t nr, int64_t *ld);
+typedef uint32_t *(*hx_row_sink)(void *user, uint32_t rank, int64_t *ld);
+HX_API int hx_rref_u32_host(const hx_modctx *ctx, uint32_t m, uint32_t n, hx_row_source src, hx_row_sink sink,
+                            void *user, uint32_t slab_rows, uint32_t *rank, uint32_t *pivots, uint32_t *pivot_rows);
+
+/* ---- batched uneven RREF (no counterpart: the reference runs one elimination per call) ---- */
+typedef struct hx_rref_job {
+    uint32_t m, n;
+    uint32_t *A;            /* device, row stride lda; overwritten by RREF(A), zero rows at the bottom */
+    int64_t lda;
+    uint32_t *pivots;       /* device, min(m, n) entries */
+    uint32_t *pivot_rows;   /* device, min(m, n) entries */
+    uint32_t *rank;         /* device, one entry */
+} hx_rref_job;
+typedef void *hx_rref_batch;
+HX_API int hx_rref_batch_create(const hx_modctx *ctx, const hx_rref_job *jobs, uint32_t njobs, int variant,
+                                hx_rref_batch *out);
+/* One persistent launch over every job (capturable once created). */
+HX_API int hx_rref_batch_run(hx_rref_batch b, hx_stream s);
+HX_API int hx_rref_batch_destroy(hx_rref_batch b);
+/* HX_E_NOT_INVERTIBLE if any job met a pivot without an inverse (synchronises the stream). */
+HX_API int hx_rref_batch_status(hx_rref_batch b, hx_stream s);
+
+/* ---- Howell form over Z/2^k, 1 <= k <= 32 (ring2k::howell, Binary/Ring2k.h:959-969; the device
+ *      side of the ring2k session, Cuda/CudaRing2k.h:28-46, generalised to whole jobs) --------- */
+typedef struct hx_howell_job {
+    uint32_t m, n;
+    const uint32_t *A;      /* device, row stride lda (entries read mod 2^k) */
+    int64_t lda;
+    uint32_t *H;            /* device, n x n, row stride n: the h Howell rows, masked to k bits */
+    uint32_t *pivots;       /* device, n entries (strictly increasing leading columns) */
+    uint32_t *pivot_v;      /* device, n entries (row i leads with exactly 2^pivot_v[i]) */
+    uint32_t *h;            /* device, one entry */
+    uint32_t *work;         /* device scratch, >= hx_howell_work_words(m, n) u32 */
+} hx_howell_job;
+HX_API size_t hx_howell_work_words(uint32_t m, uint32_t n);
+typedef void *hx_howell_batch;
+HX_API int hx_howell_batch_create(uint32_t k, const hx_howell_job *jobs, uint32_t njobs, hx_howell_batch *out);
+HX_API int hx_howell_batch_run(hx_howell_batch b, hx_stream s);
+HX_API int hx_howell_batch_destroy(hx_howell_batch b);
+
+/* ---- the learned selector's network (no counterpart in the reference: its cost model is static,
+ *      Engine/EngineCostModel.h:22-51) -----------------------------------------------------------
+ * A 2-layer MLP, FP32: h = relu(X W1^T + b1), y = h W2^T + b2.  X is B x F, W1 H x F, b1 H,
+ * W2 O x H, b2 O, y B x O (all row-major device arrays).  Only the variant / tile CHOICE comes from
+ * it; it never touches a value of the exact path. */
+HX_API int hx_mlp_forward(const float *X, uint32_t B, uint32_t F, const float *W1, const float *b1, uint32_t H,
+                          const float *W2, const float *b2, uint32_t O, float *hidden, float *y, hx_stream s);
+/* Mean-squared-error loss L = (1/(B*O)) sum (y - t)^2 with an optional per-entry mask (NULL = all);
+ * writes dW1, db1, dW2, db2 (overwritten) and the loss (device scalar). */
+HX_API int hx_mlp_backward(const float *X, uint32_t B, uint32_t F, const float *W1, uint32_t H, const float *W2,
+                           uint32_t O, const float *hidden, const float *y, const float *target, const float *mask,
+                           float *dW1, float *db1, float *dW2, float *db2, float *loss, hx_stream s);
+/* Adam step on n parameters (t = 1-based step count). */
+HX_API int hx_adam_step(float *param, const float *grad, float *m1, float *m2, uint32_t n, float lr, float beta1,
+                        float beta2, float eps, uint32_t t, hx_stream s);
+
+/* ---- fixed-seed device input generation (PerformanceSmokeValidation.cpp:64-89) ----------------
+ * out[i * ld + j] = SplitMix64 draw (i * cols + j) of a stream seeded `seed`, mod ell (the
+ * RandomEntries(rows * cols, ell, seed) matrix).  Used for synthetic inputs that never leave HBM. */
+HX_API int hx_random_entries(uint32_t *out, uint32_t rows, uint32_t cols, int64_t ld, uint32_t ell, uint64_t seed,
+                             hx_stream s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HACS_MI355X_H */

@@ -1,0 +1,115 @@
+"""GEMM mod p on the GPU against the CPU oracle: bit-exact (memcmp), both device kernels.
+
+Cases follow the reference's device gate (Cuda/CudaValidation.cpp:102-195): every op, strided /
+transposed operands, ragged sizes, k = 0, the "rank-1 x all-(p-1)" worst case, and the moduli of
+the configs (65521, 1000003, 2147483629, 2^31-1) plus 2^32-5 (plain only: no limb split fits it)
+and small moduli exercising 1-, 2- and 3-limb splits.
+"""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+PRIMES = [251, 65521, 1000003, 16777213, 2147483629, 2147483647, 4294967291]
+
+
+def _run(E, ctx, A, B, C0, op, variant):
+    dA, dB = E.to_device(A), E.to_device(B)
+    dC = E.to_device(C0)
+    E.gemm(ctx, dA, dB, dC, op, variant=variant)
+    torch.cuda.synchronize()
+    return E.from_device(dC)
+
+
+@pytest.mark.parametrize("p", PRIMES)
+@pytest.mark.parametrize("shape", [(1, 1, 1), (7, 17, 33), (64, 128, 64), (65, 129, 65), (130, 70, 300),
+                                   (257, 191, 1000), (3, 5, 0)])
+def test_gemm_matches_oracle(hx_gpu, O, p, shape):
+    from hacs_amd import engine as E
+    ctx = E.ModCtx(p)
+    m, n, k = shape
+    rng = np.random.default_rng(p ^ (m * 1000003 + n * 1009 + k))
+    A = rng.integers(0, p, (m, k), dtype=np.uint64).astype(np.uint32)
+    B = rng.integers(0, p, (k, n), dtype=np.uint64).astype(np.uint32)
+    C0 = rng.integers(0, p, (m, n), dtype=np.uint64).astype(np.uint32)
+    variants = [E.HX_GEMM_PLAIN] + ([E.HX_GEMM_LIMB] if ctx.limbs > 0 else [])
+    for op in (E.GemmOp.Set, E.GemmOp.Add, E.GemmOp.Sub):
+        want = O.gemm_fast(p, A, B, C0, op) if k else (np.zeros_like(C0) if op == 0 else C0.copy())
+        for v in variants:
+            got = _run(E, ctx, A, B, C0, op, v)
+            assert np.array_equal(got, want), f"p={p} shape={shape} op={op} variant={v}"
+
+
+@pytest.mark.parametrize("p", [65521, 2147483629, 4294967291])
+def test_gemm_worst_case_and_extremes(hx_gpu, O, p):
+    """rank-1 A x all-(p-1) B (CudaValidation.cpp:176-194) and the balanced extremes (p +- 1) / 2."""
+    from hacs_amd import engine as E
+    ctx = E.ModCtx(p)
+    m, n, k = 96, 160, 2048
+    rng = np.random.default_rng(7)
+    u = rng.integers(0, p, (m, 1), dtype=np.uint64)
+    v = rng.integers(0, p, (1, k), dtype=np.uint64)
+    A = ((u * v) % p).astype(np.uint32)
+    B = np.full((k, n), p - 1, np.uint32)
+    ext = np.array([(p - 1) // 2, (p + 1) // 2, p - 1, 0, 1], np.uint32)
+    A2 = ext[rng.integers(0, len(ext), (m, k))]
+    B2 = ext[rng.integers(0, len(ext), (k, n))]
+    variants = [E.HX_GEMM_PLAIN] + ([E.HX_GEMM_LIMB] if ctx.limbs > 0 else [])
+    for (a, b) in ((A, B), (A2, B2), (np.full((m, k), p - 1, np.uint32), B)):
+        exact = (a.astype(object).dot(b.astype(object)) % p).astype(np.uint32)
+        for var in variants:
+            got = _run(E, ctx, a, b, np.zeros((m, n), np.uint32), E.GemmOp.Set, var)
+            assert np.array_equal(got, exact), f"p={p} variant={var}"
+
+
+@pytest.mark.parametrize("variant", [1, 2])
+def test_gemm_strided_views(hx_gpu, O, variant):
+    """Transposed and windowed operands (every layout x transpose combination is one view)."""
+    from hacs_amd import engine as E
+    p = 2147483629
+    ctx = E.ModCtx(p)
+    rng = np.random.default_rng(3)
+    big = rng.integers(0, p, (300, 400), dtype=np.uint64).astype(np.uint32)
+    dbig = E.to_device(big)
+    A = dbig[10:110, 5:205]            # 100 x 200 window
+    Bt = dbig[50:180, 200:400]         # 130 x 200 -> B = Bt^T is 200 x 130
+    B = Bt.t()
+    C = torch.zeros((130, 100), dtype=torch.int32, device="cuda").t()   # 100 x 130 column-major
+    E.gemm(ctx, A, B, C, E.GemmOp.Set, variant=variant)
+    torch.cuda.synchronize()
+    want = O.gemm_fast(p, big[10:110, 5:205], big[50:180, 200:400].T.copy())
+    assert np.array_equal(E.from_device(C.contiguous()), want)
+
+
+def test_gemm_rejects_bad_input(hx_gpu):
+    from hacs_amd import engine as E
+    ctx = E.ModCtx(4294967291)
+    a = torch.zeros((4, 4), dtype=torch.int32, device="cuda")
+    b = torch.zeros((5, 4), dtype=torch.int32, device="cuda")
+    with pytest.raises(E.HxError):
+        E.gemm(ctx, a, b, torch.zeros((4, 4), dtype=torch.int32, device="cuda"))      # shape
+    with pytest.raises(E.HxError):
+        E.gemm(ctx, a, a, a)                                                          # aliasing
+    with pytest.raises(E.HxError):
+        E.gemm(ctx, a, a, torch.zeros((4, 4), dtype=torch.int32, device="cuda"), variant=E.HX_GEMM_LIMB)
+
+
+def test_config1_gemm_1024_mod_65521(hx_gpu, O):
+    """The CPU-runnable config: C = A B mod 65521, 1024^2, A/B = EngineMatrix(1024, 1024, 65521, 0x01/0x02)."""
+    from hacs_amd import engine as E
+    p, n = 65521, 1024
+    ctx = E.ModCtx(p)
+    A = O.engine_matrix(n, n, p, 0x01)
+    B = O.engine_matrix(n, n, p, 0x02)
+    want = O.gemm_fast(p, A, B)
+    dA = E.random_entries(n, n, p, O.seed_for(n, n, p, 0x01))
+    dB = E.random_entries(n, n, p, O.seed_for(n, n, p, 0x02))
+    assert np.array_equal(E.from_device(dA), A) and np.array_equal(E.from_device(dB), B)
+    for v in (E.HX_GEMM_PLAIN, E.HX_GEMM_LIMB):
+        dC = torch.empty((n, n), dtype=torch.int32, device="cuda")
+        E.gemm(ctx, dA, dB, dC, E.GemmOp.Set, variant=v)
+        torch.cuda.synchronize()
+        got = E.from_device(dC)
+        assert np.array_equal(got, want), f"variant {v}"
+        assert O.fnv_matrix(got) == O.fnv_matrix(want)

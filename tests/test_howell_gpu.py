@@ -1,0 +1,54 @@
+"""Howell form over Z/2^k on the GPU against the CPU oracle (and its special cases): H, pivots and
+pivot_v bit-exact.  Inputs follow the reference's ladder validation (Ring2kLadderValidation.cpp):
+random and planted (valuation-structured) matrices over k in {1..32}, plus the known answer
+howell([[2, 1]]) over Z/4 = [[2, 1], [0, 2]] (ENGINE_API.md:1939-1942)."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _planted(rng, m, n, k):
+    """Rows scaled by powers of two and mixed: many non-unit pivots and closure rows."""
+    A = rng.integers(0, 1 << min(k, 32), (m, n), dtype=np.uint64)
+    shifts = rng.integers(0, k, (m, 1))
+    A = (A << shifts.astype(np.uint64)) & np.uint64((1 << k) - 1 if k < 64 else ~0)
+    return A.astype(np.uint32)
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 8, 16, 31, 32])
+def test_howell_batch(hx_gpu, O, k):
+    from hacs_amd import engine as E
+    rng = np.random.default_rng(100 + k)
+    cases = [np.array([[2, 1]], np.uint32) if k == 2 else np.zeros((1, 1), np.uint32)]
+    for m, n in [(1, 1), (5, 9), (9, 5), (17, 17), (40, 33), (33, 70), (64, 64), (100, 130), (200, 150)]:
+        top = (1 << k) if k < 32 else (1 << 32)
+        cases.append(rng.integers(0, top, (m, n), dtype=np.uint64).astype(np.uint32))
+        cases.append(_planted(rng, m, n, k))
+    cases.append(np.zeros((6, 4), np.uint32))
+    mats = [E.to_device(A) for A in cases]
+    b = E.HowellBatch(k, mats)
+    b.run()
+    torch.cuda.synchronize()
+    for A, res in zip(cases, b.results()):
+        H, piv, pv = O.howell_fast(k, A)
+        assert np.array_equal(res.pivots, piv), f"k={k} {A.shape}: pivots"
+        assert np.array_equal(res.pivot_v, pv), f"k={k} {A.shape}: pivot_v"
+        assert np.array_equal(res.H, H), f"k={k} {A.shape}: H"
+    if k == 2:
+        r0 = b.results()[0]
+        assert r0.H.tolist() == [[2, 1], [0, 2]] and r0.pivots.tolist() == [0, 1] and r0.pivot_v.tolist() == [1, 1]
+
+
+def test_howell_k1_is_rref(hx_gpu, O):
+    """k = 1: the Howell form over F_2 is the RREF without zero rows (ENGINE_API.md:1988-1995)."""
+    from hacs_amd import engine as E
+    rng = np.random.default_rng(4)
+    A = rng.integers(0, 2, (70, 90), dtype=np.uint64).astype(np.uint32)
+    b = E.HowellBatch(1, [E.to_device(A)])
+    b.run()
+    torch.cuda.synchronize()
+    res = b.results()[0]
+    r, piv, R = O.rref_naive(2, A)
+    assert np.array_equal(res.H, R[:r]) and np.array_equal(res.pivots, piv) and not res.pivot_v.any()
