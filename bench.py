@@ -1,0 +1,235 @@
+#!/usr/bin/env python3
+"""Benchmark of the exact mod-p matrix engine on one MI355X.
+
+Headline workload (one "step"): C = A * B mod 2147483629 with A, B 8192 x 8192 uint32
+residues (the 8192^3 GEMM configuration), computed by the limb-MFMA kernel; inputs are
+generated on the device (SplitMix64 / SeedFor, salts 0x01 / 0x02) and stay resident in
+HBM.  The metric is exact mod-p multiply-adds per second (n^3 per step).  The same run
+also times the plain VALU kernel on the same inputs (limb vs plain), checks the two
+results are identical (and spot-checks entries against exact integer arithmetic), and
+times the batched echelon workload (4096 sparse jobs mod 1000003) as one persistent
+launch versus one launch per job (jobs / s).
+
+Prints ONE JSON line.  `--gpus N` with N > 1 is refused: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N_C3 = 8192
+PEAK_INT8_TOPS = 5000.0      # MI355X dense INT8 MFMA, tera-ops/s
+PEAK_HBM_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=N_C3)
+    ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="do not time the CPU baseline")
+    ap.add_argument("--no-batch", action="store_true", help="do not time the batched echelon workload")
+    ap.add_argument("--batch-jobs", type=int, default=4096)
+    return ap.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def gemm_timed(E, ctx, A, B, Cm, variant, steps, warmup, ws):
+    """Times `steps` GEMMs on one stream; returns (wall ms per step, main-kernel ms per step)."""
+    import torch
+    from hacs_amd import _lib
+    s = torch.cuda.current_stream()
+    h = s.cuda_stream
+    L = _lib.lib()
+    evs = []
+    for _ in range(steps):
+        a, b = C.c_void_p(), C.c_void_p()
+        _lib.check(L.hx_event_create(C.byref(a), 1))
+        _lib.check(L.hx_event_create(C.byref(b), 1))
+        evs.append((a, b))
+    o = _lib.hx_gemm_opts()
+    o.variant = variant
+    if ws is not None:
+        o.workspace = C.c_void_p(ws.data_ptr())
+        o.workspace_bytes = ws.numel()
+    for _ in range(warmup):
+        _lib.check(L.hx_gemm_u32(C.byref(ctx.c), E.view(A), E.view(B), E.view(Cm), 0, h, C.byref(o)))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        o.main_begin, o.main_end = evs[i]
+        _lib.check(L.hx_gemm_u32(C.byref(ctx.c), E.view(A), E.view(B), E.view(Cm), 0, h, C.byref(o)))
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) * 1e3 / steps
+    tot = 0.0
+    for a, b in evs:
+        ms = C.c_float()
+        _lib.check(L.hx_event_elapsed_ms(a, b, C.byref(ms)))
+        tot += ms.value
+        L.hx_event_destroy(a)
+        L.hx_event_destroy(b)
+    return wall, tot / steps
+
+
+def spot_check(C_host, A_dev, B_dev, p, count=256, seed=7):
+    """count entries of C against exact Python-integer dot products."""
+    import torch
+    rng = np.random.default_rng(seed)
+    n = C_host.shape[0]
+    rows = rng.integers(0, n, count)
+    cols = rng.integers(0, n, count)
+    Ar = A_dev[torch.as_tensor(rows, device="cuda")].cpu().numpy().view(np.uint32).astype(object)
+    Bc = B_dev[:, torch.as_tensor(cols, device="cuda")].cpu().numpy().view(np.uint32).astype(object)
+    for t in range(count):
+        exact = int(np.dot(Ar[t], Bc[:, t])) % p
+        if exact != int(C_host[rows[t], cols[t]]):
+            return False
+    return True
+
+
+def batched_echelon(E, njobs):
+    """C4: jobs / s of one persistent launch vs one launch per job."""
+    import torch
+    from hacs_amd import workloads as W
+    p = W.P_C4
+    ctx = E.ModCtx(p)
+    shapes = W.c4_shapes(njobs)
+    src = [E.to_device(W.sparse_job(j, int(m), int(n), p)) for j, (m, n) in enumerate(shapes)]
+    work = [s.clone() for s in src]
+    batch = E.RrefBatch(ctx, work)
+    batch.run()
+    batch.status()
+    torch.cuda.synchronize()
+    reps = 3
+    t_batch = []
+    for _ in range(reps):
+        for w_, s_ in zip(work, src):
+            w_.copy_(s_)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        batch.run()
+        torch.cuda.synchronize()
+        t_batch.append(time.perf_counter() - t0)
+    singles = [E.RrefBatch(ctx, [w_]) for w_ in work]
+    for w_, s_ in zip(work, src):
+        w_.copy_(s_)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for sb in singles:
+        sb.run()
+    torch.cuda.synchronize()
+    t_single = time.perf_counter() - t0
+    ranks = [r.rank for r in batch.results()]
+    work_macs = float(sum((int(m) - 1) * (r * int(n) - r * (r - 1) // 2) for (m, n), r in zip(shapes, ranks)))
+    return {
+        "jobs": njobs,
+        "jobs_per_s": njobs / min(t_batch),
+        "per_job_launch_jobs_per_s": njobs / t_single,
+        "speedup_vs_per_job": t_single / min(t_batch),
+        "elim_work_macs": work_macs,
+        "ms": min(t_batch) * 1e3,
+    }
+
+
+def cpu_baseline(p, threads):
+    """The reference's own CPU GEMM (oracle/_ref, Kernels.cpp:1417) on a bounded sample, min of 3."""
+    from oracle import oracle as O
+    from hacs_amd import workloads as W
+    m, k, n = 512, 8192, 1024
+    A = W.random_entries(m * k, p, W.seed_for(k, k, p, 0x01)).reshape(m, k)              # rows 0..m-1 of A
+    B = np.ascontiguousarray(W.random_entries(k * k, p, W.seed_for(k, k, p, 0x02)).reshape(k, k)[:, :n])
+    kind = "reference" if O.ref_available() else "port"
+    fn = O.ref_gemm if kind == "reference" else O.gemm_fast
+    best = 1e30
+    for _ in range(3):
+        t0 = time.perf_counter()
+        fn(p, A, B, None, 0, threads)
+        best = min(best, time.perf_counter() - t0)
+    return {"value": m * k * n / best, "unit": "mod-p mult-adds/s", "cores": threads, "kind": kind,
+            "sample": f"C = A B mod {p}, A {m}x{k}, B {k}x{n} (the 8192-deep product of the headline "
+                      f"config on {m}x{n} outputs), min of 3"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+    import torch
+    import hacs_amd
+    from hacs_amd import engine as E
+    from hacs_amd import workloads as W
+    if not hacs_amd.available():
+        raise SystemExit("no gfx950 device: " + hacs_amd.lib().hx_unavailable_reason().decode())
+    n, p = args.n, W.P_C3
+    ctx = E.ModCtx(p)
+    log(f"device {E.device_info()}; C3 n={n} p={p}")
+    A = E.random_entries(n, n, p, W.seed_for(n, n, p, 0x01))
+    B = E.random_entries(n, n, p, W.seed_for(n, n, p, 0x02))
+    C_limb = torch.empty((n, n), dtype=torch.int32, device="cuda")
+    C_plain = torch.empty((n, n), dtype=torch.int32, device="cuda")
+    ws = torch.empty(E.gemm_workspace_bytes(ctx, n, n, n, E.HX_GEMM_LIMB), dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+
+    wall_l, main_l = gemm_timed(E, ctx, A, B, C_limb, E.HX_GEMM_LIMB, args.steps, args.warmup, ws)
+    log(f"limb: {wall_l:.3f} ms/step (main kernel {main_l:.3f} ms)")
+    wall_p, main_p = gemm_timed(E, ctx, A, B, C_plain, E.HX_GEMM_PLAIN, args.plain_steps, 1, None)
+    log(f"plain: {wall_p:.3f} ms/step")
+    same = bool(torch.equal(C_limb, C_plain))
+    spot = spot_check(E.from_device(C_limb), A, B, p)
+    log(f"limb == plain: {same}; spot entries exact: {spot}")
+    macs = float(n) ** 3
+    value = macs / (wall_l * 1e-3)
+    achieved = 2.0 * 16.0 * macs / (main_l * 1e-3) / 1e12
+    res = {
+        "metric": "exact mod-p GEMM mult-adds/s at n=8192 (limb-MFMA vs plain); batched LU jobs/s",
+        "value": value,
+        "unit": "mod-p mult-adds/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall_l,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int8",
+        "data": "synthetic",
+        "config": {"workload": f"GEMM mod {p}, m=n=k={n}, uint32 residues (SplitMix64 salts 0x01/0x02), "
+                               "limb-MFMA kernel, inputs resident in HBM",
+                   "modulus": p, "n": n, "limbs": ctx.limbs, "parallelism": "single GPU"},
+        "plain_value": macs / (wall_p * 1e-3),
+        "plain_ms_per_step": wall_p,
+        "limb_vs_plain": wall_p / wall_l,
+        "parity": {"limb_equals_plain": same, "spot_entries_exact": spot},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_INT8_TOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_INT8_TOPS, "traffic": None,
+                     "kernel": "limb_kernel<4>", "kernel_ms": main_l,
+                     "algorithmic_ops_per_launch": 32.0 * macs},
+    }
+    if not args.no_batch:
+        b = batched_echelon(E, args.batch_jobs)
+        log(f"batched echelon: {b}")
+        res["batched_lu"] = b
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        res["cpu_baseline"] = cpu_baseline(p, threads)
+        log(f"cpu baseline: {res['cpu_baseline']}")
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

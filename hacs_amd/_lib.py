@@ -46,7 +46,7 @@ class hx_view(C.Structure):
 
 class hx_gemm_opts(C.Structure):
     _fields_ = [("variant", C.c_int), ("tile", C.c_int), ("workspace", C.c_void_p),
-                ("workspace_bytes", C.c_size_t)]
+                ("workspace_bytes", C.c_size_t), ("main_begin", C.c_void_p), ("main_end", C.c_void_p)]
 
 
 class hx_device_info(C.Structure):
@@ -57,7 +57,7 @@ class hx_device_info(C.Structure):
 
 class hx_gemm_job(C.Structure):
     _fields_ = [("m", C.c_uint32), ("n", C.c_uint32), ("k", C.c_uint32), ("op", C.c_int32),
-                ("A", C.c_void_p), ("B", C.c_void_p), ("C", C.c_void_p),
+                ("variant", C.c_int32), ("reserved", C.c_int32), ("A", C.c_void_p), ("B", C.c_void_p), ("C", C.c_void_p),
                 ("lda", C.c_int64), ("ldb", C.c_int64), ("ldc", C.c_int64)]
 
 

@@ -65,15 +65,12 @@ __device__ __forceinline__ void tile_of(uint32_t b, uint32_t ntm, uint32_t ntn, 
 // ------------------------------------------------------------------------------------------
 constexpr int PM = 64, PN = 64, PK = 32;
 
-__global__ __launch_bounds__(256) void plain_kernel(GemmArgs g, DevMod M) {
+__device__ __forceinline__ void plain_tile(const GemmArgs& g, const DevMod& M, uint32_t tm, uint32_t tn) {
     __shared__ uint32_t As[2][PK][PM + 1];
     __shared__ uint32_t Bs[2][PK][PN + 1];
     const uint32_t m = g.m_dev ? min(*g.m_dev, g.m) : g.m;
     const uint32_t k = g.k_dev ? min(*g.k_dev, g.k) : g.k;
     const uint32_t n = g.n;
-    const uint32_t ntm = (g.m + PM - 1) / PM, ntn = (n + PN - 1) / PN;
-    uint32_t tm, tn;
-    tile_of(blockIdx.x, ntm, ntn, tm, tn);
     const uint32_t m0 = tm * PM, n0 = tn * PN;
     if (m0 >= m) return;
     const int tid = threadIdx.x, tx = tid % 16, ty = tid / 16;
@@ -148,6 +145,31 @@ __global__ __launch_bounds__(256) void plain_kernel(GemmArgs g, DevMod M) {
     }
 }
 
+__global__ __launch_bounds__(256) void plain_kernel(GemmArgs g, DevMod M) {
+    const uint32_t ntm = (g.m + PM - 1) / PM, ntn = (g.n + PN - 1) / PN;
+    uint32_t tm, tn;
+    tile_of(blockIdx.x, ntm, ntn, tm, tn);
+    plain_tile(g, M, tm, tn);
+}
+
+// The workgroup's job in a batch: prefix[j] = first flat tile of job j (prefix[njobs] = total).
+__device__ __forceinline__ uint32_t job_of(const uint32_t* prefix, uint32_t njobs, uint32_t b) {
+    uint32_t lo = 0, hi = njobs;          // largest j with prefix[j] <= b
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (prefix[mid] <= b) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void plain_batch_kernel(const GemmArgs* jobs, const uint32_t* prefix, uint32_t njobs,
+                                                          DevMod M) {
+    const uint32_t j = job_of(prefix, njobs, blockIdx.x);
+    const GemmArgs g = jobs[j];
+    const uint32_t local = blockIdx.x - prefix[j], ntn = (g.n + PN - 1) / PN;
+    plain_tile(g, M, local / ntn, local % ntn);
+}
+
 // ------------------------------------------------------------------------------------------
 // limb route
 // ------------------------------------------------------------------------------------------
@@ -183,9 +205,9 @@ struct PrepArgs {
 
 // One 64 x 64 tile of the output (rows x k) per workgroup: coalesced loads along the source's
 // contiguous direction into LDS, then every thread writes 16 k-bytes of one row per plane.
-__global__ __launch_bounds__(256) void limb_prep_kernel(PrepArgs a) {
+__device__ __forceinline__ void prep_tile(const PrepArgs& a, uint32_t bx, uint32_t by) {
     __shared__ uint32_t T[64][65];
-    const uint32_t r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+    const uint32_t r0 = by * 64, c0 = bx * 64;
     const uint32_t R = a.r_dev ? min(*a.r_dev, a.R) : a.R;
     const uint32_t K = a.k_dev ? min(*a.k_dev, a.K) : a.K;
     const int tid = threadIdx.x;
@@ -227,6 +249,16 @@ __global__ __launch_bounds__(256) void limb_prep_kernel(PrepArgs a) {
     }
 }
 
+__global__ __launch_bounds__(256) void limb_prep_kernel(PrepArgs a) { prep_tile(a, blockIdx.x, blockIdx.y); }
+
+// Batched prep: prefix over the (Kp / 64) x (Rp / 64) tile grids of every operand.
+__global__ __launch_bounds__(256) void prep_batch_kernel(const PrepArgs* preps, const uint32_t* prefix, uint32_t n) {
+    const uint32_t j = job_of(prefix, n, blockIdx.x);
+    const PrepArgs a = preps[j];
+    const uint32_t local = blockIdx.x - prefix[j], gx = a.Kp / 64;
+    prep_tile(a, local % gx, local / gx);
+}
+
 constexpr int LM = 64, LN = 128, LK = 64;      // workgroup tile; k-depth of one LDS stage
 constexpr int LROW = 80;                        // LDS bytes per (plane, row): 64 k-bytes + 16 pad
 
@@ -249,12 +281,9 @@ struct LimbArgs {
 };
 
 template <int L>
-__global__ __launch_bounds__(256, 1) void limb_kernel(LimbArgs a) {
-    extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
+__device__ __forceinline__ void limb_tile(const LimbArgs& a, uint32_t tm, uint32_t tn, uint8_t* smem) {
     constexpr int A_BYTES = L * LM * LROW, B_BYTES = L * LN * LROW, STAGE = A_BYTES + B_BYTES;
     constexpr int ND = 2 * L - 1;
-    uint32_t tm, tn;
-    tile_of(blockIdx.x, a.ntm, a.ntn, tm, tn);
     const uint32_t m = a.m_dev ? min(*a.m_dev, a.m) : a.m;
     const uint32_t m0 = tm * LM, n0 = tn * LN;
     if (m0 >= m) return;
@@ -370,13 +399,41 @@ __global__ __launch_bounds__(256, 1) void limb_kernel(LimbArgs a) {
 }
 
 template <int L>
-void launch_limb(const LimbArgs& la, hipStream_t s) {
-    constexpr int STAGE = L * (LM + LN) * LROW;
-    const size_t lds = 2 * (size_t)STAGE;
+__global__ __launch_bounds__(256, 1) void limb_kernel(LimbArgs a) {
+    extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
+    uint32_t tm, tn;
+    tile_of(blockIdx.x, a.ntm, a.ntn, tm, tn);
+    limb_tile<L>(a, tm, tn, smem);
+}
+
+template <int L>
+__global__ __launch_bounds__(256, 1) void limb_batch_kernel(const LimbArgs* jobs, const uint32_t* prefix,
+                                                            uint32_t njobs) {
+    extern __shared__ __attribute__((aligned(16))) uint8_t smem[];
+    const uint32_t j = job_of(prefix, njobs, blockIdx.x);
+    const LimbArgs a = jobs[j];
+    const uint32_t local = blockIdx.x - prefix[j];
+    limb_tile<L>(a, local / a.ntn, local % a.ntn, smem);
+}
+
+template <int L>
+size_t limb_lds() { return 2 * (size_t)(L * (LM + LN) * LROW); }
+
+template <int L>
+void limb_attrs() {
     static std::once_flag once;
-    std::call_once(once, [&] {
-        (void)hipFuncSetAttribute((const void*)limb_kernel<L>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    std::call_once(once, [] {
+        (void)hipFuncSetAttribute((const void*)limb_kernel<L>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)limb_lds<L>());
+        (void)hipFuncSetAttribute((const void*)limb_batch_kernel<L>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)limb_lds<L>());
     });
+}
+
+template <int L>
+void launch_limb(const LimbArgs& la, hipStream_t s) {
+    const size_t lds = limb_lds<L>();
+    limb_attrs<L>();
     const uint32_t blocks = la.ntm * la.ntn;
     limb_kernel<L><<<blocks, 256, lds, s>>>(la);
     after_launch("limb_kernel");
@@ -397,14 +454,17 @@ size_t limb_workspace_bytes(const hx_modctx& ctx, uint32_t m, uint32_t n, uint32
     return (size_t)ctx.limbs * ((size_t)round_up(std::max<uint32_t>(m, 1), LM) + round_up(std::max<uint32_t>(n, 1), LN)) * kc;
 }
 
-void gemm_plain(const hx_modctx& ctx, const GemmArgs& g, hipStream_t s, int /*tile*/) {
+void gemm_plain(const hx_modctx& ctx, const GemmArgs& g, hipStream_t s, int /*tile*/, hipEvent_t ev0, hipEvent_t ev1) {
     if (g.m == 0 || g.n == 0) return;
     const uint32_t ntm = (g.m + PM - 1) / PM, ntn = (g.n + PN - 1) / PN;
+    if (ev0) HX_HIP(hipEventRecord(ev0, s));
     plain_kernel<<<ntm * ntn, 256, 0, s>>>(g, devmod(ctx));
     after_launch("plain_kernel");
+    if (ev1) HX_HIP(hipEventRecord(ev1, s));
 }
 
-void gemm_limb(const hx_modctx& ctx, const GemmArgs& g0, hipStream_t s, void* ws, size_t ws_bytes, int /*tile*/) {
+void gemm_limb(const hx_modctx& ctx, const GemmArgs& g0, hipStream_t s, void* ws, size_t ws_bytes, int /*tile*/,
+               hipEvent_t ev0, hipEvent_t ev1) {
     const int L = (int)ctx.limbs;
     HX_CHECK(L >= 1 && L <= 4, "the limb route needs p <= 4278124287 (ctx.limbs > 0)");
     if (g0.m == 0 || g0.n == 0) return;
@@ -480,12 +540,15 @@ void gemm_limb(const hx_modctx& ctx, const GemmArgs& g0, hipStream_t s, void* ws
         la.op = g.op;
         la.off = off;
         la.M = devmod(ctx);
+        const bool last = k0 + kc >= g.k;
+        if (ev0 && k0 == 0) HX_HIP(hipEventRecord(ev0, s));
         switch (L) {
             case 1: launch_limb<1>(la, s); break;
             case 2: launch_limb<2>(la, s); break;
             case 3: launch_limb<3>(la, s); break;
             default: launch_limb<4>(la, s); break;
         }
+        if (ev1 && last) HX_HIP(hipEventRecord(ev1, s));
         // later chunks accumulate into the C just written
         k0 += kc;
         g.op = g.op == HX_OP_SUB ? HX_OP_SUB : HX_OP_ADD;
@@ -503,12 +566,171 @@ int gemm_auto_variant(const hx_modctx& ctx, uint32_t m, uint32_t n, uint32_t k) 
 }
 
 void gemm_dispatch(const hx_modctx& ctx, const GemmArgs& g, hipStream_t s, int variant, void* ws, size_t ws_bytes,
-                   int tile) {
+                   int tile, hipEvent_t ev0, hipEvent_t ev1) {
     if (variant == HX_GEMM_AUTO) variant = gemm_auto_variant(ctx, g.m, g.n, g.k);
     if (variant == HX_GEMM_LIMB)
-        gemm_limb(ctx, g, s, ws, ws_bytes, tile);
+        gemm_limb(ctx, g, s, ws, ws_bytes, tile, ev0, ev1);
     else
-        gemm_plain(ctx, g, s, tile);
+        gemm_plain(ctx, g, s, tile, ev0, ev1);
+}
+
+// ---- batched uneven GEMM -------------------------------------------------------------------
+namespace {
+
+struct GBatch {
+    hx_modctx ctx;
+    uint32_t n_plain = 0, n_limb = 0, n_prep = 0;
+    uint32_t tiles_plain = 0, tiles_limb = 0, tiles_prep = 0;
+    GemmArgs* d_plain = nullptr;
+    LimbArgs* d_limb = nullptr;
+    PrepArgs* d_prep = nullptr;
+    uint32_t* d_prefix = nullptr;        // [plain prefix (n_plain+1) | limb prefix (n_limb+1) | prep prefix (n_prep+1)]
+    int8_t* d_ws = nullptr;
+};
+
+void gbatch_free(GBatch* b) {
+    if (!b) return;
+    (void)hipFree(b->d_plain);
+    (void)hipFree(b->d_limb);
+    (void)hipFree(b->d_prep);
+    (void)hipFree(b->d_prefix);
+    (void)hipFree(b->d_ws);
+    delete b;
+}
+
+template <class T>
+T* upload_vec(const std::vector<T>& v) {
+    T* d = nullptr;
+    if (v.empty()) return nullptr;
+    HX_HIP(hipMalloc(&d, v.size() * sizeof(T)));
+    HX_HIP(hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return d;
+}
+
+}  // namespace
+
+void* gemm_batch_create(const hx_modctx& ctx, const hx_gemm_job* jobs, uint32_t njobs, int variant) {
+    GBatch* b = new GBatch();
+    try {
+        b->ctx = ctx;
+        std::vector<GemmArgs> plain;
+        std::vector<LimbArgs> limb;
+        std::vector<PrepArgs> prep;
+        std::vector<uint32_t> pp{0}, lp{0}, prp{0};
+        size_t ws = 0;
+        const int L = (int)ctx.limbs;
+        const uint32_t kchunk = L ? limb_k_chunk(L) : 0;
+        struct Pending { size_t a_off, b_off; uint32_t Mp, Np, Kp; };
+        std::vector<Pending> pend;
+        const uint32_t p = ctx.p, half = (p - 1) / 2;
+        const int64_t off = (int64_t)((((1ull << 57) + p - 1) / p) * (uint64_t)p);
+        for (uint32_t i = 0; i < njobs; ++i) {
+            const hx_gemm_job& j = jobs[i];
+            HX_CHECK(j.op == HX_OP_SET || j.op == HX_OP_ADD || j.op == HX_OP_SUB, "gemm batch: bad op");
+            if (j.m == 0 || j.n == 0) continue;
+            HX_CHECK(j.C != nullptr && (j.k == 0 || (j.A && j.B)), "gemm batch: NULL operand");
+            HX_CHECK(j.lda >= (int64_t)j.k && j.ldb >= (int64_t)j.n && j.ldc >= (int64_t)j.n, "gemm batch: bad ld");
+            GemmArgs g;
+            g.m = j.m; g.n = j.n; g.k = j.k;
+            g.A = j.A; g.ars = j.lda; g.acs = 1;
+            g.B = j.B; g.brs = j.ldb; g.bcs = 1;
+            g.Cin = j.C; g.cirs = j.ldc; g.cics = 1;
+            g.Cout = j.C; g.cors = j.ldc; g.cocs = 1;
+            g.op = j.op;
+            int v = j.variant != HX_GEMM_AUTO ? j.variant : variant;
+            if (v == HX_GEMM_AUTO) v = gemm_auto_variant(ctx, j.m, j.n, j.k);
+            if (v == HX_GEMM_LIMB && (L == 0 || j.k > kchunk)) v = HX_GEMM_PLAIN;
+            if (v != HX_GEMM_LIMB) {
+                plain.push_back(g);
+                pp.push_back(pp.back() + ((j.m + PM - 1) / PM) * ((j.n + PN - 1) / PN));
+                continue;
+            }
+            const uint32_t Mp = round_up(j.m, LM), Np = round_up(j.n, LN), Kp = round_up(j.k, 64);
+            pend.push_back({ws, ws + (size_t)L * Mp * Kp, Mp, Np, Kp});
+            ws += (size_t)L * (Mp + Np) * Kp;
+            LimbArgs la{};
+            la.a_plane = (size_t)Mp * Kp;
+            la.b_plane = (size_t)Np * Kp;
+            la.Kp = Kp;
+            la.m = j.m; la.n = j.n;
+            la.ntm = Mp / LM; la.ntn = Np / LN;
+            la.Cin = j.C; la.cirs = j.ldc; la.cics = 1;
+            la.Cout = j.C; la.cors = j.ldc; la.cocs = 1;
+            la.op = j.op; la.off = off; la.M = devmod(ctx);
+            limb.push_back(la);
+            lp.push_back(lp.back() + la.ntm * la.ntn);
+            if (Kp > 0) {
+                PrepArgs pa{};
+                pa.src = j.A; pa.s_r = j.lda; pa.s_c = 1;
+                pa.R = j.m; pa.K = j.k; pa.Kp = Kp; pa.plane = (size_t)Mp * Kp;
+                pa.p = p; pa.half = half; pa.L = L;
+                prep.push_back(pa);
+                prp.push_back(prp.back() + (Kp / 64) * (Mp / 64));
+                PrepArgs pb = pa;
+                pb.src = j.B; pb.s_r = 1; pb.s_c = j.ldb; pb.R = j.n; pb.plane = (size_t)Np * Kp;
+                prep.push_back(pb);
+                prp.push_back(prp.back() + (Kp / 64) * (Np / 64));
+            }
+        }
+        if (ws) HX_HIP(hipMalloc(&b->d_ws, ws));
+        size_t pi = 0;
+        for (size_t i = 0; i < limb.size(); ++i) {
+            limb[i].A8 = b->d_ws + pend[i].a_off;
+            limb[i].B8 = b->d_ws + pend[i].b_off;
+            if (pend[i].Kp > 0) {
+                prep[pi++].out = b->d_ws + pend[i].a_off;
+                prep[pi++].out = b->d_ws + pend[i].b_off;
+            }
+        }
+        b->n_plain = (uint32_t)plain.size();
+        b->n_limb = (uint32_t)limb.size();
+        b->n_prep = (uint32_t)prep.size();
+        b->tiles_plain = pp.back();
+        b->tiles_limb = lp.back();
+        b->tiles_prep = prp.back();
+        b->d_plain = upload_vec(plain);
+        b->d_limb = upload_vec(limb);
+        b->d_prep = upload_vec(prep);
+        std::vector<uint32_t> pre(pp);
+        pre.insert(pre.end(), lp.begin(), lp.end());
+        pre.insert(pre.end(), prp.begin(), prp.end());
+        b->d_prefix = upload_vec(pre);
+        switch (L) {
+            case 1: limb_attrs<1>(); break;
+            case 2: limb_attrs<2>(); break;
+            case 3: limb_attrs<3>(); break;
+            case 4: limb_attrs<4>(); break;
+            default: break;
+        }
+    } catch (...) {
+        gbatch_free(b);
+        throw;
+    }
+    return b;
+}
+
+void gemm_batch_run(void* vb, hipStream_t s) {
+    GBatch* b = (GBatch*)vb;
+    const uint32_t* pre_plain = b->d_prefix;
+    const uint32_t* pre_limb = pre_plain + b->n_plain + 1;
+    const uint32_t* pre_prep = pre_limb + b->n_limb + 1;
+    if (b->tiles_prep) {
+        prep_batch_kernel<<<b->tiles_prep, 256, 0, s>>>(b->d_prep, pre_prep, b->n_prep);
+        after_launch("prep_batch_kernel");
+    }
+    if (b->tiles_limb) {
+        switch (b->ctx.limbs) {
+            case 1: limb_batch_kernel<1><<<b->tiles_limb, 256, limb_lds<1>(), s>>>(b->d_limb, pre_limb, b->n_limb); break;
+            case 2: limb_batch_kernel<2><<<b->tiles_limb, 256, limb_lds<2>(), s>>>(b->d_limb, pre_limb, b->n_limb); break;
+            case 3: limb_batch_kernel<3><<<b->tiles_limb, 256, limb_lds<3>(), s>>>(b->d_limb, pre_limb, b->n_limb); break;
+            default: limb_batch_kernel<4><<<b->tiles_limb, 256, limb_lds<4>(), s>>>(b->d_limb, pre_limb, b->n_limb); break;
+        }
+        after_launch("limb_batch_kernel");
+    }
+    if (b->tiles_plain) {
+        plain_batch_kernel<<<b->tiles_plain, 256, 0, s>>>(b->d_plain, pre_plain, b->n_plain, devmod(b->ctx));
+        after_launch("plain_batch_kernel");
+    }
 }
 
 }  // namespace hx
@@ -574,8 +796,27 @@ HX_API int hx_gemm_u32(const hx_modctx* ctx, hx_view A, hx_view B, hx_view C, in
         const int variant = opts ? opts->variant : HX_GEMM_AUTO;
         HX_CHECK(variant != HX_GEMM_LIMB || ctx->limbs > 0, "gemm: the limb route needs p <= 4278124287");
         gemm_dispatch(*ctx, args_of(A, B, C, op), (hipStream_t)s, variant, opts ? opts->workspace : nullptr,
-                      opts ? opts->workspace_bytes : 0, opts ? opts->tile : 0);
+                      opts ? opts->workspace_bytes : 0, opts ? opts->tile : 0,
+                      opts ? (hipEvent_t)opts->main_begin : nullptr, opts ? (hipEvent_t)opts->main_end : nullptr);
     });
+}
+
+HX_API int hx_gemm_batch_create(const hx_modctx* ctx, const hx_gemm_job* jobs, uint32_t njobs, int variant,
+                                hx_gemm_batch* out) {
+    return guard("hx_gemm_batch_create", [&] {
+        HX_CHECK(ctx != nullptr && out != nullptr && (jobs != nullptr || njobs == 0), "NULL argument");
+        HX_CHECK(variant != HX_GEMM_LIMB || ctx->limbs > 0, "gemm batch: the limb route needs p <= 4278124287");
+        *out = gemm_batch_create(*ctx, jobs, njobs, variant);
+    });
+}
+HX_API int hx_gemm_batch_run(hx_gemm_batch b, hx_stream s) {
+    return guard("hx_gemm_batch_run", [&] {
+        HX_CHECK(b != nullptr, "NULL batch");
+        gemm_batch_run(b, (hipStream_t)s);
+    });
+}
+HX_API int hx_gemm_batch_destroy(hx_gemm_batch b) {
+    return guard("hx_gemm_batch_destroy", [&] { gbatch_free((GBatch*)b); });
 }
 
 HX_API int hx_schur_update_u32(const hx_modctx* ctx, hx_view L, hx_view U, hx_view C, hx_stream s,

@@ -1,9 +1,478 @@
-// hx_rref_large.hip -- the multi-kernel RREF for large matrices (not built yet: every shape goes
-// through the one-workgroup path).
-#include "hx_common.h"
+// hx_rref_large.hip -- RREF over F_p (p < 2^31 prime) of one large matrix, spread over the whole GPU.
+//
+// Right-looking elimination in column panels of W = 32 columns over a compacted Schur complement S
+// (m_a x n_a, rows in original order: the i-th row of S is original row rows_orig[i]):
+//
+//  1. Panel pivots by a TOURNAMENT.  The pivot rule -- per column, the unused row of smallest
+//     original index with a non-zero Schur entry (Cuda/CudaKernels.h:255-265) -- picks, after column
+//     c, the row rank profile RRP(S[:, 0..c]).  A row in RRP of a stack lies in the RRP of its own
+//     block (a dependency on the rows above it inside the block is one in the stack), and the span
+//     of all rows above a candidate equals the span of the candidates above it, so
+//     RRP(S[:, 0..c]) = RRP(candidates[:, 0..c]) for every c: the pivots AND their assignment to
+//     columns are those of the candidate rows alone.  Level 0: one workgroup per 256-row chunk
+//     finds its chunk's candidates (division-free elimination in LDS); merge levels: one workgroup
+//     per group of 8 candidate lists re-runs the rule on their raw rows; the last level writes the
+//     panel's pivot rows P (pivot order), columns J and count k.
+//  2. X = S[P, J]^{-1} S[P, :]: every workgroup owns a slab of 256 columns and performs the same
+//     row reduction on [S[P, J] | S[P, slab]] (division-free forward, one inverse per pivot,
+//     back substitution), writing X[:, slab] into the U store (row r + t, original columns) and
+//     into a contiguous copy for the next step.  X has the identity at J.
+//  3. The rows R not in P, in order, form the next Schur complement:
+//        S'[i, :] = S[R_i, W:] - S[R_i, J] X[:, W:]
+//     one GEMM (the limb-MFMA route) with a row gather on A and C_in, a column gather on A, and
+//     output into the other S buffer (compaction for free).
+//  All sizes (m_a, k, r) live in device memory; the host only knows upper bounds, so a panel is a
+//  fixed sequence of launches.  At the end the rank rows of the U store (identity at their own
+//  panel's pivot columns, zero to the left) are cleared against each other by one unit-upper TRSM
+//  on the non-pivot columns: RREF = U_piv^{-1} U.
+#include <algorithm>
+#include <vector>
+
+#include "hx_elim.h"
+#include "hx_gemm.h"
+
 namespace hx {
-bool rref_large(const hx_modctx&, uint32_t, uint32_t, const uint32_t*, int64_t, uint32_t*, int64_t, uint32_t*,
-                uint32_t*, uint32_t*, hipStream_t) {
-    return false;
+
+void trsm_unit(const hx_modctx& ctx, hx_view T, hx_view B, int upper, hipStream_t s);
+
+namespace {
+
+constexpr int LW = 32;        // panel width
+constexpr int CH = 256;       // rows per level-0 chunk
+constexpr int GRP = 8;        // candidate lists merged per workgroup
+constexpr int LT = 256;       // threads
+
+struct LState {               // device-side counters
+    uint32_t m_a, r, k, m_new;
+    int status;
+};
+
+__device__ __forceinline__ uint32_t wmin(uint32_t x) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x = min(x, (uint32_t)__shfl_xor((int)x, off));
+    return x;
 }
+
+// The smallest-row rule on an LDS panel pan[u * ws + q] (rows in original order), division-free.
+// Returns k; s_u[t] / s_q[t] = local row / column of pivot t.  chosen[] is cleared here.
+template <class Ar>
+__device__ uint32_t lds_rule(uint32_t* pan, uint32_t rows, uint32_t wc, uint32_t ws, uint8_t* chosen, const Ar& ar,
+                             uint32_t* s_u, uint32_t* s_q) {
+    __shared__ uint32_t s_cand[2][LT / 64];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, p = ar.p;
+    for (uint32_t u = tid; u < rows; u += LT) chosen[u] = 0;
+    __syncthreads();
+    uint32_t k = 0;
+    for (uint32_t q = 0; q < wc; ++q) {
+        uint32_t best = 0xffffffffu;
+        for (uint32_t u = tid; u < rows; u += LT)
+            if (!chosen[u] && pan[u * ws + q] != 0) { best = u; break; }
+        best = wmin(best);
+        if (lane == 0) s_cand[q & 1][wave] = best;
+        __syncthreads();
+        const uint32_t bu = min(min(s_cand[q & 1][0], s_cand[q & 1][1]), min(s_cand[q & 1][2], s_cand[q & 1][3]));
+        if (bu == 0xffffffffu) continue;
+        const uint32_t* prow = pan + bu * ws;
+        const uint32_t v = prow[q];
+        for (uint32_t u = tid; u < rows; u += LT) {
+            if (u == bu) { chosen[u] = 1; continue; }
+            if (chosen[u]) continue;
+            uint32_t* row = pan + u * ws;
+            const uint32_t f = row[q];
+            if (f == 0) continue;
+            const uint32_t fneg = p - f;
+            for (uint32_t q2 = q + 1; q2 < wc; ++q2) row[q2] = ar.dfree(v, row[q2], fneg, prow[q2]);
+        }
+        if (tid == 0) { s_u[k] = bu; s_q[k] = q; }
+        ++k;
+    }
+    __syncthreads();
+    return k;
+}
+
+// Level 0: chunk `blockIdx.x` of S's rows -> its candidate rows (ascending S indices).
+template <class Ar>
+__global__ __launch_bounds__(LT) void rrp_chunk_kernel(const uint32_t* S, int64_t ld, const LState* st, uint32_t wc,
+                                                       Ar ar, uint32_t* cand, uint32_t* cnt) {
+    __shared__ uint32_t pan[CH * (LW + 1)];
+    __shared__ uint8_t chosen[CH];
+    __shared__ uint32_t s_u[LW], s_q[LW];
+    const uint32_t ma = st->m_a, r0 = blockIdx.x * CH;
+    if (r0 >= ma) {
+        if (threadIdx.x == 0) cnt[blockIdx.x] = 0;
+        return;
+    }
+    const uint32_t rows = min((uint32_t)CH, ma - r0), ws = wc | 1u;
+    for (uint32_t idx = threadIdx.x; idx < rows * wc; idx += LT) {
+        const uint32_t u = idx / wc, q = idx % wc;
+        pan[u * ws + q] = S[(int64_t)(r0 + u) * ld + q];
+    }
+    const uint32_t k = lds_rule(pan, rows, wc, ws, chosen, ar, s_u, s_q);
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (uint32_t u = 0; u < rows; ++u)
+            if (chosen[u]) cand[blockIdx.x * LW + t++] = r0 + u;
+        cnt[blockIdx.x] = k;
+    }
+}
+
+// Merge: group `blockIdx.x` of up to GRP candidate lists -> one list; the final level writes P, J, k.
+template <class Ar>
+__global__ __launch_bounds__(LT) void rrp_merge_kernel(const uint32_t* S, int64_t ld, uint32_t wc, Ar ar,
+                                                       const uint32_t* cand_in, const uint32_t* cnt_in, uint32_t nin,
+                                                       uint32_t* cand_out, uint32_t* cnt_out, int final_level,
+                                                       uint32_t* P, uint32_t* J, LState* st) {
+    __shared__ uint32_t pan[GRP * LW * (LW + 1)];
+    __shared__ uint32_t rowid[GRP * LW];
+    __shared__ uint8_t chosen[GRP * LW];
+    __shared__ uint32_t s_u[LW], s_q[LW], s_off[GRP + 1];
+    const uint32_t g0 = blockIdx.x * GRP, ng = min((uint32_t)GRP, nin - g0);
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (uint32_t i = 0; i < ng; ++i) { s_off[i] = acc; acc += cnt_in[g0 + i]; }
+        s_off[ng] = acc;
+    }
+    __syncthreads();
+    const uint32_t rows = s_off[ng], ws = wc | 1u;
+    for (uint32_t i = 0; i < ng; ++i)
+        for (uint32_t t = threadIdx.x; t < s_off[i + 1] - s_off[i]; t += LT)
+            rowid[s_off[i] + t] = cand_in[(g0 + i) * LW + t];
+    __syncthreads();
+    for (uint32_t idx = threadIdx.x; idx < rows * wc; idx += LT) {
+        const uint32_t u = idx / wc, q = idx % wc;
+        pan[u * ws + q] = S[(int64_t)rowid[u] * ld + q];
+    }
+    const uint32_t k = lds_rule(pan, rows, wc, ws, chosen, ar, s_u, s_q);
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (uint32_t u = 0; u < rows; ++u)
+            if (chosen[u]) cand_out[blockIdx.x * LW + t++] = rowid[u];
+        cnt_out[blockIdx.x] = k;
+        if (final_level) {
+            for (uint32_t i = 0; i < k; ++i) {
+                P[i] = rowid[s_u[i]];
+                J[i] = s_q[i];
+            }
+            st->k = k;
+            st->m_new = st->m_a - k;
+        }
+    }
+}
+
+// X[:, slab] = S[P, J]^{-1} S[P, slab]; also the pivots / pivot rows of this panel (block 0).
+template <class Ar>
+__global__ __launch_bounds__(LT) void panel_x_kernel(const uint32_t* S, int64_t ld, uint32_t na, const uint32_t* P,
+                                                     const uint32_t* J, LState* st, Ar ar, uint32_t* X, int64_t ldx,
+                                                     uint32_t* U, int64_t ldu, uint32_t c, const uint32_t* rows_orig,
+                                                     uint32_t* pivots, uint32_t* pivot_rows) {
+    constexpr int SL = 256;
+    __shared__ uint32_t B[LW * (LW + SL)];
+    __shared__ uint32_t s_inv[LW], s_fp[LW], s_q[LW];
+    const uint32_t k = st->k;
+    if (k == 0) return;
+    const uint32_t j0 = blockIdx.x * SL;
+    if (j0 >= na) return;
+    const uint32_t sw = min((uint32_t)SL, na - j0), bw = k + sw, p = ar.p;
+    const uint32_t r = st->r;
+    for (uint32_t idx = threadIdx.x; idx < k * bw; idx += LT) {
+        const uint32_t t = idx / bw, j = idx % bw;
+        const int64_t row = (int64_t)P[t] * ld;
+        B[t * bw + j] = j < k ? S[row + J[j]] : S[row + j0 + (j - k)];
+    }
+    if (threadIdx.x < k) s_q[threadIdx.x] = J[threadIdx.x];
+    __syncthreads();
+    // forward (division-free) in pivot order: column t of the block is pivot column J[t]
+    for (uint32_t t = 0; t + 1 < k; ++t) {
+        const uint32_t wdt = bw - t - 1, cnt = (k - t - 1) * wdt;
+        const uint32_t v = B[t * bw + t];
+        for (uint32_t it = threadIdx.x; it < cnt; it += LT) {
+            const uint32_t row = t + 1 + it / wdt, j = t + 1 + it % wdt;
+            const uint32_t f = B[row * bw + t];
+            if (f != 0) B[row * bw + j] = ar.dfree(v, B[row * bw + j], p - f, B[t * bw + j]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < k) {
+        const uint32_t iv = invm(B[threadIdx.x * bw + threadIdx.x], p);
+        if (iv == 0) atomicExch(&st->status, 1);
+        s_inv[threadIdx.x] = ar.prep(iv);
+    }
+    __syncthreads();
+    for (uint32_t idx = threadIdx.x; idx < k * bw; idx += LT) {
+        const uint32_t t = idx / bw, j = idx % bw;
+        if (j > t) B[idx] = ar.mul(B[idx], s_inv[t]);
+    }
+    __syncthreads();
+    for (uint32_t t = k - 1; t >= 1; --t) {       // back substitution (block columns > t)
+        const uint32_t wdt = bw - t - 1, cnt = t * wdt;
+        if (threadIdx.x < t) s_fp[threadIdx.x] = ar.prep(B[threadIdx.x * bw + t]);
+        __syncthreads();
+        for (uint32_t it = threadIdx.x; it < cnt; it += LT) {
+            const uint32_t row = it / wdt, j = t + 1 + it % wdt;
+            if (j < k) continue;                   // the pivot block becomes the identity below
+            const uint32_t fp = s_fp[row];
+            if (fp != 0) B[row * bw + j] = subm(B[row * bw + j], ar.mul(B[t * bw + j], fp), p);
+        }
+        __syncthreads();
+    }
+    // X[:, slab]; the slab's own pivot columns get the identity exactly (they equal it already)
+    for (uint32_t idx = threadIdx.x; idx < k * sw; idx += LT) {
+        const uint32_t t = idx / sw, j = idx % sw;
+        uint32_t x = B[t * bw + k + j];
+        for (uint32_t t2 = 0; t2 < k; ++t2)
+            if (s_q[t2] == j0 + j) x = (t2 == t) ? 1u : 0u;
+        X[(int64_t)t * ldx + j0 + j] = x;
+        U[(int64_t)(r + t) * ldu + c + j0 + j] = x;
+    }
+    if (blockIdx.x == 0 && threadIdx.x < k) {
+        pivots[r + threadIdx.x] = c + s_q[threadIdx.x];
+        pivot_rows[r + threadIdx.x] = rows_orig[P[threadIdx.x]];
+    }
+}
+
+// R = the rows of S not in P, in order; rows_orig of the next S; one workgroup.
+__global__ __launch_bounds__(1024) void rest_rows_kernel(const LState* st, const uint32_t* P, uint32_t* flags,
+                                                         uint32_t* R, const uint32_t* rows_orig, uint32_t* rows_orig_next) {
+    __shared__ uint32_t s_sum[1024];
+    const uint32_t ma = st->m_a, k = st->k, tid = threadIdx.x;
+    for (uint32_t i = tid; i < ma; i += 1024) flags[i] = 1;
+    __syncthreads();
+    for (uint32_t t = tid; t < k; t += 1024) flags[P[t]] = 0;
+    __syncthreads();
+    const uint32_t per = (ma + 1023) / 1024, s0 = min(tid * per, ma), s1 = min(s0 + per, ma);
+    uint32_t c = 0;
+    for (uint32_t i = s0; i < s1; ++i) c += flags[i];
+    s_sum[tid] = c;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {      // inclusive scan (Hillis-Steele)
+        const uint32_t x = tid >= off ? s_sum[tid - off] : 0u;
+        __syncthreads();
+        s_sum[tid] += x;
+        __syncthreads();
+    }
+    uint32_t pos = s_sum[tid] - c;
+    for (uint32_t i = s0; i < s1; ++i)
+        if (flags[i]) {
+            R[pos] = i;
+            rows_orig_next[pos] = rows_orig[i];
+            ++pos;
+        }
+}
+
+__global__ void advance_kernel(LState* st) {
+    if (threadIdx.x == 0) {
+        st->r += st->k;
+        st->m_a = st->m_new;
+        st->k = 0;
+    }
+}
+
+__global__ void init_state_kernel(LState* st, uint32_t m, uint32_t* rows_orig, uint32_t* U, size_t ucount) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) { st->m_a = m; st->r = 0; st->k = 0; st->m_new = m; st->status = 0; }
+    if (i < m) rows_orig[i] = (uint32_t)i;
+    for (size_t t = i; t < ucount; t += (size_t)gridDim.x * blockDim.x) U[t] = 0;
+}
+
+__global__ void zero_rows_kernel(uint32_t m, uint32_t n, uint32_t* R, int64_t ldr) {
+    const uint64_t total = (uint64_t)m * n;
+    for (uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (uint64_t)gridDim.x * blockDim.x)
+        R[(int64_t)(idx / n) * ldr + (idx % n)] = 0;
+}
+
+__global__ void place_rows_kernel(const uint32_t* N, int64_t ldn, const uint32_t* nonpiv, uint32_t nnp,
+                                  const uint32_t* piv, uint32_t r, uint32_t* R, int64_t ldr) {
+    const uint64_t total = (uint64_t)r * nnp;
+    for (uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t t = (uint32_t)(idx / nnp), j = (uint32_t)(idx % nnp);
+        R[(int64_t)t * ldr + nonpiv[j]] = N[(int64_t)t * ldn + j];
+    }
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < r; t += (uint64_t)gridDim.x * blockDim.x)
+        R[(int64_t)t * ldr + piv[t]] = 1;
+}
+
+__global__ void gather_cols_kernel(const uint32_t* U, int64_t ldu, uint32_t r, const uint32_t* cols, uint32_t nc,
+                                   uint32_t* out, int64_t ldo) {
+    const uint64_t total = (uint64_t)r * nc;
+    for (uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t t = (uint32_t)(idx / nc), j = (uint32_t)(idx % nc);
+        out[(int64_t)t * ldo + j] = U[(int64_t)t * ldu + cols[j]];
+    }
+}
+
+inline uint32_t gsz(uint64_t n) { return (uint32_t)std::min<uint64_t>((n + 255) / 256, 65536); }
+
+template <class Ar>
+void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const uint32_t* A, int64_t lda,
+               uint32_t* Rout, int64_t ldr, uint32_t* rank, uint32_t* pivots_h, uint32_t* pivot_rows_h, hipStream_t s) {
+    const uint32_t mn = std::min(m, n);
+    const uint32_t nchunk = (m + CH - 1) / CH;
+    // device buffers (one allocation)
+    std::vector<size_t> sizes;
+    auto add = [&](size_t words) { sizes.push_back((words + 63) / 64 * 64); };
+    add((size_t)m * n);           // 0 S0
+    add((size_t)m * n);           // 1 S1
+    add((size_t)mn * n);          // 2 U store
+    add((size_t)LW * n);          // 3 X copy
+    add(m);                       // 4 rows_orig a
+    add(m);                       // 5 rows_orig b
+    add(m);                       // 6 R list
+    add(m);                       // 7 flags
+    add((size_t)nchunk * LW);     // 8 cand a
+    add((size_t)nchunk * LW);     // 9 cand b
+    add(nchunk);                  // 10 cnt a
+    add(nchunk);                  // 11 cnt b
+    add(LW);                      // 12 P
+    add(LW);                      // 13 J
+    add(mn);                      // 14 pivots
+    add(mn);                      // 15 pivot rows
+    add(16);                      // 16 state
+    size_t total = 0;
+    std::vector<size_t> off;
+    for (size_t w : sizes) { off.push_back(total); total += w; }
+    uint32_t* base = nullptr;
+    HX_HIP(hipMalloc(&base, total * 4));
+    auto at = [&](int i) { return base + off[i]; };
+    uint32_t* Sb[2] = {at(0), at(1)};
+    uint32_t* U = at(2);
+    uint32_t* X = at(3);
+    uint32_t* ro[2] = {at(4), at(5)};
+    uint32_t *Rl = at(6), *flags = at(7);
+    uint32_t* cand[2] = {at(8), at(9)};
+    uint32_t* cnt[2] = {at(10), at(11)};
+    uint32_t *P = at(12), *J = at(13), *piv = at(14), *prow = at(15);
+    LState* st = (LState*)at(16);
+    uint32_t* work_extra = nullptr;
+    try {
+        HX_HIP(hipMemcpy2DAsync(Sb[0], (size_t)n * 4, A, (size_t)lda * 4, (size_t)n * 4, m, hipMemcpyDeviceToDevice, s));
+        init_state_kernel<<<gsz(std::max<size_t>(m, (size_t)mn * n)), 256, 0, s>>>(st, m, ro[0], U, (size_t)mn * n);
+        after_launch("init_state_kernel");
+        int cur = 0;
+        void* ws = scratch(limb_workspace_bytes(ctx, m, n, LW), 2);
+        const size_t ws_bytes = limb_workspace_bytes(ctx, m, n, LW);
+        for (uint32_t c = 0; c < n; c += LW) {
+            const uint32_t na = n - c, wc = std::min<uint32_t>(LW, na);
+            const uint32_t* S = Sb[cur];
+            // 1. tournament
+            rrp_chunk_kernel<Ar><<<nchunk, LT, 0, s>>>(S, n, st, wc, ar, cand[0], cnt[0]);
+            after_launch("rrp_chunk_kernel");
+            uint32_t nin = nchunk;
+            int lv = 0;
+            do {
+                const uint32_t nout = (nin + GRP - 1) / GRP;
+                const int fin = nout == 1;
+                rrp_merge_kernel<Ar><<<nout, LT, 0, s>>>(S, n, wc, ar, cand[lv & 1], cnt[lv & 1], nin,
+                                                         cand[(lv + 1) & 1], cnt[(lv + 1) & 1], fin, P, J, st);
+                after_launch("rrp_merge_kernel");
+                nin = nout;
+                ++lv;
+            } while (nin > 1);
+            // 2. X = S[P,J]^{-1} S[P,:] into the U store and the contiguous copy
+            panel_x_kernel<Ar><<<(na + 255) / 256, LT, 0, s>>>(S, n, na, P, J, st, ar, X, n, U, n, c, ro[cur], piv, prow);
+            after_launch("panel_x_kernel");
+            if (na > wc) {
+                // 3. the next Schur complement, compacted
+                rest_rows_kernel<<<1, 1024, 0, s>>>(st, P, flags, Rl, ro[cur], ro[cur ^ 1]);
+                after_launch("rest_rows_kernel");
+                GemmArgs g;
+                g.m = m;
+                g.n = na - wc;
+                g.k = LW;
+                g.A = S;
+                g.ars = n;
+                g.acs = 1;
+                g.a_rows = Rl;
+                g.a_cols = J;
+                g.B = X + wc;
+                g.brs = n;
+                g.bcs = 1;
+                g.Cin = S + wc;
+                g.cirs = n;
+                g.cics = 1;
+                g.c_rows = Rl;
+                g.Cout = Sb[cur ^ 1];
+                g.cors = n;
+                g.cocs = 1;
+                g.m_dev = &st->m_new;
+                g.k_dev = &st->k;
+                g.op = HX_OP_SUB;
+                gemm_dispatch(ctx, g, s, ctx.limbs ? HX_GEMM_LIMB : HX_GEMM_PLAIN, ws, ws_bytes, 0);
+                cur ^= 1;
+            }
+            advance_kernel<<<1, 64, 0, s>>>(st);
+            after_launch("advance_kernel");
+        }
+        LState h;
+        HX_HIP(hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, s));
+        HX_HIP(hipStreamSynchronize(s));
+        if (h.status != 0) throw Error{HX_E_NOT_INVERTIBLE, "a pivot had no inverse mod p (is p prime?)"};
+        const uint32_t r = h.r;
+        *rank = r;
+        std::vector<uint32_t> hp(std::max<uint32_t>(r, 1)), hr(std::max<uint32_t>(r, 1));
+        if (r) {
+            HX_HIP(hipMemcpyAsync(hp.data(), piv, (size_t)r * 4, hipMemcpyDeviceToHost, s));
+            HX_HIP(hipMemcpyAsync(hr.data(), prow, (size_t)r * 4, hipMemcpyDeviceToHost, s));
+            HX_HIP(hipStreamSynchronize(s));
+            if (pivots_h) std::copy(hp.begin(), hp.begin() + r, pivots_h);
+            if (pivot_rows_h) std::copy(hr.begin(), hr.begin() + r, pivot_rows_h);
+        }
+        if (Rout) {
+            // RREF = U_piv^{-1} U on the non-pivot columns (one unit-upper TRSM), zero rows below
+            std::vector<uint32_t> nonpiv;
+            std::vector<char> isp(n, 0);
+            for (uint32_t t = 0; t < r; ++t) isp[hp[t]] = 1;
+            for (uint32_t j = 0; j < n; ++j)
+                if (!isp[j]) nonpiv.push_back(j);
+            const uint32_t nnp = (uint32_t)nonpiv.size();
+            const size_t extra = (size_t)r * r + (size_t)r * std::max<uint32_t>(nnp, 1) + nnp + r + 64;
+            HX_HIP(hipMalloc(&work_extra, extra * 4));
+            uint32_t* T = work_extra;
+            uint32_t* N = T + (size_t)r * r;
+            uint32_t* d_np = N + (size_t)r * std::max<uint32_t>(nnp, 1);
+            uint32_t* d_pv = d_np + nnp;
+            if (nnp) HX_HIP(hipMemcpyAsync(d_np, nonpiv.data(), (size_t)nnp * 4, hipMemcpyHostToDevice, s));
+            if (r) HX_HIP(hipMemcpyAsync(d_pv, hp.data(), (size_t)r * 4, hipMemcpyHostToDevice, s));
+            zero_rows_kernel<<<gsz((uint64_t)m * n), 256, 0, s>>>(m, n, Rout, ldr);
+            after_launch("zero_rows_kernel");
+            if (r && nnp) {
+                gather_cols_kernel<<<gsz((uint64_t)r * r), 256, 0, s>>>(U, n, r, d_pv, r, T, r);
+                after_launch("gather_cols_kernel(T)");
+                gather_cols_kernel<<<gsz((uint64_t)r * nnp), 256, 0, s>>>(U, n, r, d_np, nnp, N, nnp);
+                after_launch("gather_cols_kernel(N)");
+                trsm_unit(ctx, hx_view{T, (int64_t)r, 1, r, r}, hx_view{N, (int64_t)nnp, 1, r, nnp}, 1, s);
+            }
+            if (r) {
+                place_rows_kernel<<<gsz((uint64_t)r * std::max<uint32_t>(nnp, 1)), 256, 0, s>>>(N, nnp, d_np, nnp, d_pv, r,
+                                                                                                Rout, ldr);
+                after_launch("place_rows_kernel");
+            }
+            HX_HIP(hipStreamSynchronize(s));
+        }
+    } catch (...) {
+        (void)hipStreamSynchronize(s);
+        (void)hipFree(base);
+        (void)hipFree(work_extra);
+        throw;
+    }
+    HX_HIP(hipFree(base));
+    if (work_extra) HX_HIP(hipFree(work_extra));
+}
+
+}  // namespace
+
+// The whole-GPU path takes matrices with at least 1024 rows and m * n >= 2^21; smaller ones run
+// in one workgroup (hx_rref_batch.hip).
+bool rref_large(const hx_modctx& ctx, uint32_t m, uint32_t n, const uint32_t* A, int64_t lda, uint32_t* R,
+                int64_t ldr, uint32_t* rank, uint32_t* pivots, uint32_t* pivot_rows, hipStream_t s) {
+    if (m < 1024 || (uint64_t)m * n < (1ull << 21)) return false;
+    if (ctx.p & 1u)
+        run_large(ctx, make_mont(ctx.p), m, n, A, lda, R, ldr, rank, pivots, pivot_rows, s);
+    else
+        run_large(ctx, make_barrett(ctx.p), m, n, A, lda, R, ldr, rank, pivots, pivot_rows, s);
+    return true;
+}
+
 }  // namespace hx

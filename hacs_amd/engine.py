@@ -288,3 +288,40 @@ class HowellBatch:
                 self.h = None
         except Exception:
             pass
+
+
+class GemmBatch:
+    """Uneven GEMM jobs in one launch per kernel variant (generalises GemmBatch<Elt>,
+    Cuda/CudaKernels.h:135-185, which needs one shared shape).  jobs: (A, B, C, op) row-major
+    device tensors, or (A, B, C, op, variant).  Created once (job table and limb workspace
+    uploaded: not capturable); run() is capturable."""
+
+    def __init__(self, ctx: ModCtx, jobs, variant: int = HX_GEMM_AUTO):
+        self.ctx = ctx
+        self.jobs = jobs
+        arr = (_lib.hx_gemm_job * max(1, len(jobs)))()
+        for i, jb in enumerate(jobs):
+            A, B, C_, op = jb[:4]
+            for t in (A, B, C_):
+                _u32_tensor(t)
+                if t.stride(1) != 1:
+                    raise ValueError("batched GEMM operands must be row-contiguous")
+            arr[i].m, arr[i].k = A.shape
+            arr[i].n = B.shape[1]
+            arr[i].op = op
+            arr[i].variant = jb[4] if len(jb) > 4 else HX_GEMM_AUTO
+            arr[i].A, arr[i].B, arr[i].C = C.c_void_p(A.data_ptr()), C.c_void_p(B.data_ptr()), C.c_void_p(C_.data_ptr())
+            arr[i].lda, arr[i].ldb, arr[i].ldc = A.stride(0), B.stride(0), C_.stride(0)
+        self.h = C.c_void_p()
+        check(lib().hx_gemm_batch_create(C.byref(ctx.c), arr, len(jobs), variant, C.byref(self.h)))
+
+    def run(self, stream=None) -> None:
+        check(lib().hx_gemm_batch_run(self.h, current_stream() if stream is None else stream))
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().hx_gemm_batch_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass

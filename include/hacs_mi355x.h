@@ -81,6 +81,8 @@ typedef struct hx_gemm_opts {   /* GemmOpts, CudaKernels.h:64-79 */
     int tile;               /* 0 = automatic; a variant-specific tile index (tuning only) */
     void *workspace;        /* limb route: >= hx_gemm_workspace_bytes(); NULL = library-owned (not capturable) */
     size_t workspace_bytes;
+    hx_event main_begin;    /* optional timing events recorded on the stream right before / after the */
+    hx_event main_end;      /* main product kernel (instrumentation only; NULL = none) */
 } hx_gemm_opts;
 
 typedef struct hx_device_info {   /* CudaCtx.h:55-64 DeviceInfo */
@@ -143,13 +145,17 @@ HX_API int hx_schur_update_u32(const hx_modctx *ctx, hx_view L, hx_view U, hx_vi
 typedef struct hx_gemm_job {
     uint32_t m, n, k;
     int32_t op;                 /* HX_OP_* */
+    int32_t variant;            /* HX_GEMM_*: AUTO = the batch's variant */
+    int32_t reserved;
     const uint32_t *A;          /* row-major, leading dimension lda (device) */
     const uint32_t *B;
     uint32_t *C;
     int64_t lda, ldb, ldc;
 } hx_gemm_job;
 typedef void *hx_gemm_batch;
-/* set_shape / push / upload (NOT capturable): the job table goes to device memory. */
+/* set_shape / push / upload (NOT capturable): the job table goes to device memory, with the limb
+ * planes' workspace.  variant: HX_GEMM_PLAIN / HX_GEMM_LIMB for every job, or AUTO (per-job policy);
+ * a job's own variant field overrides it.  Jobs of the limb route need k <= one limb chunk. */
 HX_API int hx_gemm_batch_create(const hx_modctx *ctx, const hx_gemm_job *jobs, uint32_t njobs, int variant,
                                 hx_gemm_batch *out);
 /* run (capturable once created): every job in one launch. */

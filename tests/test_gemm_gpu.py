@@ -113,3 +113,36 @@ def test_config1_gemm_1024_mod_65521(hx_gpu, O):
         got = E.from_device(dC)
         assert np.array_equal(got, want), f"variant {v}"
         assert O.fnv_matrix(got) == O.fnv_matrix(want)
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_gemm_batch_uneven(hx_gpu, O, variant):
+    """Uneven shapes, mixed ops and per-job variants in one batch, each job against the oracle."""
+    from hacs_amd import engine as E
+    p = 2147483629
+    ctx = E.ModCtx(p)
+    rng = np.random.default_rng(40 + variant)
+    host, jobs = [], []
+    for t in range(37):
+        m, n, k = (int(x) for x in rng.integers(1, 300, 3))
+        if t % 9 == 0:
+            k = 0
+        A = rng.integers(0, p, (m, k), dtype=np.uint64).astype(np.uint32)
+        B = rng.integers(0, p, (k, n), dtype=np.uint64).astype(np.uint32)
+        C0 = rng.integers(0, p, (m, n), dtype=np.uint64).astype(np.uint32)
+        op = t % 3
+        jv = (0, 1, 2)[t % 3] if variant == 0 else 0
+        host.append((A, B, C0, op))
+        jobs.append((E.to_device(A), E.to_device(B), E.to_device(C0), op, jv))
+    b = E.GemmBatch(ctx, jobs, variant)
+    for rep in range(2):     # run twice: the second run must not depend on the first
+        for (A, B, C0, op), jb in zip(host, jobs):
+            jb[2].copy_(E.to_device(C0))
+        b.run()
+        torch.cuda.synchronize()
+        for (A, B, C0, op), jb in zip(host, jobs):
+            if A.shape[1]:
+                want = O.gemm_fast(p, A, B, C0, op)
+            else:
+                want = np.zeros_like(C0) if op == 0 else C0
+            assert np.array_equal(E.from_device(jb[2]), want), f"rep {rep} {A.shape} x {B.shape} op {op}"

@@ -1,0 +1,59 @@
+"""CPU-only checks: the C-ABI library loads and exports every symbol the header declares (no compute
+call without a GPU), the modulus context matches ModCtx.h's constants, and the synthetic workload
+generators agree with the oracle's restatement of the reference generators."""
+import ctypes as C
+
+import numpy as np
+
+from hacs_amd import _lib
+
+
+def test_library_exports_every_header_symbol():
+    L = _lib.lib()
+    syms = _lib.header_symbols()
+    assert len(syms) >= 40
+    for s in syms:
+        assert hasattr(L, s), s
+    assert set(syms) == set(_lib.SIGNATURES), set(syms) ^ set(_lib.SIGNATURES)
+
+
+def test_modctx_constants(O):
+    L = _lib.lib()
+    for p in [2, 3, 127, 251, 255, 256, 65279, 65280, 65521, 1000003, 16711423, 16711424, 2147483629,
+              2147483647, 4278124287, 4278124288, 4294967291]:
+        c = _lib.hx_modctx()
+        _lib.check(L.hx_modctx_init(p, C.byref(c)))
+        assert c.p == p and c.barrett_m == O.barrett_m(p) and c.k_max == O.k_max(p)
+        assert c.two64_mod == (1 << 64) % p
+        half = (p - 1) // 2
+        want = next((l for l in (1, 2, 3, 4) if half <= 127 * (256 ** l - 1) // 255), 0)
+        assert c.limbs == want, (p, c.limbs, want)
+    c = _lib.hx_modctx()
+    assert L.hx_modctx_init(1, C.byref(c)) == _lib.HX_E_ARGUMENT
+
+
+def test_library_reports_no_device_without_gpu():
+    L = _lib.lib()
+    if L.hx_available():
+        return
+    assert L.hx_unavailable_reason()
+    c = _lib.hx_device_info()
+    assert L.hx_device_info_get(C.byref(c)) == _lib.HX_E_UNAVAILABLE
+
+
+def test_workload_generators_match_oracle(O):
+    from hacs_amd import workloads as W
+    for (r, c, ell, salt) in [(64, 48, 65521, 1), (8192, 8192, 2147483629, 2), (4096, 0, 1000003, 0x41)]:
+        assert W.seed_for(r, c, ell, salt) == O.seed_for(r, c, ell, salt)
+    seed = W.seed_for(100, 30, 2147483629, 1)
+    assert np.array_equal(W.random_entries(3000, 2147483629, seed), O.random_entries(3000, 2147483629, seed))
+    assert np.array_equal(W.engine_matrix(64, 48, 65521, 1), O.engine_matrix(64, 48, 65521, 1))
+    p = 2147483647
+    A = O.ranked_matrix(96, 80, 30, p)
+    assert np.array_equal(W.overwrite_rows(A, p, 8), O.overwrite_rows(A, p, 8))
+    sh = W.c4_shapes(4096)
+    assert sh.min() >= 16 and sh.max() <= 512 and abs(np.log(sh).mean() - np.log(16 * 32 ** 0.5)) < 0.05
+    J = W.sparse_job(3, 40, 50, W.P_C4)
+    nz = J[J != 0]
+    assert set(np.unique(nz)) <= {1, 2, W.P_C4 - 1, W.P_C4 - 2}
+    assert 0.02 < (J != 0).mean() < 0.09

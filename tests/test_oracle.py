@@ -1,0 +1,150 @@
+"""The oracle, pinned independently of its own formulas: published test vectors, closed forms,
+exact integer arithmetic, uniqueness properties, brute force on tiny inputs, the compiled
+reference (oracle/_ref) where it was built, and negative controls that plant a mistake in the
+oracle's inputs or rule and must be caught."""
+import itertools
+
+import numpy as np
+import pytest
+
+from tests.matrix_kinds import KINDS, make
+
+
+def test_splitmix_and_fnv_vectors(O):
+    # SplitMix64 seeded 0: the published first outputs
+    assert O.splitmix64_stream(0, 3) == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+    # FNV-1a 64 of the empty input is the offset basis; of one zero byte is basis ^ 0 * prime
+    M = np.zeros((0, 0), np.uint32)
+    h0 = O.fnv_matrix(M)
+    basis, prime = 0xCBF29CE484222325, 0x100000001B3
+    want = basis
+    for _ in range(16):          # rows = 0 and cols = 0 fed as 8 bytes each
+        want = ((want ^ 0) * prime) & ((1 << 64) - 1)
+    assert h0 == want
+
+
+def test_modular_primitives(O):
+    for p in [2, 3, 65521, 1000003, 2147483629, 2147483647, 4294967291]:
+        assert O.barrett_m(p) == (1 << 64) // p
+        k = O.k_max(p)
+        assert k * (p - 1) ** 2 <= (1 << 64) - 1 < (k + 1) * (p - 1) ** 2     # tightness
+        for x in [0, 1, p - 1, p, p * p - 1, (1 << 64) - 1, (1 << 63) + 12345]:
+            assert O.reduce_u64(p, x) == x % p
+        rng = np.random.default_rng(p)
+        for a in rng.integers(1, p, 20, dtype=np.uint64):
+            assert (int(a) * O.inv(int(a), p)) % p == 1
+    assert O.inv(3, 15) == 0 and O.inv(0, 7) == 0
+
+
+@pytest.mark.parametrize("p", [2, 251, 65521, 1000003, 2147483629, 4294967291])
+def test_gemm_exact(O, p):
+    rng = np.random.default_rng(p)
+    for (m, n, k) in [(1, 1, 1), (7, 5, 300), (33, 17, 1025)]:
+        A = rng.integers(0, p, (m, k), dtype=np.uint64).astype(np.uint32)
+        B = rng.integers(0, p, (k, n), dtype=np.uint64).astype(np.uint32)
+        Cc = rng.integers(0, p, (m, n), dtype=np.uint64).astype(np.uint32)
+        exact = A.astype(object).dot(B.astype(object))
+        for op in (0, 1, 2):
+            base = 0 if op == 0 else Cc.astype(object)
+            want = ((base + exact) if op < 2 else (base - exact)) % p
+            want = np.asarray(want, dtype=object).astype(np.uint64).astype(np.uint32)
+            assert np.array_equal(O.gemm_naive(p, A, B, Cc, op), want)
+            assert np.array_equal(O.gemm_fast(p, A, B, Cc, op, 3), want)
+    # rank-1 x all-(p-1): closed form sum_k a_i b_k (p-1) = -a_i sum_k b_k
+    u = rng.integers(0, p, (9, 1), dtype=np.uint64)
+    v = rng.integers(0, p, (1, 700), dtype=np.uint64)
+    A = ((u * v) % p).astype(np.uint32)
+    B = np.full((700, 4), p - 1, np.uint32)
+    row = [(-(int(u[i, 0]) * sum(int(x) for x in v[0])) % p) for i in range(9)]
+    assert np.array_equal(O.gemm_fast(p, A, B), np.array([[r] * 4 for r in row], np.uint32))
+
+
+def test_ranked_matrix_rank(O):
+    p = 2147483647
+    A = O.ranked_matrix(80, 70, 25, p)
+    assert O.rref_naive(p, A)[0] == 25
+    B = O.overwrite_rows(A, p, 10)
+    assert O.rref_naive(p, B)[0] == 25          # rows replaced by combinations of other rows
+
+
+@pytest.mark.parametrize("p", [2, 3, 13, 65521, 2147483647])
+def test_rref_oracles_agree_and_rules(O, p):
+    rng = np.random.default_rng(p + 1)
+    for kind in KINDS:
+        for (m, n) in [(1, 1), (12, 9), (9, 12), (40, 40)]:
+            A = make(rng, m, n, p, kind, rank=max(1, min(m, n) // 2))
+            r, piv, R = O.rref_naive(p, A)
+            r2, piv2, prow, R2 = O.rref_rule(p, A, threads=3)
+            assert r == r2 and np.array_equal(piv, piv2) and np.array_equal(R[:r], R2) and not R[r:].any()
+            # the pivot rows are the row rank profile: the pivot columns of RREF(A^T)
+            assert np.array_equal(np.sort(prow), O.rref_naive(p, A.T.copy())[1])
+            # uniqueness: a row permutation changes neither the RREF nor its pivots
+            perm = rng.permutation(m)
+            r3, piv3, R3 = O.rref_naive(p, A[perm])
+            assert r3 == r and np.array_equal(piv3, piv) and np.array_equal(R3, R)
+
+
+def test_rref_rule_negative_control(O):
+    """A planted order where the largest-row rule would disagree: pivot_rows must be [0, 2], not [1, 2]
+    (rows 0 and 1 are equal; the smallest original row wins)."""
+    p = 13
+    A = np.array([[1, 2, 3], [1, 2, 3], [0, 1, 5]], np.uint32)
+    r, piv, prow, _ = O.rref_rule(p, A)
+    assert r == 2 and piv.tolist() == [0, 1] and prow.tolist() == [0, 2]
+
+
+def test_rref_brute_force_rank_tiny(O):
+    """Rank over F_3 of every 2x3 matrix by counting the row space's size."""
+    p = 3
+    for vals in itertools.product(range(3), repeat=6):
+        A = np.array(vals, np.uint32).reshape(2, 3)
+        span = {tuple((a * A[0] + b * A[1]) % 3) for a in range(3) for b in range(3)}
+        assert O.rref_naive(p, A)[0] == {1: 0, 3: 1, 9: 2}[len(span)]
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 7, 16, 31])
+def test_howell_oracles(O, k):
+    rng = np.random.default_rng(k)
+    for (m, n) in [(1, 1), (4, 6), (7, 3), (20, 20), (30, 45)]:
+        A = rng.integers(0, 1 << k, (m, n), dtype=np.uint64).astype(np.uint32)
+        A2 = ((A.astype(np.uint64) << rng.integers(0, k, (m, 1)).astype(np.uint64)) % (1 << k)).astype(np.uint32)
+        for X in (A, A2):
+            H1, p1, v1 = O.howell_naive(k, X)
+            H2, p2, v2 = O.howell_fast(k, X)
+            assert np.array_equal(H1, H2) and np.array_equal(p1, p2) and np.array_equal(v1, v2)
+            # uniqueness: row permutations and appended zero rows give the same Howell form
+            perm = rng.permutation(m)
+            Y = np.vstack([X[perm], np.zeros((2, n), np.uint32)])
+            H3, p3, v3 = O.howell_fast(k, Y)
+            assert np.array_equal(H1, H3) and np.array_equal(p1, p3) and np.array_equal(v1, v3)
+    if k == 2:
+        H, piv, pv = O.howell_fast(2, np.array([[2, 1]], np.uint32))
+        assert H.tolist() == [[2, 1], [0, 2]] and piv.tolist() == [0, 1] and pv.tolist() == [1, 1]
+
+
+def test_howell_k1_equals_rref(O):
+    rng = np.random.default_rng(9)
+    A = rng.integers(0, 2, (25, 30), dtype=np.uint64).astype(np.uint32)
+    H, piv, pv = O.howell_fast(1, A)
+    r, piv2, R = O.rref_naive(2, A)
+    assert np.array_equal(H, R[:r]) and np.array_equal(piv, piv2) and not pv.any()
+
+
+def test_mlp_oracle_finite_differences():
+    from oracle import mlp_oracle as MO
+    rng = np.random.default_rng(2)
+    X = rng.normal(size=(16, 5))
+    W1, b1 = rng.normal(size=(7, 5)), rng.normal(size=7)
+    W2, b2 = rng.normal(size=(3, 7)), rng.normal(size=3)
+    T = rng.normal(size=(16, 3))
+    mask = (rng.random((16, 3)) < 0.6).astype(float)
+    _, dW1, db1, dW2, db2 = MO.backward(X, W1, b1, W2, b2, T, mask)
+    eps = 1e-6
+    for P, G in ((W1, dW1), (b1, db1), (W2, dW2), (b2, db2)):
+        for idx in [tuple(rng.integers(0, s) for s in P.shape) for _ in range(5)]:
+            P[idx] += eps
+            lp = MO.loss(X, W1, b1, W2, b2, T, mask)
+            P[idx] -= 2 * eps
+            lm = MO.loss(X, W1, b1, W2, b2, T, mask)
+            P[idx] += eps
+            assert abs((lp - lm) / (2 * eps) - G[idx]) < 1e-5 * max(1.0, abs(G[idx]))
