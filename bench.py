@@ -41,6 +41,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="do not time the CPU baseline")
     ap.add_argument("--no-batch", action="store_true", help="do not time the batched echelon workload")
     ap.add_argument("--batch-jobs", type=int, default=4096)
+    ap.add_argument("--no-lu", action="store_true", help="do not time the 4096^2 rank-3000 RREF")
+    ap.add_argument("--pmc-mode", action="store_true",
+                    help="for counter collection: one 1 GiB device copy (the traffic calibration), then one limb "
+                         "GEMM of the headline shape; no timing, no JSON")
     return ap.parse_args()
 
 
@@ -145,6 +149,28 @@ def batched_echelon(E, njobs):
     }
 
 
+def lu_c2(E, reps=3):
+    """C2: RREF of the 4096 x 4096 rank-3000 matrix mod 2^31-1 (rank, pivots, pivot_rows, RREF)."""
+    import torch
+    from hacs_amd import workloads as W
+    p = W.P_C2
+    ctx = E.ModCtx(p)
+    A = W.c2_device()
+    R = torch.empty_like(A)
+    res = E.rref(ctx, A, R)      # warm-up (also the parity-checked call in the tests)
+    torch.cuda.synchronize()
+    best = 1e30
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        res = E.rref(ctx, A, R)
+        torch.cuda.synchronize()
+        best = min(best, time.perf_counter() - t0)
+    m, n = A.shape
+    work = float((m - 1) * (res.rank * n - res.rank * (res.rank - 1) // 2))
+    return {"seconds": best, "rank": res.rank, "elim_work_macs": work, "elim_macs_per_s": work / best,
+            "shape": [m, n], "modulus": p}
+
+
 def cpu_baseline(p, threads):
     """The reference's own CPU GEMM (oracle/_ref, Kernels.cpp:1417) on a bounded sample, min of 3."""
     from oracle import oracle as O
@@ -177,6 +203,18 @@ def main():
         raise SystemExit("no gfx950 device: " + hacs_amd.lib().hx_unavailable_reason().decode())
     n, p = args.n, W.P_C3
     ctx = E.ModCtx(p)
+    if args.pmc_mode:
+        src = torch.ones(1 << 28, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)                  # 1 GiB read + 1 GiB written
+        torch.cuda.synchronize()
+        A = E.random_entries(n, n, p, W.seed_for(n, n, p, 0x01))
+        B = E.random_entries(n, n, p, W.seed_for(n, n, p, 0x02))
+        Cm = torch.empty((n, n), dtype=torch.int32, device="cuda")
+        E.gemm(ctx, A, B, Cm, E.GemmOp.Set, variant=E.HX_GEMM_LIMB)
+        torch.cuda.synchronize()
+        log("pmc mode done")
+        return
     log(f"device {E.device_info()}; C3 n={n} p={p}")
     A = E.random_entries(n, n, p, W.seed_for(n, n, p, 0x01))
     B = E.random_entries(n, n, p, W.seed_for(n, n, p, 0x02))
@@ -224,6 +262,10 @@ def main():
         b = batched_echelon(E, args.batch_jobs)
         log(f"batched echelon: {b}")
         res["batched_lu"] = b
+    if not args.no_lu:
+        lu = lu_c2(E)
+        log(f"C2 rref: {lu}")
+        res["lu_c2"] = lu
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         res["cpu_baseline"] = cpu_baseline(p, threads)

@@ -117,3 +117,19 @@ def c5_stream(njobs: int = 20000, salt: int = 0x51):
         n = int(_log_uniform(d[2 * j + 1:2 * j + 2], lo, hi)[0])
         out.append((("gemm", "echelon", "howell")[k], n))
     return out
+
+
+def c2_device(m: int = 4096, n: int = 4096, rank: int = 3000, p: int = P_C2, nover: int = 64):
+    """The C2 matrix on the device: RankedMatrix's factors (salts 0x21 / 0x22) generated in HBM, their
+    product by the library's GEMM (a unique residue, so any exact kernel gives these bytes), then the
+    nover row overwrites (salt 0x31) applied on the host."""
+    import torch
+    from . import engine as E
+    U = E.random_entries(m, rank, p, seed_for(m, rank, p, 0x21))
+    V = E.random_entries(rank, n, p, seed_for(rank, n, p, 0x22))
+    A = torch.empty((m, n), dtype=torch.int32, device="cuda")
+    E.gemm(E.ModCtx(p), U, V, A, E.GemmOp.Set)
+    torch.cuda.synchronize()
+    if nover:
+        A = E.to_device(overwrite_rows(E.from_device(A), p, nover))
+    return A

@@ -1,0 +1,72 @@
+"""The whole-GPU RREF path (tournament pivot search, limb-MFMA trailing updates, TRSM back
+substitution) against the oracle, and the full 4096 x 4096 rank-3000 configuration mod 2^31-1
+against the fixture written by the reference's own elim::rref (tests/golden/c2_reference.npz)."""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from tests.matrix_kinds import make
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c2_reference.npz")
+
+
+@pytest.mark.parametrize("p,m,n,kind,rank", [
+    (2147483647, 2048, 1536, "planted", 1100),
+    (1000003, 1100, 2048, "planted", 900),
+    (65521, 1536, 1536, "random", 0),
+    (3, 1280, 1700, "random", 0),
+    (2, 1024, 2100, "random", 0),
+    (2147483629, 1300, 1700, "duprows", 0),
+    (13, 1024, 2048, "boundaries", 0),
+])
+def test_rref_large_matches_oracle(hx_gpu, O, p, m, n, kind, rank):
+    from hacs_amd import engine as E
+    rng = np.random.default_rng(m + n + p)
+    A = make(rng, m, n, p, kind, rank=rank)
+    R = torch.zeros((m, n), dtype=torch.int32, device="cuda")
+    res = E.rref(E.ModCtx(p), E.to_device(A), R)
+    torch.cuda.synchronize()
+    r, piv, prow, want = O.rref_rule(p, A, threads=16)
+    assert res.rank == r, (res.rank, r)
+    assert np.array_equal(res.pivots, piv)
+    assert np.array_equal(res.pivot_rows, prow)
+    got = E.from_device(R)
+    assert np.array_equal(got[:r], want) and not got[r:].any()
+
+
+def test_rref_c2_full_size(hx_gpu, O):
+    from hacs_amd import engine as E
+    from hacs_amd import workloads as W
+    G = np.load(GOLD, allow_pickle=False)
+    p = int(G["p"][0])
+    A = W.c2_device()
+    assert O.fnv_matrix(E.from_device(A)) == int(G["input_fnv"][0]), "generator differs from the fixture's input"
+    R = torch.empty_like(A)
+    res = E.rref(E.ModCtx(p), A, R)
+    torch.cuda.synchronize()
+    assert res.rank == int(G["rank"][0]) == 3000
+    assert np.array_equal(res.pivots, G["pivots"])
+    assert np.array_equal(res.pivot_rows, G["pivot_rows"])
+    Rh = E.from_device(R)
+    assert O.fnv_matrix(Rh[:res.rank]) == int(G["rref_fnv"][0])
+    assert not Rh[res.rank:].any()
+
+
+def test_trsm_unit_matches_oracle(hx_gpu, O):
+    from hacs_amd import engine as E
+    p = 2147483629
+    ctx = E.ModCtx(p)
+    rng = np.random.default_rng(8)
+    for n, q, upper in [(70, 33, True), (70, 33, False), (300, 129, True), (257, 64, False)]:
+        T = rng.integers(0, p, (n, n), dtype=np.uint64).astype(np.uint32)
+        B = rng.integers(0, p, (n, q), dtype=np.uint64).astype(np.uint32)
+        dB = E.to_device(B)
+        E.trsm_unit(ctx, E.to_device(T), dB, upper)
+        torch.cuda.synchronize()
+        X = E.from_device(dB)
+        Tu = np.triu(T, 1) if upper else np.tril(T, -1)
+        Tu = Tu + np.eye(n, dtype=np.uint32)
+        assert np.array_equal(O.gemm_fast(p, Tu, X), B), f"T X != B for n={n} upper={upper}"
