@@ -63,13 +63,20 @@ class hx_gemm_job(C.Structure):
 
 class hx_rref_job(C.Structure):
     _fields_ = [("m", C.c_uint32), ("n", C.c_uint32), ("A", C.c_void_p), ("lda", C.c_int64),
-                ("pivots", C.c_void_p), ("pivot_rows", C.c_void_p), ("rank", C.c_void_p)]
+                ("pivots", C.c_void_p), ("pivot_rows", C.c_void_p), ("rank", C.c_void_p), ("panel", C.c_uint32)]
 
 
 class hx_howell_job(C.Structure):
     _fields_ = [("m", C.c_uint32), ("n", C.c_uint32), ("A", C.c_void_p), ("lda", C.c_int64),
                 ("H", C.c_void_p), ("pivots", C.c_void_p), ("pivot_v", C.c_void_p), ("h", C.c_void_p),
                 ("work", C.c_void_p)]
+
+
+class hx_ring2k_step_args(C.Structure):
+    _fields_ = [("m", C.c_uint32), ("r", C.c_uint32), ("np", C.c_uint32), ("pc0", C.c_uint32), ("pc1", C.c_uint32),
+                ("panel", C.c_void_p), ("ldp", C.c_size_t), ("swaps", C.c_void_p), ("nswaps", C.c_uint32),
+                ("L", C.c_void_p), ("ldl", C.c_size_t), ("u0", C.c_uint32), ("la0", C.c_uint32),
+                ("la1", C.c_uint32), ("w", C.c_uint32)]
 
 
 _V = C.c_void_p
@@ -125,6 +132,15 @@ SIGNATURES = {
     "hx_mlp_backward": (_INT, [_V, _U32, _U32, _V, _U32, _V, _U32, _V, _V, _V, _V, _V, _V, _V, _V, _V, _V]),
     "hx_adam_step": (_INT, [_V, _V, _V, _V, _U32, _F, _F, _F, _F, _U32, _V]),
     "hx_random_entries": (_INT, [_V, _U32, _U32, _I64, _U32, _U64, _V]),
+    "hx_ring2k_session_bytes": (_SZ, [_U32, _U32, _U32, _U32]),
+    "hx_ring2k_staging_bytes": (_SZ, [_U32, _U32, _U32]),
+    "hx_ring2k_open": (_INT, [_U32, _U32, _U32, _U32, C.POINTER(_V)]),
+    "hx_ring2k_put": (_INT, [_V, _V, _SZ, _U32, _U32, _U32, _U32]),
+    "hx_ring2k_get": (_INT, [_V, _V, _SZ, _U32, _U32, _U32, _U32]),
+    "hx_ring2k_step": (_INT, [_V, C.POINTER(hx_ring2k_step_args)]),
+    "hx_ring2k_fetch": (_INT, [_V, _V, _SZ]),
+    "hx_ring2k_close": (_INT, [_V]),
+    "hx_ring2k_gemm_sub_host": (_INT, [_U32, _V, _SZ, _V, _SZ, _V, _SZ, _U32, _U32, _U32]),
 }
 
 _lib = None

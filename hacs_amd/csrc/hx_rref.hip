@@ -39,7 +39,7 @@ void rref_via_batch(const hx_modctx& ctx, uint32_t m, uint32_t n, const uint32_t
         uint32_t* d_piv = dst == work ? work + (size_t)m * n : work;
         uint32_t* d_prow = d_piv + std::max<uint32_t>(mn, 1);
         uint32_t* d_rank = d_prow + std::max<uint32_t>(mn, 1);
-        hx_rref_job job{m, n, dst, ldd, d_piv, d_prow, d_rank};
+        hx_rref_job job{m, n, dst, ldd, d_piv, d_prow, d_rank, 0};
         rref_batch_create_impl(ctx, &job, 1, &b);
         rref_batch_run_impl(b, s);
         rref_batch_status_impl(b, s);

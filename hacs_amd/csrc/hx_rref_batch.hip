@@ -43,6 +43,7 @@ struct RJob {
     uint32_t* pivot_rows;
     uint32_t* rank;
     uint32_t* U;          // min(m, n) x n, row stride n
+    uint32_t panel;       // widest panel for this job (a tuning choice)
 };
 
 struct RArgs {
@@ -78,7 +79,7 @@ __device__ void rref_one(const RJob& job, const Ar& ar, const RArgs& a, uint32_t
         if (tid == 0) *job.rank = 0;
         return;
     }
-    uint32_t w = RW;
+    uint32_t w = job.panel ? min((uint32_t)RW, job.panel) : (uint32_t)RW;
     while (w > 1 && ((uint64_t)m * (w | 1) > a.region || (uint64_t)w * n > a.region)) w >>= 1;
     for (uint32_t i = tid; i < m; i += RT) list[i] = i;
     for (uint32_t j = tid; j < n; j += RT) ispiv[j] = 0;
@@ -347,7 +348,8 @@ int rref_batch_create_impl(const hx_modctx& ctx, const hx_rref_job* jobs, uint32
             HX_CHECK(j.lda >= (int64_t)j.n, "rref job: lda < n");
             HX_CHECK(j.m <= 4096, "rref job: m > 4096 (use hx_rref_u32)");
             HX_CHECK(j.rank != nullptr, "rref job: rank is NULL");
-            hj[i] = RJob{j.m, j.n, j.A, j.lda, j.pivots, j.pivot_rows, j.rank, nullptr};
+            HX_CHECK(j.panel == 0 || j.panel == 8 || j.panel == 16 || j.panel == 32, "rref job: panel must be 0, 8, 16 or 32");
+            hj[i] = RJob{j.m, j.n, j.A, j.lda, j.pivots, j.pivot_rows, j.rank, nullptr, j.panel};
             ucount += (size_t)std::min(j.m, j.n) * j.n;
             mmax = std::max(mmax, j.m);
             nmax = std::max(nmax, j.n);

@@ -177,7 +177,7 @@ class RrefBatch:
 
     The job table is uploaded once (not capturable); run() is capturable."""
 
-    def __init__(self, ctx: ModCtx, mats: list[torch.Tensor]):
+    def __init__(self, ctx: ModCtx, mats: list[torch.Tensor], panels: list[int] | None = None):
         self.ctx = ctx
         self.mats = mats
         n = len(mats)
@@ -197,6 +197,7 @@ class RrefBatch:
             jobs[i].pivots = C.c_void_p(base + 4 * off)
             jobs[i].pivot_rows = C.c_void_p(base + 4 * (off + mn))
             jobs[i].rank = C.c_void_p(base + 4 * (off + 2 * mn))
+            jobs[i].panel = 0 if panels is None else panels[i]
             self.offsets.append((off, mn))
             off += 2 * mn + 1
         self.h = C.c_void_p()
@@ -304,14 +305,15 @@ class GemmBatch:
             A, B, C_, op = jb[:4]
             for t in (A, B, C_):
                 _u32_tensor(t)
-                if t.stride(1) != 1:
+                if t.numel() and t.shape[1] > 1 and t.stride(1) != 1:
                     raise ValueError("batched GEMM operands must be row-contiguous")
             arr[i].m, arr[i].k = A.shape
             arr[i].n = B.shape[1]
             arr[i].op = op
             arr[i].variant = jb[4] if len(jb) > 4 else HX_GEMM_AUTO
             arr[i].A, arr[i].B, arr[i].C = C.c_void_p(A.data_ptr()), C.c_void_p(B.data_ptr()), C.c_void_p(C_.data_ptr())
-            arr[i].lda, arr[i].ldb, arr[i].ldc = A.stride(0), B.stride(0), C_.stride(0)
+            ld = [t.stride(0) if t.numel() and t.shape[0] > 1 else t.shape[1] for t in (A, B, C_)]
+            arr[i].lda, arr[i].ldb, arr[i].ldc = ld
         self.h = C.c_void_p()
         check(lib().hx_gemm_batch_create(C.byref(ctx.c), arr, len(jobs), variant, C.byref(self.h)))
 

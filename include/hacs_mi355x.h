@@ -192,6 +192,7 @@ typedef struct hx_rref_job {
     uint32_t *pivots;       /* device, min(m, n) entries */
     uint32_t *pivot_rows;   /* device, min(m, n) entries */
     uint32_t *rank;         /* device, one entry */
+    uint32_t panel;         /* tuning: widest column panel (8, 16 or 32; 0 = 32); never changes a value */
 } hx_rref_job;
 typedef void *hx_rref_batch;
 HX_API int hx_rref_batch_create(const hx_modctx *ctx, const hx_rref_job *jobs, uint32_t njobs, int variant,
@@ -219,6 +220,39 @@ typedef void *hx_howell_batch;
 HX_API int hx_howell_batch_create(uint32_t k, const hx_howell_job *jobs, uint32_t njobs, hx_howell_batch *out);
 HX_API int hx_howell_batch_run(hx_howell_batch b, hx_stream s);
 HX_API int hx_howell_batch_destroy(hx_howell_batch b);
+
+/* ---- the Z/2^32 / Z/2^64 LU session (Cuda/CudaRing2k.h:28-46; installed through the
+ *      ring2k::device::Ops table, Binary/Ring2kDevice.h:64-83; host driver Binary/Ring2kDeviceLU.h) ----
+ * A device-resident column window [0, cw) of an m x cw matrix of elem-byte entries (4 or 8); the host
+ * does the pivot search and hands each step its factored panel, row swaps and L slots.  Wraparound
+ * arithmetic: exact.  put / get / fetch are synchronous; step is asynchronous (two streams). */
+typedef struct hx_ring2k_step_args {   /* ring2k::device::StepArgs, Binary/Ring2kDevice.h:49-62 */
+    uint32_t m;                   /* rows of A */
+    uint32_t r, np;               /* the panel's pivot rows r .. r + np - 1 */
+    uint32_t pc0, pc1;            /* the panel's columns (the swaps skip them) */
+    const void *panel;            /* factored panel, rows r .. m-1, row-major, ld = ldp elements; NULL = none */
+    size_t ldp;
+    const uint32_t *swaps;        /* nswaps pairs (a, b) of rows, swapped in this order */
+    uint32_t nswaps;
+    const void *L;                /* L slots from row r on (ld = ldl elements), columns 0 .. np-1 read */
+    size_t ldl;
+    uint32_t u0;                  /* TRSM / GEMM columns [u0, w) (u0 >= w: no update) */
+    uint32_t la0, la1;            /* lookahead: rows r + np .. m-1 of columns [la0, la1), collected by fetch */
+    uint32_t w;                   /* window columns in use, [0, w) (0 = cw) */
+} hx_ring2k_step_args;
+typedef void *hx_ring2k;
+HX_API size_t hx_ring2k_session_bytes(uint32_t elem, uint32_t m, uint32_t cw, uint32_t panel);
+HX_API size_t hx_ring2k_staging_bytes(uint32_t elem, uint32_t m, uint32_t panel);
+HX_API int hx_ring2k_open(uint32_t elem, uint32_t m, uint32_t cw, uint32_t panel, hx_ring2k *out);
+HX_API int hx_ring2k_put(hx_ring2k s, const void *host, size_t ldh, uint32_t r0, uint32_t rows, uint32_t c0,
+                         uint32_t cols);
+HX_API int hx_ring2k_get(hx_ring2k s, void *host, size_t ldh, uint32_t r0, uint32_t rows, uint32_t c0, uint32_t cols);
+HX_API int hx_ring2k_step(hx_ring2k s, const hx_ring2k_step_args *a);
+HX_API int hx_ring2k_fetch(hx_ring2k s, void *host, size_t ldh);
+HX_API int hx_ring2k_close(hx_ring2k s);   /* never fails; frees everything */
+/* One trailing GEMM C (m x n) -= L (m x k) U (k x n) on the device, all host row-major (gemm_sub_host). */
+HX_API int hx_ring2k_gemm_sub_host(uint32_t elem, void *C, size_t ldc, const void *L, size_t ldl, const void *U,
+                                   size_t ldu, uint32_t m, uint32_t n, uint32_t k);
 
 /* ---- the learned selector's network (no counterpart in the reference: its cost model is static,
  *      Engine/EngineCostModel.h:22-51) -----------------------------------------------------------
