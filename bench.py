@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--no-batch", action="store_true", help="do not time the batched echelon workload")
     ap.add_argument("--batch-jobs", type=int, default=4096)
     ap.add_argument("--no-lu", action="store_true", help="do not time the 4096^2 rank-3000 RREF")
+    ap.add_argument("--no-mixed", action="store_true", help="do not run the 20000-job mixed stream")
+    ap.add_argument("--mixed-jobs", type=int, default=20000)
     ap.add_argument("--pmc-mode", action="store_true",
                     help="for counter collection: one 1 GiB device copy (the traffic calibration), then one limb "
                          "GEMM of the headline shape; no timing, no JSON")
@@ -50,6 +52,19 @@ def parse():
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def profiled_traffic(n, limbs):
+    """HBM bytes per main-kernel launch from the committed counter passes (profiles/), if they cover
+    this shape: (FETCH_SIZE x fetch factor + WRITE_SIZE x write factor) in bytes."""
+    path = os.path.join(ROOT, "profiles", "limb_gemm_8192_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    if t.get("n") != n or t.get("limbs") != limbs:
+        return None
+    return t["traffic_bytes"]
 
 
 def gemm_timed(E, ctx, A, B, Cm, variant, steps, warmup, ws):
@@ -171,6 +186,60 @@ def lu_c2(E, reps=3):
             "shape": [m, n], "modulus": p}
 
 
+def mixed_stream(njobs):
+    """C5: autotune a training sample exhaustively, train the MLP selector on the GPU, compare it with
+    the autotuner's best and with the static policy on held-out jobs, then run the whole stream with
+    the selector's choices (batched per kind and choice)."""
+    import torch
+    from hacs_amd import autotune as AT
+    t0 = time.perf_counter()
+    train = [AT.Job(k, n, i) for i, (k, n) in enumerate(AT.sample_jobs(240, 0x71))]
+    held = [AT.Job(k, n, i) for i, (k, n) in enumerate(AT.sample_jobs(120, 0x72))]
+    tuner = AT.Autotuner(reps=3)
+    Tt, Th = tuner.tune(train), tuner.tune(held)
+    for jb in train + held:
+        jb.free()
+    t_tune = time.perf_counter() - t0
+    sel = AT.Selector(seed=0)
+    t1 = time.perf_counter()
+    loss = sel.fit([j.kind for j in train], [j.n for j in train], Tt, steps=3000)
+    t_fit = time.perf_counter() - t1
+    slots_h = sel.choose([j.kind for j in held], [j.n for j in held])
+    stat_h = [AT.static_choice(j.kind, j.n) for j in held]
+    best = float(np.nanmin(np.where(np.isfinite(Th), Th, np.nan), axis=1).sum())
+    t_sel = float(sum(Th[i, s] for i, s in enumerate(slots_h)))
+    t_stat = float(sum(Th[i, s] for i, s in enumerate(stat_h)))
+    agree = float(np.mean([s == int(np.argmin(Th[i])) for i, s in enumerate(slots_h)]))
+    # the whole stream, in memory-bounded chunks, with the selector's and with the static choices
+    stream = AT.sample_jobs(njobs, 0x51)
+    kinds = [k for k, _ in stream]
+    ns = [n for _, n in stream]
+    slots = sel.choose(kinds, ns)
+    statics = [AT.static_choice(k, n) for k, n in stream]
+    t_run = {"selector": 0.0, "static": 0.0}
+    budget = 40 << 30
+    i = 0
+    while i < njobs:
+        chunk, used = [], 0
+        while i < njobs and (not chunk or used < budget):
+            k, n = stream[i]
+            used += (3 if k == "gemm" else 2) * 4 * n * n
+            chunk.append(i)
+            i += 1
+        jobs = [AT.Job(stream[j][0], stream[j][1], j, salt=0x90) for j in chunk]
+        t_run["selector"] += AT.run_stream(jobs, [slots[j] for j in chunk])
+        t_run["static"] += AT.run_stream(jobs, [statics[j] for j in chunk])
+        for jb in jobs:
+            jb.free()
+        torch.cuda.empty_cache()
+    counts = {k: kinds.count(k) for k in AT.KINDS}
+    return {"jobs": njobs, "kinds": counts, "seconds": t_run["selector"], "jobs_per_s": njobs / t_run["selector"],
+            "static_seconds": t_run["static"], "selector_gain_vs_static": t_run["static"] / t_run["selector"],
+            "heldout_jobs": len(held), "heldout_selector_vs_best": t_sel / best,
+            "heldout_static_vs_best": t_stat / best, "heldout_choice_agreement": agree,
+            "train_jobs": len(train), "train_loss": loss, "tune_seconds": t_tune, "fit_seconds": t_fit}
+
+
 def cpu_baseline(p, threads):
     """The reference's own CPU GEMM (oracle/_ref, Kernels.cpp:1417) on a bounded sample, min of 3."""
     from oracle import oracle as O
@@ -254,7 +323,7 @@ def main():
         "limb_vs_plain": wall_p / wall_l,
         "parity": {"limb_equals_plain": same, "spot_entries_exact": spot},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_INT8_TOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_INT8_TOPS, "traffic": None,
+                     "frac": achieved / PEAK_INT8_TOPS, "traffic": profiled_traffic(n, ctx.limbs),
                      "kernel": "limb_kernel<4>", "kernel_ms": main_l,
                      "algorithmic_ops_per_launch": 32.0 * macs},
     }
@@ -266,6 +335,10 @@ def main():
         lu = lu_c2(E)
         log(f"C2 rref: {lu}")
         res["lu_c2"] = lu
+    if not args.no_mixed:
+        mx = mixed_stream(args.mixed_jobs)
+        log(f"mixed stream: {mx}")
+        res["mixed_stream"] = mx
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         res["cpu_baseline"] = cpu_baseline(p, threads)

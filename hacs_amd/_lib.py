@@ -132,6 +132,7 @@ SIGNATURES = {
     "hx_mlp_backward": (_INT, [_V, _U32, _U32, _V, _U32, _V, _U32, _V, _V, _V, _V, _V, _V, _V, _V, _V, _V]),
     "hx_adam_step": (_INT, [_V, _V, _V, _V, _U32, _F, _F, _F, _F, _U32, _V]),
     "hx_random_entries": (_INT, [_V, _U32, _U32, _I64, _U32, _U64, _V]),
+    "hx_random_sparse": (_INT, [_V, _U32, _U32, _I64, _U32, _U64, _V]),
     "hx_ring2k_session_bytes": (_SZ, [_U32, _U32, _U32, _U32]),
     "hx_ring2k_staging_bytes": (_SZ, [_U32, _U32, _U32]),
     "hx_ring2k_open": (_INT, [_U32, _U32, _U32, _U32, C.POINTER(_V)]),

@@ -55,11 +55,11 @@ class Job:
             self.A = E.random_entries(n, n, P_GEMM, W.seed_for(n, n, P_GEMM, salt + 1) ^ j)
             self.B = E.random_entries(n, n, P_GEMM, W.seed_for(n, n, P_GEMM, salt + 2) ^ j)
             self.C = torch.empty((n, n), dtype=torch.int32, device="cuda")
-        elif kind == "echelon":
-            self.src = E.to_device(W.sparse_job(j, n, n, P_ECH, salt + 3))
+        elif kind == "echelon":     # == workloads.sparse_job(j, n, n, P_ECH, salt + 3), generated in HBM
+            self.src = E.random_sparse(n, n, P_ECH, W.seed_for(n, n, P_ECH, salt + 3) ^ j)
             self.M = self.src.clone()
-        else:
-            self.src = E.to_device(W.dense_job(j, n, n, 1 << K_HOWELL, salt + 4))
+        else:                        # == workloads.dense_job(j, n, n, 2^16, salt + 4)
+            self.src = E.random_entries(n, n, 1 << K_HOWELL, W.seed_for(n, n, 1 << K_HOWELL, salt + 4) ^ j)
 
     def free(self):
         for a in ("A", "B", "C", "src", "M"):

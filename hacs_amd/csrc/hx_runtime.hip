@@ -101,6 +101,21 @@ __global__ void random_entries_kernel(uint32_t* out, uint32_t rows, uint32_t col
     }
 }
 
+__global__ void random_sparse_kernel(uint32_t* out, uint32_t rows, uint32_t cols, int64_t ld, uint32_t p,
+                                     uint64_t seed) {
+    const uint64_t total = (uint64_t)rows * cols;
+    const uint32_t vals[4] = {(p - 2 % p) % p, p - 1, 1 % p, 2 % p};
+    for (uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t z = seed + (idx + 1) * 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        const uint32_t i = (uint32_t)(idx / cols), j = (uint32_t)(idx % cols);
+        out[(int64_t)i * ld + j] = ((z >> 32) % 20 == 0) ? vals[z & 3] : 0u;
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -268,6 +283,19 @@ HX_API int hx_random_entries(uint32_t* out, uint32_t rows, uint32_t cols, int64_
         const uint32_t blocks = (uint32_t)std::min<uint64_t>((total + 255) / 256, 65536);
         random_entries_kernel<<<blocks, 256, 0, (hipStream_t)s>>>(out, rows, cols, ld, ell, seed);
         after_launch("random_entries_kernel");
+    });
+}
+
+HX_API int hx_random_sparse(uint32_t* out, uint32_t rows, uint32_t cols, int64_t ld, uint32_t p, uint64_t seed,
+                            hx_stream s) {
+    return guard("hx_random_sparse", [&] {
+        HX_CHECK(p >= 3, "p must be >= 3");
+        HX_CHECK(ld >= (int64_t)cols, "ld < cols");
+        if ((uint64_t)rows * cols == 0) return;
+        const uint64_t total = (uint64_t)rows * cols;
+        const uint32_t blocks = (uint32_t)std::min<uint64_t>((total + 255) / 256, 65536);
+        random_sparse_kernel<<<blocks, 256, 0, (hipStream_t)s>>>(out, rows, cols, ld, p, seed);
+        after_launch("random_sparse_kernel");
     });
 }
 

@@ -121,6 +121,14 @@ def random_entries(rows: int, cols: int, ell: int, seed: int, stream=None) -> to
     return t
 
 
+def random_sparse(rows: int, cols: int, p: int, seed: int, stream=None) -> torch.Tensor:
+    """The 5 %-density {-2, -1, 1, 2} entries (the coboundary-like jobs) generated on the device."""
+    t = torch.empty((rows, cols), dtype=torch.int32, device="cuda")
+    check(lib().hx_random_sparse(C.c_void_p(t.data_ptr()), rows, cols, cols, p, seed,
+                                 current_stream() if stream is None else stream))
+    return t
+
+
 @dataclass
 class DeviceInfo:
     name: str

@@ -275,6 +275,10 @@ HX_API int hx_adam_step(float *param, const float *grad, float *m1, float *m2, u
  * RandomEntries(rows * cols, ell, seed) matrix).  Used for synthetic inputs that never leave HBM. */
 HX_API int hx_random_entries(uint32_t *out, uint32_t rows, uint32_t cols, int64_t ld, uint32_t ell, uint64_t seed,
                              hx_stream s);
+/* The sparse coboundary-like entries: draw e of the stream seeded `seed` decides entry e (row-major):
+ * non-zero iff (draw >> 32) % 20 == 0 (density 5 %), value {p-2, p-1, 1, 2}[draw & 3] reduced mod p. */
+HX_API int hx_random_sparse(uint32_t *out, uint32_t rows, uint32_t cols, int64_t ld, uint32_t p, uint64_t seed,
+                            hx_stream s);
 
 #ifdef __cplusplus
 }
