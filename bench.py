@@ -28,6 +28,7 @@ sys.path.insert(0, ROOT)
 
 N_C3 = 8192
 PEAK_INT8_TOPS = 5000.0      # MI355X dense INT8 MFMA, tera-ops/s
+LIMB_DEFAULT = 2             # the limb kernel hx_gemm_u32 runs when opts.tile is 0
 PEAK_HBM_GBS = 8000.0
 
 
@@ -38,12 +39,19 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=N_C3)
     ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--tile", type=int, default=0,
+                    help="limb kernel: 0 default, 1 register-staged LDS copy, 2 direct-to-LDS ring")
     ap.add_argument("--no-cpu", action="store_true", help="do not time the CPU baseline")
     ap.add_argument("--no-batch", action="store_true", help="do not time the batched echelon workload")
     ap.add_argument("--batch-jobs", type=int, default=4096)
     ap.add_argument("--no-lu", action="store_true", help="do not time the 4096^2 rank-3000 RREF")
+    ap.add_argument("--no-lu-cpu", action="store_true", help="do not time the reference's CPU RREF on it")
     ap.add_argument("--no-mixed", action="store_true", help="do not run the 20000-job mixed stream")
     ap.add_argument("--mixed-jobs", type=int, default=20000)
+    ap.add_argument("--no-howell", action="store_true", help="do not time the Howell batch")
+    ap.add_argument("--howell-jobs", type=int, default=6666)
+    ap.add_argument("--only", choices=["batch", "lu", "mixed", "cpu", "howell"],
+                    help="run one secondary workload alone (for profiling) and print its record")
     ap.add_argument("--pmc-mode", action="store_true",
                     help="for counter collection: one 1 GiB device copy (the traffic calibration), then one limb "
                          "GEMM of the headline shape; no timing, no JSON")
@@ -54,20 +62,20 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def profiled_traffic(n, limbs):
+def profiled_traffic(n, limbs, kernel):
     """HBM bytes per main-kernel launch from the committed counter passes (profiles/), if they cover
-    this shape: (FETCH_SIZE x fetch factor + WRITE_SIZE x write factor) in bytes."""
+    this shape and kernel: (FETCH_SIZE x fetch factor + WRITE_SIZE x write factor) in bytes."""
     path = os.path.join(ROOT, "profiles", "limb_gemm_8192_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         t = json.load(f)
-    if t.get("n") != n or t.get("limbs") != limbs:
+    if t.get("n") != n or t.get("limbs") != limbs or t.get("kernel") != kernel:
         return None
     return t["traffic_bytes"]
 
 
-def gemm_timed(E, ctx, A, B, Cm, variant, steps, warmup, ws):
+def gemm_timed(E, ctx, A, B, Cm, variant, steps, warmup, ws, tile=0):
     """Times `steps` GEMMs on one stream; returns (wall ms per step, main-kernel ms per step)."""
     import torch
     from hacs_amd import _lib
@@ -82,6 +90,7 @@ def gemm_timed(E, ctx, A, B, Cm, variant, steps, warmup, ws):
         evs.append((a, b))
     o = _lib.hx_gemm_opts()
     o.variant = variant
+    o.tile = tile
     if ws is not None:
         o.workspace = C.c_void_p(ws.data_ptr())
         o.workspace_bytes = ws.numel()
@@ -143,6 +152,32 @@ def batched_echelon(E, njobs):
         batch.run()
         torch.cuda.synchronize()
         t_batch.append(time.perf_counter() - t0)
+    eager = [w_.clone() for w_ in work]
+    ranks_eager = [r.rank for r in batch.results()]
+    # the same launch captured in a hipGraph on a side stream, replayed from resident inputs
+    import ctypes as C
+    from hacs_amd import _lib
+    L = _lib.lib()
+    gs = torch.cuda.Stream()
+    g = C.c_void_p()
+    t_graph = []
+    with torch.cuda.stream(gs):
+        h = gs.cuda_stream
+        _lib.check(L.hx_graph_begin(h))
+        batch.run(stream=h)
+        _lib.check(L.hx_graph_end(h, C.byref(g)))
+        for _ in range(reps):
+            for w_, s_ in zip(work, src):
+                w_.copy_(s_)
+            gs.synchronize()
+            t0 = time.perf_counter()
+            _lib.check(L.hx_graph_launch(g, h))
+            gs.synchronize()
+            t_graph.append(time.perf_counter() - t0)
+        _lib.check(L.hx_graph_destroy(g))
+    graph_same = all(torch.equal(a, b) for a, b in zip(work, eager)) and \
+        [r.rank for r in batch.results()] == ranks_eager
+    del eager
     singles = [E.RrefBatch(ctx, [w_]) for w_ in work]
     for w_, s_ in zip(work, src):
         w_.copy_(s_)
@@ -159,12 +194,73 @@ def batched_echelon(E, njobs):
         "jobs_per_s": njobs / min(t_batch),
         "per_job_launch_jobs_per_s": njobs / t_single,
         "speedup_vs_per_job": t_single / min(t_batch),
+        "graph_jobs_per_s": njobs / min(t_graph),
+        "graph_equals_eager": graph_same,
         "elim_work_macs": work_macs,
         "ms": min(t_batch) * 1e3,
     }
 
 
-def lu_c2(E, reps=3):
+def howell_batch(E, njobs, cpu_budget_s=8.0):
+    """C5(c): Howell forms over Z/2^16 of the stream's Howell jobs (dense uniform [0, 2^16), n
+    log-uniform in [32, 512]) -- jobs/s of one persistent launch vs one launch per job, and the
+    reference's ring2k::howell (u16 rung, oracle/_ref) on host cores over a sample of the same
+    jobs, whose outputs are compared with the device's."""
+    import torch
+    from hacs_amd import workloads as W
+    k = 16
+    shapes = [n for kind, n in W.c5_stream(3 * njobs + 3, salt=0x51) if kind == "howell"][:njobs]
+    seeds = [W.seed_for(n, n, 1 << k, 0x94) ^ j for j, n in enumerate(shapes)]
+    mats = [E.random_entries(n, n, 1 << k, sd) for n, sd in zip(shapes, seeds)]
+    b = E.HowellBatch(k, mats)
+    b.run()
+    torch.cuda.synchronize()
+    best = 1e30
+    for _ in range(3):
+        t0 = time.perf_counter()
+        b.run()
+        torch.cuda.synchronize()
+        best = min(best, time.perf_counter() - t0)
+    singles = [E.HowellBatch(k, [M]) for M in mats]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for sb in singles:
+        sb.run()
+    torch.cuda.synchronize()
+    t_single = time.perf_counter() - t0
+    out = {"jobs": len(shapes), "k": k, "jobs_per_s": len(shapes) / best, "ms": best * 1e3,
+           "per_job_launch_jobs_per_s": len(shapes) / t_single, "speedup_vs_per_job": t_single / best,
+           "mean_n": float(np.mean(shapes))}
+    from oracle import oracle as O
+    if O.ref_available():
+        done, t_cpu, same = 0, 0.0, True
+        for j in range(len(shapes)):
+            A = E.from_device(mats[j])
+            t0 = time.perf_counter()
+            H, piv, pv = O.ref_howell(k, A, "ring2k", 16)
+            t_cpu += time.perf_counter() - t0
+            g = b.results([j])[0]
+            same = same and np.array_equal(g.H, H) and np.array_equal(g.pivots, piv) and np.array_equal(g.pivot_v, pv)
+            done += 1
+            if t_cpu > cpu_budget_s:
+                break
+        sub = E.HowellBatch(k, mats[:done])
+        sub.run()
+        torch.cuda.synchronize()
+        t_sub = 1e30
+        for _ in range(3):
+            t0 = time.perf_counter()
+            sub.run()
+            torch.cuda.synchronize()
+            t_sub = min(t_sub, time.perf_counter() - t0)
+        out["cpu_reference"] = {"jobs": done, "seconds": t_cpu, "cores": 1, "jobs_per_s": done / t_cpu,
+                                "device_jobs_per_s_same_jobs": done / t_sub,
+                                "device_equals_reference": bool(same),
+                                "routine": "ring2k::howell, u16 rung (oracle/_ref), the first jobs of the list"}
+    return out
+
+
+def lu_c2(E, reps=3, cpu=True):
     """C2: RREF of the 4096 x 4096 rank-3000 matrix mod 2^31-1 (rank, pivots, pivot_rows, RREF)."""
     import torch
     from hacs_amd import workloads as W
@@ -182,8 +278,18 @@ def lu_c2(E, reps=3):
         best = min(best, time.perf_counter() - t0)
     m, n = A.shape
     work = float((m - 1) * (res.rank * n - res.rank * (res.rank - 1) // 2))
-    return {"seconds": best, "rank": res.rank, "elim_work_macs": work, "elim_macs_per_s": work / best,
-            "shape": [m, n], "modulus": p}
+    out = {"seconds": best, "rank": res.rank, "elim_work_macs": work, "elim_macs_per_s": work / best,
+           "shape": [m, n], "modulus": p}
+    from oracle import oracle as O
+    if cpu and O.ref_available():
+        Ah = E.from_device(A)
+        t0 = time.perf_counter()
+        r_ref, piv_ref, R_ref = O.ref_rref(p, Ah)
+        t_ref = time.perf_counter() - t0
+        same = r_ref == res.rank and np.array_equal(piv_ref, res.pivots) and np.array_equal(R_ref, E.from_device(R))
+        out["cpu_reference"] = {"seconds": t_ref, "cores": 1, "routine": "elim::rref<uint32_t> (oracle/_ref)",
+                                "device_equals_reference": bool(same), "speedup": t_ref / best}
+    return out
 
 
 def mixed_stream(njobs):
@@ -214,7 +320,9 @@ def mixed_stream(njobs):
     stream = AT.sample_jobs(njobs, 0x51)
     kinds = [k for k, _ in stream]
     ns = [n for _, n in stream]
+    t2 = time.perf_counter()
     slots = sel.choose(kinds, ns)
+    t_choose = time.perf_counter() - t2
     statics = [AT.static_choice(k, n) for k, n in stream]
     t_run = {"selector": 0.0, "static": 0.0}
     budget = 40 << 30
@@ -237,7 +345,9 @@ def mixed_stream(njobs):
             "static_seconds": t_run["static"], "selector_gain_vs_static": t_run["static"] / t_run["selector"],
             "heldout_jobs": len(held), "heldout_selector_vs_best": t_sel / best,
             "heldout_static_vs_best": t_stat / best, "heldout_choice_agreement": agree,
-            "train_jobs": len(train), "train_loss": loss, "tune_seconds": t_tune, "fit_seconds": t_fit}
+            "train_jobs": len(train), "train_loss": loss, "tune_seconds": t_tune, "fit_seconds": t_fit,
+            "exhaustive_tune_seconds_per_job": t_tune / (len(train) + len(held)),
+            "selector_choose_seconds": t_choose}
 
 
 def cpu_baseline(p, threads):
@@ -280,9 +390,16 @@ def main():
         A = E.random_entries(n, n, p, W.seed_for(n, n, p, 0x01))
         B = E.random_entries(n, n, p, W.seed_for(n, n, p, 0x02))
         Cm = torch.empty((n, n), dtype=torch.int32, device="cuda")
-        E.gemm(ctx, A, B, Cm, E.GemmOp.Set, variant=E.HX_GEMM_LIMB)
+        E.gemm(ctx, A, B, Cm, E.GemmOp.Set, variant=E.HX_GEMM_LIMB, tile=args.tile)
         torch.cuda.synchronize()
         log("pmc mode done")
+        return
+    if args.only:
+        leg = {"batch": lambda: batched_echelon(E, args.batch_jobs), "lu": lambda: lu_c2(E, cpu=not args.no_lu_cpu),
+               "mixed": lambda: mixed_stream(args.mixed_jobs),
+               "howell": lambda: howell_batch(E, args.howell_jobs),
+               "cpu": lambda: cpu_baseline(p, min(16, os.cpu_count() or 1))}[args.only]()
+        print(json.dumps({"only": args.only, **leg}), flush=True)
         return
     log(f"device {E.device_info()}; C3 n={n} p={p}")
     A = E.random_entries(n, n, p, W.seed_for(n, n, p, 0x01))
@@ -292,8 +409,15 @@ def main():
     ws = torch.empty(E.gemm_workspace_bytes(ctx, n, n, n, E.HX_GEMM_LIMB), dtype=torch.uint8, device="cuda")
     torch.cuda.synchronize()
 
-    wall_l, main_l = gemm_timed(E, ctx, A, B, C_limb, E.HX_GEMM_LIMB, args.steps, args.warmup, ws)
+    wall_l, main_l = gemm_timed(E, ctx, A, B, C_limb, E.HX_GEMM_LIMB, args.steps, args.warmup, ws, args.tile)
     log(f"limb: {wall_l:.3f} ms/step (main kernel {main_l:.3f} ms)")
+    kver = args.tile if args.tile in (1, 2) else LIMB_DEFAULT
+    alt = 2 if kver == 1 else 1
+    C_alt = torch.empty((n, n), dtype=torch.int32, device="cuda")
+    _, main_alt = gemm_timed(E, ctx, A, B, C_alt, E.HX_GEMM_LIMB, 5, 1, ws, alt)
+    same_alt = bool(torch.equal(C_limb, C_alt))
+    del C_alt
+    log(f"other limb kernel (tile {alt}): main kernel {main_alt:.3f} ms; same bytes: {same_alt}")
     wall_p, main_p = gemm_timed(E, ctx, A, B, C_plain, E.HX_GEMM_PLAIN, args.plain_steps, 1, None)
     log(f"plain: {wall_p:.3f} ms/step")
     same = bool(torch.equal(C_limb, C_plain))
@@ -301,6 +425,7 @@ def main():
     log(f"limb == plain: {same}; spot entries exact: {spot}")
     macs = float(n) ** 3
     value = macs / (wall_l * 1e-3)
+    kname = ("limb2_kernel<%d>" if kver == 2 else "limb_kernel<%d>") % ctx.limbs
     achieved = 2.0 * 16.0 * macs / (main_l * 1e-3) / 1e12
     res = {
         "metric": "exact mod-p GEMM mult-adds/s at n=8192 (limb-MFMA vs plain); batched LU jobs/s",
@@ -321,10 +446,12 @@ def main():
         "plain_value": macs / (wall_p * 1e-3),
         "plain_ms_per_step": wall_p,
         "limb_vs_plain": wall_p / wall_l,
-        "parity": {"limb_equals_plain": same, "spot_entries_exact": spot},
+        "parity": {"limb_equals_plain": same, "spot_entries_exact": spot, "limb_kernels_agree": same_alt},
+        "limb_other_kernel": {"tile": alt, "kernel_ms": main_alt},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_INT8_TOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_INT8_TOPS, "traffic": profiled_traffic(n, ctx.limbs),
-                     "kernel": "limb_kernel<4>", "kernel_ms": main_l,
+                     "frac": achieved / PEAK_INT8_TOPS, "traffic": profiled_traffic(n, ctx.limbs, kname),
+                     "kernel": kname,
+                     "kernel_ms": main_l,
                      "algorithmic_ops_per_launch": 32.0 * macs},
     }
     if not args.no_batch:
@@ -332,9 +459,13 @@ def main():
         log(f"batched echelon: {b}")
         res["batched_lu"] = b
     if not args.no_lu:
-        lu = lu_c2(E)
+        lu = lu_c2(E, cpu=not args.no_lu_cpu)
         log(f"C2 rref: {lu}")
         res["lu_c2"] = lu
+    if not args.no_howell:
+        hw = howell_batch(E, args.howell_jobs)
+        log(f"howell batch: {hw}")
+        res["howell_batch"] = hw
     if not args.no_mixed:
         mx = mixed_stream(args.mixed_jobs)
         log(f"mixed stream: {mx}")

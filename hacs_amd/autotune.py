@@ -148,24 +148,23 @@ class Selector:
 
 
 def run_stream(jobs: list[Job], slots: list[int]) -> float:
-    """Executes every job with its chosen slot, batched per (kind, choice): one GEMM batch per
-    variant, one persistent RREF launch per panel width, one persistent Howell launch.  Returns
-    the wall seconds of the launches (inputs already in HBM; batch tables built beforehand)."""
+    """Executes every job with its chosen slot, one batch per kind carrying the per-job choices
+    (GEMM: per-job kernel variant in one batch; echelon: per-job panel width in one persistent
+    launch; Howell: one persistent launch), so a choice never splits a launch.  Returns the wall
+    seconds of the launches (inputs already in HBM; batch tables built beforehand)."""
     ctx_g, ctx_e = E.ModCtx(P_GEMM), E.ModCtx(P_ECH)
-    groups = {}
-    for jb, s in zip(jobs, slots):
-        groups.setdefault((jb.kind, s), []).append(jb)
+    by = {k: [(jb, s) for jb, s in zip(jobs, slots) if jb.kind == k] for k in KINDS}
     runners = []
-    for (kind, s), js in groups.items():
-        if kind == "gemm":
-            v = CHOICES["gemm"][s]
-            runners.append(E.GemmBatch(ctx_g, [(jb.A, jb.B, jb.C, E.GemmOp.Set) for jb in js], v))
-        elif kind == "echelon":
-            for jb in js:
-                jb.M.copy_(jb.src)
-            runners.append(E.RrefBatch(ctx_e, [jb.M for jb in js], panels=[CHOICES["echelon"][s]] * len(js)))
-        else:
-            runners.append(E.HowellBatch(K_HOWELL, [jb.src for jb in js]))
+    if by["gemm"]:
+        runners.append(E.GemmBatch(ctx_g, [(jb.A, jb.B, jb.C, E.GemmOp.Set, CHOICES["gemm"][s])
+                                           for jb, s in by["gemm"]]))
+    if by["echelon"]:
+        for jb, _ in by["echelon"]:
+            jb.M.copy_(jb.src)
+        runners.append(E.RrefBatch(ctx_e, [jb.M for jb, _ in by["echelon"]],
+                                   panels=[CHOICES["echelon"][s] for _, s in by["echelon"]]))
+    if by["howell"]:
+        runners.append(E.HowellBatch(K_HOWELL, [jb.src for jb, _ in by["howell"]]))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for r in runners:

@@ -280,6 +280,42 @@ struct LimbArgs {
     DevMod M;
 };
 
+// Epilogue of both limb kernels: exact recombination sum_d 256^d D_d of one wave's 32 x 64
+// block (two 32 x 32 MFMA blocks; lane = column, 16 accumulator rows per lane), op into C.
+template <int L>
+__device__ __forceinline__ void limb_store(const LimbArgs& a, const v16i (&acc)[2][2 * L - 1], uint32_t m,
+                                           uint32_t r0, uint32_t c0, int lane) {
+    constexpr int ND = 2 * L - 1;
+    const int lr = lane & 31, lg = lane >> 5;
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) {
+        const uint32_t col = c0 + cb * 32 + lr;
+        if (col >= a.n) continue;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const uint32_t row = r0 + (r / 4) * 8 + lg * 4 + (r % 4);
+            if (row >= m) continue;
+            int64_t x = acc[cb][ND - 1][r];
+            int steps = 0;
+#pragma unroll
+            for (int d = ND - 2; d >= 0; --d) {
+                x = x * 256 + (int64_t)acc[cb][d][r];
+                if (++steps == 3 && d > 0) {
+                    x = (int64_t)red64((uint64_t)(x + a.off), a.M);
+                    steps = 0;
+                }
+            }
+            const uint32_t res = red64((uint64_t)(x + a.off), a.M);
+            uint32_t c = 0;
+            if (a.op != HX_OP_SET) {
+                const int64_t rr = a.c_rows ? (int64_t)a.c_rows[row] : (int64_t)row;
+                c = a.Cin[rr * a.cirs + (int64_t)col * a.cics];
+            }
+            a.Cout[(int64_t)row * a.cors + (int64_t)col * a.cocs] = apply_op(c, res, a.op, a.M.p);
+        }
+    }
+}
+
 template <int L>
 __device__ __forceinline__ void limb_tile(const LimbArgs& a, uint32_t tm, uint32_t tn, uint8_t* smem) {
     constexpr int A_BYTES = L * LM * LROW, B_BYTES = L * LN * LROW, STAGE = A_BYTES + B_BYTES;
@@ -368,34 +404,130 @@ __device__ __forceinline__ void limb_tile(const LimbArgs& a, uint32_t tm, uint32
         __syncthreads();
     }
 
-    // epilogue: exact recombination sum_d 256^d D_d, op into C
+    limb_store<L>(a, acc, m, m0 + wm * 32, n0 + wn * 64, lane);
+}
+
+// Direct-to-LDS variant of limb_tile: the global -> LDS copy is done by global_load_lds
+// (16 B per lane, 1 KB per wave instruction) into a ring of three stages, two stages in flight
+// while the MFMAs consume the third.  A stage holds, per plane, the tile's rows as 64-byte
+// k-strips; inside a row the four 16-byte chunks are stored at chunk q ^ swz(row).  A fragment
+// read takes one logical chunk from 32 consecutive rows; with the swizzle any 8 aligned rows
+// (one 128-byte bank line = 2 rows) and any 16 aligned rows (256 bytes = 4 rows) hit pairwise
+// different (row within the bank line, chunk) pairs, so a 16-byte read phase is conflict-free.
+// The loop is unrolled over the three stage arrays so that every LDS access names its array
+// statically; the waits are explicit (vmcnt counts this wave's outstanding DMA instructions:
+// 3L per stage) and the workgroup barrier carries no memory fence of its own.
+__device__ __forceinline__ int swz(int row) { return ((row >> 2) & 3) ^ (row & 2); }
+
+template <int L>
+__device__ __forceinline__ void limb_tile2(const LimbArgs& a, uint32_t tm, uint32_t tn) {
+    constexpr int SA = L * LM * 64, SB = L * LN * 64, STAGE = SA + SB;
+    constexpr int ND = 2 * L - 1, PER_WAVE = 3 * L;      // 12L 1-KB blocks per stage over 4 waves
+    __shared__ __attribute__((aligned(1024))) uint8_t s0[STAGE];
+    __shared__ __attribute__((aligned(1024))) uint8_t s1[STAGE];
+    __shared__ __attribute__((aligned(1024))) uint8_t s2[STAGE];
+    const uint32_t m = a.m_dev ? min(*a.m_dev, a.m) : a.m;
+    const uint32_t m0 = tm * LM, n0 = tn * LN;
+    if (m0 >= m) return;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm = wave & 1, wn = wave >> 1, lr = lane & 31, lg = lane >> 5;
+
+    v16i acc[2][ND];
 #pragma unroll
-    for (int cb = 0; cb < 2; ++cb) {
-        const uint32_t col = n0 + wn * 64 + cb * 32 + lr;
-        if (col >= a.n) continue;
+    for (int cb = 0; cb < 2; ++cb)
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const uint32_t row = m0 + wm * 32 + (r / 4) * 8 + lg * 4 + (r % 4);
-            if (row >= m) continue;
-            int64_t x = acc[cb][ND - 1][r];
-            int steps = 0;
+        for (int d = 0; d < ND; ++d)
 #pragma unroll
-            for (int d = ND - 2; d >= 0; --d) {
-                x = x * 256 + (int64_t)acc[cb][d][r];
-                if (++steps == 3 && d > 0) {
-                    x = (int64_t)red64((uint64_t)(x + a.off), a.M);
-                    steps = 0;
-                }
+            for (int r = 0; r < 16; ++r) acc[cb][d][r] = 0;
+
+    // lane -> (row within a 16-row block, physical chunk); it fetches the logical chunk stored there
+    const int br = lane >> 2, bq = (lane & 3) ^ swz(br);
+    const int8_t* Abase = a.A8 + (size_t)(m0 + br) * a.Kp + 16 * bq;
+    const int8_t* Bbase = a.B8 + (size_t)(n0 + br) * a.Kp + 16 * bq;
+    auto issue = [&](uint32_t kt, uint8_t* st) {
+        const size_t k0 = (size_t)kt * LK;
+#pragma unroll
+        for (int i = 0; i < PER_WAVE; ++i) {
+            const int blk = wave * PER_WAVE + i;
+            const int8_t* src;
+            uint8_t* dst;
+            if (blk < 4 * L) {
+                const int s = blk >> 2, g = blk & 3;
+                src = Abase + s * a.a_plane + (size_t)(16 * g) * a.Kp + k0;
+                dst = st + s * (LM * 64) + g * 1024;
+            } else {
+                const int s = (blk - 4 * L) >> 3, g = (blk - 4 * L) & 7;
+                src = Bbase + s * a.b_plane + (size_t)(16 * g) * a.Kp + k0;
+                dst = st + SA + s * (LN * 64) + g * 1024;
             }
-            const uint32_t res = red64((uint64_t)(x + a.off), a.M);
-            uint32_t c = 0;
-            if (a.op != HX_OP_SET) {
-                const int64_t rr = a.c_rows ? (int64_t)a.c_rows[row] : (int64_t)row;
-                c = a.Cin[rr * a.cirs + (int64_t)col * a.cics];
-            }
-            a.Cout[(int64_t)row * a.cors + (int64_t)col * a.cocs] = apply_op(c, res, a.op, a.M.p);
+            __builtin_amdgcn_global_load_lds((const void*)src, (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
         }
+    };
+    auto compute = [&](const uint8_t* sa) {
+        const uint8_t* sb = sa + SA;
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+            const int q = ks * 2 + lg;
+            v4i fa[L], fb[2][L];
+            const int ra = wm * 32 + lr;
+#pragma unroll
+            for (int s = 0; s < L; ++s) fa[s] = *(const v4i*)(sa + s * (LM * 64) + ra * 64 + 16 * (q ^ swz(ra)));
+#pragma unroll
+            for (int cb = 0; cb < 2; ++cb) {
+                const int rb = wn * 64 + cb * 32 + lr;
+#pragma unroll
+                for (int s = 0; s < L; ++s) fb[cb][s] = *(const v4i*)(sb + s * (LN * 64) + rb * 64 + 16 * (q ^ swz(rb)));
+            }
+#pragma unroll
+            for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+                for (int s = 0; s < L; ++s)
+#pragma unroll
+                    for (int t = 0; t < L; ++t)
+                        acc[cb][s + t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[s], fb[cb][t], acc[cb][s + t], 0, 0, 0);
+        }
+    };
+    const uint32_t KT = a.Kp / LK;
+    if (KT > 0) issue(0, s0);
+    if (KT > 1) issue(1, s1);
+    // stage kt: wait for this wave's DMAs of stage kt (stage kt+1 may stay in flight), barrier so
+    // every wave's DMAs of stage kt are in LDS and every wave is done reading stage kt-1, refill
+    // the slot of stage kt-1 with stage kt+2, then consume stage kt.
+    auto step = [&](uint32_t kt, const uint8_t* cur, uint8_t* next2) {
+        if (kt + 1 < KT)
+            __builtin_amdgcn_s_waitcnt(PER_WAVE | (7 << 4) | (15 << 8));   // vmcnt(3L)
+        else
+            __builtin_amdgcn_s_waitcnt((7 << 4) | (15 << 8));              // vmcnt(0)
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_barrier();
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        if (kt + 2 < KT) issue(kt + 2, next2);
+        compute(cur);
+    };
+    for (uint32_t kt = 0; kt < KT; kt += 3) {
+        step(kt, s0, s2);
+        if (kt + 1 < KT) step(kt + 1, s1, s0);
+        if (kt + 2 < KT) step(kt + 2, s2, s1);
     }
+
+    limb_store<L>(a, acc, m, m0 + wm * 32, n0 + wn * 64, lane);
+}
+
+template <int L>
+__global__ __launch_bounds__(256, 1) void limb2_kernel(LimbArgs a) {
+    uint32_t tm, tn;
+    tile_of(blockIdx.x, a.ntm, a.ntn, tm, tn);
+    limb_tile2<L>(a, tm, tn);
+}
+
+template <int L>
+__global__ __launch_bounds__(256, 1) void limb2_batch_kernel(const LimbArgs* jobs, const uint32_t* prefix,
+                                                             uint32_t njobs) {
+    const uint32_t j = job_of(prefix, njobs, blockIdx.x);
+    const LimbArgs a = jobs[j];
+    const uint32_t local = blockIdx.x - prefix[j];
+    limb_tile2<L>(a, local / a.ntn, local % a.ntn);
 }
 
 template <int L>
@@ -430,12 +562,21 @@ void limb_attrs() {
     });
 }
 
+// Limb kernel choice (hx_gemm_opts.tile for the limb variant): 1 = register-staged copy
+// (limb_kernel), 2 = direct-to-LDS copy ring (limb2_kernel); 0 = the default.
+constexpr int kLimbDefault = 2;
+inline int limb_version(int tile) { return tile == 1 || tile == 2 ? tile : kLimbDefault; }
+
 template <int L>
-void launch_limb(const LimbArgs& la, hipStream_t s) {
-    const size_t lds = limb_lds<L>();
-    limb_attrs<L>();
+void launch_limb(const LimbArgs& la, hipStream_t s, int ver) {
     const uint32_t blocks = la.ntm * la.ntn;
-    limb_kernel<L><<<blocks, 256, lds, s>>>(la);
+    if (ver == 2) {
+        limb2_kernel<L><<<blocks, 256, 0, s>>>(la);
+        after_launch("limb2_kernel");
+        return;
+    }
+    limb_attrs<L>();
+    limb_kernel<L><<<blocks, 256, limb_lds<L>(), s>>>(la);
     after_launch("limb_kernel");
 }
 
@@ -463,7 +604,7 @@ void gemm_plain(const hx_modctx& ctx, const GemmArgs& g, hipStream_t s, int /*ti
     if (ev1) HX_HIP(hipEventRecord(ev1, s));
 }
 
-void gemm_limb(const hx_modctx& ctx, const GemmArgs& g0, hipStream_t s, void* ws, size_t ws_bytes, int /*tile*/,
+void gemm_limb(const hx_modctx& ctx, const GemmArgs& g0, hipStream_t s, void* ws, size_t ws_bytes, int tile,
                hipEvent_t ev0, hipEvent_t ev1) {
     const int L = (int)ctx.limbs;
     HX_CHECK(L >= 1 && L <= 4, "the limb route needs p <= 4278124287 (ctx.limbs > 0)");
@@ -541,12 +682,13 @@ void gemm_limb(const hx_modctx& ctx, const GemmArgs& g0, hipStream_t s, void* ws
         la.off = off;
         la.M = devmod(ctx);
         const bool last = k0 + kc >= g.k;
+        const int ver = limb_version(tile);
         if (ev0 && k0 == 0) HX_HIP(hipEventRecord(ev0, s));
         switch (L) {
-            case 1: launch_limb<1>(la, s); break;
-            case 2: launch_limb<2>(la, s); break;
-            case 3: launch_limb<3>(la, s); break;
-            default: launch_limb<4>(la, s); break;
+            case 1: launch_limb<1>(la, s, ver); break;
+            case 2: launch_limb<2>(la, s, ver); break;
+            case 3: launch_limb<3>(la, s, ver); break;
+            default: launch_limb<4>(la, s, ver); break;
         }
         if (ev1 && last) HX_HIP(hipEventRecord(ev1, s));
         // later chunks accumulate into the C just written
@@ -719,11 +861,20 @@ void gemm_batch_run(void* vb, hipStream_t s) {
         after_launch("prep_batch_kernel");
     }
     if (b->tiles_limb) {
-        switch (b->ctx.limbs) {
-            case 1: limb_batch_kernel<1><<<b->tiles_limb, 256, limb_lds<1>(), s>>>(b->d_limb, pre_limb, b->n_limb); break;
-            case 2: limb_batch_kernel<2><<<b->tiles_limb, 256, limb_lds<2>(), s>>>(b->d_limb, pre_limb, b->n_limb); break;
-            case 3: limb_batch_kernel<3><<<b->tiles_limb, 256, limb_lds<3>(), s>>>(b->d_limb, pre_limb, b->n_limb); break;
-            default: limb_batch_kernel<4><<<b->tiles_limb, 256, limb_lds<4>(), s>>>(b->d_limb, pre_limb, b->n_limb); break;
+        if (kLimbDefault == 2) {
+            switch (b->ctx.limbs) {
+                case 1: limb2_batch_kernel<1><<<b->tiles_limb, 256, 0, s>>>(b->d_limb, pre_limb, b->n_limb); break;
+                case 2: limb2_batch_kernel<2><<<b->tiles_limb, 256, 0, s>>>(b->d_limb, pre_limb, b->n_limb); break;
+                case 3: limb2_batch_kernel<3><<<b->tiles_limb, 256, 0, s>>>(b->d_limb, pre_limb, b->n_limb); break;
+                default: limb2_batch_kernel<4><<<b->tiles_limb, 256, 0, s>>>(b->d_limb, pre_limb, b->n_limb); break;
+            }
+        } else {
+            switch (b->ctx.limbs) {
+                case 1: limb_batch_kernel<1><<<b->tiles_limb, 256, limb_lds<1>(), s>>>(b->d_limb, pre_limb, b->n_limb); break;
+                case 2: limb_batch_kernel<2><<<b->tiles_limb, 256, limb_lds<2>(), s>>>(b->d_limb, pre_limb, b->n_limb); break;
+                case 3: limb_batch_kernel<3><<<b->tiles_limb, 256, limb_lds<3>(), s>>>(b->d_limb, pre_limb, b->n_limb); break;
+                default: limb_batch_kernel<4><<<b->tiles_limb, 256, limb_lds<4>(), s>>>(b->d_limb, pre_limb, b->n_limb); break;
+            }
         }
         after_launch("limb_batch_kernel");
     }

@@ -281,9 +281,11 @@ class HowellBatch:
     def run(self, stream=None) -> None:
         check(lib().hx_howell_batch_run(self.h, current_stream() if stream is None else stream))
 
-    def results(self) -> list[HowellResult]:
+    def results(self, jobs: list[int] | None = None) -> list[HowellResult]:
+        """The forms of all jobs, or of the listed job indices."""
         res = []
-        for H, meta, _, n_ in self.outs:
+        for i in (range(len(self.outs)) if jobs is None else jobs):
+            H, meta, _, n_ = self.outs[i]
             mt = from_device(meta)
             nn = max(1, n_)
             h = int(mt[2 * nn])

@@ -146,3 +146,34 @@ def test_gemm_batch_uneven(hx_gpu, O, variant):
             else:
                 want = np.zeros_like(C0) if op == 0 else C0
             assert np.array_equal(E.from_device(jb[2]), want), f"rep {rep} {A.shape} x {B.shape} op {op}"
+
+
+@pytest.mark.parametrize("tile", [1, 2])
+@pytest.mark.parametrize("p", [251, 65521, 16777213, 2147483629])
+def test_limb_kernels_both_copy_paths(hx_gpu, O, p, tile):
+    """Both limb kernels (tile 1: register-staged LDS copy; tile 2: direct-to-LDS three-stage
+    ring) for every limb count, with k covering 1..5 LDS stages (the ring's unroll by three),
+    ragged m / n, every op, and a k above one limb chunk (two accumulation passes)."""
+    from hacs_amd import engine as E
+    ctx = E.ModCtx(p)
+    assert ctx.limbs in (1, 2, 3, 4)
+    rng = np.random.default_rng(p + tile)
+    shapes = [(64, 128, 64), (65, 129, 128), (200, 130, 192), (31, 300, 256), (129, 257, 320), (96, 96, 1000)]
+    for (m, n, k) in shapes:
+        A = rng.integers(0, p, (m, k), dtype=np.uint64).astype(np.uint32)
+        B = rng.integers(0, p, (k, n), dtype=np.uint64).astype(np.uint32)
+        C0 = rng.integers(0, p, (m, n), dtype=np.uint64).astype(np.uint32)
+        dA, dB = E.to_device(A), E.to_device(B)
+        for op in (E.GemmOp.Set, E.GemmOp.Add, E.GemmOp.Sub):
+            dC = E.to_device(C0)
+            E.gemm(ctx, dA, dB, dC, op, variant=E.HX_GEMM_LIMB, tile=tile)
+            torch.cuda.synchronize()
+            assert np.array_equal(E.from_device(dC), O.gemm_fast(p, A, B, C0, op)), f"{(m, n, k)} op {op}"
+    if ctx.limbs == 4:
+        m, n, k = 70, 130, 32704 + 300          # limb_k_chunk(4) = 32704
+        A = rng.integers(0, p, (m, k), dtype=np.uint64).astype(np.uint32)
+        B = rng.integers(0, p, (k, n), dtype=np.uint64).astype(np.uint32)
+        dC = torch.empty((m, n), dtype=torch.int32, device="cuda")
+        E.gemm(ctx, E.to_device(A), E.to_device(B), dC, E.GemmOp.Set, variant=E.HX_GEMM_LIMB, tile=tile)
+        torch.cuda.synchronize()
+        assert np.array_equal(E.from_device(dC), O.gemm_fast(p, A, B))
