@@ -29,6 +29,7 @@ sys.path.insert(0, ROOT)
 N_C3 = 8192
 PEAK_INT8_TOPS = 5000.0      # MI355X dense INT8 MFMA, tera-ops/s
 LIMB_DEFAULT = 2             # the limb kernel hx_gemm_u32 runs when opts.tile is 0
+LIMB_KERNELS = {1: "limb_kernel<%d>", 2: "limb2_kernel<%d>", 3: "limb3_kernel<%d>", 4: "limb4_kernel<%d>"}
 PEAK_HBM_GBS = 8000.0
 
 
@@ -40,7 +41,8 @@ def parse():
     ap.add_argument("--n", type=int, default=N_C3)
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--tile", type=int, default=0,
-                    help="limb kernel: 0 default, 1 register-staged LDS copy, 2 direct-to-LDS ring")
+                    help="limb kernel: 0 default, 1 register-staged LDS copy, 2 direct-to-LDS ring, "
+                         "3 ring with pipelined LDS reads, 4 the same on eight waves")
     ap.add_argument("--no-cpu", action="store_true", help="do not time the CPU baseline")
     ap.add_argument("--no-batch", action="store_true", help="do not time the batched echelon workload")
     ap.add_argument("--batch-jobs", type=int, default=4096)
@@ -411,13 +413,18 @@ def main():
 
     wall_l, main_l = gemm_timed(E, ctx, A, B, C_limb, E.HX_GEMM_LIMB, args.steps, args.warmup, ws, args.tile)
     log(f"limb: {wall_l:.3f} ms/step (main kernel {main_l:.3f} ms)")
-    kver = args.tile if args.tile in (1, 2) else LIMB_DEFAULT
-    alt = 2 if kver == 1 else 1
+    kver = args.tile if args.tile in LIMB_KERNELS else LIMB_DEFAULT
+    others, same_alt = [], True
     C_alt = torch.empty((n, n), dtype=torch.int32, device="cuda")
-    _, main_alt = gemm_timed(E, ctx, A, B, C_alt, E.HX_GEMM_LIMB, 5, 1, ws, alt)
-    same_alt = bool(torch.equal(C_limb, C_alt))
+    for alt in LIMB_KERNELS:
+        if alt == kver:
+            continue
+        _, main_alt = gemm_timed(E, ctx, A, B, C_alt, E.HX_GEMM_LIMB, 5, 1, ws, alt)
+        same = bool(torch.equal(C_limb, C_alt))
+        same_alt = same_alt and same
+        others.append({"tile": alt, "kernel": LIMB_KERNELS[alt] % ctx.limbs, "kernel_ms": main_alt})
+        log(f"other limb kernel (tile {alt}): main kernel {main_alt:.3f} ms; same bytes: {same}")
     del C_alt
-    log(f"other limb kernel (tile {alt}): main kernel {main_alt:.3f} ms; same bytes: {same_alt}")
     wall_p, main_p = gemm_timed(E, ctx, A, B, C_plain, E.HX_GEMM_PLAIN, args.plain_steps, 1, None)
     log(f"plain: {wall_p:.3f} ms/step")
     same = bool(torch.equal(C_limb, C_plain))
@@ -425,7 +432,7 @@ def main():
     log(f"limb == plain: {same}; spot entries exact: {spot}")
     macs = float(n) ** 3
     value = macs / (wall_l * 1e-3)
-    kname = ("limb2_kernel<%d>" if kver == 2 else "limb_kernel<%d>") % ctx.limbs
+    kname = LIMB_KERNELS[kver] % ctx.limbs
     achieved = 2.0 * 16.0 * macs / (main_l * 1e-3) / 1e12
     res = {
         "metric": "exact mod-p GEMM mult-adds/s at n=8192 (limb-MFMA vs plain); batched LU jobs/s",
@@ -447,7 +454,7 @@ def main():
         "plain_ms_per_step": wall_p,
         "limb_vs_plain": wall_p / wall_l,
         "parity": {"limb_equals_plain": same, "spot_entries_exact": spot, "limb_kernels_agree": same_alt},
-        "limb_other_kernel": {"tile": alt, "kernel_ms": main_alt},
+        "limb_other_kernels": others,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_INT8_TOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_INT8_TOPS, "traffic": profiled_traffic(n, ctx.limbs, kname),
                      "kernel": kname,

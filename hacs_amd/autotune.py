@@ -42,7 +42,7 @@ def features(kind: str, n: int) -> list[float]:
 def static_choice(kind: str, n: int) -> int:
     """The choice without the network: the library's automatic GEMM policy, the widest panel."""
     if kind == "gemm":
-        return 1 if float(n) ** 3 >= 4.0e6 else 0
+        return 1 if n >= 64 and float(n) ** 3 >= 4.0e6 else 0
     return len(CHOICES[kind]) - 1 if kind == "echelon" else 0
 
 

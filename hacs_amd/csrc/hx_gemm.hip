@@ -276,36 +276,50 @@ struct LimbArgs {
     uint32_t* Cout;
     int64_t cors, cocs;
     int op;
-    int64_t off;                 // p * ceil(2^57 / p): makes |x| < 2^57 non-negative before Barrett
+    uint64_t off;                // p * ceil(2^63 / p): makes |x| < 2^63 non-negative before Barrett
+    int32_t w[3];                // 2^(8d) mod p, balanced into (-p/2, p/2], for the diagonals d = 4, 5, 6
     DevMod M;
 };
 
+// The epilogue constants of LimbArgs for modulus p.
+inline void limb_epilogue_consts(uint32_t p, LimbArgs& la) {
+    la.off = ((1ull << 63) / p + 1) * (uint64_t)p;
+    uint64_t w = 1;
+    for (int d = 1; d <= 6; ++d) {
+        w = (w << 8) % p;
+        if (d >= 4) la.w[d - 4] = (int32_t)(w <= p / 2 ? (int64_t)w : (int64_t)w - (int64_t)p);
+    }
+}
+
 // Epilogue of both limb kernels: exact recombination sum_d 256^d D_d of one wave's 32 x 64
 // block (two 32 x 32 MFMA blocks; lane = column, 16 accumulator rows per lane), op into C.
-template <int L>
-__device__ __forceinline__ void limb_store(const LimbArgs& a, const v16i (&acc)[2][2 * L - 1], uint32_t m,
+template <int L, int NCB = 2>
+__device__ __forceinline__ void limb_store(const LimbArgs& a, const v16i (&acc)[NCB][2 * L - 1], uint32_t m,
                                            uint32_t r0, uint32_t c0, int lane) {
     constexpr int ND = 2 * L - 1;
     const int lr = lane & 31, lg = lane >> 5;
 #pragma unroll
-    for (int cb = 0; cb < 2; ++cb) {
+    for (int cb = 0; cb < NCB; ++cb) {
         const uint32_t col = c0 + cb * 32 + lr;
         if (col >= a.n) continue;
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const uint32_t row = r0 + (r / 4) * 8 + lg * 4 + (r % 4);
             if (row >= m) continue;
-            int64_t x = acc[cb][ND - 1][r];
-            int steps = 0;
+            // x == sum_d D_d 256^d (mod p), exactly in int64: the diagonals d <= 3 with their true
+            // weights (|D_d| < 2^31, so |sum| < 2^31 (1 + 2^8 + 2^16 + 2^24) < 2^56), the diagonals
+            // d >= 4 (L = 4 only; 3, 2, 1 limb products, |D_d| <= pairs(d) K 2^14 <= pairs(d) 2^29)
+            // with balanced weights |w_d| <= p/2 < 2^31: |x| < 6 * 2^60 + 2^56 < 2^63.  One reduction
+            // of x + p ceil(2^63 / p), which lies in [0, 2^64).
+            int64_t x = 0;
 #pragma unroll
-            for (int d = ND - 2; d >= 0; --d) {
-                x = x * 256 + (int64_t)acc[cb][d][r];
-                if (++steps == 3 && d > 0) {
-                    x = (int64_t)red64((uint64_t)(x + a.off), a.M);
-                    steps = 0;
-                }
+            for (int d = 0; d < ND; ++d) {
+                if (d <= 3)
+                    x += (int64_t)acc[cb][d][r] * (int64_t)(1u << (8 * d));
+                else
+                    x += (int64_t)acc[cb][d][r] * (int64_t)a.w[d - 4];
             }
-            const uint32_t res = red64((uint64_t)(x + a.off), a.M);
+            const uint32_t res = red64((uint64_t)x + a.off, a.M);
             uint32_t c = 0;
             if (a.op != HX_OP_SET) {
                 const int64_t rr = a.c_rows ? (int64_t)a.c_rows[row] : (int64_t)row;
@@ -419,6 +433,78 @@ __device__ __forceinline__ void limb_tile(const LimbArgs& a, uint32_t tm, uint32
 // 3L per stage) and the workgroup barrier carries no memory fence of its own.
 __device__ __forceinline__ int swz(int row) { return ((row >> 2) & 3) ^ (row & 2); }
 
+// Immediate of s_waitcnt: wait until at most `vm` vector-memory and `lgkm` LDS operations of this
+// wave are outstanding (expcnt not waited on).
+constexpr int wait_imm(int vm, int lgkm) { return (vm & 15) | ((vm >> 4) << 14) | (7 << 4) | ((lgkm & 15) << 8); }
+
+// One wave's MFMA operands for one 32-deep k-step: the A fragment of its 32 rows and the B
+// fragments of its two 32-column blocks, every limb plane.
+template <int L>
+struct LFrag {
+    v4i a[L];
+    v4i b[2][L];
+};
+
+// The DMA of K-stage kt (64 k-bytes of every plane row of the tile) into stage buffer st: 12L
+// blocks of 1 KB (16 rows x 64 bytes), 3L per wave, one global_load_lds (16 B per lane) each.
+// (Written with a per-block A / B branch on purpose: in this form the compiler adds no waits of
+// its own on the in-flight copies before the LDS reads; the kernels' explicit waits and
+// barriers order every copy before the reads of its stage.)
+template <int L>
+__device__ __forceinline__ void stage_issue(const LimbArgs& a, const int8_t* Abase, const int8_t* Bbase, int wave,
+                                            uint32_t kt, uint8_t* st) {
+    constexpr int SA = L * LM * 64, PER_WAVE = 3 * L;
+    const size_t k0 = (size_t)kt * LK;
+#pragma unroll
+    for (int i = 0; i < PER_WAVE; ++i) {
+        const int blk = wave * PER_WAVE + i;
+        const int8_t* src;
+        uint8_t* dst;
+        if (blk < 4 * L) {
+            const int s = blk >> 2, g = blk & 3;
+            src = Abase + s * a.a_plane + (size_t)(16 * g) * a.Kp + k0;
+            dst = st + s * (LM * 64) + g * 1024;
+        } else {
+            const int s = (blk - 4 * L) >> 3, g = (blk - 4 * L) & 7;
+            src = Bbase + s * a.b_plane + (size_t)(16 * g) * a.Kp + k0;
+            dst = st + SA + s * (LN * 64) + g * 1024;
+        }
+        __builtin_amdgcn_global_load_lds((const void*)src, (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
+    }
+}
+
+template <int L>
+__device__ __forceinline__ void stage_frag(const uint8_t* st, int ks, int wm, int wn, int lane, LFrag<L>& f) {
+    constexpr int SA = L * LM * 64;
+    const int lr = lane & 31, q = ks * 2 + (lane >> 5);
+    const int ra = wm * 32 + lr;
+#pragma unroll
+    for (int s = 0; s < L; ++s) f.a[s] = *(const v4i*)(st + s * (LM * 64) + ra * 64 + 16 * (q ^ swz(ra)));
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) {
+        const int rb = wn * 64 + cb * 32 + lr;
+#pragma unroll
+        for (int s = 0; s < L; ++s) f.b[cb][s] = *(const v4i*)(st + SA + s * (LN * 64) + rb * 64 + 16 * (q ^ swz(rb)));
+    }
+}
+
+template <int L>
+__device__ __forceinline__ void frag_mfma(const LFrag<L>& f, v16i (&acc)[2][2 * L - 1]) {
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+        for (int s = 0; s < L; ++s)
+#pragma unroll
+            for (int t = 0; t < L; ++t)
+                acc[cb][s + t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(f.a[s], f.b[cb][t], acc[cb][s + t], 0, 0, 0);
+}
+
+__device__ __forceinline__ void wg_barrier() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_barrier();
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
 template <int L>
 __device__ __forceinline__ void limb_tile2(const LimbArgs& a, uint32_t tm, uint32_t tn) {
     constexpr int SA = L * LM * 64, SB = L * LN * 64, STAGE = SA + SB;
@@ -431,7 +517,7 @@ __device__ __forceinline__ void limb_tile2(const LimbArgs& a, uint32_t tm, uint3
     if (m0 >= m) return;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int wm = wave & 1, wn = wave >> 1, lr = lane & 31, lg = lane >> 5;
+    const int wm = wave & 1, wn = wave >> 1;
 
     v16i acc[2][ND];
 #pragma unroll
@@ -445,64 +531,27 @@ __device__ __forceinline__ void limb_tile2(const LimbArgs& a, uint32_t tm, uint3
     const int br = lane >> 2, bq = (lane & 3) ^ swz(br);
     const int8_t* Abase = a.A8 + (size_t)(m0 + br) * a.Kp + 16 * bq;
     const int8_t* Bbase = a.B8 + (size_t)(n0 + br) * a.Kp + 16 * bq;
-    auto issue = [&](uint32_t kt, uint8_t* st) {
-        const size_t k0 = (size_t)kt * LK;
-#pragma unroll
-        for (int i = 0; i < PER_WAVE; ++i) {
-            const int blk = wave * PER_WAVE + i;
-            const int8_t* src;
-            uint8_t* dst;
-            if (blk < 4 * L) {
-                const int s = blk >> 2, g = blk & 3;
-                src = Abase + s * a.a_plane + (size_t)(16 * g) * a.Kp + k0;
-                dst = st + s * (LM * 64) + g * 1024;
-            } else {
-                const int s = (blk - 4 * L) >> 3, g = (blk - 4 * L) & 7;
-                src = Bbase + s * a.b_plane + (size_t)(16 * g) * a.Kp + k0;
-                dst = st + SA + s * (LN * 64) + g * 1024;
-            }
-            __builtin_amdgcn_global_load_lds((const void*)src, (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
-        }
-    };
-    auto compute = [&](const uint8_t* sa) {
-        const uint8_t* sb = sa + SA;
+    auto compute = [&](const uint8_t* st) {
+        LFrag<L> f;
 #pragma unroll
         for (int ks = 0; ks < 2; ++ks) {
-            const int q = ks * 2 + lg;
-            v4i fa[L], fb[2][L];
-            const int ra = wm * 32 + lr;
-#pragma unroll
-            for (int s = 0; s < L; ++s) fa[s] = *(const v4i*)(sa + s * (LM * 64) + ra * 64 + 16 * (q ^ swz(ra)));
-#pragma unroll
-            for (int cb = 0; cb < 2; ++cb) {
-                const int rb = wn * 64 + cb * 32 + lr;
-#pragma unroll
-                for (int s = 0; s < L; ++s) fb[cb][s] = *(const v4i*)(sb + s * (LN * 64) + rb * 64 + 16 * (q ^ swz(rb)));
-            }
-#pragma unroll
-            for (int cb = 0; cb < 2; ++cb)
-#pragma unroll
-                for (int s = 0; s < L; ++s)
-#pragma unroll
-                    for (int t = 0; t < L; ++t)
-                        acc[cb][s + t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[s], fb[cb][t], acc[cb][s + t], 0, 0, 0);
+            stage_frag<L>(st, ks, wm, wn, lane, f);
+            frag_mfma<L>(f, acc);
         }
     };
     const uint32_t KT = a.Kp / LK;
-    if (KT > 0) issue(0, s0);
-    if (KT > 1) issue(1, s1);
+    if (KT > 0) stage_issue<L>(a, Abase, Bbase, wave, 0, s0);
+    if (KT > 1) stage_issue<L>(a, Abase, Bbase, wave, 1, s1);
     // stage kt: wait for this wave's DMAs of stage kt (stage kt+1 may stay in flight), barrier so
     // every wave's DMAs of stage kt are in LDS and every wave is done reading stage kt-1, refill
     // the slot of stage kt-1 with stage kt+2, then consume stage kt.
     auto step = [&](uint32_t kt, const uint8_t* cur, uint8_t* next2) {
         if (kt + 1 < KT)
-            __builtin_amdgcn_s_waitcnt(PER_WAVE | (7 << 4) | (15 << 8));   // vmcnt(3L)
+            __builtin_amdgcn_s_waitcnt(wait_imm(PER_WAVE, 15));
         else
-            __builtin_amdgcn_s_waitcnt((7 << 4) | (15 << 8));              // vmcnt(0)
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        __builtin_amdgcn_s_barrier();
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        if (kt + 2 < KT) issue(kt + 2, next2);
+            __builtin_amdgcn_s_waitcnt(wait_imm(0, 15));
+        wg_barrier();
+        if (kt + 2 < KT) stage_issue<L>(a, Abase, Bbase, wave, kt + 2, next2);
         compute(cur);
     };
     for (uint32_t kt = 0; kt < KT; kt += 3) {
@@ -512,6 +561,237 @@ __device__ __forceinline__ void limb_tile2(const LimbArgs& a, uint32_t tm, uint3
     }
 
     limb_store<L>(a, acc, m, m0 + wm * 32, n0 + wn * 64, lane);
+}
+
+// limb_tile2 with the LDS reads software-pipelined against the MFMAs: the fragments of the next
+// 32-deep k-step are read while the MFMAs of the current one run, across stage boundaries too.
+// Per stage kt: read (kt, ks1) | MFMAs (kt, ks0) | wait until this wave's stage kt+1 DMAs and
+// (kt, ks1) reads are done, barrier (every wave holds its (kt, ks1) fragments, so slot kt is free,
+// and stage kt+1 is in LDS), DMA of stage kt+3 into slot kt | read (kt+1, ks0) | MFMAs (kt, ks1).
+// Three stages are in flight.
+template <int L>
+__device__ __forceinline__ void limb_tile3(const LimbArgs& a, uint32_t tm, uint32_t tn) {
+    constexpr int SA = L * LM * 64, SB = L * LN * 64, STAGE = SA + SB;
+    constexpr int ND = 2 * L - 1, PER_WAVE = 3 * L;
+    __shared__ __attribute__((aligned(1024))) uint8_t s0[STAGE];
+    __shared__ __attribute__((aligned(1024))) uint8_t s1[STAGE];
+    __shared__ __attribute__((aligned(1024))) uint8_t s2[STAGE];
+    const uint32_t m = a.m_dev ? min(*a.m_dev, a.m) : a.m;
+    const uint32_t m0 = tm * LM, n0 = tn * LN;
+    if (m0 >= m) return;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm = wave & 1, wn = wave >> 1;
+
+    v16i acc[2][ND];
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+        for (int d = 0; d < ND; ++d)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[cb][d][r] = 0;
+
+    const int br = lane >> 2, bq = (lane & 3) ^ swz(br);
+    const int8_t* Abase = a.A8 + (size_t)(m0 + br) * a.Kp + 16 * bq;
+    const int8_t* Bbase = a.B8 + (size_t)(n0 + br) * a.Kp + 16 * bq;
+    const uint32_t KT = a.Kp / LK;
+    if (KT > 0) {
+        stage_issue<L>(a, Abase, Bbase, wave, 0, s0);
+        if (KT > 1) stage_issue<L>(a, Abase, Bbase, wave, 1, s1);
+        if (KT > 2) stage_issue<L>(a, Abase, Bbase, wave, 2, s2);
+        if (KT > 2)
+            __builtin_amdgcn_s_waitcnt(wait_imm(2 * PER_WAVE, 15));
+        else if (KT > 1)
+            __builtin_amdgcn_s_waitcnt(wait_imm(PER_WAVE, 15));
+        else
+            __builtin_amdgcn_s_waitcnt(wait_imm(0, 15));
+        wg_barrier();
+        LFrag<L> fa, fb;
+        stage_frag<L>(s0, 0, wm, wn, lane, fa);
+        __builtin_amdgcn_s_waitcnt(wait_imm(63, 0));
+        // a stage that has a successor (kt + 1 < KT)
+        // (the scheduling barriers keep each group of LDS reads ahead of the MFMAs they overlap)
+        auto body = [&](uint32_t kt, uint8_t* cur, const uint8_t* nxt) {
+            stage_frag<L>(cur, 1, wm, wn, lane, fb);
+            __builtin_amdgcn_sched_barrier(0);
+            frag_mfma<L>(fa, acc);
+            __builtin_amdgcn_sched_barrier(0);
+            __builtin_amdgcn_s_waitcnt(wait_imm(63, 0));             // (kt, ks1) in registers
+            if (kt + 2 < KT)
+                __builtin_amdgcn_s_waitcnt(wait_imm(PER_WAVE, 15));  // stage kt+1 copied
+            else
+                __builtin_amdgcn_s_waitcnt(wait_imm(0, 15));
+            wg_barrier();
+            if (kt + 3 < KT) stage_issue<L>(a, Abase, Bbase, wave, kt + 3, cur);
+            stage_frag<L>(nxt, 0, wm, wn, lane, fa);
+            __builtin_amdgcn_sched_barrier(0);
+            frag_mfma<L>(fb, acc);
+            // the (kt+1, ks0) reads are long done here; saying so keeps the next group of reads
+            // from being followed by a full wait before the MFMAs that do not use them
+            __builtin_amdgcn_sched_barrier(0);
+            __builtin_amdgcn_s_waitcnt(wait_imm(63, 0));
+        };
+        for (uint32_t kt = 0; kt + 1 < KT; kt += 3) {
+            body(kt, s0, s1);
+            if (kt + 2 < KT) body(kt + 1, s1, s2);
+            if (kt + 3 < KT) body(kt + 2, s2, s0);
+        }
+        // the last stage (all copies are complete: nothing left in flight)
+        const uint32_t ls = (KT - 1) % 3;
+        const uint8_t* cur = ls == 0 ? s0 : (ls == 1 ? s1 : s2);
+        stage_frag<L>(cur, 1, wm, wn, lane, fb);
+        frag_mfma<L>(fa, acc);
+        frag_mfma<L>(fb, acc);
+    }
+    limb_store<L>(a, acc, m, m0 + wm * 32, n0 + wn * 64, lane);
+}
+
+// limb_tile3's pipeline with eight waves per workgroup: the same 64 x 128 tile and LDS ring, each
+// wave owning one 32 x 32 block (wm = wave & 1: rows, wn = wave >> 1: columns), so the
+// accumulators (2L-1 x 16 registers) leave room for two waves per SIMD and one wave's barrier
+// waits and epilogue overlap the other's MFMAs.  The 12L copy blocks of a stage are dealt
+// round-robin over the eight waves (blk = wave + 8 i).
+template <int L>
+__device__ __forceinline__ void stage_issue8(const LimbArgs& a, const int8_t* Abase, const int8_t* Bbase, int wave,
+                                             uint32_t kt, uint8_t* st) {
+    constexpr int SA = L * LM * 64, NBLK = 12 * L, PER = (NBLK + 7) / 8;
+    const size_t k0 = (size_t)kt * LK;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int blk = wave + 8 * i;
+        if (blk >= NBLK) break;
+        const int8_t* src;
+        uint8_t* dst;
+        if (blk < 4 * L) {
+            const int s = blk >> 2, g = blk & 3;
+            src = Abase + s * a.a_plane + (size_t)(16 * g) * a.Kp + k0;
+            dst = st + s * (LM * 64) + g * 1024;
+        } else {
+            const int s = (blk - 4 * L) >> 3, g = (blk - 4 * L) & 7;
+            src = Bbase + s * a.b_plane + (size_t)(16 * g) * a.Kp + k0;
+            dst = st + SA + s * (LN * 64) + g * 1024;
+        }
+        __builtin_amdgcn_global_load_lds((const void*)src, (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
+    }
+}
+
+template <int L>
+struct LFrag1 {
+    v4i a[L];
+    v4i b[L];
+};
+
+template <int L>
+__device__ __forceinline__ void stage_frag1(const uint8_t* st, int ks, int wm, int wn, int lane, LFrag1<L>& f) {
+    constexpr int SA = L * LM * 64;
+    const int lr = lane & 31, q = ks * 2 + (lane >> 5);
+    const int ra = wm * 32 + lr, rb = wn * 32 + lr;
+#pragma unroll
+    for (int s = 0; s < L; ++s) f.a[s] = *(const v4i*)(st + s * (LM * 64) + ra * 64 + 16 * (q ^ swz(ra)));
+#pragma unroll
+    for (int s = 0; s < L; ++s) f.b[s] = *(const v4i*)(st + SA + s * (LN * 64) + rb * 64 + 16 * (q ^ swz(rb)));
+}
+
+template <int L>
+__device__ __forceinline__ void frag_mfma1(const LFrag1<L>& f, v16i (&acc)[1][2 * L - 1]) {
+#pragma unroll
+    for (int s = 0; s < L; ++s)
+#pragma unroll
+        for (int t = 0; t < L; ++t)
+            acc[0][s + t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(f.a[s], f.b[t], acc[0][s + t], 0, 0, 0);
+}
+
+template <int L>
+__device__ __forceinline__ void limb_tile4(const LimbArgs& a, uint32_t tm, uint32_t tn) {
+    constexpr int SA = L * LM * 64, SB = L * LN * 64, STAGE = SA + SB;
+    constexpr int ND = 2 * L - 1, NBLK = 12 * L;
+    // DMA instructions per wave per stage: waves below NBLK % 8 issue one more
+    constexpr int HI = (NBLK + 7) / 8, LO = NBLK / 8, NHI = NBLK % 8 == 0 ? 8 : NBLK % 8;
+    __shared__ __attribute__((aligned(1024))) uint8_t s0[STAGE];
+    __shared__ __attribute__((aligned(1024))) uint8_t s1[STAGE];
+    __shared__ __attribute__((aligned(1024))) uint8_t s2[STAGE];
+    const uint32_t m = a.m_dev ? min(*a.m_dev, a.m) : a.m;
+    const uint32_t m0 = tm * LM, n0 = tn * LN;
+    if (m0 >= m) return;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm = wave & 1, wn = wave >> 1;
+    const bool hi = wave < NHI;
+
+    v16i acc[1][ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[0][d][r] = 0;
+
+    const int br = lane >> 2, bq = (lane & 3) ^ swz(br);
+    const int8_t* Abase = a.A8 + (size_t)(m0 + br) * a.Kp + 16 * bq;
+    const int8_t* Bbase = a.B8 + (size_t)(n0 + br) * a.Kp + 16 * bq;
+    // wait until at most `stages` stages of this wave's copies are in flight
+    auto wait_stages = [&](int stages) {
+        if (hi) {
+            if (stages == 2) __builtin_amdgcn_s_waitcnt(wait_imm(2 * HI, 15));
+            else if (stages == 1) __builtin_amdgcn_s_waitcnt(wait_imm(HI, 15));
+            else __builtin_amdgcn_s_waitcnt(wait_imm(0, 15));
+        } else {
+            if (stages == 2) __builtin_amdgcn_s_waitcnt(wait_imm(2 * LO, 15));
+            else if (stages == 1) __builtin_amdgcn_s_waitcnt(wait_imm(LO, 15));
+            else __builtin_amdgcn_s_waitcnt(wait_imm(0, 15));
+        }
+    };
+    const uint32_t KT = a.Kp / LK;
+    if (KT > 0) {
+        stage_issue8<L>(a, Abase, Bbase, wave, 0, s0);
+        if (KT > 1) stage_issue8<L>(a, Abase, Bbase, wave, 1, s1);
+        if (KT > 2) stage_issue8<L>(a, Abase, Bbase, wave, 2, s2);
+        wait_stages(KT > 2 ? 2 : (KT > 1 ? 1 : 0));
+        wg_barrier();
+        LFrag1<L> fa, fb;
+        stage_frag1<L>(s0, 0, wm, wn, lane, fa);
+        __builtin_amdgcn_s_waitcnt(wait_imm(63, 0));
+        auto body = [&](uint32_t kt, uint8_t* cur, const uint8_t* nxt) {
+            stage_frag1<L>(cur, 1, wm, wn, lane, fb);
+            __builtin_amdgcn_sched_barrier(0);
+            frag_mfma1<L>(fa, acc);
+            __builtin_amdgcn_sched_barrier(0);
+            __builtin_amdgcn_s_waitcnt(wait_imm(63, 0));
+            wait_stages(kt + 2 < KT ? 1 : 0);
+            wg_barrier();
+            if (kt + 3 < KT) stage_issue8<L>(a, Abase, Bbase, wave, kt + 3, cur);
+            stage_frag1<L>(nxt, 0, wm, wn, lane, fa);
+            __builtin_amdgcn_sched_barrier(0);
+            frag_mfma1<L>(fb, acc);
+            __builtin_amdgcn_sched_barrier(0);
+            __builtin_amdgcn_s_waitcnt(wait_imm(63, 0));
+        };
+        for (uint32_t kt = 0; kt + 1 < KT; kt += 3) {
+            body(kt, s0, s1);
+            if (kt + 2 < KT) body(kt + 1, s1, s2);
+            if (kt + 3 < KT) body(kt + 2, s2, s0);
+        }
+        const uint32_t ls = (KT - 1) % 3;
+        const uint8_t* cur = ls == 0 ? s0 : (ls == 1 ? s1 : s2);
+        stage_frag1<L>(cur, 1, wm, wn, lane, fb);
+        frag_mfma1<L>(fa, acc);
+        frag_mfma1<L>(fb, acc);
+    }
+    limb_store<L, 1>(a, acc, m, m0 + wm * 32, n0 + wn * 32, lane);
+}
+
+template <int L>
+__global__ __launch_bounds__(512, 1) void limb4_kernel(LimbArgs a) {
+    uint32_t tm, tn;
+    tile_of(blockIdx.x, a.ntm, a.ntn, tm, tn);
+    limb_tile4<L>(a, tm, tn);
+}
+
+template <int L>
+__global__ __launch_bounds__(512, 1) void limb4_batch_kernel(const LimbArgs* jobs, const uint32_t* prefix,
+                                                             uint32_t njobs) {
+    const uint32_t j = job_of(prefix, njobs, blockIdx.x);
+    const LimbArgs a = jobs[j];
+    const uint32_t local = blockIdx.x - prefix[j];
+    limb_tile4<L>(a, local / a.ntn, local % a.ntn);
 }
 
 template <int L>
@@ -528,6 +808,22 @@ __global__ __launch_bounds__(256, 1) void limb2_batch_kernel(const LimbArgs* job
     const LimbArgs a = jobs[j];
     const uint32_t local = blockIdx.x - prefix[j];
     limb_tile2<L>(a, local / a.ntn, local % a.ntn);
+}
+
+template <int L>
+__global__ __launch_bounds__(256, 1) void limb3_kernel(LimbArgs a) {
+    uint32_t tm, tn;
+    tile_of(blockIdx.x, a.ntm, a.ntn, tm, tn);
+    limb_tile3<L>(a, tm, tn);
+}
+
+template <int L>
+__global__ __launch_bounds__(256, 1) void limb3_batch_kernel(const LimbArgs* jobs, const uint32_t* prefix,
+                                                             uint32_t njobs) {
+    const uint32_t j = job_of(prefix, njobs, blockIdx.x);
+    const LimbArgs a = jobs[j];
+    const uint32_t local = blockIdx.x - prefix[j];
+    limb_tile3<L>(a, local / a.ntn, local % a.ntn);
 }
 
 template <int L>
@@ -563,9 +859,11 @@ void limb_attrs() {
 }
 
 // Limb kernel choice (hx_gemm_opts.tile for the limb variant): 1 = register-staged copy
-// (limb_kernel), 2 = direct-to-LDS copy ring (limb2_kernel); 0 = the default.
+// (limb_kernel), 2 = direct-to-LDS copy ring (limb2_kernel), 3 = the ring with the LDS reads
+// pipelined against the MFMAs (limb3_kernel), 4 = limb3's pipeline on eight waves
+// (limb4_kernel); 0 = the default.
 constexpr int kLimbDefault = 2;
-inline int limb_version(int tile) { return tile == 1 || tile == 2 ? tile : kLimbDefault; }
+inline int limb_version(int tile) { return tile >= 1 && tile <= 4 ? tile : kLimbDefault; }
 
 template <int L>
 void launch_limb(const LimbArgs& la, hipStream_t s, int ver) {
@@ -573,6 +871,16 @@ void launch_limb(const LimbArgs& la, hipStream_t s, int ver) {
     if (ver == 2) {
         limb2_kernel<L><<<blocks, 256, 0, s>>>(la);
         after_launch("limb2_kernel");
+        return;
+    }
+    if (ver == 3) {
+        limb3_kernel<L><<<blocks, 256, 0, s>>>(la);
+        after_launch("limb3_kernel");
+        return;
+    }
+    if (ver == 4) {
+        limb4_kernel<L><<<blocks, 512, 0, s>>>(la);
+        after_launch("limb4_kernel");
         return;
     }
     limb_attrs<L>();
@@ -619,8 +927,6 @@ void gemm_limb(const hx_modctx& ctx, const GemmArgs& g0, hipStream_t s, void* ws
     }
     const uint32_t Mp = round_up(g0.m, LM), Np = round_up(g0.n, LN);
     const uint32_t p = ctx.p, half = (p - 1) / 2;
-    const uint64_t two57 = 1ull << 57;
-    const int64_t off = (int64_t)(((two57 + p - 1) / p) * (uint64_t)p);
     GemmArgs g = g0;
     uint32_t k0 = 0;
     do {
@@ -679,7 +985,7 @@ void gemm_limb(const hx_modctx& ctx, const GemmArgs& g0, hipStream_t s, void* ws
         la.cors = g.cors;
         la.cocs = g.cocs;
         la.op = g.op;
-        la.off = off;
+        limb_epilogue_consts(p, la);
         la.M = devmod(ctx);
         const bool last = k0 + kc >= g.k;
         const int ver = limb_version(tile);
@@ -701,8 +1007,12 @@ void gemm_limb(const hx_modctx& ctx, const GemmArgs& g0, hipStream_t s, void* ws
     } while (k0 < g.k);
 }
 
+// The automatic choice: the limb route once the product is big enough to pay for the two
+// operand splits, and deep enough (k >= 64, one full LDS stage) for the MFMA work to outweigh the
+// recombination epilogue; thinner products (the k = 32 Schur updates of the eliminations) run
+// faster on the plain kernel.
 int gemm_auto_variant(const hx_modctx& ctx, uint32_t m, uint32_t n, uint32_t k) {
-    if (ctx.limbs == 0) return HX_GEMM_PLAIN;
+    if (ctx.limbs == 0 || k < 64) return HX_GEMM_PLAIN;
     const double work = (double)m * n * k;
     return work >= 4.0e6 ? HX_GEMM_LIMB : HX_GEMM_PLAIN;
 }
@@ -765,7 +1075,6 @@ void* gemm_batch_create(const hx_modctx& ctx, const hx_gemm_job* jobs, uint32_t 
         struct Pending { size_t a_off, b_off; uint32_t Mp, Np, Kp; };
         std::vector<Pending> pend;
         const uint32_t p = ctx.p, half = (p - 1) / 2;
-        const int64_t off = (int64_t)((((1ull << 57) + p - 1) / p) * (uint64_t)p);
         for (uint32_t i = 0; i < njobs; ++i) {
             const hx_gemm_job& j = jobs[i];
             HX_CHECK(j.op == HX_OP_SET || j.op == HX_OP_ADD || j.op == HX_OP_SUB, "gemm batch: bad op");
@@ -798,7 +1107,8 @@ void* gemm_batch_create(const hx_modctx& ctx, const hx_gemm_job* jobs, uint32_t 
             la.ntm = Mp / LM; la.ntn = Np / LN;
             la.Cin = j.C; la.cirs = j.ldc; la.cics = 1;
             la.Cout = j.C; la.cors = j.ldc; la.cocs = 1;
-            la.op = j.op; la.off = off; la.M = devmod(ctx);
+            la.op = j.op; la.M = devmod(ctx);
+            limb_epilogue_consts(p, la);
             limb.push_back(la);
             lp.push_back(lp.back() + la.ntm * la.ntn);
             if (Kp > 0) {
@@ -861,7 +1171,14 @@ void gemm_batch_run(void* vb, hipStream_t s) {
         after_launch("prep_batch_kernel");
     }
     if (b->tiles_limb) {
-        if (kLimbDefault == 2) {
+        if (kLimbDefault == 3) {
+            switch (b->ctx.limbs) {
+                case 1: limb3_batch_kernel<1><<<b->tiles_limb, 256, 0, s>>>(b->d_limb, pre_limb, b->n_limb); break;
+                case 2: limb3_batch_kernel<2><<<b->tiles_limb, 256, 0, s>>>(b->d_limb, pre_limb, b->n_limb); break;
+                case 3: limb3_batch_kernel<3><<<b->tiles_limb, 256, 0, s>>>(b->d_limb, pre_limb, b->n_limb); break;
+                default: limb3_batch_kernel<4><<<b->tiles_limb, 256, 0, s>>>(b->d_limb, pre_limb, b->n_limb); break;
+            }
+        } else if (kLimbDefault == 2) {
             switch (b->ctx.limbs) {
                 case 1: limb2_batch_kernel<1><<<b->tiles_limb, 256, 0, s>>>(b->d_limb, pre_limb, b->n_limb); break;
                 case 2: limb2_batch_kernel<2><<<b->tiles_limb, 256, 0, s>>>(b->d_limb, pre_limb, b->n_limb); break;

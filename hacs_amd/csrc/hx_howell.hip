@@ -14,6 +14,7 @@
 // The Howell form is unique (ENGINE_API.md Sec. 13.5), so this equals the reference's echelon-first
 // route (echelon_blocked + howell_pass) entry for entry.
 #include <algorithm>
+#include <mutex>
 #include <numeric>
 #include <vector>
 
@@ -149,6 +150,270 @@ __device__ void howell_one(const HJob& job, uint32_t k) {
     __syncthreads();
 }
 
+// ------------------------------------------------------------------------------------------
+// Panel-blocked pass (jobs with m + n <= kBlockedRows).  The same pass, reordered:
+//  * per panel of HW columns, the steps of the pass run on the panel columns only, in LDS
+//    (pan: the active rows' panel entries), recording for every active row the multiplier q of
+//    every step (qm) -- LU's L, with the closure rows inside it: a closure row made at step s
+//    has trailing part 2^(k-v_s) T_s, i.e. the row "0 - (-(2^(k-v_s))) T_s", so it starts from
+//    zero with qm = -2^(k-v_s) at step s and then collects its own later multipliers;
+//  * then the trailing columns are brought up to date in one sweep, one thread per column:
+//    the panel's pivot rows in step order, T_t = ui_t (row_t - sum_{s<t} qm[row_t][s] T_s)
+//    (kept in registers), and every other active row gets row -= sum_t qm[row][t] T_t.
+//  Every trailing entry is read and written once per panel instead of once per column.
+//  Reduce-above is blocked the same way: per block of HW pivots (left to right) the
+//  multipliers of every row above come from a small triangular recurrence on the block's
+//  pivot columns, then one sweep over the columns applies the block (block rows are read
+//  before they are updated, so the update uses their values from before the block, as the
+//  column-at-a-time order does).
+// ------------------------------------------------------------------------------------------
+constexpr uint32_t HW = 16;            // panel width / reduce-above block
+constexpr uint32_t kBlockedRows = 1024;
+constexpr uint32_t NONE = 0xffffffffu;
+
+__host__ __device__ constexpr size_t blocked_lds_bytes(uint32_t rc) {
+    return (size_t)rc * HW * 4 * 2 + (size_t)rc * 4 * 2 + rc;
+}
+
+__device__ void howell_blocked(const HJob& job, uint32_t k, uint32_t rc, uint32_t* lds) {
+    __shared__ uint32_t s_red[2][HT / 64];
+    __shared__ uint32_t s_pid[HW], s_ui[HW], s_v[HW], s_c[HW];
+    __shared__ uint32_t s_hb[HW * HW];
+    __shared__ uint32_t s_T[HW * HT];          // the panel's pivot rows over 256 trailing columns
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t m = job.m, n = job.n;
+    const uint32_t mask = k >= 32 ? 0xffffffffu : ((1u << k) - 1u);
+    if (n == 0) {
+        if (tid == 0) *job.h = 0;
+        return;
+    }
+    uint32_t* pan = lds;                       // [rc][HW] panel entries of active rows
+    uint32_t* qm = pan + (size_t)rc * HW;      // [rc][HW] multipliers (later: reduce-above q's)
+    uint32_t* act = qm + (size_t)rc * HW;      // active order: row ids, NONE once chosen
+    uint32_t* prow = act + rc;                 // pivot row ids in pivot order
+    uint8_t* nz = (uint8_t*)(prow + rc);       // row has a non-zero multiplier in this panel
+    uint32_t* rows = job.work;                 // row id i at rows + i * n
+    for (uint64_t idx = tid; idx < (uint64_t)m * n; idx += HT) {
+        const uint32_t i = (uint32_t)(idx / n), j = (uint32_t)(idx % n);
+        rows[(size_t)i * n + j] = job.A[(int64_t)i * job.lda + j] & mask;
+    }
+    for (uint32_t i = tid; i < m; i += HT) act[i] = i;
+    __syncthreads();
+    uint32_t na = m, h = 0, ncl = 0;
+    for (uint32_t c0 = 0; c0 < n; c0 += HW) {
+        const uint32_t wc = min(HW, n - c0), c1 = c0 + wc;
+        for (uint32_t idx = tid; idx < na * HW; idx += HT) {
+            const uint32_t pos = idx / HW, t = idx % HW, id = act[pos];
+            if (id == NONE) continue;
+            pan[id * HW + t] = t < wc ? rows[(size_t)id * n + c0 + t] : 0u;
+            qm[id * HW + t] = 0;
+            if (t == 0) nz[id] = 0;
+        }
+        if (tid < HW) s_pid[tid] = NONE;
+        const uint32_t ncl0 = ncl;
+        __syncthreads();
+        // (A) the pass on the panel columns
+        for (uint32_t t = 0; t < wc; ++t) {
+            uint32_t best = NONE;
+            for (uint32_t pos = tid; pos < na; pos += HT) {
+                const uint32_t id = act[pos];
+                if (id == NONE) continue;
+                const uint32_t x = pan[id * HW + t] & mask;
+                if (x != 0) best = min(best, ((uint32_t)__builtin_ctz(x) << 24) | pos);
+            }
+            best = wave_min_u32(best);
+            if (lane == 0) s_red[t & 1][wave] = best;
+            __syncthreads();
+            best = min(min(s_red[t & 1][0], s_red[t & 1][1]), min(s_red[t & 1][2], s_red[t & 1][3]));
+            if (best == NONE) continue;
+            const uint32_t bpos = best & 0xffffffu, v = best >> 24, pid = act[bpos];
+            uint32_t* P = pan + pid * HW;
+            const uint32_t ui = inv_odd((P[t] & mask) >> v);
+            __syncthreads();                                   // everyone has read act[bpos] and P[t]
+            if (tid < wc - t) P[t + tid] *= ui;
+            if (tid == 0) {
+                act[bpos] = NONE;
+                s_pid[t] = pid;
+                s_ui[t] = ui;
+                prow[h] = pid;
+                job.pivots[h] = c0 + t;
+                job.pivot_v[h] = v;
+            }
+            __syncthreads();
+            for (uint32_t pos = tid; pos < na; pos += HT) {
+                const uint32_t id = act[pos];
+                if (id == NONE) continue;
+                uint32_t* R = pan + id * HW;
+                const uint32_t q = (R[t] & mask) >> v;
+                if (q == 0) continue;
+                qm[id * HW + t] = q;
+                nz[id] = 1;
+                for (uint32_t t2 = t; t2 < wc; ++t2) R[t2] -= q * P[t2];
+            }
+            if (v > 0) {                                       // closure row 2^(k-v) P at the bottom
+                const uint32_t cid = m + ncl, sh = k - v;      // 1 <= sh <= 31
+                if (tid < HW) {
+                    pan[cid * HW + tid] = (tid <= t || tid >= wc) ? 0u : (P[tid] << sh);
+                    qm[cid * HW + tid] = tid == t ? (0u - (1u << sh)) : 0u;
+                }
+                if (tid == 0) {
+                    act[na] = cid;
+                    nz[cid] = 1;
+                }
+                ++na;
+                ++ncl;
+            }
+            ++h;
+            __syncthreads();
+        }
+        // the panel's pivot rows keep their panel entries; new closure rows start from zero
+        for (uint32_t idx = tid; idx < HW * HW; idx += HT) {
+            const uint32_t t = idx / HW, t2 = idx % HW, pid = s_pid[t];
+            if (pid != NONE && t2 < wc) rows[(size_t)pid * n + c0 + t2] = pan[pid * HW + t2];
+        }
+        const uint32_t tw = n - c1;
+        for (uint64_t idx = tid; idx < (uint64_t)(ncl - ncl0) * tw; idx += HT)
+            rows[(size_t)(m + ncl0 + idx / tw) * n + c1 + idx % tw] = 0;
+        __syncthreads();
+        // (B) the trailing columns, 256 at a time: (B1) one thread per column brings the panel's
+        // pivot rows up to date (T_t, in step order) into s_T and back to their rows; (B2) every
+        // wave sweeps a quarter of the active rows over the 256 columns, 4 columns per lane (64
+        // apart: coalesced) and 8 rows at a time, so 32 independent loads are in flight per lane.
+        for (uint32_t j0 = c1; j0 < n; j0 += HT) {
+            const uint32_t j = j0 + tid;
+            if (j < n) {
+                uint32_t T[HW];
+#pragma unroll
+                for (uint32_t t = 0; t < HW; ++t) {
+                    T[t] = 0;
+                    const uint32_t pid = s_pid[t];
+                    if (pid == NONE) continue;
+                    uint32_t x = rows[(size_t)pid * n + j];
+                    const uint32_t* qr = qm + pid * HW;
+#pragma unroll
+                    for (uint32_t s2 = 0; s2 < t; ++s2) x -= qr[s2] * T[s2];
+                    x *= s_ui[t];
+                    T[t] = x;
+                    rows[(size_t)pid * n + j] = x;
+                }
+#pragma unroll
+                for (uint32_t t = 0; t < HW; ++t) s_T[t * HT + tid] = T[t];
+            }
+            __syncthreads();
+            uint32_t Tr[HW][4];
+#pragma unroll
+            for (uint32_t t = 0; t < HW; ++t)
+#pragma unroll
+                for (uint32_t u = 0; u < 4; ++u) Tr[t][u] = s_T[t * HT + lane + 64 * u];
+            bool in[4];
+#pragma unroll
+            for (uint32_t u = 0; u < 4; ++u) in[u] = j0 + lane + 64 * u < n;
+            for (uint32_t pos = wave * 8; pos < na; pos += 32) {
+                uint32_t ids[8], xs[8][4];
+#pragma unroll
+                for (int r = 0; r < 8; ++r) {
+                    const uint32_t id = pos + r < na ? act[pos + r] : NONE;
+                    ids[r] = (id != NONE && nz[id]) ? id : NONE;
+                }
+#pragma unroll
+                for (int r = 0; r < 8; ++r)
+#pragma unroll
+                    for (uint32_t u = 0; u < 4; ++u)
+                        if (ids[r] != NONE && in[u]) xs[r][u] = rows[(size_t)ids[r] * n + j0 + lane + 64 * u];
+#pragma unroll
+                for (int r = 0; r < 8; ++r) {
+                    if (ids[r] == NONE) continue;
+                    const uint32_t* qr = qm + ids[r] * HW;
+                    uint32_t q[HW];
+#pragma unroll
+                    for (uint32_t t = 0; t < HW; ++t) q[t] = qr[t];
+#pragma unroll
+                    for (uint32_t u = 0; u < 4; ++u) {
+                        if (!in[u]) continue;
+                        uint32_t x = xs[r][u];
+#pragma unroll
+                        for (uint32_t t = 0; t < HW; ++t) x -= q[t] * Tr[t][u];
+                        rows[(size_t)ids[r] * n + j0 + lane + 64 * u] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // H rows in pivot order (masked).  Left of its pivot column a pivot row is zero mod 2^k; its
+    // stored entries there may be stale (panel entries are written back for pivot rows only).
+    for (uint64_t idx = tid; idx < (uint64_t)h * n; idx += HT) {
+        const uint32_t i = (uint32_t)(idx / n), j = (uint32_t)(idx % n);
+        job.H[(size_t)i * n + j] = j < job.pivots[i] ? 0u : rows[(size_t)prow[i] * n + j] & mask;
+    }
+    __syncthreads();
+    // blocked reduce above
+    for (uint32_t i0 = 0; i0 < h; i0 += HW) {
+        const uint32_t bw = min(HW, h - i0), i1 = i0 + bw;
+        if (tid < HW) {
+            s_c[tid] = tid < bw ? job.pivots[i0 + tid] : 0u;
+            s_v[tid] = tid < bw ? job.pivot_v[i0 + tid] : 0u;
+        }
+        __syncthreads();
+        for (uint32_t idx = tid; idx < HW * HW; idx += HT) {
+            const uint32_t t = idx / HW, t2 = idx % HW;
+            s_hb[idx] = (t < bw && t2 < bw) ? job.H[(size_t)(i0 + t) * n + s_c[t2]] : 0u;
+        }
+        __syncthreads();
+        for (uint32_t ip = tid; ip < i1; ip += HT) {
+            uint32_t cur[HW];
+#pragma unroll
+            for (uint32_t t2 = 0; t2 < HW; ++t2) cur[t2] = t2 < bw ? job.H[(size_t)ip * n + s_c[t2]] : 0u;
+#pragma unroll 1
+            for (uint32_t t = 0; t < HW; ++t) {
+                uint32_t q = 0;
+                if (t < bw && i0 + t > ip) q = (cur[t] & mask) >> s_v[t];
+#pragma unroll
+                for (uint32_t t2 = t + 1; t2 < HW; ++t2) cur[t2] -= q * s_hb[t * HW + t2];
+                qm[ip * HW + t] = q;
+            }
+        }
+        __syncthreads();
+        for (uint32_t j = s_c[0] + tid; j < n; j += HT) {
+            uint32_t Hr[HW];
+#pragma unroll
+            for (uint32_t t = 0; t < HW; ++t) Hr[t] = t < bw ? job.H[(size_t)(i0 + t) * n + j] : 0u;
+            for (uint32_t ip0 = 0; ip0 < i1; ip0 += 8) {
+                uint32_t xs[8];
+#pragma unroll
+                for (uint32_t u = 0; u < 8; ++u)
+                    if (ip0 + u < i1) xs[u] = job.H[(size_t)(ip0 + u) * n + j];
+#pragma unroll
+                for (uint32_t u = 0; u < 8; ++u) {
+                    if (ip0 + u >= i1) continue;
+                    const uint32_t* qr = qm + (ip0 + u) * HW;
+                    uint32_t x = xs[u];
+#pragma unroll
+                    for (uint32_t t = 0; t < HW; ++t) x -= qr[t] * Hr[t];
+                    job.H[(size_t)(ip0 + u) * n + j] = x & mask;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) *job.h = h;
+    __syncthreads();
+}
+
+// Persistent: one workgroup per job of this row-capacity class, jobs from the class's counter.
+__global__ __launch_bounds__(HT) void howell_blocked_kernel(HArgs a, uint32_t rc) {
+    extern __shared__ __attribute__((aligned(16))) uint32_t hlds[];
+    __shared__ uint32_t s_job;
+    for (;;) {
+        if (threadIdx.x == 0) s_job = atomicAdd(a.next, 1u);
+        __syncthreads();
+        const uint32_t jj = s_job;
+        __syncthreads();
+        if (jj >= a.njobs) break;
+        howell_blocked(a.jobs[a.order[jj]], a.k, rc, hlds);
+    }
+}
+
 __global__ __launch_bounds__(HT) void howell_batch_kernel(HArgs a) {
     __shared__ uint32_t s_job;
     for (;;) {
@@ -161,22 +426,34 @@ __global__ __launch_bounds__(HT) void howell_batch_kernel(HArgs a) {
     }
 }
 
-__global__ void zero_counter_kernel(uint32_t* x) {
-    if (threadIdx.x == 0) x[0] = 0;
+__global__ void zero_counters_kernel(uint32_t* x, uint32_t n) {
+    if (threadIdx.x < n) x[threadIdx.x] = 0;
 }
 
-struct HBatch {
-    uint32_t k = 0, njobs = 0, grid = 0;
-    HJob* d_jobs = nullptr;
+// Job classes by row capacity m + n: three blocked classes (LDS sized to the class) and the
+// unblocked kernel for the rest.  Each class is one persistent launch with its own counter.
+constexpr int NCLASS = 4;
+constexpr uint32_t kClassRows[NCLASS - 1] = {256, 512, kBlockedRows};
+
+struct HClass {
+    uint32_t njobs = 0, grid = 0, rc = 0;
+    size_t lds = 0;
     uint32_t* d_order = nullptr;
-    uint32_t* d_next = nullptr;
+};
+
+struct HBatch {
+    uint32_t k = 0, njobs = 0;
+    HJob* d_jobs = nullptr;
+    uint32_t* d_next = nullptr;          // one counter per class
+    HClass cls[NCLASS];
 };
 
 void destroy(HBatch* b) {
     if (!b) return;
     if (b->d_jobs) (void)hipFree(b->d_jobs);
-    if (b->d_order) (void)hipFree(b->d_order);
     if (b->d_next) (void)hipFree(b->d_next);
+    for (HClass& c : b->cls)
+        if (c.d_order) (void)hipFree(c.d_order);
     delete b;
 }
 
@@ -214,17 +491,46 @@ HX_API int hx_howell_batch_create(uint32_t k, const hx_howell_job* jobs, uint32_
             std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
                 return (double)hj[x].m * hj[x].n * hj[x].n > (double)hj[y].m * hj[y].n * hj[y].n;
             });
-            HX_HIP(hipMalloc(&b->d_jobs, std::max<size_t>(njobs, 1) * sizeof(HJob)));
-            HX_HIP(hipMalloc(&b->d_order, std::max<size_t>(njobs, 1) * sizeof(uint32_t)));
-            HX_HIP(hipMalloc(&b->d_next, sizeof(uint32_t)));
-            if (njobs) {
-                HX_HIP(hipMemcpy(b->d_jobs, hj.data(), njobs * sizeof(HJob), hipMemcpyHostToDevice));
-                HX_HIP(hipMemcpy(b->d_order, order.data(), njobs * sizeof(uint32_t), hipMemcpyHostToDevice));
+            std::vector<uint32_t> per[NCLASS];
+            for (uint32_t i : order) {
+                const uint32_t rows = hj[i].m + hj[i].n;
+                int c = NCLASS - 1;
+                for (int t = 0; t < NCLASS - 1; ++t)
+                    if (rows <= kClassRows[t]) { c = t; break; }
+                per[c].push_back(i);
             }
+            HX_HIP(hipMalloc(&b->d_jobs, std::max<size_t>(njobs, 1) * sizeof(HJob)));
+            HX_HIP(hipMalloc(&b->d_next, NCLASS * sizeof(uint32_t)));
+            if (njobs) HX_HIP(hipMemcpy(b->d_jobs, hj.data(), njobs * sizeof(HJob), hipMemcpyHostToDevice));
             int cus = 256;
             hipDeviceProp_t prop;
             if (hipGetDeviceProperties(&prop, 0) == hipSuccess) cus = prop.multiProcessorCount;
-            b->grid = std::max<uint32_t>(1, std::min<uint32_t>(njobs, (uint32_t)cus * 4));
+            static std::once_flag once;
+            std::call_once(once, [] {
+                (void)hipFuncSetAttribute((const void*)howell_blocked_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)blocked_lds_bytes(kBlockedRows));
+            });
+            for (int c = 0; c < NCLASS; ++c) {
+                HClass& cl = b->cls[c];
+                cl.njobs = (uint32_t)per[c].size();
+                if (!cl.njobs) continue;
+                HX_HIP(hipMalloc(&cl.d_order, cl.njobs * sizeof(uint32_t)));
+                HX_HIP(hipMemcpy(cl.d_order, per[c].data(), cl.njobs * sizeof(uint32_t), hipMemcpyHostToDevice));
+                int per_cu = 4;
+                if (c < NCLASS - 1) {
+                    uint32_t rc = 0;
+                    for (uint32_t i : per[c]) rc = std::max(rc, hj[i].m + hj[i].n);
+                    cl.rc = rc;
+                    cl.lds = blocked_lds_bytes(rc);
+                    int nb = 0;
+                    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, howell_blocked_kernel, HT, cl.lds) == hipSuccess &&
+                        nb > 0)
+                        per_cu = nb;
+                    else
+                        per_cu = 1;
+                }
+                cl.grid = std::max<uint32_t>(1, std::min<uint32_t>(cl.njobs, (uint32_t)cus * per_cu));
+            }
         } catch (...) {
             destroy(b);
             throw;
@@ -238,11 +544,20 @@ HX_API int hx_howell_batch_run(hx_howell_batch vb, hx_stream s) {
         HX_CHECK(vb != nullptr, "NULL batch");
         HBatch* b = (HBatch*)vb;
         if (b->njobs == 0) return;
-        zero_counter_kernel<<<1, 64, 0, (hipStream_t)s>>>(b->d_next);
-        after_launch("zero_counter_kernel");
-        HArgs a{b->d_jobs, b->d_order, b->njobs, b->d_next, b->k};
-        howell_batch_kernel<<<b->grid, HT, 0, (hipStream_t)s>>>(a);
-        after_launch("howell_batch_kernel");
+        zero_counters_kernel<<<1, 64, 0, (hipStream_t)s>>>(b->d_next, NCLASS);
+        after_launch("zero_counters_kernel");
+        for (int c = NCLASS - 1; c >= 0; --c) {          // largest jobs first
+            const HClass& cl = b->cls[c];
+            if (!cl.njobs) continue;
+            HArgs a{b->d_jobs, cl.d_order, cl.njobs, b->d_next + c, b->k};
+            if (c == NCLASS - 1) {
+                howell_batch_kernel<<<cl.grid, HT, 0, (hipStream_t)s>>>(a);
+                after_launch("howell_batch_kernel");
+            } else {
+                howell_blocked_kernel<<<cl.grid, HT, cl.lds, (hipStream_t)s>>>(a, cl.rc);
+                after_launch("howell_blocked_kernel");
+            }
+        }
     });
 }
 

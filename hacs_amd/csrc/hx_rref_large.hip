@@ -54,45 +54,58 @@ __device__ __forceinline__ uint32_t wmin(uint32_t x) {
 }
 
 // The smallest-row rule on an LDS panel pan[u * ws + q] (rows in original order), division-free.
-// Returns k; s_u[t] / s_q[t] = local row / column of pivot t.  chosen[] is cleared here.
+// Four threads per row: thread (u, part) owns the columns q == part (mod 4) of row u, so a column's
+// elimination is spread over all 4 * rows threads.  The owner of column q scans it (it wrote it
+// last); one barrier per column.  Returns k; s_u[t] / s_q[t] = local row / column of pivot t;
+// chosen[u] = 1 for the rows picked.
+constexpr int RT = 1024;      // threads of the tournament kernels (rows <= RT / 4)
+
 template <class Ar>
 __device__ uint32_t lds_rule(uint32_t* pan, uint32_t rows, uint32_t wc, uint32_t ws, uint8_t* chosen, const Ar& ar,
                              uint32_t* s_u, uint32_t* s_q) {
-    __shared__ uint32_t s_cand[2][LT / 64];
+    __shared__ uint32_t s_cand[2][RT / 64];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, p = ar.p;
-    for (uint32_t u = tid; u < rows; u += LT) chosen[u] = 0;
-    __syncthreads();
+    const uint32_t u = tid >> 2, part = tid & 3;
+    const bool mine = u < rows;
+    uint32_t* row = pan + u * ws;
+    bool done = false;
     uint32_t k = 0;
+    static_assert(CH <= RT / 4 && GRP * LW <= RT / 4, "one row per four threads");
+    __syncthreads();                                   // the panel is in LDS
     for (uint32_t q = 0; q < wc; ++q) {
         uint32_t best = 0xffffffffu;
-        for (uint32_t u = tid; u < rows; u += LT)
-            if (!chosen[u] && pan[u * ws + q] != 0) { best = u; break; }
+        if (mine && part == (q & 3) && !done && row[q] != 0) best = u;
         best = wmin(best);
         if (lane == 0) s_cand[q & 1][wave] = best;
         __syncthreads();
-        const uint32_t bu = min(min(s_cand[q & 1][0], s_cand[q & 1][1]), min(s_cand[q & 1][2], s_cand[q & 1][3]));
+        uint32_t bu = s_cand[q & 1][0];
+#pragma unroll
+        for (int w = 1; w < RT / 64; ++w) bu = min(bu, s_cand[q & 1][w]);
         if (bu == 0xffffffffu) continue;
-        const uint32_t* prow = pan + bu * ws;
-        const uint32_t v = prow[q];
-        for (uint32_t u = tid; u < rows; u += LT) {
-            if (u == bu) { chosen[u] = 1; continue; }
-            if (chosen[u]) continue;
-            uint32_t* row = pan + u * ws;
-            const uint32_t f = row[q];
-            if (f == 0) continue;
-            const uint32_t fneg = p - f;
-            for (uint32_t q2 = q + 1; q2 < wc; ++q2) row[q2] = ar.dfree(v, row[q2], fneg, prow[q2]);
+        if (mine && !done) {
+            if (u == bu) {
+                done = true;
+            } else {
+                const uint32_t f = row[q];
+                if (f != 0) {
+                    const uint32_t* prow = pan + bu * ws;
+                    const uint32_t v = prow[q], fneg = p - f;
+                    for (uint32_t q2 = q + 1 + ((part - (q + 1)) & 3u); q2 < wc; q2 += 4)
+                        row[q2] = ar.dfree(v, row[q2], fneg, prow[q2]);
+                }
+            }
         }
         if (tid == 0) { s_u[k] = bu; s_q[k] = q; }
         ++k;
     }
+    if (mine && part == 0) chosen[u] = done ? 1 : 0;
     __syncthreads();
     return k;
 }
 
 // Level 0: chunk `blockIdx.x` of S's rows -> its candidate rows (ascending S indices).
 template <class Ar>
-__global__ __launch_bounds__(LT) void rrp_chunk_kernel(const uint32_t* S, int64_t ld, const LState* st, uint32_t wc,
+__global__ __launch_bounds__(RT) void rrp_chunk_kernel(const uint32_t* S, int64_t ld, const LState* st, uint32_t wc,
                                                        Ar ar, uint32_t* cand, uint32_t* cnt) {
     __shared__ uint32_t pan[CH * (LW + 1)];
     __shared__ uint8_t chosen[CH];
@@ -103,7 +116,7 @@ __global__ __launch_bounds__(LT) void rrp_chunk_kernel(const uint32_t* S, int64_
         return;
     }
     const uint32_t rows = min((uint32_t)CH, ma - r0), ws = wc | 1u;
-    for (uint32_t idx = threadIdx.x; idx < rows * wc; idx += LT) {
+    for (uint32_t idx = threadIdx.x; idx < rows * wc; idx += RT) {
         const uint32_t u = idx / wc, q = idx % wc;
         pan[u * ws + q] = S[(int64_t)(r0 + u) * ld + q];
     }
@@ -118,7 +131,7 @@ __global__ __launch_bounds__(LT) void rrp_chunk_kernel(const uint32_t* S, int64_
 
 // Merge: group `blockIdx.x` of up to GRP candidate lists -> one list; the final level writes P, J, k.
 template <class Ar>
-__global__ __launch_bounds__(LT) void rrp_merge_kernel(const uint32_t* S, int64_t ld, uint32_t wc, Ar ar,
+__global__ __launch_bounds__(RT) void rrp_merge_kernel(const uint32_t* S, int64_t ld, uint32_t wc, Ar ar,
                                                        const uint32_t* cand_in, const uint32_t* cnt_in, uint32_t nin,
                                                        uint32_t* cand_out, uint32_t* cnt_out, int final_level,
                                                        uint32_t* P, uint32_t* J, LState* st) {
@@ -135,10 +148,10 @@ __global__ __launch_bounds__(LT) void rrp_merge_kernel(const uint32_t* S, int64_
     __syncthreads();
     const uint32_t rows = s_off[ng], ws = wc | 1u;
     for (uint32_t i = 0; i < ng; ++i)
-        for (uint32_t t = threadIdx.x; t < s_off[i + 1] - s_off[i]; t += LT)
+        for (uint32_t t = threadIdx.x; t < s_off[i + 1] - s_off[i]; t += RT)
             rowid[s_off[i] + t] = cand_in[(g0 + i) * LW + t];
     __syncthreads();
-    for (uint32_t idx = threadIdx.x; idx < rows * wc; idx += LT) {
+    for (uint32_t idx = threadIdx.x; idx < rows * wc; idx += RT) {
         const uint32_t u = idx / wc, q = idx % wc;
         pan[u * ws + q] = S[(int64_t)rowid[u] * ld + q];
     }
@@ -160,12 +173,14 @@ __global__ __launch_bounds__(LT) void rrp_merge_kernel(const uint32_t* S, int64_
 }
 
 // X[:, slab] = S[P, J]^{-1} S[P, slab]; also the pivots / pivot rows of this panel (block 0).
+// Every workgroup reduces the same k x k pivot block beside its own SL-column slab: narrow slabs
+// spread the (multiply-latency-bound) elimination over many CUs.
+constexpr int SL = 32;
 template <class Ar>
 __global__ __launch_bounds__(LT) void panel_x_kernel(const uint32_t* S, int64_t ld, uint32_t na, const uint32_t* P,
                                                      const uint32_t* J, LState* st, Ar ar, uint32_t* X, int64_t ldx,
                                                      uint32_t* U, int64_t ldu, uint32_t c, const uint32_t* rows_orig,
                                                      uint32_t* pivots, uint32_t* pivot_rows) {
-    constexpr int SL = 256;
     __shared__ uint32_t B[LW * (LW + SL)];
     __shared__ uint32_t s_inv[LW], s_fp[LW], s_q[LW];
     const uint32_t k = st->k;
@@ -351,27 +366,29 @@ void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const
         init_state_kernel<<<gsz(std::max<size_t>(m, (size_t)mn * n)), 256, 0, s>>>(st, m, ro[0], U, (size_t)mn * n);
         after_launch("init_state_kernel");
         int cur = 0;
-        void* ws = scratch(limb_workspace_bytes(ctx, m, n, LW), 2);
-        const size_t ws_bytes = limb_workspace_bytes(ctx, m, n, LW);
+        // the trailing updates are k = 32 products: the automatic choice (the plain kernel)
+        const int tvar = gemm_auto_variant(ctx, m, n, LW);
+        const size_t ws_bytes = tvar == HX_GEMM_LIMB ? limb_workspace_bytes(ctx, m, n, LW) : 0;
+        void* ws = ws_bytes ? scratch(ws_bytes, 2) : nullptr;
         for (uint32_t c = 0; c < n; c += LW) {
             const uint32_t na = n - c, wc = std::min<uint32_t>(LW, na);
             const uint32_t* S = Sb[cur];
             // 1. tournament
-            rrp_chunk_kernel<Ar><<<nchunk, LT, 0, s>>>(S, n, st, wc, ar, cand[0], cnt[0]);
+            rrp_chunk_kernel<Ar><<<nchunk, RT, 0, s>>>(S, n, st, wc, ar, cand[0], cnt[0]);
             after_launch("rrp_chunk_kernel");
             uint32_t nin = nchunk;
             int lv = 0;
             do {
                 const uint32_t nout = (nin + GRP - 1) / GRP;
                 const int fin = nout == 1;
-                rrp_merge_kernel<Ar><<<nout, LT, 0, s>>>(S, n, wc, ar, cand[lv & 1], cnt[lv & 1], nin,
+                rrp_merge_kernel<Ar><<<nout, RT, 0, s>>>(S, n, wc, ar, cand[lv & 1], cnt[lv & 1], nin,
                                                          cand[(lv + 1) & 1], cnt[(lv + 1) & 1], fin, P, J, st);
                 after_launch("rrp_merge_kernel");
                 nin = nout;
                 ++lv;
             } while (nin > 1);
             // 2. X = S[P,J]^{-1} S[P,:] into the U store and the contiguous copy
-            panel_x_kernel<Ar><<<(na + 255) / 256, LT, 0, s>>>(S, n, na, P, J, st, ar, X, n, U, n, c, ro[cur], piv, prow);
+            panel_x_kernel<Ar><<<(na + SL - 1) / SL, LT, 0, s>>>(S, n, na, P, J, st, ar, X, n, U, n, c, ro[cur], piv, prow);
             after_launch("panel_x_kernel");
             if (na > wc) {
                 // 3. the next Schur complement, compacted
@@ -399,7 +416,7 @@ void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const
                 g.m_dev = &st->m_new;
                 g.k_dev = &st->k;
                 g.op = HX_OP_SUB;
-                gemm_dispatch(ctx, g, s, ctx.limbs ? HX_GEMM_LIMB : HX_GEMM_PLAIN, ws, ws_bytes, 0);
+                gemm_dispatch(ctx, g, s, tvar, ws, ws_bytes, 0);
                 cur ^= 1;
             }
             advance_kernel<<<1, 64, 0, s>>>(st);

@@ -52,3 +52,34 @@ def test_howell_k1_is_rref(hx_gpu, O):
     res = b.results()[0]
     r, piv, R = O.rref_naive(2, A)
     assert np.array_equal(res.H, R[:r]) and np.array_equal(res.pivots, piv) and not res.pivot_v.any()
+
+
+@pytest.mark.parametrize("k", [5, 16, 32])
+def test_howell_row_classes_and_panel_edges(hx_gpu, O, k):
+    """Every job class of the batch (m + n <= 256, <= 512, <= 1024 on the panel-blocked kernel,
+    larger on the column-at-a-time kernel) in ONE batch; widths around the 16-column panel
+    (3, 15, 16, 17, 33), zero columns (panel steps without a pivot), zero rows, sparse and
+    planted inputs."""
+    from hacs_amd import engine as E
+    rng = np.random.default_rng(700 + k)
+    top = (1 << k) if k < 32 else (1 << 32)
+    cases = []
+    for m, n in [(3, 16), (16, 3), (20, 17), (31, 32), (15, 15), (40, 33), (300, 200), (600, 400),
+                 (700, 400), (40, 600)]:
+        cases.append(rng.integers(0, top, (m, n), dtype=np.uint64).astype(np.uint32))
+        cases.append(_planted(rng, m, n, k))
+    S = rng.integers(0, top, (90, 70), dtype=np.uint64).astype(np.uint32)
+    S[rng.random((90, 70)) > 0.05] = 0
+    S[:, 10:30] = 0                      # whole panels without a pivot
+    S[40:50, :] = 0
+    cases.append(S)
+    mats = [E.to_device(A) for A in cases]
+    b = E.HowellBatch(k, mats)
+    for rep in range(2):                 # a second run over the same inputs gives the same forms
+        b.run()
+        torch.cuda.synchronize()
+        for A, res in zip(cases, b.results()):
+            H, piv, pv = O.howell_fast(k, A)
+            assert np.array_equal(res.pivots, piv), f"rep {rep} k={k} {A.shape}: pivots"
+            assert np.array_equal(res.pivot_v, pv), f"rep {rep} k={k} {A.shape}: pivot_v"
+            assert np.array_equal(res.H, H), f"rep {rep} k={k} {A.shape}: H"
