@@ -82,15 +82,29 @@ __device__ __forceinline__ void plain_tile(const GemmArgs& g, const DevMod& M, u
 #pragma unroll
         for (int j = 0; j < 4; ++j) { lo[i][j] = 0; hi[i][j] = 0; }
 
+    // Each thread loads the same 8 A rows (tid / PK + 8 t) and the same B column (tid % PN) at every
+    // k-step: their offsets, row gathers included, are computed once; per k-step only the column
+    // gather (one index) remains.
+    int64_t a_row[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        const uint32_t row = m0 + (tid + 256 * t) / PK;
+        a_row[t] = row < m ? (g.a_rows ? (int64_t)g.a_rows[row] : (int64_t)row) * g.ars : -1;
+    }
+    const uint32_t a_c = tid % PK, b_c = tid % PN, b_r0 = tid / PN;
+    const bool b_ok = n0 + b_c < n;
+    const uint32_t* Bcol = g.B + (int64_t)(n0 + b_c) * g.bcs;
     uint32_t ra[8], rb[8];
     auto fetch = [&](uint32_t k0) {
+        const uint32_t kc = k0 + a_c;
+        const bool k_ok = kc < k;
+        const int64_t acol = k_ok ? (g.a_cols ? (int64_t)g.a_cols[kc] : (int64_t)kc) * g.acs : 0;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) ra[t] = (k_ok && a_row[t] >= 0) ? g.A[a_row[t] + acol] : 0u;
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
-            const int e = tid + 256 * t;
-            const uint32_t r = e / PK, c = e % PK;
-            ra[t] = (m0 + r < m && k0 + c < k) ? ld_a(g, m0 + r, k0 + c) : 0u;
-            const uint32_t br = e / PN, bc = e % PN;
-            rb[t] = (k0 + br < k && n0 + bc < n) ? g.B[(int64_t)(k0 + br) * g.brs + (int64_t)(n0 + bc) * g.bcs] : 0u;
+            const uint32_t kr = k0 + b_r0 + 4 * t;          // (tid + 256 t) / PN
+            rb[t] = (b_ok && kr < k) ? Bcol[(int64_t)kr * g.brs] : 0u;
         }
     };
     auto stash = [&](int buf) {

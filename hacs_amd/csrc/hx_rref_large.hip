@@ -10,17 +10,21 @@
 //     of all rows above a candidate equals the span of the candidates above it, so
 //     RRP(S[:, 0..c]) = RRP(candidates[:, 0..c]) for every c: the pivots AND their assignment to
 //     columns are those of the candidate rows alone.  Level 0: one workgroup per 256-row chunk
-//     finds its chunk's candidates (division-free elimination in LDS); merge levels: one workgroup
-//     per group of 8 candidate lists re-runs the rule on their raw rows; the last level writes the
-//     panel's pivot rows P (pivot order), columns J and count k.
-//  2. X = S[P, J]^{-1} S[P, :]: every workgroup owns a slab of 256 columns and performs the same
+//     finds its chunk's candidates (division-free elimination in LDS, four threads per row);
+//     merge levels: one workgroup per group of 16 candidate lists re-runs the rule on their raw
+//     rows (two threads per row); the last level writes the panel's pivot rows P (pivot order),
+//     columns J and count k.
+//  2. X = S[P, J]^{-1} S[P, :]: every workgroup owns a slab of 16 columns and performs the same
 //     row reduction on [S[P, J] | S[P, slab]] (division-free forward, one inverse per pivot,
 //     back substitution), writing X[:, slab] into the U store (row r + t, original columns) and
 //     into a contiguous copy for the next step.  X has the identity at J.
 //  3. The rows R not in P, in order, form the next Schur complement:
 //        S'[i, :] = S[R_i, W:] - S[R_i, J] X[:, W:]
-//     one GEMM (the limb-MFMA route) with a row gather on A and C_in, a column gather on A, and
-//     output into the other S buffer (compaction for free).
+//     as GEMMs with a row gather on A and C_in, a column gather on A, and output into the other S
+//     buffer (compaction for free).  Lookahead: the next panel's W columns are computed first on
+//     the caller's stream, the rest on a second stream, overlapping the next panel's tournament
+//     (which reads only those W columns); the next panel's step 2 waits for it.  P, J and the
+//     update's sizes are double-buffered by panel parity for the overlap.
 //  All sizes (m_a, k, r) live in device memory; the host only knows upper bounds, so a panel is a
 //  fixed sequence of launches.  At the end the rank rows of the U store (identity at their own
 //  panel's pivot columns, zero to the left) are cleared against each other by one unit-upper TRSM
@@ -39,12 +43,13 @@ namespace {
 
 constexpr int LW = 32;        // panel width
 constexpr int CH = 256;       // rows per level-0 chunk
-constexpr int GRP = 8;        // candidate lists merged per workgroup
+constexpr int GRP = 16;       // candidate lists merged per workgroup
 constexpr int LT = 256;       // threads
 
 struct LState {               // device-side counters
     uint32_t m_a, r, k, m_new;
     int status;
+    uint32_t snap_k[2], snap_m[2];   // the sizes of panel i's trailing update, slot i & 1
 };
 
 __device__ __forceinline__ uint32_t wmin(uint32_t x) {
@@ -54,27 +59,27 @@ __device__ __forceinline__ uint32_t wmin(uint32_t x) {
 }
 
 // The smallest-row rule on an LDS panel pan[u * ws + q] (rows in original order), division-free.
-// Four threads per row: thread (u, part) owns the columns q == part (mod 4) of row u, so a column's
-// elimination is spread over all 4 * rows threads.  The owner of column q scans it (it wrote it
-// last); one barrier per column.  Returns k; s_u[t] / s_q[t] = local row / column of pivot t;
-// chosen[u] = 1 for the rows picked.
-constexpr int RT = 1024;      // threads of the tournament kernels (rows <= RT / 4)
+// TPR threads per row: thread (u, part) owns the columns q == part (mod TPR) of row u, so a
+// column's elimination is spread over all TPR * rows threads.  The owner of column q scans it (it
+// wrote it last); one barrier per column.  Returns k; s_u[t] / s_q[t] = local row / column of
+// pivot t; chosen[u] = 1 for the rows picked.
+constexpr int RT = 1024;      // threads of the tournament kernels
 
-template <class Ar>
+template <int TPR, class Ar>
 __device__ uint32_t lds_rule(uint32_t* pan, uint32_t rows, uint32_t wc, uint32_t ws, uint8_t* chosen, const Ar& ar,
                              uint32_t* s_u, uint32_t* s_q) {
+    static_assert((TPR & (TPR - 1)) == 0, "threads per row: a power of two");
     __shared__ uint32_t s_cand[2][RT / 64];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, p = ar.p;
-    const uint32_t u = tid >> 2, part = tid & 3;
+    const uint32_t u = tid / TPR, part = tid % TPR;
     const bool mine = u < rows;
     uint32_t* row = pan + u * ws;
     bool done = false;
     uint32_t k = 0;
-    static_assert(CH <= RT / 4 && GRP * LW <= RT / 4, "one row per four threads");
     __syncthreads();                                   // the panel is in LDS
     for (uint32_t q = 0; q < wc; ++q) {
         uint32_t best = 0xffffffffu;
-        if (mine && part == (q & 3) && !done && row[q] != 0) best = u;
+        if (mine && part == (q % TPR) && !done && row[q] != 0) best = u;
         best = wmin(best);
         if (lane == 0) s_cand[q & 1][wave] = best;
         __syncthreads();
@@ -90,7 +95,7 @@ __device__ uint32_t lds_rule(uint32_t* pan, uint32_t rows, uint32_t wc, uint32_t
                 if (f != 0) {
                     const uint32_t* prow = pan + bu * ws;
                     const uint32_t v = prow[q], fneg = p - f;
-                    for (uint32_t q2 = q + 1 + ((part - (q + 1)) & 3u); q2 < wc; q2 += 4)
+                    for (uint32_t q2 = q + 1 + ((part - (q + 1)) & (TPR - 1)); q2 < wc; q2 += TPR)
                         row[q2] = ar.dfree(v, row[q2], fneg, prow[q2]);
                 }
             }
@@ -120,7 +125,8 @@ __global__ __launch_bounds__(RT) void rrp_chunk_kernel(const uint32_t* S, int64_
         const uint32_t u = idx / wc, q = idx % wc;
         pan[u * ws + q] = S[(int64_t)(r0 + u) * ld + q];
     }
-    const uint32_t k = lds_rule(pan, rows, wc, ws, chosen, ar, s_u, s_q);
+    static_assert(CH * 4 <= RT, "four threads per chunk row");
+    const uint32_t k = lds_rule<4>(pan, rows, wc, ws, chosen, ar, s_u, s_q);
     if (threadIdx.x == 0) {
         uint32_t t = 0;
         for (uint32_t u = 0; u < rows; ++u)
@@ -155,7 +161,8 @@ __global__ __launch_bounds__(RT) void rrp_merge_kernel(const uint32_t* S, int64_
         const uint32_t u = idx / wc, q = idx % wc;
         pan[u * ws + q] = S[(int64_t)rowid[u] * ld + q];
     }
-    const uint32_t k = lds_rule(pan, rows, wc, ws, chosen, ar, s_u, s_q);
+    static_assert(GRP * LW * 2 <= RT, "two threads per candidate row");
+    const uint32_t k = lds_rule<2>(pan, rows, wc, ws, chosen, ar, s_u, s_q);
     if (threadIdx.x == 0) {
         uint32_t t = 0;
         for (uint32_t u = 0; u < rows; ++u)
@@ -175,9 +182,10 @@ __global__ __launch_bounds__(RT) void rrp_merge_kernel(const uint32_t* S, int64_
 // X[:, slab] = S[P, J]^{-1} S[P, slab]; also the pivots / pivot rows of this panel (block 0).
 // Every workgroup reduces the same k x k pivot block beside its own SL-column slab: narrow slabs
 // spread the (multiply-latency-bound) elimination over many CUs.
-constexpr int SL = 32;
+constexpr int SL = 16;
+constexpr int PXT = 512;     // threads of panel_x_kernel (two waves per SIMD)
 template <class Ar>
-__global__ __launch_bounds__(LT) void panel_x_kernel(const uint32_t* S, int64_t ld, uint32_t na, const uint32_t* P,
+__global__ __launch_bounds__(PXT) void panel_x_kernel(const uint32_t* S, int64_t ld, uint32_t na, const uint32_t* P,
                                                      const uint32_t* J, LState* st, Ar ar, uint32_t* X, int64_t ldx,
                                                      uint32_t* U, int64_t ldu, uint32_t c, const uint32_t* rows_orig,
                                                      uint32_t* pivots, uint32_t* pivot_rows) {
@@ -189,7 +197,7 @@ __global__ __launch_bounds__(LT) void panel_x_kernel(const uint32_t* S, int64_t 
     if (j0 >= na) return;
     const uint32_t sw = min((uint32_t)SL, na - j0), bw = k + sw, p = ar.p;
     const uint32_t r = st->r;
-    for (uint32_t idx = threadIdx.x; idx < k * bw; idx += LT) {
+    for (uint32_t idx = threadIdx.x; idx < k * bw; idx += PXT) {
         const uint32_t t = idx / bw, j = idx % bw;
         const int64_t row = (int64_t)P[t] * ld;
         B[t * bw + j] = j < k ? S[row + J[j]] : S[row + j0 + (j - k)];
@@ -200,7 +208,7 @@ __global__ __launch_bounds__(LT) void panel_x_kernel(const uint32_t* S, int64_t 
     for (uint32_t t = 0; t + 1 < k; ++t) {
         const uint32_t wdt = bw - t - 1, cnt = (k - t - 1) * wdt;
         const uint32_t v = B[t * bw + t];
-        for (uint32_t it = threadIdx.x; it < cnt; it += LT) {
+        for (uint32_t it = threadIdx.x; it < cnt; it += PXT) {
             const uint32_t row = t + 1 + it / wdt, j = t + 1 + it % wdt;
             const uint32_t f = B[row * bw + t];
             if (f != 0) B[row * bw + j] = ar.dfree(v, B[row * bw + j], p - f, B[t * bw + j]);
@@ -213,7 +221,7 @@ __global__ __launch_bounds__(LT) void panel_x_kernel(const uint32_t* S, int64_t 
         s_inv[threadIdx.x] = ar.prep(iv);
     }
     __syncthreads();
-    for (uint32_t idx = threadIdx.x; idx < k * bw; idx += LT) {
+    for (uint32_t idx = threadIdx.x; idx < k * bw; idx += PXT) {
         const uint32_t t = idx / bw, j = idx % bw;
         if (j > t) B[idx] = ar.mul(B[idx], s_inv[t]);
     }
@@ -222,7 +230,7 @@ __global__ __launch_bounds__(LT) void panel_x_kernel(const uint32_t* S, int64_t 
         const uint32_t wdt = bw - t - 1, cnt = t * wdt;
         if (threadIdx.x < t) s_fp[threadIdx.x] = ar.prep(B[threadIdx.x * bw + t]);
         __syncthreads();
-        for (uint32_t it = threadIdx.x; it < cnt; it += LT) {
+        for (uint32_t it = threadIdx.x; it < cnt; it += PXT) {
             const uint32_t row = it / wdt, j = t + 1 + it % wdt;
             if (j < k) continue;                   // the pivot block becomes the identity below
             const uint32_t fp = s_fp[row];
@@ -231,7 +239,7 @@ __global__ __launch_bounds__(LT) void panel_x_kernel(const uint32_t* S, int64_t 
         __syncthreads();
     }
     // X[:, slab]; the slab's own pivot columns get the identity exactly (they equal it already)
-    for (uint32_t idx = threadIdx.x; idx < k * sw; idx += LT) {
+    for (uint32_t idx = threadIdx.x; idx < k * sw; idx += PXT) {
         const uint32_t t = idx / sw, j = idx % sw;
         uint32_t x = B[t * bw + k + j];
         for (uint32_t t2 = 0; t2 < k; ++t2)
@@ -272,6 +280,15 @@ __global__ __launch_bounds__(1024) void rest_rows_kernel(const LState* st, const
             rows_orig_next[pos] = rows_orig[i];
             ++pos;
         }
+}
+
+// The trailing update of a panel may still run (second stream) while the next panel's
+// tournament rewrites k / m_new: it reads its own copy.
+__global__ void snapshot_kernel(LState* st, int slot) {
+    if (threadIdx.x == 0) {
+        st->snap_k[slot] = st->k;
+        st->snap_m[slot] = st->m_new;
+    }
 }
 
 __global__ void advance_kernel(LState* st) {
@@ -340,8 +357,8 @@ void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const
     add((size_t)nchunk * LW);     // 9 cand b
     add(nchunk);                  // 10 cnt a
     add(nchunk);                  // 11 cnt b
-    add(LW);                      // 12 P
-    add(LW);                      // 13 J
+    add(2 * LW);                  // 12 P (two panels: slot i & 1)
+    add(2 * LW);                  // 13 J
     add(mn);                      // 14 pivots
     add(mn);                      // 15 pivot rows
     add(16);                      // 16 state
@@ -358,9 +375,17 @@ void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const
     uint32_t *Rl = at(6), *flags = at(7);
     uint32_t* cand[2] = {at(8), at(9)};
     uint32_t* cnt[2] = {at(10), at(11)};
-    uint32_t *P = at(12), *J = at(13), *piv = at(14), *prow = at(15);
+    uint32_t *Pb = at(12), *Jb = at(13), *piv = at(14), *prow = at(15);
     LState* st = (LState*)at(16);
     uint32_t* work_extra = nullptr;
+    // lookahead: the trailing update of panel i is split into the next panel's LW columns (on s)
+    // and the rest (on s2, overlapping panel i+1's tournament); panel i+1's X waits for it.
+    hipStream_t s2 = nullptr;
+    hipEvent_t ev_next = nullptr, ev_rest = nullptr;
+    HX_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+    HX_HIP(hipEventCreateWithFlags(&ev_next, hipEventDisableTiming));
+    HX_HIP(hipEventCreateWithFlags(&ev_rest, hipEventDisableTiming));
+    bool rest_pending = false;
     try {
         HX_HIP(hipMemcpy2DAsync(Sb[0], (size_t)n * 4, A, (size_t)lda * 4, (size_t)n * 4, m, hipMemcpyDeviceToDevice, s));
         init_state_kernel<<<gsz(std::max<size_t>(m, (size_t)mn * n)), 256, 0, s>>>(st, m, ro[0], U, (size_t)mn * n);
@@ -368,12 +393,17 @@ void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const
         int cur = 0;
         // the trailing updates are k = 32 products: the automatic choice (the plain kernel)
         const int tvar = gemm_auto_variant(ctx, m, n, LW);
-        const size_t ws_bytes = tvar == HX_GEMM_LIMB ? limb_workspace_bytes(ctx, m, n, LW) : 0;
+        const size_t ws_bytes = tvar == HX_GEMM_LIMB ? limb_workspace_bytes(ctx, m, LW, LW) : 0;
+        const size_t ws2_bytes = tvar == HX_GEMM_LIMB ? limb_workspace_bytes(ctx, m, n, LW) : 0;
         void* ws = ws_bytes ? scratch(ws_bytes, 2) : nullptr;
-        for (uint32_t c = 0; c < n; c += LW) {
+        void* ws2 = ws2_bytes ? scratch(ws2_bytes, 3) : nullptr;
+        for (uint32_t c = 0, panel = 0; c < n; c += LW, ++panel) {
             const uint32_t na = n - c, wc = std::min<uint32_t>(LW, na);
             const uint32_t* S = Sb[cur];
-            // 1. tournament
+            const int slot = (int)(panel & 1);
+            uint32_t* P = Pb + slot * LW;
+            uint32_t* J = Jb + slot * LW;
+            // 1. tournament (reads only this panel's LW columns, already up to date)
             rrp_chunk_kernel<Ar><<<nchunk, RT, 0, s>>>(S, n, st, wc, ar, cand[0], cnt[0]);
             after_launch("rrp_chunk_kernel");
             uint32_t nin = nchunk;
@@ -387,40 +417,63 @@ void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const
                 nin = nout;
                 ++lv;
             } while (nin > 1);
+            // the previous panel's update must be complete before its rows are read whole
+            if (rest_pending) {
+                HX_HIP(hipStreamWaitEvent(s, ev_rest, 0));
+                rest_pending = false;
+            }
             // 2. X = S[P,J]^{-1} S[P,:] into the U store and the contiguous copy
-            panel_x_kernel<Ar><<<(na + SL - 1) / SL, LT, 0, s>>>(S, n, na, P, J, st, ar, X, n, U, n, c, ro[cur], piv, prow);
+            panel_x_kernel<Ar><<<(na + SL - 1) / SL, PXT, 0, s>>>(S, n, na, P, J, st, ar, X, n, U, n, c, ro[cur], piv, prow);
             after_launch("panel_x_kernel");
             if (na > wc) {
-                // 3. the next Schur complement, compacted
+                // 3. the next Schur complement, compacted: first the next panel's columns, then the rest
                 rest_rows_kernel<<<1, 1024, 0, s>>>(st, P, flags, Rl, ro[cur], ro[cur ^ 1]);
                 after_launch("rest_rows_kernel");
+                snapshot_kernel<<<1, 64, 0, s>>>(st, slot);
+                after_launch("snapshot_kernel");
+                const uint32_t nt = na - wc, n1 = std::min<uint32_t>(LW, nt);
+                if (nt > n1) HX_HIP(hipEventRecord(ev_next, s));     // X, the row list and the sizes are ready
                 GemmArgs g;
                 g.m = m;
-                g.n = na - wc;
                 g.k = LW;
                 g.A = S;
                 g.ars = n;
                 g.acs = 1;
                 g.a_rows = Rl;
                 g.a_cols = J;
-                g.B = X + wc;
                 g.brs = n;
                 g.bcs = 1;
-                g.Cin = S + wc;
                 g.cirs = n;
                 g.cics = 1;
                 g.c_rows = Rl;
-                g.Cout = Sb[cur ^ 1];
                 g.cors = n;
                 g.cocs = 1;
-                g.m_dev = &st->m_new;
-                g.k_dev = &st->k;
+                g.m_dev = &st->snap_m[slot];
+                g.k_dev = &st->snap_k[slot];
                 g.op = HX_OP_SUB;
+                g.n = n1;
+                g.B = X + wc;
+                g.Cin = S + wc;
+                g.Cout = Sb[cur ^ 1];
                 gemm_dispatch(ctx, g, s, tvar, ws, ws_bytes, 0);
+                if (nt > n1) {
+                    HX_HIP(hipStreamWaitEvent(s2, ev_next, 0));
+                    g.n = nt - n1;
+                    g.B = X + wc + n1;
+                    g.Cin = S + wc + n1;
+                    g.Cout = Sb[cur ^ 1] + n1;
+                    gemm_dispatch(ctx, g, s2, tvar, ws2, ws2_bytes, 0);
+                    HX_HIP(hipEventRecord(ev_rest, s2));
+                    rest_pending = true;
+                }
                 cur ^= 1;
             }
             advance_kernel<<<1, 64, 0, s>>>(st);
             after_launch("advance_kernel");
+        }
+        if (rest_pending) {
+            HX_HIP(hipStreamWaitEvent(s, ev_rest, 0));
+            rest_pending = false;
         }
         LState h;
         HX_HIP(hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, s));
@@ -470,10 +523,18 @@ void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const
         }
     } catch (...) {
         (void)hipStreamSynchronize(s);
+        (void)hipStreamSynchronize(s2);
+        (void)hipEventDestroy(ev_next);
+        (void)hipEventDestroy(ev_rest);
+        (void)hipStreamDestroy(s2);
         (void)hipFree(base);
         (void)hipFree(work_extra);
         throw;
     }
+    HX_HIP(hipStreamSynchronize(s2));
+    HX_HIP(hipEventDestroy(ev_next));
+    HX_HIP(hipEventDestroy(ev_rest));
+    HX_HIP(hipStreamDestroy(s2));
     HX_HIP(hipFree(base));
     if (work_extra) HX_HIP(hipFree(work_extra));
 }

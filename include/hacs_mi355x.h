@@ -71,14 +71,16 @@ enum { HX_OP_SET = 0, HX_OP_ADD = 1, HX_OP_SUB = 2 };
 
 /* GEMM kernel variants (no counterpart: the reference has one u32 kernel and the p <= 127 IMMA route) */
 enum {
-    HX_GEMM_AUTO = 0,   /* the selector picks (never changes a value) */
+    HX_GEMM_AUTO = 0,   /* the library picks: LIMB when m*n*k >= 4e6 and k >= 64, else PLAIN (never changes a value) */
     HX_GEMM_PLAIN = 1,  /* u32 x u32 -> exact u64 (+carry word) accumulation on the VALU, Barrett epilogue */
     HX_GEMM_LIMB = 2    /* balanced residues as signed 8-bit limbs on the INT8 MFMA cores, exact recombination */
 };
 
 typedef struct hx_gemm_opts {   /* GemmOpts, CudaKernels.h:64-79 */
     int variant;            /* HX_GEMM_* */
-    int tile;               /* 0 = automatic; a variant-specific tile index (tuning only) */
+    int tile;               /* 0 = automatic; a variant-specific choice (tuning only, same bytes): for LIMB,
+                               1 = register-staged LDS copy, 2 = direct-to-LDS copy ring (the default),
+                               3 = the ring with LDS reads pipelined against the MFMAs, 4 = that on eight waves */
     void *workspace;        /* limb route: >= hx_gemm_workspace_bytes(); NULL = library-owned (not capturable) */
     size_t workspace_bytes;
     hx_event main_begin;    /* optional timing events recorded on the stream right before / after the */
