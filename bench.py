@@ -52,7 +52,8 @@ def parse():
     ap.add_argument("--mixed-jobs", type=int, default=20000)
     ap.add_argument("--no-howell", action="store_true", help="do not time the Howell batch")
     ap.add_argument("--howell-jobs", type=int, default=6666)
-    ap.add_argument("--only", choices=["batch", "lu", "mixed", "cpu", "howell"],
+    ap.add_argument("--no-ring2k", action="store_true", help="do not time the Z/2^32 LU device session")
+    ap.add_argument("--only", choices=["batch", "lu", "mixed", "cpu", "howell", "ring2k"],
                     help="run one secondary workload alone (for profiling) and print its record")
     ap.add_argument("--pmc-mode", action="store_true",
                     help="for counter collection: one 1 GiB device copy (the traffic calibration), then one limb "
@@ -262,6 +263,46 @@ def howell_batch(E, njobs, cpu_budget_s=8.0):
     return out
 
 
+def ring2k_session(E, m=8192, panel=64):
+    """The Z/2^32 LU device session (hx_ring2k_*, the reference's ring2k::device Ops) over an m x m
+    window: one step per panel of `panel` columns with `panel` pivots -- the factored panel and L
+    uploaded from host buffers, TRSM on the pivot rows, trailing wraparound GEMM on the device.
+    The step data are synthetic (the device work does not depend on the values; the tests check the
+    session's results); the rate includes the per-step host -> device copies of the panel and L."""
+    import ctypes as C
+    from hacs_amd import _lib
+    L = _lib.lib()
+    rng = np.random.default_rng(5)
+    A = rng.integers(0, 1 << 32, (m, m), dtype=np.uint64).astype(np.uint32)
+    Pn = rng.integers(0, 1 << 32, (m, panel), dtype=np.uint64).astype(np.uint32)
+    Lh = rng.integers(0, 1 << 32, (m, panel), dtype=np.uint64).astype(np.uint32)
+    sw = np.zeros(2, np.uint32)
+    h = C.c_void_p()
+    _lib.check(L.hx_ring2k_open(4, m, m, panel, C.byref(h)))
+    try:
+        out = np.zeros((1, 1), np.uint32)
+        best = 1e30
+        for rep in range(2):
+            _lib.check(L.hx_ring2k_put(h, A.ctypes.data, m, 0, m, 0, m))
+            _lib.check(L.hx_ring2k_get(h, out.ctypes.data, 1, 0, 1, 0, 1))        # synchronises
+            t0 = time.perf_counter()
+            for r in range(0, m, panel):
+                c1 = min(r + panel, m)
+                st = _lib.hx_ring2k_step_args(m=m, r=r, np=c1 - r, pc0=r, pc1=c1, panel=Pn.ctypes.data, ldp=panel,
+                                              swaps=sw.ctypes.data, nswaps=0, L=Lh.ctypes.data, ldl=panel,
+                                              u0=c1, la0=0, la1=0, w=0)
+                _lib.check(L.hx_ring2k_step(h, C.byref(st)))
+            _lib.check(L.hx_ring2k_get(h, out.ctypes.data, 1, m - 1, 1, m - 1, 1))
+            best = min(best, time.perf_counter() - t0)
+    finally:
+        L.hx_ring2k_close(h)
+    macs = float(sum((m - min(r + panel, m)) * (m - min(r + panel, m)) * min(panel, m - r) for r in range(0, m, panel)))
+    h2d = float(sum(2 * (m - r) * panel * 4 for r in range(0, m, panel)))
+    return {"m": m, "panel": panel, "seconds": best, "steps": (m + panel - 1) // panel,
+            "trailing_wraparound_macs": macs, "macs_per_s": macs / best, "h2d_bytes": h2d,
+            "note": "Z/2^32, synthetic step data, per-step host panel + L uploads included"}
+
+
 def lu_c2(E, reps=3, cpu=True):
     """C2: RREF of the 4096 x 4096 rank-3000 matrix mod 2^31-1 (rank, pivots, pivot_rows, RREF)."""
     import torch
@@ -327,6 +368,7 @@ def mixed_stream(njobs):
     t_choose = time.perf_counter() - t2
     statics = [AT.static_choice(k, n) for k, n in stream]
     t_run = {"selector": 0.0, "static": 0.0}
+    t_kind = {}
     budget = 40 << 30
     i = 0
     while i < njobs:
@@ -338,6 +380,7 @@ def mixed_stream(njobs):
             i += 1
         jobs = [AT.Job(stream[j][0], stream[j][1], j, salt=0x90) for j in chunk]
         t_run["selector"] += AT.run_stream(jobs, [slots[j] for j in chunk])
+        AT.run_stream(jobs, [slots[j] for j in chunk], per_kind=t_kind)
         t_run["static"] += AT.run_stream(jobs, [statics[j] for j in chunk])
         for jb in jobs:
             jb.free()
@@ -348,6 +391,7 @@ def mixed_stream(njobs):
             "heldout_jobs": len(held), "heldout_selector_vs_best": t_sel / best,
             "heldout_static_vs_best": t_stat / best, "heldout_choice_agreement": agree,
             "train_jobs": len(train), "train_loss": loss, "tune_seconds": t_tune, "fit_seconds": t_fit,
+            "seconds_by_kind": t_kind,
             "exhaustive_tune_seconds_per_job": t_tune / (len(train) + len(held)),
             "selector_choose_seconds": t_choose}
 
@@ -400,6 +444,7 @@ def main():
         leg = {"batch": lambda: batched_echelon(E, args.batch_jobs), "lu": lambda: lu_c2(E, cpu=not args.no_lu_cpu),
                "mixed": lambda: mixed_stream(args.mixed_jobs),
                "howell": lambda: howell_batch(E, args.howell_jobs),
+               "ring2k": lambda: ring2k_session(E),
                "cpu": lambda: cpu_baseline(p, min(16, os.cpu_count() or 1))}[args.only]()
         print(json.dumps({"only": args.only, **leg}), flush=True)
         return
@@ -473,6 +518,10 @@ def main():
         hw = howell_batch(E, args.howell_jobs)
         log(f"howell batch: {hw}")
         res["howell_batch"] = hw
+    if not args.no_ring2k:
+        r2 = ring2k_session(E)
+        log(f"ring2k session: {r2}")
+        res["ring2k_session"] = r2
     if not args.no_mixed:
         mx = mixed_stream(args.mixed_jobs)
         log(f"mixed stream: {mx}")

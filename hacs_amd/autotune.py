@@ -147,28 +147,35 @@ class Selector:
         return out
 
 
-def run_stream(jobs: list[Job], slots: list[int]) -> float:
+def run_stream(jobs: list[Job], slots: list[int], per_kind: dict | None = None) -> float:
     """Executes every job with its chosen slot, one batch per kind carrying the per-job choices
     (GEMM: per-job kernel variant in one batch; echelon: per-job panel width in one persistent
     launch; Howell: one persistent launch), so a choice never splits a launch.  Returns the wall
-    seconds of the launches (inputs already in HBM; batch tables built beforehand)."""
+    seconds of the launches (inputs already in HBM; batch tables built beforehand); with
+    `per_kind`, also adds each kind's own seconds to it (each kind then timed on its own)."""
     ctx_g, ctx_e = E.ModCtx(P_GEMM), E.ModCtx(P_ECH)
     by = {k: [(jb, s) for jb, s in zip(jobs, slots) if jb.kind == k] for k in KINDS}
     runners = []
     if by["gemm"]:
-        runners.append(E.GemmBatch(ctx_g, [(jb.A, jb.B, jb.C, E.GemmOp.Set, CHOICES["gemm"][s])
-                                           for jb, s in by["gemm"]]))
+        runners.append(("gemm", E.GemmBatch(ctx_g, [(jb.A, jb.B, jb.C, E.GemmOp.Set, CHOICES["gemm"][s])
+                                                    for jb, s in by["gemm"]])))
     if by["echelon"]:
         for jb, _ in by["echelon"]:
             jb.M.copy_(jb.src)
-        runners.append(E.RrefBatch(ctx_e, [jb.M for jb, _ in by["echelon"]],
-                                   panels=[CHOICES["echelon"][s] for _, s in by["echelon"]]))
+        runners.append(("echelon", E.RrefBatch(ctx_e, [jb.M for jb, _ in by["echelon"]],
+                                               panels=[CHOICES["echelon"][s] for _, s in by["echelon"]])))
     if by["howell"]:
-        runners.append(E.HowellBatch(K_HOWELL, [jb.src for jb, _ in by["howell"]]))
+        runners.append(("howell", E.HowellBatch(K_HOWELL, [jb.src for jb, _ in by["howell"]])))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for r in runners:
-        r.run()
+    for kind, r in runners:
+        if per_kind is not None:
+            t1 = time.perf_counter()
+            r.run()
+            torch.cuda.synchronize()
+            per_kind[kind] = per_kind.get(kind, 0.0) + time.perf_counter() - t1
+        else:
+            r.run()
     torch.cuda.synchronize()
     return time.perf_counter() - t0
 

@@ -24,6 +24,7 @@
 //
 // No floating point anywhere; no atomics on results.
 #include "hx_gemm.h"
+#include "hx_limbs.h"
 
 #include <algorithm>
 #include <mutex>
@@ -187,21 +188,6 @@ __global__ __launch_bounds__(256) void plain_batch_kernel(const GemmArgs* jobs, 
 // ------------------------------------------------------------------------------------------
 // limb route
 // ------------------------------------------------------------------------------------------
-// Balanced signed base-256 digits of a canonical residue x (x < p): L digits in [-128, 127].
-__device__ __forceinline__ void limbs_of(uint32_t x, uint32_t p, uint32_t half, int L, int32_t* d) {
-    int32_t v = x > half ? (int32_t)((int64_t)x - (int64_t)p) : (int32_t)x;
-#pragma unroll
-    for (int s = 0; s < 3; ++s) {
-        if (s >= L - 1) break;
-        const int32_t t = ((v + 128) & 255) - 128;
-        d[s] = t;
-        v = (v - t) >> 8;
-    }
-#pragma unroll
-    for (int s = 0; s < 4; ++s)
-        if (s == L - 1) d[s] = v;
-}
-
 struct PrepArgs {
     const uint32_t* src;
     int64_t s_r, s_c;             // source strides along output rows / columns
@@ -290,20 +276,11 @@ struct LimbArgs {
     uint32_t* Cout;
     int64_t cors, cocs;
     int op;
-    uint64_t off;                // p * ceil(2^63 / p): makes |x| < 2^63 non-negative before Barrett
-    int32_t w[3];                // 2^(8d) mod p, balanced into (-p/2, p/2], for the diagonals d = 4, 5, 6
+    LimbConsts lc;               // the recombination constants (hx_limbs.h)
     DevMod M;
 };
 
-// The epilogue constants of LimbArgs for modulus p.
-inline void limb_epilogue_consts(uint32_t p, LimbArgs& la) {
-    la.off = ((1ull << 63) / p + 1) * (uint64_t)p;
-    uint64_t w = 1;
-    for (int d = 1; d <= 6; ++d) {
-        w = (w << 8) % p;
-        if (d >= 4) la.w[d - 4] = (int32_t)(w <= p / 2 ? (int64_t)w : (int64_t)w - (int64_t)p);
-    }
-}
+inline void limb_epilogue_consts(uint32_t p, LimbArgs& la) { la.lc = limb_consts(p); }
 
 // Epilogue of both limb kernels: exact recombination sum_d 256^d D_d of one wave's 32 x 64
 // block (two 32 x 32 MFMA blocks; lane = column, 16 accumulator rows per lane), op into C.
@@ -320,20 +297,10 @@ __device__ __forceinline__ void limb_store(const LimbArgs& a, const v16i (&acc)[
         for (int r = 0; r < 16; ++r) {
             const uint32_t row = r0 + (r / 4) * 8 + lg * 4 + (r % 4);
             if (row >= m) continue;
-            // x == sum_d D_d 256^d (mod p), exactly in int64: the diagonals d <= 3 with their true
-            // weights (|D_d| < 2^31, so |sum| < 2^31 (1 + 2^8 + 2^16 + 2^24) < 2^56), the diagonals
-            // d >= 4 (L = 4 only; 3, 2, 1 limb products, |D_d| <= pairs(d) K 2^14 <= pairs(d) 2^29)
-            // with balanced weights |w_d| <= p/2 < 2^31: |x| < 6 * 2^60 + 2^56 < 2^63.  One reduction
-            // of x + p ceil(2^63 / p), which lies in [0, 2^64).
-            int64_t x = 0;
+            int32_t D[ND];
 #pragma unroll
-            for (int d = 0; d < ND; ++d) {
-                if (d <= 3)
-                    x += (int64_t)acc[cb][d][r] * (int64_t)(1u << (8 * d));
-                else
-                    x += (int64_t)acc[cb][d][r] * (int64_t)a.w[d - 4];
-            }
-            const uint32_t res = red64((uint64_t)x + a.off, a.M);
+            for (int d = 0; d < ND; ++d) D[d] = acc[cb][d][r];
+            const uint32_t res = limb_recombine<ND>(D, a.lc, a.M);   // exact (bounds in hx_limbs.h)
             uint32_t c = 0;
             if (a.op != HX_OP_SET) {
                 const int64_t rr = a.c_rows ? (int64_t)a.c_rows[row] : (int64_t)row;

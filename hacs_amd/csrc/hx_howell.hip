@@ -175,6 +175,16 @@ __host__ __device__ constexpr size_t blocked_lds_bytes(uint32_t rc) {
     return (size_t)rc * HW * 4 * 2 + (size_t)rc * 4 * 2 + rc;
 }
 
+// Multiplication of the pass mod 2^32 (only the value mod 2^k matters).  For k <= 24 every
+// multiplier operand is kept reduced mod 2^k (< 2^24), and the full-rate 24-bit multiply gives
+// the exact low 32 bits of the product; otherwise the quarter-rate 32-bit multiply.
+template <bool SMALLK>
+__device__ __forceinline__ uint32_t kmul(uint32_t a, uint32_t b) {
+    if constexpr (SMALLK) return __umul24(a, b);
+    else return a * b;
+}
+
+template <bool SMALLK>
 __device__ void howell_blocked(const HJob& job, uint32_t k, uint32_t rc, uint32_t* lds) {
     __shared__ uint32_t s_red[2][HT / 64];
     __shared__ uint32_t s_pid[HW], s_ui[HW], s_v[HW], s_c[HW];
@@ -205,19 +215,23 @@ __device__ void howell_blocked(const HJob& job, uint32_t k, uint32_t rc, uint32_
         for (uint32_t idx = tid; idx < na * HW; idx += HT) {
             const uint32_t pos = idx / HW, t = idx % HW, id = act[pos];
             if (id == NONE) continue;
-            pan[id * HW + t] = t < wc ? rows[(size_t)id * n + c0 + t] : 0u;
+            pan[id * HW + t] = t < wc ? rows[(size_t)id * n + c0 + t] & mask : 0u;
             qm[id * HW + t] = 0;
             if (t == 0) nz[id] = 0;
         }
         if (tid < HW) s_pid[tid] = NONE;
         const uint32_t ncl0 = ncl;
         __syncthreads();
-        // (A) the pass on the panel columns
+        // (A) the pass on the panel columns.  Row position pos belongs to thread pos % HT for the
+        // whole panel (scan, elimination, closure rows appended there, the chosen mark), and the
+        // pivot row is left unscaled (the eliminating multiplier is q u^-1 instead), so the only
+        // exchange per step is the pivot search: one barrier per step.
+        uint32_t chosen = 0;                                   // bit j: position tid + j HT picked
         for (uint32_t t = 0; t < wc; ++t) {
             uint32_t best = NONE;
-            for (uint32_t pos = tid; pos < na; pos += HT) {
-                const uint32_t id = act[pos];
-                if (id == NONE) continue;
+            for (uint32_t pos = tid, j = 0; pos < na; pos += HT, ++j) {
+                const uint32_t id = act[pos];                  // NONE: chosen in an earlier panel
+                if (id == NONE || (chosen >> j & 1u)) continue;
                 const uint32_t x = pan[id * HW + t] & mask;
                 if (x != 0) best = min(best, ((uint32_t)__builtin_ctz(x) << 24) | pos);
             }
@@ -227,49 +241,49 @@ __device__ void howell_blocked(const HJob& job, uint32_t k, uint32_t rc, uint32_
             best = min(min(s_red[t & 1][0], s_red[t & 1][1]), min(s_red[t & 1][2], s_red[t & 1][3]));
             if (best == NONE) continue;
             const uint32_t bpos = best & 0xffffffu, v = best >> 24, pid = act[bpos];
-            uint32_t* P = pan + pid * HW;
-            const uint32_t ui = inv_odd((P[t] & mask) >> v);
-            __syncthreads();                                   // everyone has read act[bpos] and P[t]
-            if (tid < wc - t) P[t + tid] *= ui;
+            const uint32_t* P = pan + pid * HW;                // unscaled: P[t] = u 2^v
+            const uint32_t ui = inv_odd((P[t] & mask) >> v) & mask;
+            if (bpos % HT == tid) chosen |= 1u << (bpos / HT);
             if (tid == 0) {
-                act[bpos] = NONE;
                 s_pid[t] = pid;
                 s_ui[t] = ui;
                 prow[h] = pid;
                 job.pivots[h] = c0 + t;
                 job.pivot_v[h] = v;
             }
-            __syncthreads();
-            for (uint32_t pos = tid; pos < na; pos += HT) {
+            for (uint32_t pos = tid, j = 0; pos < na; pos += HT, ++j) {
                 const uint32_t id = act[pos];
-                if (id == NONE) continue;
+                if (id == NONE || (chosen >> j & 1u)) continue;
                 uint32_t* R = pan + id * HW;
                 const uint32_t q = (R[t] & mask) >> v;
                 if (q == 0) continue;
-                qm[id * HW + t] = q;
+                qm[id * HW + t] = q;                           // w.r.t. the scaled pivot row
                 nz[id] = 1;
-                for (uint32_t t2 = t; t2 < wc; ++t2) R[t2] -= q * P[t2];
+                const uint32_t qe = kmul<SMALLK>(q, ui) & mask;
+                for (uint32_t t2 = t; t2 < wc; ++t2) R[t2] = (R[t2] - kmul<SMALLK>(qe, P[t2])) & mask;
             }
-            if (v > 0) {                                       // closure row 2^(k-v) P at the bottom
+            if (v > 0) {                                       // closure row 2^(k-v) u^-1 P at the bottom
                 const uint32_t cid = m + ncl, sh = k - v;      // 1 <= sh <= 31
-                if (tid < HW) {
-                    pan[cid * HW + tid] = (tid <= t || tid >= wc) ? 0u : (P[tid] << sh);
-                    qm[cid * HW + tid] = tid == t ? (0u - (1u << sh)) : 0u;
-                }
-                if (tid == 0) {
+                if (na % HT == tid) {
                     act[na] = cid;
                     nz[cid] = 1;
+                    for (uint32_t t2 = 0; t2 < HW; ++t2) {
+                        pan[cid * HW + t2] = (t2 <= t || t2 >= wc) ? 0u : (kmul<SMALLK>(P[t2], ui) << sh) & mask;
+                        qm[cid * HW + t2] = t2 == t ? (0u - (1u << sh)) & mask : 0u;
+                    }
                 }
                 ++na;
                 ++ncl;
             }
             ++h;
-            __syncthreads();
         }
+        __syncthreads();
+        for (uint32_t pos = tid, j = 0; pos < na; pos += HT, ++j)
+            if (chosen >> j & 1u) act[pos] = NONE;
         // the panel's pivot rows keep their panel entries; new closure rows start from zero
         for (uint32_t idx = tid; idx < HW * HW; idx += HT) {
             const uint32_t t = idx / HW, t2 = idx % HW, pid = s_pid[t];
-            if (pid != NONE && t2 < wc) rows[(size_t)pid * n + c0 + t2] = pan[pid * HW + t2];
+            if (pid != NONE && t2 < wc) rows[(size_t)pid * n + c0 + t2] = kmul<SMALLK>(pan[pid * HW + t2], s_ui[t]) & mask;
         }
         const uint32_t tw = n - c1;
         for (uint64_t idx = tid; idx < (uint64_t)(ncl - ncl0) * tw; idx += HT)
@@ -291,8 +305,8 @@ __device__ void howell_blocked(const HJob& job, uint32_t k, uint32_t rc, uint32_
                     uint32_t x = rows[(size_t)pid * n + j];
                     const uint32_t* qr = qm + pid * HW;
 #pragma unroll
-                    for (uint32_t s2 = 0; s2 < t; ++s2) x -= qr[s2] * T[s2];
-                    x *= s_ui[t];
+                    for (uint32_t s2 = 0; s2 < t; ++s2) x -= kmul<SMALLK>(qr[s2], T[s2]);
+                    x = kmul<SMALLK>(x & mask, s_ui[t] & mask) & mask;
                     T[t] = x;
                     rows[(size_t)pid * n + j] = x;
                 }
@@ -332,7 +346,7 @@ __device__ void howell_blocked(const HJob& job, uint32_t k, uint32_t rc, uint32_
                         if (!in[u]) continue;
                         uint32_t x = xs[r][u];
 #pragma unroll
-                        for (uint32_t t = 0; t < HW; ++t) x -= q[t] * Tr[t][u];
+                        for (uint32_t t = 0; t < HW; ++t) x -= kmul<SMALLK>(q[t], Tr[t][u]);
                         rows[(size_t)ids[r] * n + j0 + lane + 64 * u] = x;
                     }
                 }
@@ -369,7 +383,7 @@ __device__ void howell_blocked(const HJob& job, uint32_t k, uint32_t rc, uint32_
                 uint32_t q = 0;
                 if (t < bw && i0 + t > ip) q = (cur[t] & mask) >> s_v[t];
 #pragma unroll
-                for (uint32_t t2 = t + 1; t2 < HW; ++t2) cur[t2] -= q * s_hb[t * HW + t2];
+                for (uint32_t t2 = t + 1; t2 < HW; ++t2) cur[t2] -= kmul<SMALLK>(q, s_hb[t * HW + t2]);
                 qm[ip * HW + t] = q;
             }
         }
@@ -389,7 +403,7 @@ __device__ void howell_blocked(const HJob& job, uint32_t k, uint32_t rc, uint32_
                     const uint32_t* qr = qm + (ip0 + u) * HW;
                     uint32_t x = xs[u];
 #pragma unroll
-                    for (uint32_t t = 0; t < HW; ++t) x -= qr[t] * Hr[t];
+                    for (uint32_t t = 0; t < HW; ++t) x -= kmul<SMALLK>(qr[t], Hr[t]);
                     job.H[(size_t)(ip0 + u) * n + j] = x & mask;
                 }
             }
@@ -401,6 +415,7 @@ __device__ void howell_blocked(const HJob& job, uint32_t k, uint32_t rc, uint32_
 }
 
 // Persistent: one workgroup per job of this row-capacity class, jobs from the class's counter.
+template <bool SMALLK>
 __global__ __launch_bounds__(HT) void howell_blocked_kernel(HArgs a, uint32_t rc) {
     extern __shared__ __attribute__((aligned(16))) uint32_t hlds[];
     __shared__ uint32_t s_job;
@@ -410,7 +425,7 @@ __global__ __launch_bounds__(HT) void howell_blocked_kernel(HArgs a, uint32_t rc
         const uint32_t jj = s_job;
         __syncthreads();
         if (jj >= a.njobs) break;
-        howell_blocked(a.jobs[a.order[jj]], a.k, rc, hlds);
+        howell_blocked<SMALLK>(a.jobs[a.order[jj]], a.k, rc, hlds);
     }
 }
 
@@ -507,8 +522,10 @@ HX_API int hx_howell_batch_create(uint32_t k, const hx_howell_job* jobs, uint32_
             if (hipGetDeviceProperties(&prop, 0) == hipSuccess) cus = prop.multiProcessorCount;
             static std::once_flag once;
             std::call_once(once, [] {
-                (void)hipFuncSetAttribute((const void*)howell_blocked_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                          (int)blocked_lds_bytes(kBlockedRows));
+                (void)hipFuncSetAttribute((const void*)howell_blocked_kernel<true>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)blocked_lds_bytes(kBlockedRows));
+                (void)hipFuncSetAttribute((const void*)howell_blocked_kernel<false>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)blocked_lds_bytes(kBlockedRows));
             });
             for (int c = 0; c < NCLASS; ++c) {
                 HClass& cl = b->cls[c];
@@ -523,7 +540,8 @@ HX_API int hx_howell_batch_create(uint32_t k, const hx_howell_job* jobs, uint32_
                     cl.rc = rc;
                     cl.lds = blocked_lds_bytes(rc);
                     int nb = 0;
-                    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, howell_blocked_kernel, HT, cl.lds) == hipSuccess &&
+                    const auto kern = k <= 24 ? howell_blocked_kernel<true> : howell_blocked_kernel<false>;
+                    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, HT, cl.lds) == hipSuccess &&
                         nb > 0)
                         per_cu = nb;
                     else
@@ -554,7 +572,10 @@ HX_API int hx_howell_batch_run(hx_howell_batch vb, hx_stream s) {
                 howell_batch_kernel<<<cl.grid, HT, 0, (hipStream_t)s>>>(a);
                 after_launch("howell_batch_kernel");
             } else {
-                howell_blocked_kernel<<<cl.grid, HT, cl.lds, (hipStream_t)s>>>(a, cl.rc);
+                if (b->k <= 24)
+                    howell_blocked_kernel<true><<<cl.grid, HT, cl.lds, (hipStream_t)s>>>(a, cl.rc);
+                else
+                    howell_blocked_kernel<false><<<cl.grid, HT, cl.lds, (hipStream_t)s>>>(a, cl.rc);
                 after_launch("howell_blocked_kernel");
             }
         }

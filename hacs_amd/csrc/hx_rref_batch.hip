@@ -14,8 +14,9 @@
 //         to X = S[P,J]^{-1} S[P, c:n]: division-free forward over the pivot order, one inverse per
 //         pivot (all in parallel), back substitution -- X has the identity at the panel's pivot columns.
 //     (d) trailing update of every other unused row: S[i, j] -= sum_t S[i, J_t] X[t, j] for j past the
-//         panel, exact u64+carry dot products, one Barrett reduction per entry (the Schur complement
-//         w.r.t. the panel pivots; the panel's own columns of those rows become zero and are dropped).
+//         panel on the INT8 matrix cores (both operands split into balanced 8-bit limbs on the fly,
+//         exact int32 limb products, exact recombination; trailing_mfma) -- the Schur complement
+//         w.r.t. the panel pivots; the panel's own columns of those rows become zero and are dropped.
 //     (e) X is stored as the next k rows of the U workspace (pivot order).
 //   back substitution: U's rows have the identity at their own panel's pivot columns; panels are
 //   cleared against all later rows from the bottom (non-pivot columns only: pivot columns are 0 / 1).
@@ -28,6 +29,7 @@
 #include <vector>
 
 #include "hx_elim.h"
+#include "hx_limbs.h"
 
 namespace hx {
 namespace {
@@ -55,7 +57,88 @@ struct RArgs {
     uint32_t region;      // LDS words for the panel / pivot-row block
     uint32_t mmax, nmax;
     DevMod M;
+    LimbConsts lc;        // the limb split of the MFMA trailing update (hx_limbs.h)
+    uint32_t half;        // (p - 1) / 2
+    int L;                // limbs per residue (1..4)
 };
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+// (d) on the INT8 matrix cores: S[list[u], c + j] -= sum_{t<k} S[list[u], c + J_t] X[t, j] for the
+// unused rows u and the trailing columns j in [wc, nb), k <= 32 (one 32-deep MFMA k-step).  Both
+// operands are split into L balanced 8-bit limbs on the fly (the X fragments from LDS, the row
+// fragments from the job's matrix), the L^2 limb products accumulate per anti-diagonal in int32,
+// and the exact recombination (hx_limbs.h) gives the residue subtracted from the entry.  Each wave
+// takes 32-row blocks; per block it walks the 32-column blocks of the trailing columns.
+template <int L>
+__device__ void trailing_mfma(const RJob& job, const RArgs& a, const uint32_t* B, uint32_t nb, uint32_t wc,
+                              uint32_t k, uint32_t c, uint32_t mu, const uint32_t* list, const uint8_t* chosen,
+                              const uint32_t* s_q) {
+    constexpr int ND = 2 * L - 1;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, lr = lane & 31, lg = lane >> 5;
+    const uint32_t p = a.M.p, half = a.half, tc = nb - wc;
+    const uint32_t nrb = (mu + 31) / 32, ncb = (tc + 31) / 32;
+    // the limb planes of 16 residues (one per k-byte) into L fragment registers
+    auto split_into = [&](uint32_t x, int i, v4i (&f)[L]) {
+        int32_t d[4] = {0, 0, 0, 0};
+        limbs_of(x, p, half, L, d);
+#pragma unroll
+        for (int s = 0; s < L; ++s) f[s][i >> 2] |= (int)(((uint32_t)d[s] & 0xffu) << (8 * (i & 3)));
+    };
+    for (uint32_t rb = wave; rb < nrb; rb += RT / 64) {
+        // this lane's row of the block and its 16 k-values (k-bytes 16 lg .. 16 lg + 15)
+        const uint32_t u = rb * 32 + lr;
+        const bool live = u < mu && !chosen[u];
+        const int64_t rowoff = live ? (int64_t)list[u] * job.lda + c : 0;
+        v4i fa[L];
+#pragma unroll
+        for (int s = 0; s < L; ++s) fa[s] = v4i{0, 0, 0, 0};
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const uint32_t t = 16 * lg + i;
+            split_into((live && t < k) ? job.A[rowoff + s_q[t]] : 0u, i, fa);
+        }
+        for (uint32_t cb = 0; cb < ncb; ++cb) {
+            const uint32_t j = wc + cb * 32 + lr;
+            v4i fb[L];
+#pragma unroll
+            for (int s = 0; s < L; ++s) fb[s] = v4i{0, 0, 0, 0};
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const uint32_t t = 16 * lg + i;
+                split_into((j < nb && t < k) ? B[t * nb + j] : 0u, i, fb);
+            }
+            v16i acc[ND];
+#pragma unroll
+            for (int d = 0; d < ND; ++d)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) acc[d][r] = 0;
+#pragma unroll
+            for (int s = 0; s < L; ++s)
+#pragma unroll
+                for (int t = 0; t < L; ++t)
+                    acc[s + t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[s], fb[t], acc[s + t], 0, 0, 0);
+            if (j >= nb) continue;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const uint32_t ur = rb * 32 + (r / 4) * 8 + lg * 4 + (r % 4);
+                if (ur >= mu || chosen[ur]) continue;
+                int32_t D[ND];
+                bool nzd = false;
+#pragma unroll
+                for (int d = 0; d < ND; ++d) {
+                    D[d] = acc[d][r];
+                    nzd |= D[d] != 0;
+                }
+                if (!nzd) continue;                                   // a zero update leaves the entry
+                const uint32_t res = limb_recombine<ND>(D, a.lc, a.M);
+                uint32_t* x = job.A + (int64_t)list[ur] * job.lda + c + j;
+                *x = subm(*x, res, p);
+            }
+        }
+    }
+}
 
 __device__ __forceinline__ uint32_t wave_min(uint32_t x) {
 #pragma unroll
@@ -179,34 +262,13 @@ __device__ void rref_one(const RJob& job, const Ar& ar, const RArgs& a, uint32_t
             job.pivots[r + tid] = c + s_q[tid];
             job.pivot_rows[r + tid] = s_P[tid];
         }
-        // (d) trailing update of the other unused rows: S[i, j] -= sum_t S[i, J_t] X[t, j]
-        const uint32_t tc = nb - wc;
-        if (tc > 0) {
-            for (uint32_t u = wave; u < mu; u += RT / 64) {
-                if (chosen[u]) continue;
-                const int64_t row = (int64_t)list[u] * lda;
-                uint32_t L[RW];
-                bool any = false;
-#pragma unroll
-                for (int t = 0; t < RW; ++t) {
-                    L[t] = (uint32_t)t < k ? job.A[row + c + s_q[t]] : 0u;
-                    any |= L[t] != 0;
-                }
-                if (!any) continue;
-                for (uint32_t jj = lane; jj < tc; jj += 64) {
-                    const uint32_t j = wc + jj;
-                    uint64_t lo = 0;
-                    uint32_t hi = 0;
-#pragma unroll
-                    for (int t = 0; t < RW; ++t) {
-                        if ((uint32_t)t >= k) break;
-                        const uint64_t prod = (uint64_t)L[t] * B[t * nb + j];
-                        lo += prod;
-                        hi += lo < prod ? 1u : 0u;
-                    }
-                    uint32_t* x = job.A + row + c + j;
-                    *x = subm(*x, red96(hi, lo, a.M), p);
-                }
+        // (d) trailing update of the other unused rows: S[i, j] -= sum_t S[i, J_t] X[t, j] (MFMA)
+        if (nb > wc) {
+            switch (a.L) {
+                case 1: trailing_mfma<1>(job, a, B, nb, wc, k, c, mu, list, chosen, s_q); break;
+                case 2: trailing_mfma<2>(job, a, B, nb, wc, k, c, mu, list, chosen, s_q); break;
+                case 3: trailing_mfma<3>(job, a, B, nb, wc, k, c, mu, list, chosen, s_q); break;
+                default: trailing_mfma<4>(job, a, B, nb, wc, k, c, mu, list, chosen, s_q); break;
             }
         }
         __syncthreads();
@@ -245,16 +307,25 @@ __device__ void rref_one(const RJob& job, const Ar& ar, const RArgs& a, uint32_t
         uint32_t t0 = t1 - 1;
         while (t0 > 0 && piv[t0 - 1] / w == b) --t0;
         if (t1 < r) {
-            const uint32_t jstart = (b + 1) * w;
+            const uint32_t jstart = (b + 1) * w, nc = r - t1;
+            // the block's coefficients U[t][piv[t2]] (t in [t0, t1), t2 in [t1, r)) into LDS first:
+            // (t1 - t0) * (r - t1) <= w * n words, inside the panel region
+            uint32_t* coefs = pan;
+            for (uint32_t idx = tid; idx < (t1 - t0) * nc; idx += RT) {
+                const uint32_t tt = idx / nc, t2 = t1 + idx % nc;
+                coefs[idx] = job.U[(size_t)(t0 + tt) * n + piv[t2]];
+            }
+            __syncthreads();
             for (uint32_t t = t0 + wave; t < t1; t += RT / 64) {
                 uint32_t* Ut = job.U + (size_t)t * n;
+                const uint32_t* ct = coefs + (size_t)(t - t0) * nc;
                 for (uint32_t j0 = jstart; j0 < n; j0 += 64 * 8) {
                     uint64_t lo[8];
                     uint32_t hi[8];
 #pragma unroll
                     for (int s = 0; s < 8; ++s) { lo[s] = 0; hi[s] = 0; }
                     for (uint32_t t2 = t1; t2 < r; ++t2) {
-                        const uint32_t coef = Ut[piv[t2]];
+                        const uint32_t coef = ct[t2 - t1];
                         if (coef == 0) continue;
                         const uint32_t* U2 = job.U + (size_t)t2 * n;
 #pragma unroll
@@ -293,7 +364,7 @@ __device__ void rref_one(const RJob& job, const Ar& ar, const RArgs& a, uint32_t
 }
 
 template <class Ar>
-__global__ __launch_bounds__(RT) void rref_batch_kernel(RArgs a, Ar ar) {
+__global__ __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(2, 8))) void rref_batch_kernel(RArgs a, Ar ar) {
     extern __shared__ __attribute__((aligned(16))) uint32_t sm[];
     __shared__ uint32_t s_job;
     for (;;) {
@@ -420,6 +491,10 @@ int rref_batch_run_impl(void* vb, hipStream_t s) {
     a.mmax = b->mmax;
     a.nmax = b->nmax;
     a.M = devmod(b->ctx);
+    a.lc = limb_consts(b->ctx.p);
+    a.half = (b->ctx.p - 1) / 2;
+    const uint32_t pp = b->ctx.p;                       // limbs from p itself (balanced digits)
+    a.L = pp <= 255u ? 1 : (pp <= 65279u ? 2 : (pp <= 16711423u ? 3 : 4));
     if (b->ctx.p & 1u) {
         rref_batch_kernel<ArMont><<<b->grid, RT, b->lds, s>>>(a, make_mont(b->ctx.p));
     } else {
