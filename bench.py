@@ -53,7 +53,8 @@ def parse():
     ap.add_argument("--no-howell", action="store_true", help="do not time the Howell batch")
     ap.add_argument("--howell-jobs", type=int, default=6666)
     ap.add_argument("--no-ring2k", action="store_true", help="do not time the Z/2^32 LU device session")
-    ap.add_argument("--only", choices=["batch", "lu", "mixed", "cpu", "howell", "ring2k"],
+    ap.add_argument("--no-level1", action="store_true", help="do not time the HBM-bound level-1/2 kernels")
+    ap.add_argument("--only", choices=["batch", "lu", "mixed", "cpu", "howell", "ring2k", "level1"],
                     help="run one secondary workload alone (for profiling) and print its record")
     ap.add_argument("--pmc-mode", action="store_true",
                     help="for counter collection: one 1 GiB device copy (the traffic calibration), then one limb "
@@ -396,6 +397,47 @@ def mixed_stream(njobs):
             "selector_choose_seconds": t_choose}
 
 
+def level1_kernels(E, n=8192, reps=20):
+    """The HBM-bound level-1/2 kernels on n x n residues (256 MiB per matrix at n = 8192): average
+    kernel time over `reps` launches (torch events on the launching stream) and the algorithmic bytes
+    each moves (every input read once, every output written once), against the 8 TB/s HBM peak;
+    torch's own device copy of the same bytes is timed beside them."""
+    import torch
+    from hacs_amd import workloads as W
+    p = W.P_C3
+    ctx = E.ModCtx(p)
+    X = E.random_entries(n, n, p, 0x51)
+    Y = E.random_entries(n, n, p, 0x52)
+    Z = torch.empty_like(X)
+    cols = torch.from_numpy(np.random.default_rng(1).permutation(n).astype(np.int32)).cuda().reshape(-1, 1)
+    xv = X[:, 0:1].contiguous()
+    yv = torch.empty((n, 1), dtype=torch.int32, device="cuda")
+    word = 4.0 * n * n
+    cases = {
+        "copy": (lambda: E.copy(X, Z), 2 * word),
+        "axpy": (lambda: E.axpy(ctx, 12345, X, Y), 3 * word),
+        "scal": (lambda: E.scal(ctx, 3, Y), 2 * word),
+        "transpose": (lambda: E.transpose(X, Z), 2 * word),
+        "gather": (lambda: E.gather(ctx, 1, 7, X, 0, cols, Z, False), 2 * word),
+        "gemv": (lambda: E.gemv(ctx, X, xv, yv), word),
+        "torch_copy": (lambda: Z.copy_(X), 2 * word),
+    }
+    out = {}
+    for name, (fn, nbytes) in cases.items():
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        out[name] = {"ms": ms, "bytes": nbytes, "GB_per_s": gbs, "frac_hbm": gbs / PEAK_HBM_GBS}
+    return {"n": n, "kernels": out}
+
+
 def cpu_baseline(p, threads):
     """The reference's own CPU GEMM (oracle/_ref, Kernels.cpp:1417) on a bounded sample, min of 3."""
     from oracle import oracle as O
@@ -445,6 +487,7 @@ def main():
                "mixed": lambda: mixed_stream(args.mixed_jobs),
                "howell": lambda: howell_batch(E, args.howell_jobs),
                "ring2k": lambda: ring2k_session(E),
+               "level1": lambda: level1_kernels(E),
                "cpu": lambda: cpu_baseline(p, min(16, os.cpu_count() or 1))}[args.only]()
         print(json.dumps({"only": args.only, **leg}), flush=True)
         return
@@ -522,6 +565,10 @@ def main():
         r2 = ring2k_session(E)
         log(f"ring2k session: {r2}")
         res["ring2k_session"] = r2
+    if not args.no_level1:
+        l1 = level1_kernels(E)
+        log(f"level-1/2 kernels: {l1}")
+        res["level1"] = l1
     if not args.no_mixed:
         mx = mixed_stream(args.mixed_jobs)
         log(f"mixed stream: {mx}")

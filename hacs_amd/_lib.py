@@ -72,6 +72,13 @@ class hx_howell_job(C.Structure):
                 ("work", C.c_void_p)]
 
 
+class hx_stats(C.Structure):
+    _fields_ = [("kernel_launches", C.c_uint64), ("gemm_launches", C.c_uint64), ("batched_instances", C.c_uint64),
+                ("limb_launches", C.c_uint64), ("h2d_bytes", C.c_uint64), ("d2h_bytes", C.c_uint64),
+                ("d2d_bytes", C.c_uint64), ("h2d_calls", C.c_uint64), ("d2h_calls", C.c_uint64),
+                ("mallocs", C.c_uint64), ("frees", C.c_uint64)]
+
+
 class hx_ring2k_step_args(C.Structure):
     _fields_ = [("m", C.c_uint32), ("r", C.c_uint32), ("np", C.c_uint32), ("pc0", C.c_uint32), ("pc1", C.c_uint32),
                 ("panel", C.c_void_p), ("ldp", C.c_size_t), ("swaps", C.c_void_p), ("nswaps", C.c_uint32),
@@ -118,6 +125,19 @@ SIGNATURES = {
     "hx_gemm_batch_run": (_INT, [_V, _V]),
     "hx_gemm_batch_destroy": (_INT, [_V]),
     "hx_trsm_unit_u32": (_INT, [C.POINTER(hx_modctx), hx_view, hx_view, _INT, _V]),
+    "hx_stats_get": (_INT, [C.POINTER(hx_stats)]),
+    "hx_stats_reset": (_INT, []),
+    "hx_gemm_nt_u32": (_INT, [C.POINTER(hx_modctx), hx_view, hx_view, hx_view, _INT, _V, C.POINTER(hx_gemm_opts)]),
+    "hx_gemv_u32": (_INT, [C.POINTER(hx_modctx), hx_view, hx_view, hx_view, _INT, _V]),
+    "hx_axpy_u32": (_INT, [C.POINTER(hx_modctx), _U32, hx_view, hx_view, _INT, _V]),
+    "hx_scal_u32": (_INT, [C.POINTER(hx_modctx), _U32, hx_view, _V]),
+    "hx_copy_u32": (_INT, [hx_view, hx_view, _V]),
+    "hx_transpose_u32": (_INT, [hx_view, hx_view, _V]),
+    "hx_fill_u32": (_INT, [hx_view, _U32, _V]),
+    "hx_axpy_identity_u32": (_INT, [C.POINTER(hx_modctx), _U32, hx_view, _INT, _V]),
+    "hx_gather_u32": (_INT, [C.POINTER(hx_modctx), _U32, _U32, hx_view, _U32, _V, hx_view, _INT, _V, _V]),
+    "hx_canonicalise_u32": (_INT, [C.POINTER(hx_modctx), hx_view, _V]),
+    "hx_canonicalise_acc64_u32": (_INT, [C.POINTER(hx_modctx), hx_view, hx_view, _V]),
     "hx_rref_u32": (_INT, [C.POINTER(hx_modctx), _U32, _U32, _V, _I64, _V, _I64, _U32P, _U32P, _U32P, _V]),
     "hx_rref_u32_host": (_INT, [C.POINTER(hx_modctx), _U32, _U32, _V, _V, _V, _U32, _U32P, _U32P, _U32P]),
     "hx_rref_batch_create": (_INT, [C.POINTER(hx_modctx), C.POINTER(hx_rref_job), _U32, _INT, C.POINTER(_V)]),

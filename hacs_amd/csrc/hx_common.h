@@ -47,10 +47,39 @@ int guard(const char* where, F&& body) {
     }
 }
 
-// Checks the launch just issued.
+// Process-wide diagnostic counters (launch_stats, Cuda/CudaKernels.h:299-306; transfer counters,
+// Cuda/CudaCtx.h:94-101).  Relaxed atomics: counts only, never synchronisation.
+enum StatId {
+    ST_KERNELS, ST_GEMMS, ST_BATCHED, ST_LIMB, ST_H2D_BYTES, ST_D2H_BYTES, ST_D2D_BYTES, ST_H2D_CALLS,
+    ST_D2H_CALLS, ST_MALLOCS, ST_FREES, ST_COUNT
+};
+void stat_add(StatId id, uint64_t n);
+
+// Checks the launch just issued (and counts it).
 inline void after_launch(const char* what) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) throw Error{HX_E_RUNTIME, std::string(what) + ": " + hipGetErrorString(e)};
+    stat_add(ST_KERNELS, 1);
+}
+
+// Whether the byte ranges spanned by two views intersect (windows_overlap, cuda_kernels.cu:384-394,
+// taken conservatively over the bounding byte ranges); eb = bytes per element of each view.
+inline bool views_overlap(const hx_view& a, const hx_view& b, int eba = 4, int ebb = 4) {
+    if (a.rows == 0 || a.cols == 0 || b.rows == 0 || b.cols == 0) return false;
+    auto span = [](const hx_view& v, int eb, uintptr_t& lo, uintptr_t& hi) {
+        const int64_t i1 = (int64_t)(v.rows - 1) * v.rs, j1 = (int64_t)(v.cols - 1) * v.cs;
+        int64_t mn = 0, mx = 0;
+        for (int64_t e : {i1, j1, i1 + j1}) {
+            mn = e < mn ? e : mn;
+            mx = e > mx ? e : mx;
+        }
+        lo = (uintptr_t)v.base + mn * eb;
+        hi = (uintptr_t)v.base + mx * eb + eb;
+    };
+    uintptr_t alo, ahi, blo, bhi;
+    span(a, eba, alo, ahi);
+    span(b, ebb, blo, bhi);
+    return !(ahi <= blo || bhi <= alo);
 }
 
 // Library-owned device scratch (host side): grows, never shrinks, per device.

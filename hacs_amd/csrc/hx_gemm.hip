@@ -1005,6 +1005,10 @@ void gemm_dispatch(const hx_modctx& ctx, const GemmArgs& g, hipStream_t s, int v
         gemm_limb(ctx, g, s, ws, ws_bytes, tile, ev0, ev1);
     else
         gemm_plain(ctx, g, s, tile, ev0, ev1);
+    if (g.m != 0 && g.n != 0) {
+        stat_add(ST_GEMMS, 1);
+        if (variant == HX_GEMM_LIMB) stat_add(ST_LIMB, 1);
+    }
 }
 
 // ---- batched uneven GEMM -------------------------------------------------------------------
@@ -1147,6 +1151,9 @@ void gemm_batch_run(void* vb, hipStream_t s) {
     const uint32_t* pre_plain = b->d_prefix;
     const uint32_t* pre_limb = pre_plain + b->n_plain + 1;
     const uint32_t* pre_prep = pre_limb + b->n_limb + 1;
+    stat_add(ST_BATCHED, (uint64_t)b->n_plain + b->n_limb);
+    stat_add(ST_GEMMS, (b->tiles_plain ? 1 : 0) + (b->tiles_limb ? 1 : 0));
+    if (b->tiles_limb) stat_add(ST_LIMB, 1);
     if (b->tiles_prep) {
         prep_batch_kernel<<<b->tiles_prep, 256, 0, s>>>(b->d_prep, pre_prep, b->n_prep);
         after_launch("prep_batch_kernel");
@@ -1188,21 +1195,6 @@ using namespace hx;
 
 namespace {
 
-bool overlap(const hx_view& a, const hx_view& b) {
-    if (a.rows == 0 || a.cols == 0 || b.rows == 0 || b.cols == 0) return false;
-    auto span = [](const hx_view& v, uintptr_t& lo, uintptr_t& hi) {
-        const int64_t i0 = 0, i1 = (int64_t)(v.rows - 1) * v.rs, j1 = (int64_t)(v.cols - 1) * v.cs;
-        const int64_t mn = std::min<int64_t>(i0, std::min<int64_t>(i1, std::min<int64_t>(j1, i1 + j1)));
-        const int64_t mx = std::max<int64_t>(i0, std::max<int64_t>(i1, std::max<int64_t>(j1, i1 + j1)));
-        lo = (uintptr_t)v.base + mn * 4;
-        hi = (uintptr_t)v.base + mx * 4 + 4;
-    };
-    uintptr_t alo, ahi, blo, bhi;
-    span(a, alo, ahi);
-    span(b, blo, bhi);
-    return !(ahi <= blo || bhi <= alo);
-}
-
 GemmArgs args_of(hx_view A, hx_view B, hx_view C, int op) {
     GemmArgs g;
     g.m = A.rows;
@@ -1240,7 +1232,7 @@ HX_API int hx_gemm_u32(const hx_modctx* ctx, hx_view A, hx_view B, hx_view C, in
         HX_CHECK(ctx != nullptr && ctx->p >= 2, "bad modulus context");
         HX_CHECK(A.cols == B.rows && C.rows == A.rows && C.cols == B.cols, "gemm: shape mismatch");
         HX_CHECK(op == HX_OP_SET || op == HX_OP_ADD || op == HX_OP_SUB, "gemm: bad op");
-        HX_CHECK(!overlap(C, A) && !overlap(C, B), "gemm: C aliases an operand");
+        HX_CHECK(!views_overlap(C, A) && !views_overlap(C, B), "gemm: C aliases an operand");
         HX_CHECK(C.rows == 0 || C.cols == 0 || C.base != nullptr, "gemm: C is NULL");
         const int variant = opts ? opts->variant : HX_GEMM_AUTO;
         HX_CHECK(variant != HX_GEMM_LIMB || ctx->limbs > 0, "gemm: the limb route needs p <= 4278124287");

@@ -17,9 +17,10 @@
 //         panel on the INT8 matrix cores (both operands split into balanced 8-bit limbs on the fly,
 //         exact int32 limb products, exact recombination; trailing_mfma) -- the Schur complement
 //         w.r.t. the panel pivots; the panel's own columns of those rows become zero and are dropped.
+//         In the same pass the pivot rows stored so far (U rows 0 .. r-1) are reduced against this
+//         panel's pivots, U[t, c:n] -= U[t, c + J] X (Gauss-Jordan), so they are already RREF rows
+//         when the forward pass ends and no back substitution is left.
 //     (e) X is stored as the next k rows of the U workspace (pivot order).
-//   back substitution: U's rows have the identity at their own panel's pivot columns; panels are
-//   cleared against all later rows from the bottom (non-pivot columns only: pivot columns are 0 / 1).
 //   The rank rows of U are written over the job's matrix, then zero rows (RREF with zero rows last).
 //
 // Every value is exact; the RREF, its pivot columns and the smallest-row pivot_rows are unique
@@ -65,20 +66,24 @@ struct RArgs {
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
-// (d) on the INT8 matrix cores: S[list[u], c + j] -= sum_{t<k} S[list[u], c + J_t] X[t, j] for the
-// unused rows u and the trailing columns j in [wc, nb), k <= 32 (one 32-deep MFMA k-step).  Both
-// operands are split into L balanced 8-bit limbs on the fly (the X fragments from LDS, the row
-// fragments from the job's matrix), the L^2 limb products accumulate per anti-diagonal in int32,
-// and the exact recombination (hx_limbs.h) gives the residue subtracted from the entry.  Each wave
-// takes 32-row blocks; per block it walks the 32-column blocks of the trailing columns.
+// (d) on the INT8 matrix cores, k <= 32 (one 32-deep MFMA k-step), for two row sets:
+//   the unused rows u:        S[list[u], c + j] -= sum_{t<k} S[list[u], c + J_t] X[t, j],  j in [wc, nb)
+//   the stored pivot rows t:  U[t, c + j]       -= sum_{t'<k} U[t, c + J_t'] X[t', j],  j in [0, nb)
+// (the second set is the Gauss-Jordan half: the earlier pivot rows are reduced against this panel's
+// pivots as the panel is made, so no back substitution is left at the end; X is the identity on
+// the pivot columns, so those entries become exactly zero).  Both operands are split into L
+// balanced 8-bit limbs on the fly (the X fragments from LDS, the row fragments from HBM), the L^2
+// limb products accumulate per anti-diagonal in int32, and the exact recombination (hx_limbs.h)
+// gives the residue subtracted from the entry.  A wave takes a 32-row block of either set, loads
+// the block's k coefficients into registers before any write, then walks the 32-column blocks.
 template <int L>
 __device__ void trailing_mfma(const RJob& job, const RArgs& a, const uint32_t* B, uint32_t nb, uint32_t wc,
                               uint32_t k, uint32_t c, uint32_t mu, const uint32_t* list, const uint8_t* chosen,
-                              const uint32_t* s_q) {
+                              const uint32_t* s_q, uint32_t r) {
     constexpr int ND = 2 * L - 1;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, lr = lane & 31, lg = lane >> 5;
-    const uint32_t p = a.M.p, half = a.half, tc = nb - wc;
-    const uint32_t nrb = (mu + 31) / 32, ncb = (tc + 31) / 32;
+    const uint32_t p = a.M.p, half = a.half;
+    const uint32_t nrb_u = nb > wc ? (mu + 31) / 32 : 0, nrb = nrb_u + (r + 31) / 32;
     // the limb planes of 16 residues (one per k-byte) into L fragment registers
     auto split_into = [&](uint32_t x, int i, v4i (&f)[L]) {
         int32_t d[4] = {0, 0, 0, 0};
@@ -87,20 +92,27 @@ __device__ void trailing_mfma(const RJob& job, const RArgs& a, const uint32_t* B
         for (int s = 0; s < L; ++s) f[s][i >> 2] |= (int)(((uint32_t)d[s] & 0xffu) << (8 * (i & 3)));
     };
     for (uint32_t rb = wave; rb < nrb; rb += RT / 64) {
+        const bool urows = rb >= nrb_u;                       // a block of stored pivot rows
+        const uint32_t b0 = (urows ? rb - nrb_u : rb) * 32, nrows = urows ? r : mu, jbeg = urows ? 0 : wc;
+        uint32_t* base = urows ? job.U : job.A;
+        const int64_t ld = urows ? (int64_t)job.n : job.lda;
+        auto row_of = [&](uint32_t u) -> int64_t { return urows ? (int64_t)u : (int64_t)list[u]; };
+        auto live_row = [&](uint32_t u) { return u < nrows && (urows || !chosen[u]); };
         // this lane's row of the block and its 16 k-values (k-bytes 16 lg .. 16 lg + 15)
-        const uint32_t u = rb * 32 + lr;
-        const bool live = u < mu && !chosen[u];
-        const int64_t rowoff = live ? (int64_t)list[u] * job.lda + c : 0;
+        const uint32_t u = b0 + lr;
+        const bool live = live_row(u);
+        const int64_t rowoff = live ? row_of(u) * ld + c : 0;
         v4i fa[L];
 #pragma unroll
         for (int s = 0; s < L; ++s) fa[s] = v4i{0, 0, 0, 0};
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
             const uint32_t t = 16 * lg + i;
-            split_into((live && t < k) ? job.A[rowoff + s_q[t]] : 0u, i, fa);
+            split_into((live && t < k) ? base[rowoff + s_q[t]] : 0u, i, fa);
         }
+        const uint32_t ncb = (nb - jbeg + 31) / 32;
         for (uint32_t cb = 0; cb < ncb; ++cb) {
-            const uint32_t j = wc + cb * 32 + lr;
+            const uint32_t j = jbeg + cb * 32 + lr;
             v4i fb[L];
 #pragma unroll
             for (int s = 0; s < L; ++s) fb[s] = v4i{0, 0, 0, 0};
@@ -113,7 +125,7 @@ __device__ void trailing_mfma(const RJob& job, const RArgs& a, const uint32_t* B
 #pragma unroll
             for (int d = 0; d < ND; ++d)
 #pragma unroll
-                for (int r = 0; r < 16; ++r) acc[d][r] = 0;
+                for (int e = 0; e < 16; ++e) acc[d][e] = 0;
 #pragma unroll
             for (int s = 0; s < L; ++s)
 #pragma unroll
@@ -121,19 +133,19 @@ __device__ void trailing_mfma(const RJob& job, const RArgs& a, const uint32_t* B
                     acc[s + t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[s], fb[t], acc[s + t], 0, 0, 0);
             if (j >= nb) continue;
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const uint32_t ur = rb * 32 + (r / 4) * 8 + lg * 4 + (r % 4);
-                if (ur >= mu || chosen[ur]) continue;
+            for (int e = 0; e < 16; ++e) {
+                const uint32_t ur = b0 + (e / 4) * 8 + lg * 4 + (e % 4);
+                if (!live_row(ur)) continue;
                 int32_t D[ND];
                 bool nzd = false;
 #pragma unroll
                 for (int d = 0; d < ND; ++d) {
-                    D[d] = acc[d][r];
+                    D[d] = acc[d][e];
                     nzd |= D[d] != 0;
                 }
                 if (!nzd) continue;                                   // a zero update leaves the entry
                 const uint32_t res = limb_recombine<ND>(D, a.lc, a.M);
-                uint32_t* x = job.A + (int64_t)list[ur] * job.lda + c + j;
+                uint32_t* x = base + row_of(ur) * ld + c + j;
                 *x = subm(*x, res, p);
             }
         }
@@ -253,7 +265,15 @@ __device__ void rref_one(const RJob& job, const Ar& ar, const RArgs& a, uint32_t
         }
         __syncthreads();
 
-        // (e) U rows, pivots, pivot rows
+        // (d) trailing update of the other unused rows, S[i, j] -= sum_t S[i, J_t] X[t, j], and the
+        // reduction of the stored pivot rows 0 .. r-1 against this panel's pivots (MFMA)
+        switch (a.L) {
+            case 1: trailing_mfma<1>(job, a, B, nb, wc, k, c, mu, list, chosen, s_q, r); break;
+            case 2: trailing_mfma<2>(job, a, B, nb, wc, k, c, mu, list, chosen, s_q, r); break;
+            case 3: trailing_mfma<3>(job, a, B, nb, wc, k, c, mu, list, chosen, s_q, r); break;
+            default: trailing_mfma<4>(job, a, B, nb, wc, k, c, mu, list, chosen, s_q, r); break;
+        }
+        // (e) this panel's pivot rows (X) become U rows r .. r+k-1; pivots, pivot rows
         for (uint32_t idx = tid; idx < k * n; idx += RT) {
             const uint32_t t = idx / n, j = idx % n;
             job.U[(size_t)(r + t) * n + j] = j < c ? 0u : B[t * nb + (j - c)];
@@ -261,15 +281,6 @@ __device__ void rref_one(const RJob& job, const Ar& ar, const RArgs& a, uint32_t
         if (tid < k) {
             job.pivots[r + tid] = c + s_q[tid];
             job.pivot_rows[r + tid] = s_P[tid];
-        }
-        // (d) trailing update of the other unused rows: S[i, j] -= sum_t S[i, J_t] X[t, j] (MFMA)
-        if (nb > wc) {
-            switch (a.L) {
-                case 1: trailing_mfma<1>(job, a, B, nb, wc, k, c, mu, list, chosen, s_q); break;
-                case 2: trailing_mfma<2>(job, a, B, nb, wc, k, c, mu, list, chosen, s_q); break;
-                case 3: trailing_mfma<3>(job, a, B, nb, wc, k, c, mu, list, chosen, s_q); break;
-                default: trailing_mfma<4>(job, a, B, nb, wc, k, c, mu, list, chosen, s_q); break;
-            }
         }
         __syncthreads();
         // stable compaction of the unused list (chosen rows leave)
@@ -297,63 +308,6 @@ __device__ void rref_one(const RJob& job, const Ar& ar, const RArgs& a, uint32_t
         r += k;
     }
 
-    // back substitution across panels (U rows in pivot order; panel of pivot t = pivots[t] / w)
-    uint32_t* piv = list;
-    for (uint32_t t = tid; t < r; t += RT) piv[t] = job.pivots[t];
-    __syncthreads();
-    uint32_t t1 = r;
-    while (t1 > 0) {
-        const uint32_t b = piv[t1 - 1] / w;
-        uint32_t t0 = t1 - 1;
-        while (t0 > 0 && piv[t0 - 1] / w == b) --t0;
-        if (t1 < r) {
-            const uint32_t jstart = (b + 1) * w, nc = r - t1;
-            // the block's coefficients U[t][piv[t2]] (t in [t0, t1), t2 in [t1, r)) into LDS first:
-            // (t1 - t0) * (r - t1) <= w * n words, inside the panel region
-            uint32_t* coefs = pan;
-            for (uint32_t idx = tid; idx < (t1 - t0) * nc; idx += RT) {
-                const uint32_t tt = idx / nc, t2 = t1 + idx % nc;
-                coefs[idx] = job.U[(size_t)(t0 + tt) * n + piv[t2]];
-            }
-            __syncthreads();
-            for (uint32_t t = t0 + wave; t < t1; t += RT / 64) {
-                uint32_t* Ut = job.U + (size_t)t * n;
-                const uint32_t* ct = coefs + (size_t)(t - t0) * nc;
-                for (uint32_t j0 = jstart; j0 < n; j0 += 64 * 8) {
-                    uint64_t lo[8];
-                    uint32_t hi[8];
-#pragma unroll
-                    for (int s = 0; s < 8; ++s) { lo[s] = 0; hi[s] = 0; }
-                    for (uint32_t t2 = t1; t2 < r; ++t2) {
-                        const uint32_t coef = ct[t2 - t1];
-                        if (coef == 0) continue;
-                        const uint32_t* U2 = job.U + (size_t)t2 * n;
-#pragma unroll
-                        for (int s = 0; s < 8; ++s) {
-                            const uint32_t j = j0 + lane + 64 * s;
-                            if (j < n) {
-                                const uint64_t prod = (uint64_t)coef * U2[j];
-                                lo[s] += prod;
-                                hi[s] += lo[s] < prod ? 1u : 0u;
-                            }
-                        }
-                    }
-#pragma unroll
-                    for (int s = 0; s < 8; ++s) {
-                        const uint32_t j = j0 + lane + 64 * s;
-                        if (j < n && !ispiv[j]) Ut[j] = subm(Ut[j], red96(hi[s], lo[s], a.M), p);
-                    }
-                }
-            }
-            __syncthreads();
-            for (uint32_t idx = tid; idx < (t1 - t0) * (r - t1); idx += RT) {
-                const uint32_t t = t0 + idx / (r - t1), t2 = t1 + idx % (r - t1);
-                job.U[(size_t)t * n + piv[t2]] = 0;
-            }
-            __syncthreads();
-        }
-        t1 = t0;
-    }
     // the RREF over the job's matrix: rank rows of U, then zero rows
     for (uint64_t idx = tid; idx < (uint64_t)m * n; idx += RT) {
         const uint32_t i = (uint32_t)(idx / n), j = (uint32_t)(idx % n);

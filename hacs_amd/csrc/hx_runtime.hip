@@ -5,6 +5,7 @@
 // runtime stages pageable copies itself, and pinned buffers are the caller's choice.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -15,6 +16,10 @@
 namespace hx {
 
 static thread_local std::string g_last_error;
+
+static std::atomic<uint64_t> g_stat[ST_COUNT];
+
+void stat_add(StatId id, uint64_t n) { g_stat[id].fetch_add(n, std::memory_order_relaxed); }
 
 void set_error(const std::string& msg) { g_last_error = msg; }
 int fail(int code, const std::string& msg) {
@@ -207,6 +212,28 @@ HX_API int hx_event_destroy(hx_event e) {
     return guard("hx_event_destroy", [&] { HX_HIP(hipEventDestroy((hipEvent_t)e)); });
 }
 
+HX_API int hx_stats_get(hx_stats* out) {
+    return guard("hx_stats_get", [&] {
+        HX_CHECK(out != nullptr, "out is NULL");
+        auto g = [](StatId id) { return g_stat[id].load(std::memory_order_relaxed); };
+        out->kernel_launches = g(ST_KERNELS);
+        out->gemm_launches = g(ST_GEMMS);
+        out->batched_instances = g(ST_BATCHED);
+        out->limb_launches = g(ST_LIMB);
+        out->h2d_bytes = g(ST_H2D_BYTES);
+        out->d2h_bytes = g(ST_D2H_BYTES);
+        out->d2d_bytes = g(ST_D2D_BYTES);
+        out->h2d_calls = g(ST_H2D_CALLS);
+        out->d2h_calls = g(ST_D2H_CALLS);
+        out->mallocs = g(ST_MALLOCS);
+        out->frees = g(ST_FREES);
+    });
+}
+HX_API int hx_stats_reset(void) {
+    for (auto& c : g_stat) c.store(0, std::memory_order_relaxed);
+    return HX_OK;
+}
+
 HX_API int hx_malloc(void** ptr, size_t bytes) {
     return guard("hx_malloc", [&] {
         HX_CHECK(ptr != nullptr, "ptr is NULL");
@@ -216,11 +243,15 @@ HX_API int hx_malloc(void** ptr, size_t bytes) {
             (void)hipGetLastError();
             throw Error{HX_E_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed"};
         }
+        stat_add(ST_MALLOCS, 1);
     });
 }
 HX_API int hx_free(void* ptr) {
     return guard("hx_free", [&] {
-        if (ptr) HX_HIP(hipFree(ptr));
+        if (ptr) {
+            HX_HIP(hipFree(ptr));
+            stat_add(ST_FREES, 1);
+        }
     });
 }
 HX_API int hx_upload(void* dev, const void* host, size_t bytes, hx_stream s) {
@@ -228,6 +259,8 @@ HX_API int hx_upload(void* dev, const void* host, size_t bytes, hx_stream s) {
         if (bytes == 0) return;
         HX_HIP(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, (hipStream_t)s));
         HX_HIP(hipStreamSynchronize((hipStream_t)s));
+        stat_add(ST_H2D_BYTES, bytes);
+        stat_add(ST_H2D_CALLS, 1);
     });
 }
 HX_API int hx_download(void* host, const void* dev, size_t bytes, hx_stream s) {
@@ -235,12 +268,15 @@ HX_API int hx_download(void* host, const void* dev, size_t bytes, hx_stream s) {
         if (bytes == 0) return;
         HX_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, (hipStream_t)s));
         HX_HIP(hipStreamSynchronize((hipStream_t)s));
+        stat_add(ST_D2H_BYTES, bytes);
+        stat_add(ST_D2H_CALLS, 1);
     });
 }
 HX_API int hx_copy_d2d(void* dst, const void* src, size_t bytes, hx_stream s) {
     return guard("hx_copy_d2d", [&] {
         if (bytes == 0) return;
         HX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)s));
+        stat_add(ST_D2D_BYTES, bytes);
     });
 }
 HX_API int hx_memset(void* dst, int value, size_t bytes, hx_stream s) {

@@ -101,7 +101,90 @@ def gemm(ctx: ModCtx, A: torch.Tensor, B: torch.Tensor, C_: torch.Tensor, op: in
 
 def gemm_nt(ctx: ModCtx, A, B, C_, op: int = GemmOp.Set, stream=None, variant: int = HX_GEMM_AUTO) -> None:
     """C op= A * B^T with B given n x k (engine::cuda::gemm_nt, CudaKernels.h:125-128)."""
-    gemm(ctx, A, B.t(), C_, op, stream, variant)
+    o = _opts(variant)
+    check(lib().hx_gemm_nt_u32(C.byref(ctx.c), view(A), view(B), view(C_), op,
+                               current_stream() if stream is None else stream, C.byref(o)))
+
+
+def _s(stream):
+    return current_stream() if stream is None else stream
+
+
+def _ptr(t: torch.Tensor | None):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def view64(t: torch.Tensor) -> _lib.hx_view:
+    """hx_view of a 2-D device tensor of 64-bit accumulators (strides in 8-byte elements)."""
+    if t.dtype not in (torch.int64, torch.uint64) or not t.is_cuda or t.dim() != 2:
+        raise TypeError("expected a 2-D int64/uint64 device tensor")
+    return _lib.hx_view(C.c_void_p(t.data_ptr()), t.stride(0), t.stride(1), t.shape[0], t.shape[1])
+
+
+def gemv(ctx: ModCtx, A, x, y, op: int = GemmOp.Set, stream=None) -> None:
+    """y op= A x with x a k x 1 and y an m x 1 view (engine::cuda::gemv, CudaKernels.h:187-191)."""
+    check(lib().hx_gemv_u32(C.byref(ctx.c), view(A), view(x), view(y), op, _s(stream)))
+
+
+def axpy(ctx: ModCtx, alpha: int, x, y, subtract: bool = False, stream=None) -> None:
+    """y = y +/- alpha x (engine::cuda::axpy, CudaKernels.h:193-196)."""
+    check(lib().hx_axpy_u32(C.byref(ctx.c), alpha, view(x), view(y), 1 if subtract else 0, _s(stream)))
+
+
+def scal(ctx: ModCtx, alpha: int, x, stream=None) -> None:
+    """x = alpha x (engine::cuda::scal, CudaKernels.h:198-200)."""
+    check(lib().hx_scal_u32(C.byref(ctx.c), alpha, view(x), _s(stream)))
+
+
+def copy(src, dst, stream=None) -> None:
+    """dst = src, any strides (engine::cuda::copy, CudaKernels.h:202-204)."""
+    check(lib().hx_copy_u32(view(src), view(dst), _s(stream)))
+
+
+def transpose(src, dst, stream=None) -> None:
+    """dst(i, j) = src(j, i) (engine::cuda::transpose, CudaKernels.h:206-208)."""
+    check(lib().hx_transpose_u32(view(src), view(dst), _s(stream)))
+
+
+def fill(x, value: int, stream=None) -> None:
+    """x(i, j) = value (engine::cuda::fill, CudaKernels.h:211-213)."""
+    check(lib().hx_fill_u32(view(x), value, _s(stream)))
+
+
+def axpy_identity(ctx: ModCtx, alpha: int, y, subtract: bool = False, stream=None) -> None:
+    """y(i, i) = y(i, i) +/- alpha (engine::cuda::axpy_identity, CudaKernels.h:215-219)."""
+    check(lib().hx_axpy_identity_u32(C.byref(ctx.c), alpha, view(y), 1 if subtract else 0, _s(stream)))
+
+
+def gather(ctx: ModCtx, alpha: int, beta: int, a, row_off: int, cols, c, accumulate: bool = False,
+           rows=None, stream=None) -> None:
+    """c(i, t) [+]= alpha [row_off + r(i) == col(t)] + beta a(r(i), col(t)) (engine::cuda::gather,
+    CudaKernels.h:221-230).  a None = the identity source; cols / rows device index tensors or None."""
+    av = _lib.hx_view(None, 0, 0, 0, 0) if a is None else view(a)
+    check(lib().hx_gather_u32(C.byref(ctx.c), alpha, beta, av, row_off, _ptr(cols), view(c),
+                              1 if accumulate else 0, _ptr(rows), _s(stream)))
+
+
+def canonicalise(ctx: ModCtx, v, stream=None) -> None:
+    """Lazy2p -> canonical in place (engine::cuda::canonicalise, CudaKernels.h:232-234)."""
+    check(lib().hx_canonicalise_u32(C.byref(ctx.c), view(v), _s(stream)))
+
+
+def canonicalise_acc64(ctx: ModCtx, acc, dst, stream=None) -> None:
+    """dst = acc mod p for 64-bit accumulators (engine::cuda::canonicalise, CudaKernels.h:236-238)."""
+    check(lib().hx_canonicalise_acc64_u32(C.byref(ctx.c), view64(acc), view(dst), _s(stream)))
+
+
+def launch_stats() -> dict:
+    """The library's diagnostic counters (launch_stats, CudaKernels.h:299-306; TransferStats,
+    CudaCtx.h:94-101)."""
+    st = _lib.hx_stats()
+    check(lib().hx_stats_get(C.byref(st)))
+    return {name: int(getattr(st, name)) for name, _ in st._fields_}
+
+
+def reset_launch_stats() -> None:
+    check(lib().hx_stats_reset())
 
 
 def schur_update(ctx: ModCtx, L, U, C_, stream=None, variant: int = HX_GEMM_AUTO) -> None:

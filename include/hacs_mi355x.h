@@ -125,6 +125,20 @@ HX_API int hx_download(void *host, const void *dev, size_t bytes, hx_stream s); 
 HX_API int hx_copy_d2d(void *dst, const void *src, size_t bytes, hx_stream s);
 HX_API int hx_memset(void *dst, int value, size_t bytes, hx_stream s);
 
+/* ---- diagnostic counters (launch_stats / reset_launch_stats, CudaKernels.h:299-306;
+ *      TransferStats, CudaCtx.h:94-101).  Process-wide, counts only. ------------------- */
+typedef struct hx_stats {
+    uint64_t kernel_launches;    /* every kernel the library launched */
+    uint64_t gemm_launches;      /* GEMM product launches (single or batched; a batch counts per kernel) */
+    uint64_t batched_instances;  /* GEMM jobs run through batches */
+    uint64_t limb_launches;      /* launches of the INT8-MFMA limb route (the IMMA counter's counterpart) */
+    uint64_t h2d_bytes, d2h_bytes, d2d_bytes;   /* hx_upload / hx_download / hx_copy_d2d */
+    uint64_t h2d_calls, d2h_calls;
+    uint64_t mallocs, frees;     /* hx_malloc / hx_free */
+} hx_stats;
+HX_API int hx_stats_get(hx_stats *out);
+HX_API int hx_stats_reset(void);
+
 /* ---- graph capture (CudaCtx.h:223-253 GraphCapture) ---------------------- */
 HX_API int hx_graph_begin(hx_stream s);
 HX_API int hx_graph_end(hx_stream s, hx_graph *g);
@@ -167,6 +181,40 @@ HX_API int hx_gemm_batch_destroy(hx_gemm_batch b);
 /* ---- triangular solve (CudaKernels.h:240-247 trsm_unit) ------------------ */
 /* B <- T^{-1} B in place, T square unit triangular (only the strict triangle is read). */
 HX_API int hx_trsm_unit_u32(const hx_modctx *ctx, hx_view T, hx_view B, int upper, hx_stream s);
+
+/* ---- level-1/2 and tape-support kernels (CudaKernels.h:186-238) --------------------------
+ * Views are any-stride windows (hx_view); every operand is canonical (< p) unless stated, and an
+ * output must not overlap an input (HX_E_ARGUMENT), as in cuda_kernels.cu:950-1087.  All are
+ * asynchronous on s, allocate nothing and are capturable. */
+/* C op= A * B^T mod p (gemm_nt, CudaKernels.h:125-128): B is n x k. */
+HX_API int hx_gemm_nt_u32(const hx_modctx *ctx, hx_view A, hx_view B, hx_view C, int op, hx_stream s,
+                          const hx_gemm_opts *opts);
+/* y op= A * x (gemv, CudaKernels.h:187-191): x a k x 1 view, y an m x 1 view. */
+HX_API int hx_gemv_u32(const hx_modctx *ctx, hx_view A, hx_view x, hx_view y, int op, hx_stream s);
+/* y = y + alpha x, or y - alpha x when subtract != 0 (axpy, CudaKernels.h:193-196). */
+HX_API int hx_axpy_u32(const hx_modctx *ctx, uint32_t alpha, hx_view x, hx_view y, int subtract, hx_stream s);
+/* x = alpha x (scal, CudaKernels.h:198-200). */
+HX_API int hx_scal_u32(const hx_modctx *ctx, uint32_t alpha, hx_view x, hx_stream s);
+/* dst(i, j) = src(i, j), any strides (copy, CudaKernels.h:202-204); identical views are a no-op. */
+HX_API int hx_copy_u32(hx_view src, hx_view dst, hx_stream s);
+/* dst(i, j) = src(j, i) (transpose, CudaKernels.h:206-208). */
+HX_API int hx_transpose_u32(hx_view src, hx_view dst, hx_stream s);
+/* x(i, j) = value (fill, CudaKernels.h:211-213). */
+HX_API int hx_fill_u32(hx_view x, uint32_t value, hx_stream s);
+/* y(i, i) = y(i, i) +/- alpha for i < min(rows, cols) (axpy_identity, CudaKernels.h:215-219). */
+HX_API int hx_axpy_identity_u32(const hx_modctx *ctx, uint32_t alpha, hx_view y, int subtract, hx_stream s);
+/* The tape's GATHER (gather, CudaKernels.h:221-230):
+ *   c(i, t) [+]= alpha * [row_off + r(i) == col(t)] + beta * a(r(i), col(t))
+ * col(t) = cols[t] (device list of c.cols entries) or t when cols == NULL; r(i) = rows[i] (device
+ * list of c.rows entries) or i when rows == NULL.  a may be empty (a.base == NULL: identity source);
+ * accumulate == 0 is Set.  a is read only when a.base != NULL and beta != 0. */
+HX_API int hx_gather_u32(const hx_modctx *ctx, uint32_t alpha, uint32_t beta, hx_view a, uint32_t row_off,
+                         const uint32_t *cols, hx_view c, int accumulate, const uint32_t *rows, hx_stream s);
+/* Lazy2p -> canonical in place: v >= p ? v - p : v (canonicalise, CudaKernels.h:232-234). */
+HX_API int hx_canonicalise_u32(const hx_modctx *ctx, hx_view v, hx_stream s);
+/* dst(i, j) = acc(i, j) mod p for a uint64 accumulator view (strides in 8-byte elements)
+ * (canonicalise of an Accum64 view, CudaKernels.h:236-238). */
+HX_API int hx_canonicalise_acc64_u32(const hx_modctx *ctx, hx_view acc, hx_view dst, hx_stream s);
 
 /* ---- device RREF over F_p (CudaKernels.h:249-298 imma_rref; DenseElim.h:111-123 hook) ----
  * p prime, 2 <= p < 2^31.  A (m x n, row stride lda, device) is not modified.  R (device, row

@@ -246,6 +246,68 @@ def howell_fast(k, A):
 
 
 # ---------------------------------------------------------------------------
+# level-1/2 and tape-support kernels (Cuda/cuda_kernels.cu:590-713, 950-1087): exact numpy
+# restatements on uint64 (every product of two residues < 2^32 fits)
+# ---------------------------------------------------------------------------
+def _u64(M):
+    return np.asarray(M, dtype=np.uint64)
+
+
+def axpy(p, alpha, x, y, subtract=False):
+    """y +/- alpha x mod p (axpy_kernel, cuda_kernels.cu:609-623: red32(alpha x), then apply_op)."""
+    t = (np.uint64(alpha) * _u64(x)) % np.uint64(p)
+    y = _u64(y)
+    r = (y + np.uint64(p) - t) if subtract else (y + t)
+    return (r % np.uint64(p)).astype(np.uint32)
+
+
+def scal(p, alpha, x):
+    """alpha x mod p (scal_kernel, cuda_kernels.cu:624-633)."""
+    return ((np.uint64(alpha) * _u64(x)) % np.uint64(p)).astype(np.uint32)
+
+
+def canonicalise(p, v):
+    """v >= p ? v - p : v on a Lazy2p view (canon_kernel, cuda_kernels.cu:634-643)."""
+    v = np.asarray(v, dtype=np.uint32)
+    return np.where(v >= np.uint32(p), v - np.uint32(p), v).astype(np.uint32)
+
+
+def canonicalise_acc64(p, acc):
+    """acc mod p of 64-bit accumulators (canon64_kernel, cuda_kernels.cu:644-649)."""
+    return (_u64(acc) % np.uint64(p)).astype(np.uint32)
+
+
+def axpy_identity(p, alpha, y, subtract=False):
+    """y(i, i) +/- alpha on the diagonal of the window (axpy_identity_kernel, cuda_kernels.cu:666-672)."""
+    y = np.array(y, dtype=np.uint32)
+    n = min(y.shape)
+    d = _u64(y[np.arange(n), np.arange(n)])
+    d = (d + np.uint64(p) - np.uint64(alpha)) if subtract else (d + np.uint64(alpha))
+    y[np.arange(n), np.arange(n)] = (d % np.uint64(p)).astype(np.uint32)
+    return y
+
+
+def gather(p, alpha, beta, a, row_off, cols, c, accumulate=False, rows=None):
+    """c(i, t) [+]= alpha [row_off + r(i) == col(t)] + beta a(r(i), col(t)), r(i) = rows[i] or i,
+    col(t) = cols[t] or t; a None or beta == 0: a is not read (gather_kernel, cuda_kernels.cu:673-695).
+    row_off + r(i) is a 32-bit sum, as in the kernel."""
+    c = np.array(c, dtype=np.uint32)
+    m, n = c.shape
+    r = np.arange(m, dtype=np.uint64) if rows is None else _u64(rows)
+    col = np.arange(n, dtype=np.uint64) if cols is None else _u64(cols)
+    v = np.zeros((m, n), np.uint64)
+    if a is not None and beta != 0:
+        av = _u64(a)[r.astype(np.int64)[:, None], col.astype(np.int64)[None, :]]
+        v = (np.uint64(beta) * av) % np.uint64(p)
+    if alpha != 0:
+        hit = ((r[:, None] + np.uint64(row_off)) & np.uint64(0xFFFFFFFF)) == col[None, :]
+        v = np.where(hit, (v + np.uint64(alpha)) % np.uint64(p), v)
+    if accumulate:
+        v = (_u64(c) + v) % np.uint64(p)
+    return v.astype(np.uint32)
+
+
+# ---------------------------------------------------------------------------
 # the compiled reference (None when oracle/_ref was not built)
 # ---------------------------------------------------------------------------
 def ref_gemm(p, A, B, C=None, op=0, threads=8):

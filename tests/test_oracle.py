@@ -148,3 +148,61 @@ def test_mlp_oracle_finite_differences():
             lm = MO.loss(X, W1, b1, W2, b2, T, mask)
             P[idx] += eps
             assert abs((lp - lm) / (2 * eps) - G[idx]) < 1e-5 * max(1.0, abs(G[idx]))
+
+
+@pytest.mark.parametrize("p", [2, 65521, 2147483647, 4294967291])
+def test_level1_oracles_through_gemm(O, p):
+    """axpy / scal / gather / axpy_identity of the oracle, each re-derived as a k <= 2 product through
+    the oracle's C GEMM (a separate code path with its own reduction) and checked against Python
+    integers; plus canonicalise on the whole Lazy2p range and the 64-bit reduction."""
+    rng = np.random.default_rng(p)
+    x = rng.integers(0, p, (7, 9), dtype=np.uint64).astype(np.uint32)
+    y = rng.integers(0, p, (7, 9), dtype=np.uint64).astype(np.uint32)
+    for alpha in {0, 1, p - 1, int(rng.integers(0, p))}:
+        for sub in (False, True):
+            got = O.axpy(p, alpha, x, y, subtract=sub)
+            coef = np.array([[(p - alpha) % p if sub else alpha], [1]], np.uint32)
+            want = O.gemm_naive(p, np.stack([x.ravel(), y.ravel()], 1), coef).reshape(x.shape)
+            assert np.array_equal(got, want)
+            i, j = 3, 5
+            assert int(got[i, j]) == (int(y[i, j]) + (-1 if sub else 1) * alpha * int(x[i, j])) % p
+        assert np.array_equal(O.scal(p, alpha, x), O.gemm_naive(p, x.reshape(-1, 1), np.array([[alpha]], np.uint32)).reshape(x.shape))
+    # canonicalise: every v < 2p maps to v mod p
+    v = np.concatenate([np.arange(0, min(2 * p, 1 << 32), max(1, (2 * p) // 97), dtype=np.uint64),
+                        np.array([p - 1, p, min(2 * p - 1, (1 << 32) - 1)], np.uint64)]).astype(np.uint32)
+    assert [int(t) for t in O.canonicalise(p, v)] == [int(t) % p for t in v]
+    acc = np.array([[0, 1, (1 << 64) - 1], [p, p * p - 1, (1 << 63) + 7]], dtype=np.uint64)
+    assert [int(t) for t in O.canonicalise_acc64(p, acc).ravel()] == [int(t) % p for t in acc.ravel()]
+    # gather: the selected entries times beta plus alpha on the shifted identity, as one k = 2 product
+    a = rng.integers(0, p, (11, 13), dtype=np.uint64).astype(np.uint32)
+    rows = np.array([10, 0, 4, 4, 7], np.uint32)
+    cols = np.array([12, 3, 0, 9, 1, 5], np.uint32)
+    c = rng.integers(0, p, (5, 6), dtype=np.uint64).astype(np.uint32)
+    alpha, beta, off = int(rng.integers(1, p)) if p > 2 else 1, int(rng.integers(1, p)) if p > 2 else 1, 1
+    E = np.array([[1 if off + rows[i] == cols[t] else 0 for t in range(6)] for i in range(5)], np.uint32)
+    assert E.sum() > 0
+    S = a[rows][:, cols]
+    for acc_ in (False, True):
+        got = O.gather(p, alpha, beta, a, off, cols, c, accumulate=acc_, rows=rows)
+        Aop = np.stack([E.ravel(), S.ravel()] + ([c.ravel()] if acc_ else []), 1)
+        coef = np.array([[alpha], [beta]] + ([[1]] if acc_ else []), np.uint32)
+        assert np.array_equal(got, O.gemm_naive(p, Aop, coef).reshape(c.shape))
+    # identity source (a absent): alpha on the cells with row_off + i == t
+    got = O.gather(p, alpha, beta, None, 2, None, c)
+    assert np.array_equal(got, ((np.eye(5, 6, 2, dtype=np.uint64) * alpha) % p).astype(np.uint32))
+    # axpy_identity == axpy with an identity x on the window's diagonal
+    y2 = rng.integers(0, p, (4, 6), dtype=np.uint64).astype(np.uint32)
+    for sub in (False, True):
+        assert np.array_equal(O.axpy_identity(p, alpha, y2, sub),
+                              O.axpy(p, alpha, np.eye(4, 6, dtype=np.uint32), y2, subtract=sub))
+
+
+def test_level1_oracle_negative_control(O):
+    """A planted mistake (the identity tested against r(i) without row_off) must be caught by the
+    product form above."""
+    p = 65521
+    c = np.zeros((4, 4), np.uint32)
+    good = O.gather(p, 5, 0, None, 1, None, c)
+    planted = O.gather(p, 5, 0, None, 0, None, c)
+    assert not np.array_equal(good, planted)
+    assert np.array_equal(good, (np.eye(4, 4, 1, dtype=np.uint32) * 5))
