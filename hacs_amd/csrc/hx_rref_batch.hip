@@ -158,10 +158,106 @@ __device__ __forceinline__ uint32_t wave_min(uint32_t x) {
     return x;
 }
 
-template <class Ar>
+// (c) X = S[P, J]^{-1} S[P, c:n] for the panel's k pivot rows P (pivot columns J = c + s_q).
+//   c1: [K | I] with K = S[P, J] into LDS (two 32 x 64 buffers at the start of the panel region,
+//       whose panel copy is dead by now);
+//   c2: division-free Gauss-Jordan in the pivot order, row_i <- K_tt row_i - K_it row_t (i != t):
+//       every leading principal block of K is non-singular in this order (each pivot was chosen
+//       non-zero after eliminating the earlier ones), the left block ends diagonal, D, and the
+//       right block is D K^{-1}; a row left alone (K_it = 0) keeps its own scale, which is fine,
+//       since only the ratio right / left of a row is used.  One barrier per step (the buffers
+//       alternate), then the k inverses of D at once;
+//   c3: X = W S[P, c:n] on the INT8 matrix cores, W = D^{-1} (D K^{-1}) split into limbs once per
+//       wave (all waves hold the same A fragment), S[P, c + j] gathered from HBM per 32-column
+//       block; the exact recombination is X's entry.  X overwrites the region (after a barrier,
+//       so every wave holds W first), k rows of nb words.
+template <int L, class Ar>
+__device__ void pivot_solve(const RJob& job, const RArgs& a, const Ar& ar, uint32_t* pan, uint32_t k, uint32_t c,
+                            uint32_t nb, const uint32_t* s_P, const uint32_t* s_q, uint32_t* s_inv) {
+    constexpr int ND = 2 * L - 1, GS = 64;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, lr = lane & 31, lg = lane >> 5;
+    const uint32_t p = ar.p, k2 = 2 * k;
+    uint32_t* G0 = pan;
+    uint32_t* G1 = pan + 32 * GS;
+    for (uint32_t idx = tid; idx < k * k2; idx += RT) {
+        const uint32_t i = idx / k2, j = idx % k2;
+        G0[i * GS + j] = j < k ? job.A[(int64_t)s_P[i] * job.lda + c + s_q[j]] : (j - k == i ? 1u : 0u);
+    }
+    __syncthreads();
+    for (uint32_t t = 0; t < k; ++t) {
+        const uint32_t* src = (t & 1) ? G1 : G0;
+        uint32_t* dst = (t & 1) ? G0 : G1;
+        const uint32_t v = src[t * GS + t];
+        for (uint32_t idx = tid; idx < k * k2; idx += RT) {
+            const uint32_t i = idx / k2, j = idx % k2;
+            const uint32_t x = src[i * GS + j], f = src[i * GS + t];
+            dst[i * GS + j] = (i == t || f == 0) ? x : ar.dfree(v, x, p - f, src[t * GS + j]);
+        }
+        __syncthreads();
+    }
+    const uint32_t* Gk = (k & 1) ? G1 : G0;
+    if (tid < k) {
+        const uint32_t iv = invm(Gk[tid * GS + tid], p);
+        if (iv == 0) atomicExch(a.status, 1);
+        s_inv[tid] = ar.prep(iv);
+    }
+    __syncthreads();
+    auto split_into = [&](uint32_t x, int i, v4i (&f)[L]) {
+        int32_t d[4] = {0, 0, 0, 0};
+        limbs_of(x, p, a.half, L, d);
+#pragma unroll
+        for (int s = 0; s < L; ++s) f[s][i >> 2] |= (int)(((uint32_t)d[s] & 0xffu) << (8 * (i & 3)));
+    };
+    // W's fragment: row lr, k-bytes 16 lg .. 16 lg + 15
+    v4i fa[L];
+#pragma unroll
+    for (int s = 0; s < L; ++s) fa[s] = v4i{0, 0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const uint32_t t2 = 16 * lg + i;
+        split_into((lr < k && t2 < k) ? ar.mul(Gk[lr * GS + k + t2], s_inv[lr]) : 0u, i, fa);
+    }
+    __syncthreads();                                  // every wave holds W: the region may be overwritten
+    uint32_t* B = pan;
+    const uint32_t ncb = (nb + 31) / 32;
+    for (uint32_t cb = wave; cb < ncb; cb += RT / 64) {
+        const uint32_t j = cb * 32 + lr;
+        v4i fb[L];
+#pragma unroll
+        for (int s = 0; s < L; ++s) fb[s] = v4i{0, 0, 0, 0};
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const uint32_t t2 = 16 * lg + i;
+            split_into((j < nb && t2 < k) ? job.A[(int64_t)s_P[t2] * job.lda + c + j] : 0u, i, fb);
+        }
+        v16i acc[ND];
+#pragma unroll
+        for (int d = 0; d < ND; ++d)
+#pragma unroll
+            for (int e = 0; e < 16; ++e) acc[d][e] = 0;
+#pragma unroll
+        for (int s = 0; s < L; ++s)
+#pragma unroll
+            for (int t = 0; t < L; ++t)
+                acc[s + t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[s], fb[t], acc[s + t], 0, 0, 0);
+        if (j >= nb) continue;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            const uint32_t t = (e / 4) * 8 + lg * 4 + (e % 4);
+            if (t >= k) continue;
+            int32_t D[ND];
+#pragma unroll
+            for (int d = 0; d < ND; ++d) D[d] = acc[d][e];
+            B[t * nb + j] = limb_recombine<ND>(D, a.lc, a.M);
+        }
+    }
+    __syncthreads();
+}
+
+template <class Ar, int L>
 __device__ void rref_one(const RJob& job, const Ar& ar, const RArgs& a, uint32_t* sm) {
     __shared__ uint32_t s_cand[2][RT / 64];
-    __shared__ uint32_t s_P[RW], s_q[RW], s_inv[RW], s_fp[RW];
+    __shared__ uint32_t s_P[RW], s_q[RW], s_inv[RW];
     __shared__ uint32_t s_chunk[RT];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t m = job.m, n = job.n, p = ar.p;
@@ -169,7 +265,6 @@ __device__ void rref_one(const RJob& job, const Ar& ar, const RArgs& a, uint32_t
     uint32_t* pan = sm;
     uint32_t* list = sm + a.region;
     uint8_t* chosen = (uint8_t*)(list + a.mmax);
-    uint8_t* ispiv = chosen + a.mmax;
     if (m == 0 || n == 0) {
         if (tid == 0) *job.rank = 0;
         return;
@@ -177,7 +272,6 @@ __device__ void rref_one(const RJob& job, const Ar& ar, const RArgs& a, uint32_t
     uint32_t w = job.panel ? min((uint32_t)RW, job.panel) : (uint32_t)RW;
     while (w > 1 && ((uint64_t)m * (w | 1) > a.region || (uint64_t)w * n > a.region)) w >>= 1;
     for (uint32_t i = tid; i < m; i += RT) list[i] = i;
-    for (uint32_t j = tid; j < n; j += RT) ispiv[j] = 0;
     __syncthreads();
 
     uint32_t mu = m, r = 0;
@@ -217,62 +311,15 @@ __device__ void rref_one(const RJob& job, const Ar& ar, const RArgs& a, uint32_t
         __syncthreads();
         if (k == 0) continue;
 
-        // (c) the pivot rows, reduced to X (identity at the panel's pivot columns)
+        // (c) X = W S[P, c:n] with W = S[P, J]^{-1}: the k x k pivot block is inverted by
+        //     division-free Gauss-Jordan on [K | I] in LDS (one barrier per step), then the product
+        //     runs on the matrix cores into LDS; X is exactly the identity at the pivot columns.
         uint32_t* B = pan;
-        for (uint32_t idx = tid; idx < k * nb; idx += RT) {
-            const uint32_t t = idx / nb, j = idx % nb;
-            B[t * nb + j] = job.A[(int64_t)s_P[t] * lda + c + j];
-        }
-        if (tid < k) ispiv[c + s_q[tid]] = 1;
-        __syncthreads();
-        for (uint32_t t = 0; t + 1 < k; ++t) {       // division-free forward over the pivot order
-            const uint32_t qt = s_q[t], wdt = nb - qt - 1, cnt = (k - t - 1) * wdt;
-            const uint32_t v = B[t * nb + qt];
-            for (uint32_t it = tid; it < cnt; it += RT) {
-                const uint32_t row = t + 1 + it / wdt, j = qt + 1 + it % wdt;
-                const uint32_t f = B[row * nb + qt];
-                if (f != 0) B[row * nb + j] = ar.dfree(v, B[row * nb + j], p - f, B[t * nb + j]);
-            }
-            __syncthreads();
-        }
-        if (tid < k) {                                // one inverse per pivot, all at once
-            const uint32_t d = B[tid * nb + s_q[tid]];
-            const uint32_t iv = invm(d, p);
-            if (iv == 0) atomicExch(a.status, 1);
-            s_inv[tid] = ar.prep(iv);
-        }
-        __syncthreads();
-        for (uint32_t idx = tid; idx < k * nb; idx += RT) {
-            const uint32_t t = idx / nb, j = idx % nb;
-            if (j > s_q[t]) B[idx] = ar.mul(B[idx], s_inv[t]);
-        }
-        __syncthreads();
-        for (uint32_t t = k - 1; t >= 1; --t) {       // back substitution, non-pivot columns only
-            const uint32_t qt = s_q[t], wdt = nb - qt - 1, cnt = t * wdt;
-            if (tid < t) s_fp[tid] = ar.prep(B[tid * nb + qt]);
-            __syncthreads();
-            for (uint32_t it = tid; it < cnt; it += RT) {
-                const uint32_t row = it / wdt, j = qt + 1 + it % wdt;
-                if (ispiv[c + j]) continue;
-                const uint32_t fp = s_fp[row];
-                if (fp != 0) B[row * nb + j] = subm(B[row * nb + j], ar.mul(B[t * nb + j], fp), p);
-            }
-            __syncthreads();
-        }
-        for (uint32_t idx = tid; idx < k * k; idx += RT) {
-            const uint32_t t2 = idx / k, t = idx % k;
-            B[t2 * nb + s_q[t]] = t2 == t ? 1u : 0u;
-        }
-        __syncthreads();
+        pivot_solve<L>(job, a, ar, pan, k, c, nb, s_P, s_q, s_inv);
 
         // (d) trailing update of the other unused rows, S[i, j] -= sum_t S[i, J_t] X[t, j], and the
         // reduction of the stored pivot rows 0 .. r-1 against this panel's pivots (MFMA)
-        switch (a.L) {
-            case 1: trailing_mfma<1>(job, a, B, nb, wc, k, c, mu, list, chosen, s_q, r); break;
-            case 2: trailing_mfma<2>(job, a, B, nb, wc, k, c, mu, list, chosen, s_q, r); break;
-            case 3: trailing_mfma<3>(job, a, B, nb, wc, k, c, mu, list, chosen, s_q, r); break;
-            default: trailing_mfma<4>(job, a, B, nb, wc, k, c, mu, list, chosen, s_q, r); break;
-        }
+        trailing_mfma<L>(job, a, B, nb, wc, k, c, mu, list, chosen, s_q, r);
         // (e) this panel's pivot rows (X) become U rows r .. r+k-1; pivots, pivot rows
         for (uint32_t idx = tid; idx < k * n; idx += RT) {
             const uint32_t t = idx / n, j = idx % n;
@@ -317,7 +364,7 @@ __device__ void rref_one(const RJob& job, const Ar& ar, const RArgs& a, uint32_t
     __syncthreads();
 }
 
-template <class Ar>
+template <class Ar, int L>
 __global__ __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(2, 8))) void rref_batch_kernel(RArgs a, Ar ar) {
     extern __shared__ __attribute__((aligned(16))) uint32_t sm[];
     __shared__ uint32_t s_job;
@@ -327,7 +374,7 @@ __global__ __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(2, 8))) void
         const uint32_t jj = s_job;
         __syncthreads();
         if (jj >= a.njobs) break;
-        rref_one(a.jobs[a.order[jj]], ar, a, sm);
+        rref_one<Ar, L>(a.jobs[a.order[jj]], ar, a, sm);
     }
 }
 
@@ -381,12 +428,14 @@ int rref_batch_create_impl(const hx_modctx& ctx, const hx_rref_job* jobs, uint32
         }
         b->mmax = (mmax + 3) / 4 * 4;
         b->nmax = (nmax + 3) / 4 * 4;
-        const size_t fixed = (size_t)b->mmax * 4 + b->mmax + b->nmax + 64;
+        const size_t fixed = (size_t)b->mmax * 4 + b->mmax + 64;      // the unused-row list and flags
         HX_CHECK(fixed + 4 * 64 < kLdsCap, "rref batch: job too tall for the LDS row lists");
-        size_t want = std::max<size_t>((size_t)mmax * (RW | 1), (size_t)RW * nmax);
+        // the panel copy (m x (w|1)), the pivot rows' X (w x n), the two [K | I] buffers (2 x 32 x 64)
+        size_t want = std::max<size_t>(std::max<size_t>((size_t)mmax * (RW | 1), (size_t)RW * nmax), 2 * 32 * 64);
         const size_t cap_words = (kLdsCap - fixed) / 4;
         b->region = (uint32_t)std::min(want, cap_words);
-        HX_CHECK(b->region >= std::max<size_t>(mmax + 1, nmax), "rref batch: shape exceeds the LDS panel");
+        HX_CHECK(b->region >= std::max<size_t>(std::max<size_t>(mmax + 1, nmax), 2 * 32 * 64),
+                 "rref batch: shape exceeds the LDS panel");
         b->lds = (size_t)b->region * 4 + fixed;
         HX_HIP(hipMalloc(&b->d_U, std::max<size_t>(ucount, 1) * 4));
         size_t off = 0;
@@ -416,10 +465,11 @@ int rref_batch_create_impl(const hx_modctx& ctx, const hx_rref_job* jobs, uint32
         b->grid = std::max<uint32_t>(1, std::min<uint32_t>(njobs, (uint32_t)cus * std::min<uint32_t>(per_cu, 4)));
         static bool attr_done = false;
         if (!attr_done) {
-            (void)hipFuncSetAttribute((const void*)rref_batch_kernel<ArMont>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)kLdsCap);
-            (void)hipFuncSetAttribute((const void*)rref_batch_kernel<ArBarrett>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsCap);
+            for (const void* f : {(const void*)rref_batch_kernel<ArMont, 1>, (const void*)rref_batch_kernel<ArMont, 2>,
+                                  (const void*)rref_batch_kernel<ArMont, 3>, (const void*)rref_batch_kernel<ArMont, 4>,
+                                  (const void*)rref_batch_kernel<ArBarrett, 1>, (const void*)rref_batch_kernel<ArBarrett, 2>,
+                                  (const void*)rref_batch_kernel<ArBarrett, 3>, (const void*)rref_batch_kernel<ArBarrett, 4>})
+                (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsCap);
             attr_done = true;
         }
     } catch (...) {
@@ -449,11 +499,20 @@ int rref_batch_run_impl(void* vb, hipStream_t s) {
     a.half = (b->ctx.p - 1) / 2;
     const uint32_t pp = b->ctx.p;                       // limbs from p itself (balanced digits)
     a.L = pp <= 255u ? 1 : (pp <= 65279u ? 2 : (pp <= 16711423u ? 3 : 4));
-    if (b->ctx.p & 1u) {
-        rref_batch_kernel<ArMont><<<b->grid, RT, b->lds, s>>>(a, make_mont(b->ctx.p));
-    } else {
-        rref_batch_kernel<ArBarrett><<<b->grid, RT, b->lds, s>>>(a, make_barrett(b->ctx.p));
-    }
+    // one kernel per limb count (the MFMA fragments' size is a compile-time constant)
+    auto launch = [&](auto ar) {
+        using Ar = decltype(ar);
+        switch (a.L) {
+            case 1: rref_batch_kernel<Ar, 1><<<b->grid, RT, b->lds, s>>>(a, ar); break;
+            case 2: rref_batch_kernel<Ar, 2><<<b->grid, RT, b->lds, s>>>(a, ar); break;
+            case 3: rref_batch_kernel<Ar, 3><<<b->grid, RT, b->lds, s>>>(a, ar); break;
+            default: rref_batch_kernel<Ar, 4><<<b->grid, RT, b->lds, s>>>(a, ar); break;
+        }
+    };
+    if (b->ctx.p & 1u)
+        launch(make_mont(b->ctx.p));
+    else
+        launch(make_barrett(b->ctx.p));
     after_launch("rref_batch_kernel");
     return HX_OK;
 }
