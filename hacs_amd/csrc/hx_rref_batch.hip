@@ -77,9 +77,9 @@ typedef int v16i __attribute__((ext_vector_type(16)));
 // gives the residue subtracted from the entry.  A wave takes a 32-row block of either set, loads
 // the block's k coefficients into registers before any write, then walks the 32-column blocks.
 template <int L>
-__device__ void trailing_mfma(const RJob& job, const RArgs& a, const uint32_t* B, uint32_t nb, uint32_t wc,
-                              uint32_t k, uint32_t c, uint32_t mu, const uint32_t* list, const uint8_t* chosen,
-                              const uint32_t* s_q, uint32_t r) {
+__device__ void trailing_mfma(const RJob& job, const RArgs& a, const uint32_t* B, bool planes, uint32_t nb,
+                              uint32_t wc, uint32_t k, uint32_t c, uint32_t mu, const uint32_t* list,
+                              const uint8_t* chosen, const uint32_t* s_q, uint32_t r) {
     constexpr int ND = 2 * L - 1;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, lr = lane & 31, lg = lane >> 5;
     const uint32_t p = a.M.p, half = a.half;
@@ -96,12 +96,11 @@ __device__ void trailing_mfma(const RJob& job, const RArgs& a, const uint32_t* B
         const uint32_t b0 = (urows ? rb - nrb_u : rb) * 32, nrows = urows ? r : mu, jbeg = urows ? 0 : wc;
         uint32_t* base = urows ? job.U : job.A;
         const int64_t ld = urows ? (int64_t)job.n : job.lda;
-        auto row_of = [&](uint32_t u) -> int64_t { return urows ? (int64_t)u : (int64_t)list[u]; };
         auto live_row = [&](uint32_t u) { return u < nrows && (urows || !chosen[u]); };
         // this lane's row of the block and its 16 k-values (k-bytes 16 lg .. 16 lg + 15)
         const uint32_t u = b0 + lr;
         const bool live = live_row(u);
-        const int64_t rowoff = live ? row_of(u) * ld + c : 0;
+        const int64_t rowoff = live ? (urows ? (int64_t)u : (int64_t)list[u]) * ld + c : 0;
         v4i fa[L];
 #pragma unroll
         for (int s = 0; s < L; ++s) fa[s] = v4i{0, 0, 0, 0};
@@ -110,16 +109,38 @@ __device__ void trailing_mfma(const RJob& job, const RArgs& a, const uint32_t* B
             const uint32_t t = 16 * lg + i;
             split_into((live && t < k) ? base[rowoff + s_q[t]] : 0u, i, fa);
         }
+        // the 16 output rows this lane touches (accumulator element e), as matrix row indices;
+        // NOROW for rows that take no update
+        constexpr uint32_t NOROW = 0xffffffffu;
+        uint32_t orow[16];
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            const uint32_t ur = b0 + (e / 4) * 8 + lg * 4 + (e % 4);
+            orow[e] = live_row(ur) ? (urows ? ur : list[ur]) : NOROW;
+        }
+        uint32_t* const basec = base + c;
         const uint32_t ncb = (nb - jbeg + 31) / 32;
         for (uint32_t cb = 0; cb < ncb; ++cb) {
             const uint32_t j = jbeg + cb * 32 + lr;
+            // the block's entries are loaded before the MFMAs, so their latency overlaps them
+            uint32_t cv[16];
+#pragma unroll
+            for (int e = 0; e < 16; ++e) cv[e] = (j < nb && orow[e] != NOROW) ? basec[(int64_t)orow[e] * ld + j] : 0u;
             v4i fb[L];
 #pragma unroll
             for (int s = 0; s < L; ++s) fb[s] = v4i{0, 0, 0, 0};
+            if (planes) {
+                if (j < nb) {
+                    const uint8_t* P8 = (const uint8_t*)B + 32 * j + 16 * lg;
 #pragma unroll
-            for (int i = 0; i < 16; ++i) {
-                const uint32_t t = 16 * lg + i;
-                split_into((j < nb && t < k) ? B[t * nb + j] : 0u, i, fb);
+                    for (int s = 0; s < L; ++s) fb[s] = *(const v4i*)(P8 + (size_t)s * 32 * nb);
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    const uint32_t t = 16 * lg + i;
+                    split_into((j < nb && t < k) ? B[t * nb + j] : 0u, i, fb);
+                }
             }
             v16i acc[ND];
 #pragma unroll
@@ -131,22 +152,20 @@ __device__ void trailing_mfma(const RJob& job, const RArgs& a, const uint32_t* B
 #pragma unroll
                 for (int t = 0; t < L; ++t)
                     acc[s + t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[s], fb[t], acc[s + t], 0, 0, 0);
-            if (j >= nb) continue;
+            if (j < nb) {
 #pragma unroll
-            for (int e = 0; e < 16; ++e) {
-                const uint32_t ur = b0 + (e / 4) * 8 + lg * 4 + (e % 4);
-                if (!live_row(ur)) continue;
-                int32_t D[ND];
-                bool nzd = false;
+                for (int e = 0; e < 16; ++e) {
+                    if (orow[e] == NOROW) continue;
+                    int32_t D[ND];
+                    bool nzd = false;
 #pragma unroll
-                for (int d = 0; d < ND; ++d) {
-                    D[d] = acc[d][e];
-                    nzd |= D[d] != 0;
+                    for (int d = 0; d < ND; ++d) {
+                        D[d] = acc[d][e];
+                        nzd |= D[d] != 0;
+                    }
+                    if (!nzd) continue;                           // a zero update leaves the entry
+                    basec[(int64_t)orow[e] * ld + j] = subm(cv[e], limb_recombine<ND>(D, a.lc, a.M), p);
                 }
-                if (!nzd) continue;                                   // a zero update leaves the entry
-                const uint32_t res = limb_recombine<ND>(D, a.lc, a.M);
-                uint32_t* x = base + row_of(ur) * ld + c + j;
-                *x = subm(*x, res, p);
             }
         }
     }
@@ -173,7 +192,8 @@ __device__ __forceinline__ uint32_t wave_min(uint32_t x) {
 //       so every wave holds W first), k rows of nb words.
 template <int L, class Ar>
 __device__ void pivot_solve(const RJob& job, const RArgs& a, const Ar& ar, uint32_t* pan, uint32_t k, uint32_t c,
-                            uint32_t nb, const uint32_t* s_P, const uint32_t* s_q, uint32_t* s_inv) {
+                            uint32_t nb, uint32_t r, bool planes, const uint32_t* s_P, const uint32_t* s_q,
+                            uint32_t* s_inv) {
     constexpr int ND = 2 * L - 1, GS = 64;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, lr = lane & 31, lg = lane >> 5;
     const uint32_t p = ar.p, k2 = 2 * k;
@@ -241,14 +261,35 @@ __device__ void pivot_solve(const RJob& job, const RArgs& a, const Ar& ar, uint3
             for (int t = 0; t < L; ++t)
                 acc[s + t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[s], fb[t], acc[s + t], 0, 0, 0);
         if (j >= nb) continue;
+        uint32_t word[4][L];                          // limb bytes of rows g*8 + lg*4 .. +3, per plane
 #pragma unroll
         for (int e = 0; e < 16; ++e) {
             const uint32_t t = (e / 4) * 8 + lg * 4 + (e % 4);
-            if (t >= k) continue;
             int32_t D[ND];
 #pragma unroll
             for (int d = 0; d < ND; ++d) D[d] = acc[d][e];
-            B[t * nb + j] = limb_recombine<ND>(D, a.lc, a.M);
+            const uint32_t x = t < k ? limb_recombine<ND>(D, a.lc, a.M) : 0u;
+            if (t < k) job.U[(size_t)(r + t) * job.n + c + j] = x;
+            if (planes) {
+                int32_t dg[4] = {0, 0, 0, 0};
+                limbs_of(x, p, a.half, L, dg);
+#pragma unroll
+                for (int sp = 0; sp < L; ++sp) {
+                    const uint32_t byte = ((uint32_t)dg[sp] & 0xffu) << (8 * (e % 4));
+                    word[e / 4][sp] = (e % 4) == 0 ? byte : (word[e / 4][sp] | byte);
+                }
+            } else if (t < k) {
+                B[t * nb + j] = x;
+            }
+        }
+        if (planes) {
+            // plane sp, column j: 32 k-bytes (rows t = 0..31) at byte sp * 32 nb + 32 j + t
+            uint8_t* P8 = (uint8_t*)B;
+#pragma unroll
+            for (int g = 0; g < 4; ++g)
+#pragma unroll
+                for (int sp = 0; sp < L; ++sp)
+                    *(uint32_t*)(P8 + (size_t)sp * 32 * nb + 32 * j + g * 8 + lg * 4) = word[g][sp];
         }
     }
     __syncthreads();
@@ -315,15 +356,19 @@ __device__ void rref_one(const RJob& job, const Ar& ar, const RArgs& a, uint32_t
         //     division-free Gauss-Jordan on [K | I] in LDS (one barrier per step), then the product
         //     runs on the matrix cores into LDS; X is exactly the identity at the pivot columns.
         uint32_t* B = pan;
-        pivot_solve<L>(job, a, ar, pan, k, c, nb, s_P, s_q, s_inv);
+        // X's limb planes (L x nb x 32 bytes) replace X in LDS when they fit (then the trailing
+        // pass reads its B fragments directly); X itself goes straight to U rows r .. r+k-1
+        const bool planes = (uint64_t)8 * L * nb <= a.region;
+        pivot_solve<L>(job, a, ar, pan, k, c, nb, r, planes, s_P, s_q, s_inv);
 
         // (d) trailing update of the other unused rows, S[i, j] -= sum_t S[i, J_t] X[t, j], and the
         // reduction of the stored pivot rows 0 .. r-1 against this panel's pivots (MFMA)
-        trailing_mfma<L>(job, a, B, nb, wc, k, c, mu, list, chosen, s_q, r);
-        // (e) this panel's pivot rows (X) become U rows r .. r+k-1; pivots, pivot rows
-        for (uint32_t idx = tid; idx < k * n; idx += RT) {
-            const uint32_t t = idx / n, j = idx % n;
-            job.U[(size_t)(r + t) * n + j] = j < c ? 0u : B[t * nb + (j - c)];
+        trailing_mfma<L>(job, a, B, planes, nb, wc, k, c, mu, list, chosen, s_q, r);
+        // (e) this panel's pivot rows (X, already in U rows r .. r+k-1 from c on): zeros left of c;
+        // pivots, pivot rows
+        for (uint32_t idx = tid; idx < k * c; idx += RT) {
+            const uint32_t t = idx / c, j = idx % c;
+            job.U[(size_t)(r + t) * n + j] = 0u;
         }
         if (tid < k) {
             job.pivots[r + tid] = c + s_q[tid];
