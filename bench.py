@@ -370,15 +370,22 @@ def mixed_stream(njobs):
     statics = [AT.static_choice(k, n) for k, n in stream]
     t_run = {"selector": 0.0, "static": 0.0}
     t_kind = {}
+    # batched per kind (as a scheduler would), in chunks of at most 40 GB of inputs and outputs: the
+    # GEMMs (~160 GB) take several chunks, the echelon and Howell jobs one each
     budget = 40 << 30
-    i = 0
-    while i < njobs:
-        chunk, used = [], 0
-        while i < njobs and (not chunk or used < budget):
-            k, n = stream[i]
-            used += (3 if k == "gemm" else 2) * 4 * n * n
-            chunk.append(i)
-            i += 1
+    chunks = []
+    for kind in AT.KINDS:
+        idx = [j for j in range(njobs) if stream[j][0] == kind]
+        i = 0
+        while i < len(idx):
+            chunk, used = [], 0
+            while i < len(idx) and (not chunk or used < budget):
+                n = stream[idx[i]][1]
+                used += (3 if kind == "gemm" else 2) * 4 * n * n
+                chunk.append(idx[i])
+                i += 1
+            chunks.append(chunk)
+    for chunk in chunks:
         jobs = [AT.Job(stream[j][0], stream[j][1], j, salt=0x90) for j in chunk]
         t_run["selector"] += AT.run_stream(jobs, [slots[j] for j in chunk])
         AT.run_stream(jobs, [slots[j] for j in chunk], per_kind=t_kind)

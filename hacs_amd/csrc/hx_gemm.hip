@@ -27,6 +27,7 @@
 #include "hx_limbs.h"
 
 #include <algorithm>
+#include <numeric>
 #include <mutex>
 
 namespace hx {
@@ -772,7 +773,10 @@ __global__ __launch_bounds__(512, 1) void limb4_batch_kernel(const LimbArgs* job
     const uint32_t j = job_of(prefix, njobs, blockIdx.x);
     const LimbArgs a = jobs[j];
     const uint32_t local = blockIdx.x - prefix[j];
-    limb_tile4<L>(a, local / a.ntn, local % a.ntn);
+    if (local >= a.ntm * a.ntn) return;          // alignment padding of the job's block range
+    uint32_t tm, tn;
+    tile_of(local, a.ntm, a.ntn, tm, tn);
+    limb_tile4<L>(a, tm, tn);
 }
 
 template <int L>
@@ -788,7 +792,10 @@ __global__ __launch_bounds__(256, 1) void limb2_batch_kernel(const LimbArgs* job
     const uint32_t j = job_of(prefix, njobs, blockIdx.x);
     const LimbArgs a = jobs[j];
     const uint32_t local = blockIdx.x - prefix[j];
-    limb_tile2<L>(a, local / a.ntn, local % a.ntn);
+    if (local >= a.ntm * a.ntn) return;          // alignment padding of the job's block range
+    uint32_t tm, tn;
+    tile_of(local, a.ntm, a.ntn, tm, tn);
+    limb_tile2<L>(a, tm, tn);
 }
 
 template <int L>
@@ -804,7 +811,10 @@ __global__ __launch_bounds__(256, 1) void limb3_batch_kernel(const LimbArgs* job
     const uint32_t j = job_of(prefix, njobs, blockIdx.x);
     const LimbArgs a = jobs[j];
     const uint32_t local = blockIdx.x - prefix[j];
-    limb_tile3<L>(a, local / a.ntn, local % a.ntn);
+    if (local >= a.ntm * a.ntn) return;          // alignment padding of the job's block range
+    uint32_t tm, tn;
+    tile_of(local, a.ntm, a.ntn, tm, tn);
+    limb_tile3<L>(a, tm, tn);
 }
 
 template <int L>
@@ -822,7 +832,10 @@ __global__ __launch_bounds__(256, 1) void limb_batch_kernel(const LimbArgs* jobs
     const uint32_t j = job_of(prefix, njobs, blockIdx.x);
     const LimbArgs a = jobs[j];
     const uint32_t local = blockIdx.x - prefix[j];
-    limb_tile<L>(a, local / a.ntn, local % a.ntn, smem);
+    if (local >= a.ntm * a.ntn) return;          // alignment padding of the job's block range
+    uint32_t tm, tn;
+    tile_of(local, a.ntm, a.ntn, tm, tn);
+    limb_tile<L>(a, tm, tn, smem);
 }
 
 template <int L>
@@ -1060,8 +1073,15 @@ void* gemm_batch_create(const hx_modctx& ctx, const hx_gemm_job* jobs, uint32_t 
         struct Pending { size_t a_off, b_off; uint32_t Mp, Np, Kp; };
         std::vector<Pending> pend;
         const uint32_t p = ctx.p, half = (p - 1) / 2;
-        for (uint32_t i = 0; i < njobs; ++i) {
-            const hx_gemm_job& j = jobs[i];
+        // largest products first, so the big jobs' tiles are dispatched first and the small ones fill
+        // the tail (the order of the job table does not change any value)
+        std::vector<uint32_t> ord(njobs);
+        std::iota(ord.begin(), ord.end(), 0u);
+        std::stable_sort(ord.begin(), ord.end(), [&](uint32_t x, uint32_t y) {
+            return (uint64_t)jobs[x].m * jobs[x].n * jobs[x].k > (uint64_t)jobs[y].m * jobs[y].n * jobs[y].k;
+        });
+        for (uint32_t oi = 0; oi < njobs; ++oi) {
+            const hx_gemm_job& j = jobs[ord[oi]];
             HX_CHECK(j.op == HX_OP_SET || j.op == HX_OP_ADD || j.op == HX_OP_SUB, "gemm batch: bad op");
             if (j.m == 0 || j.n == 0) continue;
             HX_CHECK(j.C != nullptr && (j.k == 0 || (j.A && j.B)), "gemm batch: NULL operand");
@@ -1095,6 +1115,9 @@ void* gemm_batch_create(const hx_modctx& ctx, const hx_gemm_job* jobs, uint32_t 
             la.op = j.op; la.M = devmod(ctx);
             limb_epilogue_consts(p, la);
             limb.push_back(la);
+            // a job of 64+ tiles starts at a multiple of 8 blocks, so block b runs on XCD b mod 8 with
+            // the job-local XCD-aware tile order (tile_of) intact; the skipped blocks exit at once
+            if (la.ntm * la.ntn >= 64) lp.back() = (lp.back() + 7) / 8 * 8;
             lp.push_back(lp.back() + la.ntm * la.ntn);
             if (Kp > 0) {
                 PrepArgs pa{};

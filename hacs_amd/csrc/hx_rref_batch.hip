@@ -37,6 +37,7 @@ namespace {
 
 constexpr int RT = 256;   // threads per job workgroup
 constexpr int RW = 32;    // widest panel
+constexpr uint32_t kWavePanelRows = 128;   // panels of at most this many unused rows: one wave, no barriers
 
 struct RJob {
     uint32_t m, n;
@@ -300,6 +301,7 @@ __device__ void rref_one(const RJob& job, const Ar& ar, const RArgs& a, uint32_t
     __shared__ uint32_t s_cand[2][RT / 64];
     __shared__ uint32_t s_P[RW], s_q[RW], s_inv[RW];
     __shared__ uint32_t s_chunk[RT];
+    __shared__ uint32_t s_k;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t m = job.m, n = job.n, p = ar.p;
     const int64_t lda = job.lda;
@@ -319,13 +321,57 @@ __device__ void rref_one(const RJob& job, const Ar& ar, const RArgs& a, uint32_t
     for (uint32_t c = 0; c < n && mu > 0; c += w) {
         const uint32_t wc = min(w, n - c), ws = wc | 1u, nb = n - c;
         // (a) panel of the unused rows, pivot search by the smallest-row rule
-        for (uint32_t idx = tid; idx < mu * wc; idx += RT) {
-            const uint32_t u = idx / wc, q = idx % wc;
-            pan[u * ws + q] = job.A[(int64_t)list[u] * lda + c + q];
+        for (uint32_t i0 = 0; i0 < mu * wc; i0 += 4 * RT) {      // four loads per thread in flight
+            uint32_t v[4], at[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const uint32_t idx = i0 + e * RT + tid;
+                if (idx < mu * wc) {
+                    const uint32_t u = wc == RW ? idx / RW : idx / wc, q = idx - u * wc;
+                    v[e] = job.A[(int64_t)list[u] * lda + c + q];
+                    at[e] = u * ws + q;
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                if (i0 + e * RT + tid < mu * wc) pan[at[e]] = v[e];
         }
         for (uint32_t u = tid; u < mu; u += RT) chosen[u] = 0;
         __syncthreads();
         uint32_t k = 0;
+        if (mu <= kWavePanelRows) {
+            // a short panel is factored by wave 0 alone, with no barrier per column: lane l owns the
+            // rows l, l + 64, ...; the pivot is the lowest set bit of a ballot (rows are in original
+            // order, so this is the smallest-row rule); the wave's LDS accesses complete in issue
+            // order, so every lane sees the pivot row as its owner last wrote it
+            if (wave == 0) {
+                for (uint32_t q = 0; q < wc; ++q) {
+                    uint32_t bu = 0xffffffffu;
+                    for (uint32_t u0 = 0; u0 < mu && bu == 0xffffffffu; u0 += 64) {
+                        const uint32_t u = u0 + lane;
+                        const uint64_t bal = __ballot(u < mu && !chosen[u] && pan[u * ws + q] != 0);
+                        if (bal) bu = u0 + (uint32_t)(__ffsll((long long)bal) - 1);
+                    }
+                    if (bu == 0xffffffffu) continue;
+                    const uint32_t* prow = pan + bu * ws;
+                    const uint32_t v = prow[q];
+                    for (uint32_t u = lane; u < mu; u += 64) {
+                        if (u == bu) { chosen[u] = 1; continue; }
+                        if (chosen[u]) continue;
+                        uint32_t* row = pan + u * ws;
+                        const uint32_t f = row[q];
+                        if (f == 0) continue;
+                        const uint32_t fneg = p - f;
+                        for (uint32_t q2 = q + 1; q2 < wc; ++q2) row[q2] = ar.dfree(v, row[q2], fneg, prow[q2]);
+                    }
+                    if (lane == 0) { s_P[k] = list[bu]; s_q[k] = q; }
+                    ++k;
+                }
+                if (lane == 0) s_k = k;
+            }
+            __syncthreads();
+            k = s_k;
+        } else
         for (uint32_t q = 0; q < wc; ++q) {
             uint32_t best = 0xffffffffu;
             for (uint32_t u = tid; u < mu; u += RT)
@@ -366,10 +412,8 @@ __device__ void rref_one(const RJob& job, const Ar& ar, const RArgs& a, uint32_t
         trailing_mfma<L>(job, a, B, planes, nb, wc, k, c, mu, list, chosen, s_q, r);
         // (e) this panel's pivot rows (X, already in U rows r .. r+k-1 from c on): zeros left of c;
         // pivots, pivot rows
-        for (uint32_t idx = tid; idx < k * c; idx += RT) {
-            const uint32_t t = idx / c, j = idx % c;
-            job.U[(size_t)(r + t) * n + j] = 0u;
-        }
+        for (uint32_t t = wave; t < k; t += RT / 64)
+            for (uint32_t j = lane; j < c; j += 64) job.U[(size_t)(r + t) * n + j] = 0u;
         if (tid < k) {
             job.pivots[r + tid] = c + s_q[tid];
             job.pivot_rows[r + tid] = s_P[tid];
@@ -401,9 +445,17 @@ __device__ void rref_one(const RJob& job, const Ar& ar, const RArgs& a, uint32_t
     }
 
     // the RREF over the job's matrix: rank rows of U, then zero rows
-    for (uint64_t idx = tid; idx < (uint64_t)m * n; idx += RT) {
-        const uint32_t i = (uint32_t)(idx / n), j = (uint32_t)(idx % n);
-        job.A[(int64_t)i * lda + j] = i < r ? job.U[(size_t)i * n + j] : 0u;
+    for (uint32_t i = wave; i < m; i += RT / 64) {          // a wave per row, four words per lane in flight
+        uint32_t* Ai = job.A + (int64_t)i * lda;
+        const uint32_t* Ui = job.U + (size_t)i * n;
+        for (uint32_t j0 = lane; j0 < n; j0 += 256) {
+            uint32_t v[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) v[e] = (i < r && j0 + 64 * e < n) ? Ui[j0 + 64 * e] : 0u;
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                if (j0 + 64 * e < n) Ai[j0 + 64 * e] = v[e];
+        }
     }
     if (tid == 0) *job.rank = r;
     __syncthreads();
@@ -477,7 +529,21 @@ int rref_batch_create_impl(const hx_modctx& ctx, const hx_rref_job* jobs, uint32
         HX_CHECK(fixed + 4 * 64 < kLdsCap, "rref batch: job too tall for the LDS row lists");
         // the panel copy (m x (w|1)), the pivot rows' X (w x n), the two [K | I] buffers (2 x 32 x 64)
         size_t want = std::max<size_t>(std::max<size_t>((size_t)mmax * (RW | 1), (size_t)RW * nmax), 2 * 32 * 64);
-        const size_t cap_words = (kLdsCap - fixed) / 4;
+        // Two workgroups per CU (half the LDS each) when every job still gets a panel of at least 16
+        // columns there AND the batch holds enough work to keep both busy: each workgroup then runs
+        // at ~0.57x the speed of one alone on its CU, so the largest job's own time grows while the
+        // total throughput rises; with total work >= 512x the largest job's, the throughput term
+        // dominates (measured on the 6667 echelon jobs of the mixed stream: 110 -> 103 ms).
+        double wsum = 0.0, wmax = 0.0;
+        for (const RJob& jj : hj) {
+            const double w = (double)jj.m * jj.n * std::min(jj.m, jj.n);
+            wsum += w;
+            wmax = std::max(wmax, w);
+        }
+        const size_t half_words = kLdsCap / 2 > fixed ? (kLdsCap / 2 - fixed) / 4 : 0;
+        const bool two_per_cu = (size_t)mmax * 17 <= half_words && (size_t)16 * nmax <= half_words &&
+                                half_words >= 2 * 32 * 64 && wsum >= 512.0 * wmax;
+        const size_t cap_words = ((two_per_cu ? kLdsCap / 2 : kLdsCap) - fixed) / 4;
         b->region = (uint32_t)std::min(want, cap_words);
         HX_CHECK(b->region >= std::max<size_t>(std::max<size_t>(mmax + 1, nmax), 2 * 32 * 64),
                  "rref batch: shape exceeds the LDS panel");
