@@ -25,10 +25,14 @@
 //     the caller's stream, the rest on a second stream, overlapping the next panel's tournament
 //     (which reads only those W columns); the next panel's step 2 waits for it.  P, J and the
 //     update's sizes are double-buffered by panel parity for the overlap.
+//  4. Gauss-Jordan: the pivot rows stored so far (U rows 0 .. r-1) are reduced against the panel's
+//     pivots, U[0:r, c:n] -= U[0:r, c + J] X, on the second stream after the rest of the update
+//     (their panel columns gathered first, since the product overwrites them); X is exactly the
+//     identity at J, so those entries become zero.  When the last panel is done the U rows are the
+//     RREF's rows.  The X copy is double-buffered by panel parity, so the next panel's X does not
+//     overwrite the one this product reads.
 //  All sizes (m_a, k, r) live in device memory; the host only knows upper bounds, so a panel is a
-//  fixed sequence of launches.  At the end the rank rows of the U store (identity at their own
-//  panel's pivot columns, zero to the left) are cleared against each other by one unit-upper TRSM
-//  on the non-pivot columns: RREF = U_piv^{-1} U.
+//  fixed sequence of launches.
 #include <algorithm>
 #include <vector>
 
@@ -36,8 +40,6 @@
 #include "hx_gemm.h"
 
 namespace hx {
-
-void trsm_unit(const hx_modctx& ctx, hx_view T, hx_view B, int upper, hipStream_t s);
 
 namespace {
 
@@ -50,6 +52,7 @@ struct LState {               // device-side counters
     uint32_t m_a, r, k, m_new;
     int status;
     uint32_t snap_k[2], snap_m[2];   // the sizes of panel i's trailing update, slot i & 1
+    uint32_t snap_r[2];              // the pivot rows stored before panel i (its Gauss-Jordan rows)
 };
 
 __device__ __forceinline__ uint32_t wmin(uint32_t x) {
@@ -288,6 +291,7 @@ __global__ void snapshot_kernel(LState* st, int slot) {
     if (threadIdx.x == 0) {
         st->snap_k[slot] = st->k;
         st->snap_m[slot] = st->m_new;
+        st->snap_r[slot] = st->r;
     }
 }
 
@@ -306,32 +310,24 @@ __global__ void init_state_kernel(LState* st, uint32_t m, uint32_t* rows_orig, u
     for (size_t t = i; t < ucount; t += (size_t)gridDim.x * blockDim.x) U[t] = 0;
 }
 
-__global__ void zero_rows_kernel(uint32_t m, uint32_t n, uint32_t* R, int64_t ldr) {
+// T[t, q] = U[t, c + J[q]] for the r pivot rows stored before the panel (the Gauss-Jordan product's
+// left operand; zero past the panel's k pivots).
+__global__ void gj_gather_kernel(const uint32_t* U, int64_t ldu, uint32_t c, const uint32_t* J, const LState* st,
+                                 int slot, uint32_t* T) {
+    const uint32_t r = st->snap_r[slot], k = st->snap_k[slot];
+    for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < r * LW; idx += gridDim.x * blockDim.x) {
+        const uint32_t t = idx / LW, q = idx % LW;
+        T[idx] = q < k ? U[(int64_t)t * ldu + c + J[q]] : 0u;
+    }
+}
+
+// R = the r rank rows of U, then zero rows (the RREF with its zero rows last).
+__global__ void rref_out_kernel(const uint32_t* U, uint32_t n, uint32_t r, uint32_t m, uint32_t* R, int64_t ldr) {
     const uint64_t total = (uint64_t)m * n;
     for (uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-         idx += (uint64_t)gridDim.x * blockDim.x)
-        R[(int64_t)(idx / n) * ldr + (idx % n)] = 0;
-}
-
-__global__ void place_rows_kernel(const uint32_t* N, int64_t ldn, const uint32_t* nonpiv, uint32_t nnp,
-                                  const uint32_t* piv, uint32_t r, uint32_t* R, int64_t ldr) {
-    const uint64_t total = (uint64_t)r * nnp;
-    for (uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
          idx += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t t = (uint32_t)(idx / nnp), j = (uint32_t)(idx % nnp);
-        R[(int64_t)t * ldr + nonpiv[j]] = N[(int64_t)t * ldn + j];
-    }
-    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < r; t += (uint64_t)gridDim.x * blockDim.x)
-        R[(int64_t)t * ldr + piv[t]] = 1;
-}
-
-__global__ void gather_cols_kernel(const uint32_t* U, int64_t ldu, uint32_t r, const uint32_t* cols, uint32_t nc,
-                                   uint32_t* out, int64_t ldo) {
-    const uint64_t total = (uint64_t)r * nc;
-    for (uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-         idx += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t t = (uint32_t)(idx / nc), j = (uint32_t)(idx % nc);
-        out[(int64_t)t * ldo + j] = U[(int64_t)t * ldu + cols[j]];
+        const uint32_t i = (uint32_t)(idx / n), j = (uint32_t)(idx % n);
+        R[(int64_t)i * ldr + j] = i < r ? U[(size_t)i * n + j] : 0u;
     }
 }
 
@@ -348,7 +344,7 @@ void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const
     add((size_t)m * n);           // 0 S0
     add((size_t)m * n);           // 1 S1
     add((size_t)mn * n);          // 2 U store
-    add((size_t)LW * n);          // 3 X copy
+    add((size_t)2 * LW * n);      // 3 X copy (two panels: slot i & 1)
     add(m);                       // 4 rows_orig a
     add(m);                       // 5 rows_orig b
     add(m);                       // 6 R list
@@ -362,6 +358,7 @@ void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const
     add(mn);                      // 14 pivots
     add(mn);                      // 15 pivot rows
     add(16);                      // 16 state
+    add((size_t)mn * LW);         // 17 Gauss-Jordan left operand T
     size_t total = 0;
     std::vector<size_t> off;
     for (size_t w : sizes) { off.push_back(total); total += w; }
@@ -370,16 +367,17 @@ void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const
     auto at = [&](int i) { return base + off[i]; };
     uint32_t* Sb[2] = {at(0), at(1)};
     uint32_t* U = at(2);
-    uint32_t* X = at(3);
+    uint32_t* Xb = at(3);
+    uint32_t* T = at(17);
     uint32_t* ro[2] = {at(4), at(5)};
     uint32_t *Rl = at(6), *flags = at(7);
     uint32_t* cand[2] = {at(8), at(9)};
     uint32_t* cnt[2] = {at(10), at(11)};
     uint32_t *Pb = at(12), *Jb = at(13), *piv = at(14), *prow = at(15);
     LState* st = (LState*)at(16);
-    uint32_t* work_extra = nullptr;
     // lookahead: the trailing update of panel i is split into the next panel's LW columns (on s)
-    // and the rest (on s2, overlapping panel i+1's tournament); panel i+1's X waits for it.
+    // and the rest (on s2, overlapping panel i+1's tournament), followed on s2 by the Gauss-Jordan
+    // gather and product; panel i+1's X waits for s2 up to the gather (ev_rest).
     hipStream_t s2 = nullptr;
     hipEvent_t ev_next = nullptr, ev_rest = nullptr;
     HX_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
@@ -403,6 +401,7 @@ void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const
             const int slot = (int)(panel & 1);
             uint32_t* P = Pb + slot * LW;
             uint32_t* J = Jb + slot * LW;
+            uint32_t* X = Xb + (size_t)slot * LW * n;
             // 1. tournament (reads only this panel's LW columns, already up to date)
             rrp_chunk_kernel<Ar><<<nchunk, RT, 0, s>>>(S, n, st, wc, ar, cand[0], cnt[0]);
             after_launch("rrp_chunk_kernel");
@@ -425,14 +424,17 @@ void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const
             // 2. X = S[P,J]^{-1} S[P,:] into the U store and the contiguous copy
             panel_x_kernel<Ar><<<(na + SL - 1) / SL, PXT, 0, s>>>(S, n, na, P, J, st, ar, X, n, U, n, c, ro[cur], piv, prow);
             after_launch("panel_x_kernel");
+            // 3. the next Schur complement, compacted: first the next panel's columns, then the rest
             if (na > wc) {
-                // 3. the next Schur complement, compacted: first the next panel's columns, then the rest
                 rest_rows_kernel<<<1, 1024, 0, s>>>(st, P, flags, Rl, ro[cur], ro[cur ^ 1]);
                 after_launch("rest_rows_kernel");
-                snapshot_kernel<<<1, 64, 0, s>>>(st, slot);
-                after_launch("snapshot_kernel");
+            }
+            snapshot_kernel<<<1, 64, 0, s>>>(st, slot);
+            after_launch("snapshot_kernel");
+            HX_HIP(hipEventRecord(ev_next, s));     // X, the row list and the sizes are ready
+            HX_HIP(hipStreamWaitEvent(s2, ev_next, 0));
+            if (na > wc) {
                 const uint32_t nt = na - wc, n1 = std::min<uint32_t>(LW, nt);
-                if (nt > n1) HX_HIP(hipEventRecord(ev_next, s));     // X, the row list and the sizes are ready
                 GemmArgs g;
                 g.m = m;
                 g.k = LW;
@@ -457,24 +459,52 @@ void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const
                 g.Cout = Sb[cur ^ 1];
                 gemm_dispatch(ctx, g, s, tvar, ws, ws_bytes, 0);
                 if (nt > n1) {
-                    HX_HIP(hipStreamWaitEvent(s2, ev_next, 0));
                     g.n = nt - n1;
                     g.B = X + wc + n1;
                     g.Cin = S + wc + n1;
                     g.Cout = Sb[cur ^ 1] + n1;
                     gemm_dispatch(ctx, g, s2, tvar, ws2, ws2_bytes, 0);
-                    HX_HIP(hipEventRecord(ev_rest, s2));
-                    rest_pending = true;
                 }
                 cur ^= 1;
+            }
+            // 4. Gauss-Jordan on the stored pivot rows (at most min(c, mn) of them), on s2; not needed
+            //    when only the rank, pivots and pivot rows are asked for
+            const uint32_t rmax = Rout ? std::min<uint32_t>(c, mn) : 0u;
+            if (rmax > 0) {
+                gj_gather_kernel<<<gsz((uint64_t)rmax * LW), 256, 0, s2>>>(U, n, c, J, st, slot, T);
+                after_launch("gj_gather_kernel");
+            }
+            HX_HIP(hipEventRecord(ev_rest, s2));
+            rest_pending = true;
+            if (rmax > 0) {
+                GemmArgs g;
+                g.m = rmax;
+                g.k = LW;
+                g.n = na;
+                g.A = T;
+                g.ars = LW;
+                g.acs = 1;
+                g.B = X;
+                g.brs = n;
+                g.bcs = 1;
+                g.Cin = U + c;
+                g.cirs = n;
+                g.cics = 1;
+                g.Cout = U + c;
+                g.cors = n;
+                g.cocs = 1;
+                g.m_dev = &st->snap_r[slot];
+                g.k_dev = &st->snap_k[slot];
+                g.op = HX_OP_SUB;
+                gemm_dispatch(ctx, g, s2, HX_GEMM_PLAIN, nullptr, 0, 0);
             }
             advance_kernel<<<1, 64, 0, s>>>(st);
             after_launch("advance_kernel");
         }
-        if (rest_pending) {
-            HX_HIP(hipStreamWaitEvent(s, ev_rest, 0));
-            rest_pending = false;
-        }
+        // every product on s2 (the last Gauss-Jordan product included) before U is read
+        HX_HIP(hipEventRecord(ev_rest, s2));
+        HX_HIP(hipStreamWaitEvent(s, ev_rest, 0));
+        rest_pending = false;
         LState h;
         HX_HIP(hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, s));
         HX_HIP(hipStreamSynchronize(s));
@@ -490,35 +520,8 @@ void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const
             if (pivot_rows_h) std::copy(hr.begin(), hr.begin() + r, pivot_rows_h);
         }
         if (Rout) {
-            // RREF = U_piv^{-1} U on the non-pivot columns (one unit-upper TRSM), zero rows below
-            std::vector<uint32_t> nonpiv;
-            std::vector<char> isp(n, 0);
-            for (uint32_t t = 0; t < r; ++t) isp[hp[t]] = 1;
-            for (uint32_t j = 0; j < n; ++j)
-                if (!isp[j]) nonpiv.push_back(j);
-            const uint32_t nnp = (uint32_t)nonpiv.size();
-            const size_t extra = (size_t)r * r + (size_t)r * std::max<uint32_t>(nnp, 1) + nnp + r + 64;
-            HX_HIP(hipMalloc(&work_extra, extra * 4));
-            uint32_t* T = work_extra;
-            uint32_t* N = T + (size_t)r * r;
-            uint32_t* d_np = N + (size_t)r * std::max<uint32_t>(nnp, 1);
-            uint32_t* d_pv = d_np + nnp;
-            if (nnp) HX_HIP(hipMemcpyAsync(d_np, nonpiv.data(), (size_t)nnp * 4, hipMemcpyHostToDevice, s));
-            if (r) HX_HIP(hipMemcpyAsync(d_pv, hp.data(), (size_t)r * 4, hipMemcpyHostToDevice, s));
-            zero_rows_kernel<<<gsz((uint64_t)m * n), 256, 0, s>>>(m, n, Rout, ldr);
-            after_launch("zero_rows_kernel");
-            if (r && nnp) {
-                gather_cols_kernel<<<gsz((uint64_t)r * r), 256, 0, s>>>(U, n, r, d_pv, r, T, r);
-                after_launch("gather_cols_kernel(T)");
-                gather_cols_kernel<<<gsz((uint64_t)r * nnp), 256, 0, s>>>(U, n, r, d_np, nnp, N, nnp);
-                after_launch("gather_cols_kernel(N)");
-                trsm_unit(ctx, hx_view{T, (int64_t)r, 1, r, r}, hx_view{N, (int64_t)nnp, 1, r, nnp}, 1, s);
-            }
-            if (r) {
-                place_rows_kernel<<<gsz((uint64_t)r * std::max<uint32_t>(nnp, 1)), 256, 0, s>>>(N, nnp, d_np, nnp, d_pv, r,
-                                                                                                Rout, ldr);
-                after_launch("place_rows_kernel");
-            }
+            rref_out_kernel<<<gsz((uint64_t)m * n), 256, 0, s>>>(U, n, r, m, Rout, ldr);
+            after_launch("rref_out_kernel");
             HX_HIP(hipStreamSynchronize(s));
         }
     } catch (...) {
@@ -528,7 +531,6 @@ void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const
         (void)hipEventDestroy(ev_rest);
         (void)hipStreamDestroy(s2);
         (void)hipFree(base);
-        (void)hipFree(work_extra);
         throw;
     }
     HX_HIP(hipStreamSynchronize(s2));
@@ -536,7 +538,6 @@ void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const
     HX_HIP(hipEventDestroy(ev_rest));
     HX_HIP(hipStreamDestroy(s2));
     HX_HIP(hipFree(base));
-    if (work_extra) HX_HIP(hipFree(work_extra));
 }
 
 }  // namespace
