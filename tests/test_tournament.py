@@ -1,0 +1,81 @@
+"""CPU check of the pivot tournament used by the large RREF (hx_rref_large.hip): the row rank profile
+of a panel found on 256-row chunks (level 0), then on groups of up to 16 candidate lists (merge
+levels, repeated until one list is left), re-running the smallest-row rule on the raw candidate rows
+at every merge, must give the same pivot rows, in the same pivot order, as the rule on the whole
+panel (the oracle's hxo_rref_rule).  The claim holds for any merge tree; a pairwise tree over 64-row
+leaves is checked as well.  Planted ranks, zero rows and a negative control where a chunk drops its
+candidates."""
+import numpy as np
+import pytest
+
+P = 101
+
+
+def _rule(rows, A):
+    """The smallest-row rule on the listed rows (ascending) of panel A, division-free: the chosen rows
+    (ascending) and the pivot order [(row, column)]."""
+    M = A[rows].astype(np.int64) % P
+    done = np.zeros(len(rows), bool)
+    order = []
+    for q in range(A.shape[1]):
+        cand = [i for i in range(len(rows)) if not done[i] and M[i, q] != 0]
+        if not cand:
+            continue
+        b = cand[0]
+        done[b] = True
+        for i in range(len(rows)):
+            if i != b and not done[i] and M[i, q] != 0:
+                M[i] = (M[b, q] * M[i] - M[i, q] * M[b]) % P
+        order.append((rows[b], q))
+    return [r for r, d in zip(rows, done) if d], order
+
+
+def _pairwise(lists, A):
+    while len(lists) > 1:
+        lists = [_rule(sorted(lists[i] + (lists[i + 1] if i + 1 < len(lists) else [])), A)[0]
+                 for i in range(0, len(lists), 2)]
+    return lists[0] if lists else []
+
+
+def _tournament(A, drop_leaf=None):
+    """The kernels' tree: 256-row chunks, then merges of up to 16 lists until one is left."""
+    m = A.shape[0]
+    lists = [_rule(list(range(r, min(r + 256, m))), A)[0] for r in range(0, m, 256)]
+    if drop_leaf is not None and drop_leaf < len(lists):
+        lists[drop_leaf] = []
+    while len(lists) > 1:
+        lists = [_rule(sorted(sum(lists[i:i + 16], [])), A)[0] for i in range(0, len(lists), 16)]
+    final = lists[0] if lists else []
+    return _rule(final, A)[1]
+
+
+def _pairwise_tree(A):
+    m = A.shape[0]
+    leaves = [_rule(list(range(r, min(r + 64, m))), A)[0] for r in range(0, m, 64)]
+    return _rule(_pairwise(leaves, A), A)[1]
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_tree_tournament_equals_rule(O, seed):
+    rng = np.random.default_rng(seed)
+    m, w = int(rng.integers(1, 1400)), 32
+    rank = int(rng.integers(0, 33))
+    A = ((rng.integers(0, P, (m, rank)) @ rng.integers(0, P, (rank, w))) % P).astype(np.uint32)
+    A[rng.random(m) < 0.25] = 0
+    r, piv, prow, _ = O.rref_rule(P, A, want_rref=False)
+    want = list(zip(prow.tolist(), piv.tolist()))
+    assert _tournament(A) == want
+    assert _pairwise_tree(A) == want
+
+
+def test_tree_tournament_negative_control(O):
+    """Dropping one leaf's candidates (a plausible indexing mistake) changes the pivots."""
+    rng = np.random.default_rng(99)
+    A = ((rng.integers(0, P, (300, 6)) @ rng.integers(0, P, (6, 32))) % P).astype(np.uint32)
+    A = np.concatenate([A, A[:40]])
+    A[:256] = 0
+    A[256:262] = ((rng.integers(0, P, (6, 6)) @ rng.integers(0, P, (6, 32))) % P).astype(np.uint32)
+    r, piv, prow, _ = O.rref_rule(P, A, want_rref=False)
+    want = list(zip(prow.tolist(), piv.tolist()))
+    assert _tournament(A) == want
+    assert _tournament(A, drop_leaf=1) != want
