@@ -28,7 +28,7 @@ sys.path.insert(0, ROOT)
 
 N_C3 = 8192
 PEAK_INT8_TOPS = 5000.0      # MI355X dense INT8 MFMA, tera-ops/s
-LIMB_DEFAULT = 2             # the limb kernel hx_gemm_u32 runs when opts.tile is 0
+LIMB_DEFAULT = 4             # the limb kernel hx_gemm_u32 runs when opts.tile is 0
 LIMB_KERNELS = {1: "limb_kernel<%d>", 2: "limb2_kernel<%d>", 3: "limb3_kernel<%d>", 4: "limb4_kernel<%d>"}
 PEAK_HBM_GBS = 8000.0
 

@@ -856,7 +856,7 @@ void limb_attrs() {
 // (limb_kernel), 2 = direct-to-LDS copy ring (limb2_kernel), 3 = the ring with the LDS reads
 // pipelined against the MFMAs (limb3_kernel), 4 = limb3's pipeline on eight waves
 // (limb4_kernel); 0 = the default.
-constexpr int kLimbDefault = 2;
+constexpr int kLimbDefault = 4;
 inline int limb_version(int tile) { return tile >= 1 && tile <= 4 ? tile : kLimbDefault; }
 
 template <int L>
@@ -1188,6 +1188,13 @@ void gemm_batch_run(void* vb, hipStream_t s) {
                 case 2: limb3_batch_kernel<2><<<b->tiles_limb, 256, 0, s>>>(b->d_limb, pre_limb, b->n_limb); break;
                 case 3: limb3_batch_kernel<3><<<b->tiles_limb, 256, 0, s>>>(b->d_limb, pre_limb, b->n_limb); break;
                 default: limb3_batch_kernel<4><<<b->tiles_limb, 256, 0, s>>>(b->d_limb, pre_limb, b->n_limb); break;
+            }
+        } else if (kLimbDefault == 4) {
+            switch (b->ctx.limbs) {
+                case 1: limb4_batch_kernel<1><<<b->tiles_limb, 512, 0, s>>>(b->d_limb, pre_limb, b->n_limb); break;
+                case 2: limb4_batch_kernel<2><<<b->tiles_limb, 512, 0, s>>>(b->d_limb, pre_limb, b->n_limb); break;
+                case 3: limb4_batch_kernel<3><<<b->tiles_limb, 512, 0, s>>>(b->d_limb, pre_limb, b->n_limb); break;
+                default: limb4_batch_kernel<4><<<b->tiles_limb, 512, 0, s>>>(b->d_limb, pre_limb, b->n_limb); break;
             }
         } else if (kLimbDefault == 2) {
             switch (b->ctx.limbs) {

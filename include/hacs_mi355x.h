@@ -79,8 +79,9 @@ enum {
 typedef struct hx_gemm_opts {   /* GemmOpts, CudaKernels.h:64-79 */
     int variant;            /* HX_GEMM_* */
     int tile;               /* 0 = automatic; a variant-specific choice (tuning only, same bytes): for LIMB,
-                               1 = register-staged LDS copy, 2 = direct-to-LDS copy ring (the default),
-                               3 = the ring with LDS reads pipelined against the MFMAs, 4 = that on eight waves */
+                               1 = register-staged LDS copy, 2 = direct-to-LDS copy ring,
+                               3 = the ring with LDS reads pipelined against the MFMAs, 4 = the ring on eight
+                               waves, one 32 x 32 block each (the default) */
     void *workspace;        /* limb route: >= hx_gemm_workspace_bytes(); NULL = library-owned (not capturable) */
     size_t workspace_bytes;
     hx_event main_begin;    /* optional timing events recorded on the stream right before / after the */
