@@ -203,9 +203,17 @@ __device__ void howell_blocked(const HJob& job, uint32_t k, uint32_t rc, uint32_
     uint32_t* prow = act + rc;                 // pivot row ids in pivot order
     uint8_t* nz = (uint8_t*)(prow + rc);       // row has a non-zero multiplier in this panel
     uint32_t* rows = job.work;                 // row id i at rows + i * n
-    for (uint64_t idx = tid; idx < (uint64_t)m * n; idx += HT) {
-        const uint32_t i = (uint32_t)(idx / n), j = (uint32_t)(idx % n);
-        rows[(size_t)i * n + j] = job.A[(int64_t)i * job.lda + j] & mask;
+    for (uint32_t i = wave; i < m; i += HT / 64) {     // a wave per row, four words per lane in flight
+        const uint32_t* Ai = job.A + (int64_t)i * job.lda;
+        uint32_t* Ri = rows + (size_t)i * n;
+        for (uint32_t j0 = lane; j0 < n; j0 += 256) {
+            uint32_t v[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) v[e] = j0 + 64 * e < n ? Ai[j0 + 64 * e] : 0u;
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                if (j0 + 64 * e < n) Ri[j0 + 64 * e] = v[e] & mask;
+        }
     }
     for (uint32_t i = tid; i < m; i += HT) act[i] = i;
     __syncthreads();
@@ -260,7 +268,16 @@ __device__ void howell_blocked(const HJob& job, uint32_t k, uint32_t rc, uint32_
                 qm[id * HW + t] = q;                           // w.r.t. the scaled pivot row
                 nz[id] = 1;
                 const uint32_t qe = kmul<SMALLK>(q, ui) & mask;
-                for (uint32_t t2 = t; t2 < wc; ++t2) R[t2] = (R[t2] - kmul<SMALLK>(qe, P[t2])) & mask;
+                uint32_t rv[HW], pv[HW];                       // all reads before any write (same LDS array)
+#pragma unroll
+                for (uint32_t t2 = 0; t2 < HW; ++t2)
+                    if (t2 >= t && t2 < wc) {
+                        rv[t2] = R[t2];
+                        pv[t2] = P[t2];
+                    }
+#pragma unroll
+                for (uint32_t t2 = 0; t2 < HW; ++t2)
+                    if (t2 >= t && t2 < wc) R[t2] = (rv[t2] - kmul<SMALLK>(qe, pv[t2])) & mask;
             }
             if (v > 0) {                                       // closure row 2^(k-v) u^-1 P at the bottom
                 const uint32_t cid = m + ncl, sh = k - v;      // 1 <= sh <= 31
@@ -296,19 +313,28 @@ __device__ void howell_blocked(const HJob& job, uint32_t k, uint32_t rc, uint32_
         for (uint32_t j0 = c1; j0 < n; j0 += HT) {
             const uint32_t j = j0 + tid;
             if (j < n) {
+                // all HW pivot-row entries are loaded before any is written back (the rows are
+                // distinct), so the loads are in flight together instead of one after another
                 uint32_t T[HW];
 #pragma unroll
                 for (uint32_t t = 0; t < HW; ++t) {
-                    T[t] = 0;
                     const uint32_t pid = s_pid[t];
-                    if (pid == NONE) continue;
-                    uint32_t x = rows[(size_t)pid * n + j];
+                    T[t] = pid == NONE ? 0u : rows[(size_t)pid * n + j];
+                }
+#pragma unroll
+                for (uint32_t t = 0; t < HW; ++t) {
+                    const uint32_t pid = s_pid[t];
+                    if (pid == NONE) { T[t] = 0; continue; }
+                    uint32_t x = T[t];
                     const uint32_t* qr = qm + pid * HW;
 #pragma unroll
                     for (uint32_t s2 = 0; s2 < t; ++s2) x -= kmul<SMALLK>(qr[s2], T[s2]);
-                    x = kmul<SMALLK>(x & mask, s_ui[t] & mask) & mask;
-                    T[t] = x;
-                    rows[(size_t)pid * n + j] = x;
+                    T[t] = kmul<SMALLK>(x & mask, s_ui[t] & mask) & mask;
+                }
+#pragma unroll
+                for (uint32_t t = 0; t < HW; ++t) {
+                    const uint32_t pid = s_pid[t];
+                    if (pid != NONE) rows[(size_t)pid * n + j] = T[t];
                 }
 #pragma unroll
                 for (uint32_t t = 0; t < HW; ++t) s_T[t * HT + tid] = T[t];
@@ -322,18 +348,25 @@ __device__ void howell_blocked(const HJob& job, uint32_t k, uint32_t rc, uint32_
             bool in[4];
 #pragma unroll
             for (uint32_t u = 0; u < 4; ++u) in[u] = j0 + lane + 64 * u < n;
-            for (uint32_t pos = wave * 8; pos < na; pos += 32) {
-                uint32_t ids[8], xs[8][4];
+            // groups of 8 active rows per wave; the next group's ids and entries are loaded before
+            // this group's entries are written back (different rows)
+            uint32_t ids[8], xs[8][4];
+            auto load_group = [&](uint32_t pos, uint32_t (&gid)[8], uint32_t (&gx)[8][4]) {
 #pragma unroll
                 for (int r = 0; r < 8; ++r) {
                     const uint32_t id = pos + r < na ? act[pos + r] : NONE;
-                    ids[r] = (id != NONE && nz[id]) ? id : NONE;
+                    gid[r] = (id != NONE && nz[id]) ? id : NONE;
                 }
 #pragma unroll
                 for (int r = 0; r < 8; ++r)
 #pragma unroll
                     for (uint32_t u = 0; u < 4; ++u)
-                        if (ids[r] != NONE && in[u]) xs[r][u] = rows[(size_t)ids[r] * n + j0 + lane + 64 * u];
+                        gx[r][u] = (gid[r] != NONE && in[u]) ? rows[(size_t)gid[r] * n + j0 + lane + 64 * u] : 0u;
+            };
+            if (wave * 8 < na) load_group(wave * 8, ids, xs);
+            for (uint32_t pos = wave * 8; pos < na; pos += 32) {
+                uint32_t nids[8], nxs[8][4];
+                if (pos + 32 < na) load_group(pos + 32, nids, nxs);
 #pragma unroll
                 for (int r = 0; r < 8; ++r) {
                     if (ids[r] == NONE) continue;
@@ -350,15 +383,35 @@ __device__ void howell_blocked(const HJob& job, uint32_t k, uint32_t rc, uint32_
                         rows[(size_t)ids[r] * n + j0 + lane + 64 * u] = x;
                     }
                 }
+                if (pos + 32 < na) {
+#pragma unroll
+                    for (int r = 0; r < 8; ++r) {
+                        ids[r] = nids[r];
+#pragma unroll
+                        for (uint32_t u = 0; u < 4; ++u) xs[r][u] = nxs[r][u];
+                    }
+                }
             }
             __syncthreads();
         }
     }
     // H rows in pivot order (masked).  Left of its pivot column a pivot row is zero mod 2^k; its
     // stored entries there may be stale (panel entries are written back for pivot rows only).
-    for (uint64_t idx = tid; idx < (uint64_t)h * n; idx += HT) {
-        const uint32_t i = (uint32_t)(idx / n), j = (uint32_t)(idx % n);
-        job.H[(size_t)i * n + j] = j < job.pivots[i] ? 0u : rows[(size_t)prow[i] * n + j] & mask;
+    for (uint32_t i = wave; i < h; i += HT / 64) {     // a wave per row, four words per lane in flight
+        const uint32_t* Ri = rows + (size_t)prow[i] * n;
+        uint32_t* Hi = job.H + (size_t)i * n;
+        const uint32_t pc = job.pivots[i];
+        for (uint32_t j0 = lane; j0 < n; j0 += 256) {
+            uint32_t v[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const uint32_t j = j0 + 64 * e;
+                v[e] = (j < n && j >= pc) ? Ri[j] & mask : 0u;
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                if (j0 + 64 * e < n) Hi[j0 + 64 * e] = v[e];
+        }
     }
     __syncthreads();
     // blocked reduce above
@@ -392,11 +445,16 @@ __device__ void howell_blocked(const HJob& job, uint32_t k, uint32_t rc, uint32_
             uint32_t Hr[HW];
 #pragma unroll
             for (uint32_t t = 0; t < HW; ++t) Hr[t] = t < bw ? job.H[(size_t)(i0 + t) * n + j] : 0u;
+            // rows in groups of 8; the next group's entries are loaded before this group's are
+            // written back (different rows), so a load's latency overlaps the previous group
+            uint32_t xs[8];
+#pragma unroll
+            for (uint32_t u = 0; u < 8; ++u) xs[u] = u < i1 ? job.H[(size_t)u * n + j] : 0u;
             for (uint32_t ip0 = 0; ip0 < i1; ip0 += 8) {
-                uint32_t xs[8];
+                uint32_t xn[8];
 #pragma unroll
                 for (uint32_t u = 0; u < 8; ++u)
-                    if (ip0 + u < i1) xs[u] = job.H[(size_t)(ip0 + u) * n + j];
+                    xn[u] = ip0 + 8 + u < i1 ? job.H[(size_t)(ip0 + 8 + u) * n + j] : 0u;
 #pragma unroll
                 for (uint32_t u = 0; u < 8; ++u) {
                     if (ip0 + u >= i1) continue;
@@ -406,6 +464,8 @@ __device__ void howell_blocked(const HJob& job, uint32_t k, uint32_t rc, uint32_
                     for (uint32_t t = 0; t < HW; ++t) x -= kmul<SMALLK>(qr[t], Hr[t]);
                     job.H[(size_t)(ip0 + u) * n + j] = x & mask;
                 }
+#pragma unroll
+                for (uint32_t u = 0; u < 8; ++u) xs[u] = xn[u];
             }
         }
         __syncthreads();
