@@ -268,16 +268,7 @@ __device__ void howell_blocked(const HJob& job, uint32_t k, uint32_t rc, uint32_
                 qm[id * HW + t] = q;                           // w.r.t. the scaled pivot row
                 nz[id] = 1;
                 const uint32_t qe = kmul<SMALLK>(q, ui) & mask;
-                uint32_t rv[HW], pv[HW];                       // all reads before any write (same LDS array)
-#pragma unroll
-                for (uint32_t t2 = 0; t2 < HW; ++t2)
-                    if (t2 >= t && t2 < wc) {
-                        rv[t2] = R[t2];
-                        pv[t2] = P[t2];
-                    }
-#pragma unroll
-                for (uint32_t t2 = 0; t2 < HW; ++t2)
-                    if (t2 >= t && t2 < wc) R[t2] = (rv[t2] - kmul<SMALLK>(qe, pv[t2])) & mask;
+                for (uint32_t t2 = t; t2 < wc; ++t2) R[t2] = (R[t2] - kmul<SMALLK>(qe, P[t2])) & mask;
             }
             if (v > 0) {                                       // closure row 2^(k-v) u^-1 P at the bottom
                 const uint32_t cid = m + ncl, sh = k - v;      // 1 <= sh <= 31
