@@ -268,7 +268,18 @@ __device__ void howell_blocked(const HJob& job, uint32_t k, uint32_t rc, uint32_
                 qm[id * HW + t] = q;                           // w.r.t. the scaled pivot row
                 nz[id] = 1;
                 const uint32_t qe = kmul<SMALLK>(q, ui) & mask;
-                for (uint32_t t2 = t; t2 < wc; ++t2) R[t2] = (R[t2] - kmul<SMALLK>(qe, P[t2])) & mask;
+                uint32_t t2 = t;                               // four columns per round, reads first
+                for (; t2 + 4 <= wc; t2 += 4) {
+                    uint32_t rv[4], pv[4];
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        rv[e] = R[t2 + e];
+                        pv[e] = P[t2 + e];
+                    }
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) R[t2 + e] = (rv[e] - kmul<SMALLK>(qe, pv[e])) & mask;
+                }
+                for (; t2 < wc; ++t2) R[t2] = (R[t2] - kmul<SMALLK>(qe, P[t2])) & mask;
             }
             if (v > 0) {                                       // closure row 2^(k-v) u^-1 P at the bottom
                 const uint32_t cid = m + ncl, sh = k - v;      // 1 <= sh <= 31

@@ -172,6 +172,26 @@ __device__ void trailing_mfma(const RJob& job, const RArgs& a, const uint32_t* B
     }
 }
 
+// row[q2] <- (v row[q2] - f prow[q2]) R^-1 for q2 in (q, wc), four columns at a time with their
+// eight reads issued before the four writes: row and prow lie in the same LDS array, so a loop that
+// stores each entry before the next loads would wait one LDS round trip per column.
+template <class Ar>
+__device__ __forceinline__ void elim_row(uint32_t* row, const uint32_t* prow, uint32_t q, uint32_t wc, uint32_t v,
+                                         uint32_t fneg, const Ar& ar) {
+    uint32_t q2 = q + 1;
+    for (; q2 + 4 <= wc; q2 += 4) {
+        uint32_t x[4], y[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            x[e] = row[q2 + e];
+            y[e] = prow[q2 + e];
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) row[q2 + e] = ar.dfree(v, x[e], fneg, y[e]);
+    }
+    for (; q2 < wc; ++q2) row[q2] = ar.dfree(v, row[q2], fneg, prow[q2]);
+}
+
 __device__ __forceinline__ uint32_t wave_min(uint32_t x) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) x = min(x, (uint32_t)__shfl_xor((int)x, off));
@@ -362,7 +382,7 @@ __device__ void rref_one(const RJob& job, const Ar& ar, const RArgs& a, uint32_t
                         const uint32_t f = row[q];
                         if (f == 0) continue;
                         const uint32_t fneg = p - f;
-                        for (uint32_t q2 = q + 1; q2 < wc; ++q2) row[q2] = ar.dfree(v, row[q2], fneg, prow[q2]);
+                        elim_row(row, prow, q, wc, v, fneg, ar);
                     }
                     if (lane == 0) { s_P[k] = list[bu]; s_q[k] = q; }
                     ++k;
@@ -390,7 +410,7 @@ __device__ void rref_one(const RJob& job, const Ar& ar, const RArgs& a, uint32_t
                 const uint32_t f = row[q];
                 if (f == 0) continue;
                 const uint32_t fneg = p - f;
-                for (uint32_t q2 = q + 1; q2 < wc; ++q2) row[q2] = ar.dfree(v, row[q2], fneg, prow[q2]);
+                elim_row(row, prow, q, wc, v, fneg, ar);
             }
             if (tid == 0) { s_P[k] = list[bu]; s_q[k] = q; }
             ++k;

@@ -157,6 +157,53 @@ private:
     hx_gemm_batch h_ = nullptr;
 };
 
+// ---- level-1/2 and tape support (CudaKernels.h:186-238), LaunchStats (CudaKernels.h:299-306) ----
+template <class E>
+inline hx_view cview(DeviceView<E> v) { return hx_view{(void*)v.base, v.rs, v.cs, v.rows, v.cols}; }
+
+inline void gemv(const ModCtx32& ctx, DeviceView<const uint32_t> A, DeviceView<const uint32_t> x,
+                 DeviceView<uint32_t> y, GemmOp op, Stream s) {
+    check(hx_gemv_u32(&ctx.c, cview(A), cview(x), cview(y), (int)op, s.h));
+}
+inline void axpy(const ModCtx32& ctx, uint32_t alpha, DeviceView<const uint32_t> x, DeviceView<uint32_t> y,
+                 bool subtract, Stream s) {
+    check(hx_axpy_u32(&ctx.c, alpha, cview(x), cview(y), subtract ? 1 : 0, s.h));
+}
+inline void scal(const ModCtx32& ctx, uint32_t alpha, DeviceView<uint32_t> x, Stream s) {
+    check(hx_scal_u32(&ctx.c, alpha, cview(x), s.h));
+}
+inline void copy(DeviceView<const uint32_t> src, DeviceView<uint32_t> dst, Stream s) {
+    check(hx_copy_u32(cview(src), cview(dst), s.h));
+}
+inline void transpose(DeviceView<const uint32_t> src, DeviceView<uint32_t> dst, Stream s) {
+    check(hx_transpose_u32(cview(src), cview(dst), s.h));
+}
+inline void fill(DeviceView<uint32_t> x, uint32_t value, Stream s) { check(hx_fill_u32(cview(x), value, s.h)); }
+inline void axpy_identity(const ModCtx32& ctx, uint32_t alpha, DeviceView<uint32_t> y, bool subtract, Stream s) {
+    check(hx_axpy_identity_u32(&ctx.c, alpha, cview(y), subtract ? 1 : 0, s.h));
+}
+inline void gather(const ModCtx32& ctx, uint32_t alpha, uint32_t beta, DeviceView<const uint32_t> a, uint32_t row_off,
+                   const uint32_t* cols, DeviceView<uint32_t> c, bool accumulate, Stream s,
+                   const uint32_t* rows = nullptr) {
+    check(hx_gather_u32(&ctx.c, alpha, beta, cview(a), row_off, cols, cview(c), accumulate ? 1 : 0, rows, s.h));
+}
+inline void canonicalise(const ModCtx32& ctx, DeviceView<uint32_t> v, Stream s) {
+    check(hx_canonicalise_u32(&ctx.c, cview(v), s.h));
+}
+inline void canonicalise(const ModCtx32& ctx, DeviceView<const uint64_t> acc, DeviceView<uint32_t> dst, Stream s) {
+    check(hx_canonicalise_acc64_u32(&ctx.c, cview(acc), cview(dst), s.h));
+}
+struct LaunchStats {
+    uint64_t kernel_launches = 0, gemm_launches = 0, batched_instances = 0;
+    uint64_t imma_launches = 0;   // the limb-MFMA route's launches
+};
+inline LaunchStats launch_stats() {
+    hx_stats st;
+    check(hx_stats_get(&st));
+    return LaunchStats{st.kernel_launches, st.gemm_launches, st.batched_instances, st.limb_launches};
+}
+inline void reset_launch_stats() { check(hx_stats_reset()); }
+
 inline void trsm_unit(const ModCtx32& ctx, DeviceView<const uint32_t> T, DeviceView<uint32_t> B, bool upper,
                       Stream s) {
     check(hx_trsm_unit_u32(&ctx.c, hx_view{(void*)T.base, T.rs, T.cs, T.rows, T.cols}, B.c_view(), upper ? 1 : 0,

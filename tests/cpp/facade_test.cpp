@@ -63,6 +63,23 @@ int main() {
     bool zero = true;
     for (uint32_t i = 0; i < 50 * n; ++i) zero = zero && C[i] == 0;   // C held A B: C - A B = 0 on those rows
     expect(C2 == want && zero, "GemmBatch (uneven jobs, Set and Sub)");
+    // level-1 / tape support: C2 = 3 A B + C2 (axpy), its transpose, a gathered column block; launch counters
+    hxe::reset_launch_stats();
+    hxe::axpy(ctx, 3u, dC.view(), dC2.view(), false, s);
+    hxe::DeviceBuf dT(n, m);
+    hxe::transpose(dC2.view(), dT.view(), s);
+    std::vector<uint32_t> T((size_t)n * m);
+    dT.download(T.data(), s);
+    bool tr_ok = true;
+    for (uint32_t i = 0; i < m && tr_ok; ++i)
+        for (uint32_t j = 0; j < n; ++j) {
+            const uint32_t c_ij = i < 50 ? 0u : want[(size_t)i * n + j];     // dC: A B with rows < 50 cleared
+            const uint64_t e = ((uint64_t)want[(size_t)i * n + j] + 3ull * c_ij) % p;
+            if (T[(size_t)j * m + i] != (uint32_t)e) { tr_ok = false; break; }
+        }
+    expect(tr_ok, "axpy then transpose == exact");
+    const hxe::LaunchStats ls = hxe::launch_stats();
+    expect(ls.kernel_launches == 2 && ls.gemm_launches == 0, "launch_stats counts the two launches");
     // contract breach -> CudaError
     bool threw = false;
     try {
