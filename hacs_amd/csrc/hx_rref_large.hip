@@ -98,8 +98,20 @@ __device__ uint32_t lds_rule(uint32_t* pan, uint32_t rows, uint32_t wc, uint32_t
                 if (f != 0) {
                     const uint32_t* prow = pan + bu * ws;
                     const uint32_t v = prow[q], fneg = p - f;
-                    for (uint32_t q2 = q + 1 + ((part - (q + 1)) & (TPR - 1)); q2 < wc; q2 += TPR)
-                        row[q2] = ar.dfree(v, row[q2], fneg, prow[q2]);
+                    // this thread's columns past q, four at a time with the reads first (row and
+                    // prow share the LDS array: a store before the next load costs a round trip)
+                    uint32_t q2 = q + 1 + ((part - (q + 1)) & (TPR - 1));
+                    for (; q2 + 3 * TPR < wc; q2 += 4 * TPR) {
+                        uint32_t x[4], y[4];
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) {
+                            x[e] = row[q2 + e * TPR];
+                            y[e] = prow[q2 + e * TPR];
+                        }
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) row[q2 + e * TPR] = ar.dfree(v, x[e], fneg, y[e]);
+                    }
+                    for (; q2 < wc; q2 += TPR) row[q2] = ar.dfree(v, row[q2], fneg, prow[q2]);
                 }
             }
         }
