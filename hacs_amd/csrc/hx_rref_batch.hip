@@ -37,7 +37,6 @@ namespace {
 
 constexpr int RT = 256;   // threads per job workgroup
 constexpr int RW = 32;    // widest panel
-constexpr uint32_t kWavePanelRows = 128;   // panels of at most this many unused rows: one wave, no barriers
 
 struct RJob {
     uint32_t m, n;
@@ -321,7 +320,6 @@ __device__ void rref_one(const RJob& job, const Ar& ar, const RArgs& a, uint32_t
     __shared__ uint32_t s_cand[2][RT / 64];
     __shared__ uint32_t s_P[RW], s_q[RW], s_inv[RW];
     __shared__ uint32_t s_chunk[RT];
-    __shared__ uint32_t s_k;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t m = job.m, n = job.n, p = ar.p;
     const int64_t lda = job.lda;
@@ -359,39 +357,6 @@ __device__ void rref_one(const RJob& job, const Ar& ar, const RArgs& a, uint32_t
         for (uint32_t u = tid; u < mu; u += RT) chosen[u] = 0;
         __syncthreads();
         uint32_t k = 0;
-        if (mu <= kWavePanelRows) {
-            // a short panel is factored by wave 0 alone, with no barrier per column: lane l owns the
-            // rows l, l + 64, ...; the pivot is the lowest set bit of a ballot (rows are in original
-            // order, so this is the smallest-row rule); the wave's LDS accesses complete in issue
-            // order, so every lane sees the pivot row as its owner last wrote it
-            if (wave == 0) {
-                for (uint32_t q = 0; q < wc; ++q) {
-                    uint32_t bu = 0xffffffffu;
-                    for (uint32_t u0 = 0; u0 < mu && bu == 0xffffffffu; u0 += 64) {
-                        const uint32_t u = u0 + lane;
-                        const uint64_t bal = __ballot(u < mu && !chosen[u] && pan[u * ws + q] != 0);
-                        if (bal) bu = u0 + (uint32_t)(__ffsll((long long)bal) - 1);
-                    }
-                    if (bu == 0xffffffffu) continue;
-                    const uint32_t* prow = pan + bu * ws;
-                    const uint32_t v = prow[q];
-                    for (uint32_t u = lane; u < mu; u += 64) {
-                        if (u == bu) { chosen[u] = 1; continue; }
-                        if (chosen[u]) continue;
-                        uint32_t* row = pan + u * ws;
-                        const uint32_t f = row[q];
-                        if (f == 0) continue;
-                        const uint32_t fneg = p - f;
-                        elim_row(row, prow, q, wc, v, fneg, ar);
-                    }
-                    if (lane == 0) { s_P[k] = list[bu]; s_q[k] = q; }
-                    ++k;
-                }
-                if (lane == 0) s_k = k;
-            }
-            __syncthreads();
-            k = s_k;
-        } else
         for (uint32_t q = 0; q < wc; ++q) {
             uint32_t best = 0xffffffffu;
             for (uint32_t u = tid; u < mu; u += RT)
