@@ -523,6 +523,10 @@ struct HBatch {
     HJob* d_jobs = nullptr;
     uint32_t* d_next = nullptr;          // one counter per class
     HClass cls[NCLASS];
+    // the two smaller classes run on a second stream beside the larger ones (forked from and joined
+    // back into the caller's stream by events, so a capture on the caller's stream records both)
+    hipStream_t s2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 void destroy(HBatch* b) {
@@ -531,6 +535,9 @@ void destroy(HBatch* b) {
     if (b->d_next) (void)hipFree(b->d_next);
     for (HClass& c : b->cls)
         if (c.d_order) (void)hipFree(c.d_order);
+    if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+    if (b->ev_join) (void)hipEventDestroy(b->ev_join);
+    if (b->s2) (void)hipStreamDestroy(b->s2);
     delete b;
 }
 
@@ -611,6 +618,9 @@ HX_API int hx_howell_batch_create(uint32_t k, const hx_howell_job* jobs, uint32_
                 }
                 cl.grid = std::max<uint32_t>(1, std::min<uint32_t>(cl.njobs, (uint32_t)cus * per_cu));
             }
+            HX_HIP(hipStreamCreateWithFlags(&b->s2, hipStreamNonBlocking));
+            HX_HIP(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
+            HX_HIP(hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming));
         } catch (...) {
             destroy(b);
             throw;
@@ -624,23 +634,31 @@ HX_API int hx_howell_batch_run(hx_howell_batch vb, hx_stream s) {
         HX_CHECK(vb != nullptr, "NULL batch");
         HBatch* b = (HBatch*)vb;
         if (b->njobs == 0) return;
-        zero_counters_kernel<<<1, 64, 0, (hipStream_t)s>>>(b->d_next, NCLASS);
+        const hipStream_t s1 = (hipStream_t)s;
+        zero_counters_kernel<<<1, 64, 0, s1>>>(b->d_next, NCLASS);
         after_launch("zero_counters_kernel");
-        for (int c = NCLASS - 1; c >= 0; --c) {          // largest jobs first
+        // classes 3 and 2 (the largest jobs first) on the caller's stream, classes 1 and 0 on the
+        // second stream: the smaller jobs fill the CUs that the large classes' tails leave idle
+        HX_HIP(hipEventRecord(b->ev_fork, s1));
+        HX_HIP(hipStreamWaitEvent(b->s2, b->ev_fork, 0));
+        for (int c = NCLASS - 1; c >= 0; --c) {
             const HClass& cl = b->cls[c];
             if (!cl.njobs) continue;
+            const hipStream_t sc = c >= 2 ? s1 : b->s2;
             HArgs a{b->d_jobs, cl.d_order, cl.njobs, b->d_next + c, b->k};
             if (c == NCLASS - 1) {
-                howell_batch_kernel<<<cl.grid, HT, 0, (hipStream_t)s>>>(a);
+                howell_batch_kernel<<<cl.grid, HT, 0, sc>>>(a);
                 after_launch("howell_batch_kernel");
             } else {
                 if (b->k <= 24)
-                    howell_blocked_kernel<true><<<cl.grid, HT, cl.lds, (hipStream_t)s>>>(a, cl.rc);
+                    howell_blocked_kernel<true><<<cl.grid, HT, cl.lds, sc>>>(a, cl.rc);
                 else
-                    howell_blocked_kernel<false><<<cl.grid, HT, cl.lds, (hipStream_t)s>>>(a, cl.rc);
+                    howell_blocked_kernel<false><<<cl.grid, HT, cl.lds, sc>>>(a, cl.rc);
                 after_launch("howell_blocked_kernel");
             }
         }
+        HX_HIP(hipEventRecord(b->ev_join, b->s2));
+        HX_HIP(hipStreamWaitEvent(s1, b->ev_join, 0));
     });
 }
 

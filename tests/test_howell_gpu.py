@@ -83,3 +83,33 @@ def test_howell_row_classes_and_panel_edges(hx_gpu, O, k):
             assert np.array_equal(res.pivots, piv), f"rep {rep} k={k} {A.shape}: pivots"
             assert np.array_equal(res.pivot_v, pv), f"rep {rep} k={k} {A.shape}: pivot_v"
             assert np.array_equal(res.H, H), f"rep {rep} k={k} {A.shape}: H"
+
+
+def test_howell_batch_graph_replay(hx_gpu, O):
+    """A batch spanning every job class (the large classes run on the caller's stream, the small
+    ones on the batch's second stream, forked and joined by events) captured in one hipGraph on a
+    side stream and replayed three times; every replay equals the oracle."""
+    import ctypes as C
+    from hacs_amd import engine as E
+    from hacs_amd import _lib
+    k = 16
+    rng = np.random.default_rng(4242)
+    cases = [rng.integers(0, 1 << k, (m, n), dtype=np.uint64).astype(np.uint32)
+             for m, n in [(10, 12), (100, 90), (200, 250), (400, 500), (700, 400), (30, 31)]]
+    want = [O.howell_fast(k, A) for A in cases]
+    mats = [E.to_device(A) for A in cases]
+    b = E.HowellBatch(k, mats)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        h = s.cuda_stream
+        g = C.c_void_p()
+        _lib.check(_lib.lib().hx_graph_begin(h))
+        b.run(stream=h)
+        _lib.check(_lib.lib().hx_graph_end(h, C.byref(g)))
+        for rep in range(3):
+            _lib.check(_lib.lib().hx_graph_launch(g, h))
+            s.synchronize()
+            for A, (H, piv, pv), res in zip(cases, want, b.results()):
+                assert np.array_equal(res.pivots, piv) and np.array_equal(res.pivot_v, pv), (rep, A.shape)
+                assert np.array_equal(res.H, H), (rep, A.shape)
+        _lib.check(_lib.lib().hx_graph_destroy(g))
