@@ -171,8 +171,11 @@ constexpr uint32_t HW = 16;            // panel width / reduce-above block
 constexpr uint32_t kBlockedRows = 1024;
 constexpr uint32_t NONE = 0xffffffffu;
 
+// The panel region holds max(rc, HT) rows: during the trailing sweep it holds the panel's pivot
+// rows over HT columns instead (the panel entries are dead by then).
+__host__ __device__ constexpr uint32_t pan_rows(uint32_t rc) { return rc > HT ? rc : HT; }
 __host__ __device__ constexpr size_t blocked_lds_bytes(uint32_t rc) {
-    return (size_t)rc * HW * 4 * 2 + (size_t)rc * 4 * 2 + rc;
+    return (size_t)pan_rows(rc) * HW * 4 + (size_t)rc * HW * 4 + (size_t)rc * 4 * 2 + rc;
 }
 
 // Multiplication of the pass mod 2^32 (only the value mod 2^k matters).  For k <= 24 every
@@ -189,7 +192,6 @@ __device__ void howell_blocked(const HJob& job, uint32_t k, uint32_t rc, uint32_
     __shared__ uint32_t s_red[2][HT / 64];
     __shared__ uint32_t s_pid[HW], s_ui[HW], s_v[HW], s_c[HW];
     __shared__ uint32_t s_hb[HW * HW];
-    __shared__ uint32_t s_T[HW * HT];          // the panel's pivot rows over 256 trailing columns
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t m = job.m, n = job.n;
     const uint32_t mask = k >= 32 ? 0xffffffffu : ((1u << k) - 1u);
@@ -198,7 +200,9 @@ __device__ void howell_blocked(const HJob& job, uint32_t k, uint32_t rc, uint32_
         return;
     }
     uint32_t* pan = lds;                       // [rc][HW] panel entries of active rows
-    uint32_t* qm = pan + (size_t)rc * HW;      // [rc][HW] multipliers (later: reduce-above q's)
+    uint32_t* s_T = lds;                       // [HW][HT] the panel's pivot rows over HT trailing columns
+                                               // (phase B only, when the panel entries are dead)
+    uint32_t* qm = pan + (size_t)pan_rows(rc) * HW;   // [rc][HW] multipliers (later: reduce-above q's)
     uint32_t* act = qm + (size_t)rc * HW;      // active order: row ids, NONE once chosen
     uint32_t* prow = act + rc;                 // pivot row ids in pivot order
     uint8_t* nz = (uint8_t*)(prow + rc);       // row has a non-zero multiplier in this panel
@@ -637,14 +641,15 @@ HX_API int hx_howell_batch_run(hx_howell_batch vb, hx_stream s) {
         const hipStream_t s1 = (hipStream_t)s;
         zero_counters_kernel<<<1, 64, 0, s1>>>(b->d_next, NCLASS);
         after_launch("zero_counters_kernel");
-        // classes 3 and 2 (the largest jobs first) on the caller's stream, classes 1 and 0 on the
-        // second stream: the smaller jobs fill the CUs that the large classes' tails leave idle
+        // class 1 (m + n <= 512) on the second stream beside classes 3 and 2 (the largest jobs
+        // first) on the caller's stream, then class 0 there: the two heavy blocked classes share
+        // the CUs and the small jobs follow the one that ends first
         HX_HIP(hipEventRecord(b->ev_fork, s1));
         HX_HIP(hipStreamWaitEvent(b->s2, b->ev_fork, 0));
         for (int c = NCLASS - 1; c >= 0; --c) {
             const HClass& cl = b->cls[c];
             if (!cl.njobs) continue;
-            const hipStream_t sc = c >= 2 ? s1 : b->s2;
+            const hipStream_t sc = c == 1 ? b->s2 : s1;
             HArgs a{b->d_jobs, cl.d_order, cl.njobs, b->d_next + c, b->k};
             if (c == NCLASS - 1) {
                 howell_batch_kernel<<<cl.grid, HT, 0, sc>>>(a);
