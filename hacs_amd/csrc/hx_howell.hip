@@ -171,11 +171,15 @@ constexpr uint32_t HW = 16;            // panel width / reduce-above block
 constexpr uint32_t kBlockedRows = 1024;
 constexpr uint32_t NONE = 0xffffffffu;
 
-// The panel region holds max(rc, HT) rows: during the trailing sweep it holds the panel's pivot
-// rows over HT columns instead (the panel entries are dead by then).
-__host__ __device__ constexpr uint32_t pan_rows(uint32_t rc) { return rc > HT ? rc : HT; }
-__host__ __device__ constexpr size_t blocked_lds_bytes(uint32_t rc) {
-    return (size_t)pan_rows(rc) * HW * 4 + (size_t)rc * HW * 4 + (size_t)rc * 4 * 2 + rc;
+// Panel entries and multipliers are stored in elements of `eb` bytes: 2 when k <= 16 (every stored
+// value is reduced mod 2^k), else 4; the halved store lets the largest job class (m + n <= 1024)
+// run two workgroups per CU.  The panel region holds at least HW x HT words: during the trailing
+// sweep it holds the panel's pivot rows over HT columns instead (the panel entries are dead then).
+__host__ __device__ constexpr uint32_t pan_rows(uint32_t rc, uint32_t eb) {
+    return rc * eb >= HT * 4u ? rc : HT * 4u / eb;
+}
+__host__ __device__ constexpr size_t blocked_lds_bytes(uint32_t rc, uint32_t eb) {
+    return (size_t)pan_rows(rc, eb) * HW * eb + (size_t)rc * HW * eb + (size_t)rc * 4 * 2 + rc;
 }
 
 // Multiplication of the pass mod 2^32 (only the value mod 2^k matters).  For k <= 24 every
@@ -187,7 +191,7 @@ __device__ __forceinline__ uint32_t kmul(uint32_t a, uint32_t b) {
     else return a * b;
 }
 
-template <bool SMALLK>
+template <bool SMALLK, class PT>
 __device__ void howell_blocked(const HJob& job, uint32_t k, uint32_t rc, uint32_t* lds) {
     __shared__ uint32_t s_red[2][HT / 64];
     __shared__ uint32_t s_pid[HW], s_ui[HW], s_v[HW], s_c[HW];
@@ -199,11 +203,11 @@ __device__ void howell_blocked(const HJob& job, uint32_t k, uint32_t rc, uint32_
         if (tid == 0) *job.h = 0;
         return;
     }
-    uint32_t* pan = lds;                       // [rc][HW] panel entries of active rows
+    PT* pan = (PT*)lds;                        // [rc][HW] panel entries of active rows
     uint32_t* s_T = lds;                       // [HW][HT] the panel's pivot rows over HT trailing columns
                                                // (phase B only, when the panel entries are dead)
-    uint32_t* qm = pan + (size_t)pan_rows(rc) * HW;   // [rc][HW] multipliers (later: reduce-above q's)
-    uint32_t* act = qm + (size_t)rc * HW;      // active order: row ids, NONE once chosen
+    PT* qm = pan + (size_t)pan_rows(rc, sizeof(PT)) * HW;   // [rc][HW] multipliers (later: reduce-above q's)
+    uint32_t* act = (uint32_t*)(qm + (size_t)rc * HW);     // active order: row ids, NONE once chosen
     uint32_t* prow = act + rc;                 // pivot row ids in pivot order
     uint8_t* nz = (uint8_t*)(prow + rc);       // row has a non-zero multiplier in this panel
     uint32_t* rows = job.work;                 // row id i at rows + i * n
@@ -253,7 +257,7 @@ __device__ void howell_blocked(const HJob& job, uint32_t k, uint32_t rc, uint32_
             best = min(min(s_red[t & 1][0], s_red[t & 1][1]), min(s_red[t & 1][2], s_red[t & 1][3]));
             if (best == NONE) continue;
             const uint32_t bpos = best & 0xffffffu, v = best >> 24, pid = act[bpos];
-            const uint32_t* P = pan + pid * HW;                // unscaled: P[t] = u 2^v
+            const PT* P = pan + pid * HW;                      // unscaled: P[t] = u 2^v
             const uint32_t ui = inv_odd((P[t] & mask) >> v) & mask;
             if (bpos % HT == tid) chosen |= 1u << (bpos / HT);
             if (tid == 0) {
@@ -266,7 +270,7 @@ __device__ void howell_blocked(const HJob& job, uint32_t k, uint32_t rc, uint32_
             for (uint32_t pos = tid, j = 0; pos < na; pos += HT, ++j) {
                 const uint32_t id = act[pos];
                 if (id == NONE || (chosen >> j & 1u)) continue;
-                uint32_t* R = pan + id * HW;
+                PT* R = pan + id * HW;
                 const uint32_t q = (R[t] & mask) >> v;
                 if (q == 0) continue;
                 qm[id * HW + t] = q;                           // w.r.t. the scaled pivot row
@@ -332,7 +336,7 @@ __device__ void howell_blocked(const HJob& job, uint32_t k, uint32_t rc, uint32_
                     const uint32_t pid = s_pid[t];
                     if (pid == NONE) { T[t] = 0; continue; }
                     uint32_t x = T[t];
-                    const uint32_t* qr = qm + pid * HW;
+                    const PT* qr = qm + pid * HW;
 #pragma unroll
                     for (uint32_t s2 = 0; s2 < t; ++s2) x -= kmul<SMALLK>(qr[s2], T[s2]);
                     T[t] = kmul<SMALLK>(x & mask, s_ui[t] & mask) & mask;
@@ -376,7 +380,7 @@ __device__ void howell_blocked(const HJob& job, uint32_t k, uint32_t rc, uint32_
 #pragma unroll
                 for (int r = 0; r < 8; ++r) {
                     if (ids[r] == NONE) continue;
-                    const uint32_t* qr = qm + ids[r] * HW;
+                    const PT* qr = qm + ids[r] * HW;
                     uint32_t q[HW];
 #pragma unroll
                     for (uint32_t t = 0; t < HW; ++t) q[t] = qr[t];
@@ -464,7 +468,7 @@ __device__ void howell_blocked(const HJob& job, uint32_t k, uint32_t rc, uint32_
 #pragma unroll
                 for (uint32_t u = 0; u < 8; ++u) {
                     if (ip0 + u >= i1) continue;
-                    const uint32_t* qr = qm + (ip0 + u) * HW;
+                    const PT* qr = qm + (ip0 + u) * HW;
                     uint32_t x = xs[u];
 #pragma unroll
                     for (uint32_t t = 0; t < HW; ++t) x -= kmul<SMALLK>(qr[t], Hr[t]);
@@ -481,7 +485,7 @@ __device__ void howell_blocked(const HJob& job, uint32_t k, uint32_t rc, uint32_
 }
 
 // Persistent: one workgroup per job of this row-capacity class, jobs from the class's counter.
-template <bool SMALLK>
+template <bool SMALLK, class PT>
 __global__ __launch_bounds__(HT) void howell_blocked_kernel(HArgs a, uint32_t rc) {
     extern __shared__ __attribute__((aligned(16))) uint32_t hlds[];
     __shared__ uint32_t s_job;
@@ -491,9 +495,17 @@ __global__ __launch_bounds__(HT) void howell_blocked_kernel(HArgs a, uint32_t rc
         const uint32_t jj = s_job;
         __syncthreads();
         if (jj >= a.njobs) break;
-        howell_blocked<SMALLK>(a.jobs[a.order[jj]], a.k, rc, hlds);
+        howell_blocked<SMALLK, PT>(a.jobs[a.order[jj]], a.k, rc, hlds);
     }
 }
+
+// The blocked kernel for this k: 16-bit stores for k <= 16, the 24-bit multiply for k <= 24.
+inline const void* blocked_kernel_for(uint32_t k) {
+    if (k <= 16) return (const void*)howell_blocked_kernel<true, uint16_t>;
+    if (k <= 24) return (const void*)howell_blocked_kernel<true, uint32_t>;
+    return (const void*)howell_blocked_kernel<false, uint32_t>;
+}
+inline uint32_t blocked_elt_bytes(uint32_t k) { return k <= 16 ? 2u : 4u; }
 
 __global__ __launch_bounds__(HT) void howell_batch_kernel(HArgs a) {
     __shared__ uint32_t s_job;
@@ -595,10 +607,9 @@ HX_API int hx_howell_batch_create(uint32_t k, const hx_howell_job* jobs, uint32_
             if (hipGetDeviceProperties(&prop, 0) == hipSuccess) cus = prop.multiProcessorCount;
             static std::once_flag once;
             std::call_once(once, [] {
-                (void)hipFuncSetAttribute((const void*)howell_blocked_kernel<true>,
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)blocked_lds_bytes(kBlockedRows));
-                (void)hipFuncSetAttribute((const void*)howell_blocked_kernel<false>,
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)blocked_lds_bytes(kBlockedRows));
+                for (uint32_t kk : {16u, 24u, 32u})
+                    (void)hipFuncSetAttribute(blocked_kernel_for(kk), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                              (int)blocked_lds_bytes(kBlockedRows, blocked_elt_bytes(kk)));
             });
             for (int c = 0; c < NCLASS; ++c) {
                 HClass& cl = b->cls[c];
@@ -611,9 +622,9 @@ HX_API int hx_howell_batch_create(uint32_t k, const hx_howell_job* jobs, uint32_
                     uint32_t rc = 0;
                     for (uint32_t i : per[c]) rc = std::max(rc, hj[i].m + hj[i].n);
                     cl.rc = rc;
-                    cl.lds = blocked_lds_bytes(rc);
+                    cl.lds = blocked_lds_bytes(rc, blocked_elt_bytes(k));
                     int nb = 0;
-                    const auto kern = k <= 24 ? howell_blocked_kernel<true> : howell_blocked_kernel<false>;
+                    const void* kern = blocked_kernel_for(k);
                     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, HT, cl.lds) == hipSuccess &&
                         nb > 0)
                         per_cu = nb;
@@ -655,10 +666,12 @@ HX_API int hx_howell_batch_run(hx_howell_batch vb, hx_stream s) {
                 howell_batch_kernel<<<cl.grid, HT, 0, sc>>>(a);
                 after_launch("howell_batch_kernel");
             } else {
-                if (b->k <= 24)
-                    howell_blocked_kernel<true><<<cl.grid, HT, cl.lds, sc>>>(a, cl.rc);
+                if (b->k <= 16)
+                    howell_blocked_kernel<true, uint16_t><<<cl.grid, HT, cl.lds, sc>>>(a, cl.rc);
+                else if (b->k <= 24)
+                    howell_blocked_kernel<true, uint32_t><<<cl.grid, HT, cl.lds, sc>>>(a, cl.rc);
                 else
-                    howell_blocked_kernel<false><<<cl.grid, HT, cl.lds, sc>>>(a, cl.rc);
+                    howell_blocked_kernel<false, uint32_t><<<cl.grid, HT, cl.lds, sc>>>(a, cl.rc);
                 after_launch("howell_blocked_kernel");
             }
         }
