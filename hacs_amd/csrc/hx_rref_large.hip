@@ -123,6 +123,20 @@ __device__ uint32_t lds_rule(uint32_t* pan, uint32_t rows, uint32_t wc, uint32_t
     return k;
 }
 
+// The rows u < rows with flag[u] set, ascending: emit(position, u) for each, positions 0, 1, ...
+// A ballot per wave and the waves' counts in LDS (one barrier), instead of a one-thread loop.
+template <class F>
+__device__ __forceinline__ void block_compact(const uint8_t* flag, uint32_t rows, uint32_t* s_wcnt, F emit) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const bool f = tid < rows && flag[tid] != 0;
+    const uint64_t bal = __ballot(f);
+    if (lane == 0) s_wcnt[wave] = (uint32_t)__popcll(bal);
+    __syncthreads();
+    uint32_t off = 0;
+    for (uint32_t w = 0; w < wave; ++w) off += s_wcnt[w];
+    if (f) emit(off + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull)), tid);
+}
+
 // Level 0: chunk `blockIdx.x` of S's rows -> its candidate rows (ascending S indices).
 template <class Ar>
 __global__ __launch_bounds__(RT) void rrp_chunk_kernel(const uint32_t* S, int64_t ld, const LState* st, uint32_t wc,
@@ -142,12 +156,9 @@ __global__ __launch_bounds__(RT) void rrp_chunk_kernel(const uint32_t* S, int64_
     }
     static_assert(CH * 4 <= RT, "four threads per chunk row");
     const uint32_t k = lds_rule<4>(pan, rows, wc, ws, chosen, ar, s_u, s_q);
-    if (threadIdx.x == 0) {
-        uint32_t t = 0;
-        for (uint32_t u = 0; u < rows; ++u)
-            if (chosen[u]) cand[blockIdx.x * LW + t++] = r0 + u;
-        cnt[blockIdx.x] = k;
-    }
+    __shared__ uint32_t s_wcnt[RT / 64];
+    block_compact(chosen, rows, s_wcnt, [&](uint32_t pos, uint32_t u) { cand[blockIdx.x * LW + pos] = r0 + u; });
+    if (threadIdx.x == 0) cnt[blockIdx.x] = k;
 }
 
 // Merge: group `blockIdx.x` of up to GRP candidate lists -> one list; the final level writes P, J, k.
@@ -161,10 +172,11 @@ __global__ __launch_bounds__(RT) void rrp_merge_kernel(const uint32_t* S, int64_
     __shared__ uint8_t chosen[GRP * LW];
     __shared__ uint32_t s_u[LW], s_q[LW], s_off[GRP + 1];
     const uint32_t g0 = blockIdx.x * GRP, ng = min((uint32_t)GRP, nin - g0);
+    if (threadIdx.x < ng) s_off[threadIdx.x + 1] = cnt_in[g0 + threadIdx.x];   // the list lengths, in parallel
+    __syncthreads();
     if (threadIdx.x == 0) {
-        uint32_t acc = 0;
-        for (uint32_t i = 0; i < ng; ++i) { s_off[i] = acc; acc += cnt_in[g0 + i]; }
-        s_off[ng] = acc;
+        s_off[0] = 0;
+        for (uint32_t i = 0; i < ng; ++i) s_off[i + 1] += s_off[i];
     }
     __syncthreads();
     const uint32_t rows = s_off[ng], ws = wc | 1u;
@@ -178,16 +190,16 @@ __global__ __launch_bounds__(RT) void rrp_merge_kernel(const uint32_t* S, int64_
     }
     static_assert(GRP * LW * 2 <= RT, "two threads per candidate row");
     const uint32_t k = lds_rule<2>(pan, rows, wc, ws, chosen, ar, s_u, s_q);
-    if (threadIdx.x == 0) {
-        uint32_t t = 0;
-        for (uint32_t u = 0; u < rows; ++u)
-            if (chosen[u]) cand_out[blockIdx.x * LW + t++] = rowid[u];
-        cnt_out[blockIdx.x] = k;
-        if (final_level) {
-            for (uint32_t i = 0; i < k; ++i) {
-                P[i] = rowid[s_u[i]];
-                J[i] = s_q[i];
-            }
+    __shared__ uint32_t s_wcnt[RT / 64];
+    block_compact(chosen, rows, s_wcnt,
+                  [&](uint32_t pos, uint32_t u) { cand_out[blockIdx.x * LW + pos] = rowid[u]; });
+    if (threadIdx.x == 0) cnt_out[blockIdx.x] = k;
+    if (final_level) {
+        if (threadIdx.x < k) {
+            P[threadIdx.x] = rowid[s_u[threadIdx.x]];
+            J[threadIdx.x] = s_q[threadIdx.x];
+        }
+        if (threadIdx.x == 0) {
             st->k = k;
             st->m_new = st->m_a - k;
         }
