@@ -319,7 +319,7 @@ template <class Ar, int L>
 __device__ void rref_one(const RJob& job, const Ar& ar, const RArgs& a, uint32_t* sm) {
     __shared__ uint32_t s_cand[2][RT / 64];
     __shared__ uint32_t s_P[RW], s_q[RW], s_inv[RW];
-    __shared__ uint32_t s_chunk[RT];
+    __shared__ uint32_t s_keep[RT / 64];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t m = job.m, n = job.n, p = ar.p;
     const int64_t lda = job.lda;
@@ -404,26 +404,28 @@ __device__ void rref_one(const RJob& job, const Ar& ar, const RArgs& a, uint32_t
             job.pivot_rows[r + tid] = s_P[tid];
         }
         __syncthreads();
-        // stable compaction of the unused list (chosen rows leave)
+        // stable compaction of the unused list (chosen rows leave), 256 positions per round: a
+        // ballot per wave gives each kept row its place (writes only ever move left, past rows
+        // this round has already read)
         {
-            const uint32_t per = (mu + RT - 1) / RT, s0 = min(tid * per, mu), s1 = min(s0 + per, mu);
-            uint32_t keep = 0;
-            for (uint32_t u = s0; u < s1; ++u) keep += chosen[u] ? 0u : 1u;
-            s_chunk[tid] = keep;
-            __syncthreads();
-            if (tid == 0) {
-                uint32_t acc = 0;
-                for (int t = 0; t < RT; ++t) { const uint32_t x = s_chunk[t]; s_chunk[t] = acc; acc += x; }
+            uint32_t base = 0;
+            for (uint32_t u0 = 0; u0 < mu; u0 += RT) {
+                const uint32_t u = u0 + tid;
+                const bool keep = u < mu && !chosen[u];
+                const uint32_t val = keep ? list[u] : 0u;
+                const uint64_t bal = __ballot(keep);
+                if (lane == 0) s_keep[wave] = (uint32_t)__popcll(bal);
+                __syncthreads();
+                uint32_t off = base, tot = 0;
+#pragma unroll
+                for (uint32_t w = 0; w < RT / 64; ++w) {
+                    off += w < wave ? s_keep[w] : 0u;
+                    tot += s_keep[w];
+                }
+                if (keep) list[off + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = val;
+                base += tot;
+                __syncthreads();
             }
-            __syncthreads();
-            uint32_t vals[16];
-            uint32_t nv = 0;
-            for (uint32_t u = s0; u < s1; ++u)
-                if (!chosen[u]) vals[nv++] = list[u];
-            __syncthreads();
-            const uint32_t base = s_chunk[tid];
-            for (uint32_t t = 0; t < nv; ++t) list[base + t] = vals[t];
-            __syncthreads();
         }
         mu -= k;
         r += k;
