@@ -216,22 +216,51 @@ __device__ void pivot_solve(const RJob& job, const RArgs& a, const Ar& ar, uint3
                             uint32_t* s_inv) {
     constexpr int ND = 2 * L - 1, GS = 64;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, lr = lane & 31, lg = lane >> 5;
-    const uint32_t p = ar.p, k2 = 2 * k;
+    const uint32_t p = ar.p;
     uint32_t* G0 = pan;
     uint32_t* G1 = pan + 32 * GS;
-    for (uint32_t idx = tid; idx < k * k2; idx += RT) {
-        const uint32_t i = idx / k2, j = idx % k2;
-        G0[i * GS + j] = j < k ? job.A[(int64_t)s_P[i] * job.lda + c + s_q[j]] : (j - k == i ? 1u : 0u);
+    {   // K's k^2 <= 4 x 256 entries gathered with all loads in flight, then [K | I] stored
+        const uint32_t kk = k * k;
+        uint32_t v[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const uint32_t idx = e * RT + tid, i = idx / k, j = idx - i * k;
+            v[e] = idx < kk ? job.A[(int64_t)s_P[i] * job.lda + c + s_q[j]] : 0u;
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const uint32_t idx = e * RT + tid, i = idx / k, j = idx - i * k;
+            if (idx < kk) {
+                G0[i * GS + j] = v[e];
+                G0[i * GS + k + j] = i == j ? 1u : 0u;
+            }
+        }
     }
     __syncthreads();
+    // Step t touches only the entries it can change.  Before step t a row i holds explicitly: the
+    // left columns [t, k), the right columns [0, t), its own diagonal entry (i, i) when i < t and
+    // its own identity entry (i, k + i) when i >= t; every other entry is zero (left columns < t
+    // were eliminated, right columns > t are still identity columns).  A zero stays zero under
+    // row_i <- v row_i - f row_t (row t is zero there too), so step t writes k + 1 entries per row:
+    // left (t, k), right [0, t] (right column t read as zero in rows i != t) and the own entry
+    // (row t is zero at both own positions of a row i != t).  Eight threads share a row.
+    const uint32_t gi = tid >> 3, gsub = tid & 7;
     for (uint32_t t = 0; t < k; ++t) {
         const uint32_t* src = (t & 1) ? G1 : G0;
         uint32_t* dst = (t & 1) ? G0 : G1;
-        const uint32_t v = src[t * GS + t];
-        for (uint32_t idx = tid; idx < k * k2; idx += RT) {
-            const uint32_t i = idx / k2, j = idx % k2;
-            const uint32_t x = src[i * GS + j], f = src[i * GS + t];
-            dst[i * GS + j] = (i == t || f == 0) ? x : ar.dfree(v, x, p - f, src[t * GS + j]);
+        if (gi < k) {
+            const uint32_t* srow = src + gi * GS;
+            const uint32_t* trow = src + t * GS;
+            uint32_t* drow = dst + gi * GS;
+            const uint32_t v = trow[t], f = srow[t], fneg = p - f, nl = k - t - 1;
+            const bool upd = gi != t && f != 0;
+            for (uint32_t sl = gsub; sl <= k; sl += 8) {
+                const bool own = sl == k;
+                const uint32_t col = sl < nl ? t + 1 + sl : (!own ? k + (sl - nl) : (gi <= t ? gi : k + gi));
+                const uint32_t x = (col == k + t && gi != t) ? 0u : srow[col];
+                const uint32_t y = own ? 0u : trow[col];
+                drow[col] = upd ? ar.dfree(v, x, fneg, y) : x;
+            }
         }
         __syncthreads();
     }
@@ -339,10 +368,10 @@ __device__ void rref_one(const RJob& job, const Ar& ar, const RArgs& a, uint32_t
     for (uint32_t c = 0; c < n && mu > 0; c += w) {
         const uint32_t wc = min(w, n - c), ws = wc | 1u, nb = n - c;
         // (a) panel of the unused rows, pivot search by the smallest-row rule
-        for (uint32_t i0 = 0; i0 < mu * wc; i0 += 4 * RT) {      // four loads per thread in flight
-            uint32_t v[4], at[4];
+        for (uint32_t i0 = 0; i0 < mu * wc; i0 += 8 * RT) {      // eight loads per thread in flight
+            uint32_t v[8], at[8];
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
+            for (int e = 0; e < 8; ++e) {
                 const uint32_t idx = i0 + e * RT + tid;
                 if (idx < mu * wc) {
                     const uint32_t u = wc == RW ? idx / RW : idx / wc, q = idx - u * wc;
@@ -351,7 +380,7 @@ __device__ void rref_one(const RJob& job, const Ar& ar, const RArgs& a, uint32_t
                 }
             }
 #pragma unroll
-            for (int e = 0; e < 4; ++e)
+            for (int e = 0; e < 8; ++e)
                 if (i0 + e * RT + tid < mu * wc) pan[at[e]] = v[e];
         }
         for (uint32_t u = tid; u < mu; u += RT) chosen[u] = 0;
@@ -395,10 +424,8 @@ __device__ void rref_one(const RJob& job, const Ar& ar, const RArgs& a, uint32_t
         // (d) trailing update of the other unused rows, S[i, j] -= sum_t S[i, J_t] X[t, j], and the
         // reduction of the stored pivot rows 0 .. r-1 against this panel's pivots (MFMA)
         trailing_mfma<L>(job, a, B, planes, nb, wc, k, c, mu, list, chosen, s_q, r);
-        // (e) this panel's pivot rows (X, already in U rows r .. r+k-1 from c on): zeros left of c;
-        // pivots, pivot rows
-        for (uint32_t t = wave; t < k; t += RT / 64)
-            for (uint32_t j = lane; j < c; j += 64) job.U[(size_t)(r + t) * n + j] = 0u;
+        // (e) this panel's pivot rows are X, already in U rows r .. r+k-1 from column c on (the
+        // final copy writes their zeros left of c); pivots, pivot rows
         if (tid < k) {
             job.pivots[r + tid] = c + s_q[tid];
             job.pivot_rows[r + tid] = s_P[tid];
@@ -427,18 +454,23 @@ __device__ void rref_one(const RJob& job, const Ar& ar, const RArgs& a, uint32_t
                 __syncthreads();
             }
         }
+        // U rows r .. r+k-1 start at column c: kept in the list's free tail (positions mu-k .. mu-1,
+        // past the compacted list) as list[m - 1 - row]
+        if (tid < k) list[m - 1 - (r + tid)] = c;
         mu -= k;
         r += k;
     }
+    __syncthreads();
 
     // the RREF over the job's matrix: rank rows of U, then zero rows
     for (uint32_t i = wave; i < m; i += RT / 64) {          // a wave per row, four words per lane in flight
         uint32_t* Ai = job.A + (int64_t)i * lda;
         const uint32_t* Ui = job.U + (size_t)i * n;
+        const uint32_t c0 = i < r ? list[m - 1 - i] : n;     // zeros left of the row's panel
         for (uint32_t j0 = lane; j0 < n; j0 += 256) {
             uint32_t v[4];
 #pragma unroll
-            for (int e = 0; e < 4; ++e) v[e] = (i < r && j0 + 64 * e < n) ? Ui[j0 + 64 * e] : 0u;
+            for (int e = 0; e < 4; ++e) v[e] = (j0 + 64 * e >= c0 && j0 + 64 * e < n) ? Ui[j0 + 64 * e] : 0u;
 #pragma unroll
             for (int e = 0; e < 4; ++e)
                 if (j0 + 64 * e < n) Ai[j0 + 64 * e] = v[e];
