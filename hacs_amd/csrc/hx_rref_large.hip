@@ -53,6 +53,7 @@ struct LState {               // device-side counters
     int status;
     uint32_t snap_k[2], snap_m[2];   // the sizes of panel i's trailing update, slot i & 1
     uint32_t snap_r[2];              // the pivot rows stored before panel i (its Gauss-Jordan rows)
+    uint32_t spec;                   // 1: this panel's pivots came from its first SPR rows alone
 };
 
 __device__ __forceinline__ uint32_t wmin(uint32_t x) {
@@ -137,6 +138,39 @@ __device__ __forceinline__ void block_compact(const uint8_t* flag, uint32_t rows
     if (f) emit(off + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull)), tid);
 }
 
+// The rule on the first SPR rows of S alone.  If it finds a pivot in every panel column, those
+// are the panel's pivots over all rows: by induction over the columns, the earlier pivots agree,
+// so every row is eliminated alike, and column q's pivot, the smallest row with a non-zero entry,
+// lies among the first SPR rows, below every later row.  Then P, J and k are written and the
+// tournament kernels return at once (st->spec = 1); otherwise the tournament decides (st->spec = 0).
+constexpr int SPR = 64;
+template <class Ar>
+__global__ __launch_bounds__(RT) void rrp_spec_kernel(const uint32_t* S, int64_t ld, LState* st, uint32_t wc, Ar ar,
+                                                      uint32_t* P, uint32_t* J) {
+    __shared__ uint32_t pan[SPR * (LW + 1)];
+    __shared__ uint8_t chosen[SPR];
+    __shared__ uint32_t s_u[LW], s_q[LW];
+    const uint32_t ma = st->m_a, rows = min((uint32_t)SPR, ma), ws = wc | 1u;
+    for (uint32_t idx = threadIdx.x; idx < rows * wc; idx += RT) {
+        const uint32_t u = idx / wc, q = idx % wc;
+        pan[u * ws + q] = S[(int64_t)u * ld + q];
+    }
+    static_assert(SPR * 16 == RT, "sixteen threads per row");
+    const uint32_t k = lds_rule<16>(pan, rows, wc, ws, chosen, ar, s_u, s_q);
+    const bool ok = k == wc;
+    if (ok && threadIdx.x < k) {
+        P[threadIdx.x] = s_u[threadIdx.x];
+        J[threadIdx.x] = s_q[threadIdx.x];
+    }
+    if (threadIdx.x == 0) {
+        st->spec = ok ? 1u : 0u;
+        if (ok) {
+            st->k = k;
+            st->m_new = ma - k;
+        }
+    }
+}
+
 // Level 0: chunk `blockIdx.x` of S's rows -> its candidate rows (ascending S indices).
 template <class Ar>
 __global__ __launch_bounds__(RT) void rrp_chunk_kernel(const uint32_t* S, int64_t ld, const LState* st, uint32_t wc,
@@ -144,6 +178,7 @@ __global__ __launch_bounds__(RT) void rrp_chunk_kernel(const uint32_t* S, int64_
     __shared__ uint32_t pan[CH * (LW + 1)];
     __shared__ uint8_t chosen[CH];
     __shared__ uint32_t s_u[LW], s_q[LW];
+    if (st->spec) return;                              // the first rows settled the panel
     const uint32_t ma = st->m_a, r0 = blockIdx.x * CH;
     if (r0 >= ma) {
         if (threadIdx.x == 0) cnt[blockIdx.x] = 0;
@@ -171,6 +206,7 @@ __global__ __launch_bounds__(RT) void rrp_merge_kernel(const uint32_t* S, int64_
     __shared__ uint32_t rowid[GRP * LW];
     __shared__ uint8_t chosen[GRP * LW];
     __shared__ uint32_t s_u[LW], s_q[LW], s_off[GRP + 1];
+    if (st->spec) return;                              // the first rows settled the panel
     const uint32_t g0 = blockIdx.x * GRP, ng = min((uint32_t)GRP, nin - g0);
     if (threadIdx.x < ng) s_off[threadIdx.x + 1] = cnt_in[g0 + threadIdx.x];   // the list lengths, in parallel
     __syncthreads();
@@ -329,7 +365,7 @@ __global__ void advance_kernel(LState* st) {
 
 __global__ void init_state_kernel(LState* st, uint32_t m, uint32_t* rows_orig, uint32_t* U, size_t ucount) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) { st->m_a = m; st->r = 0; st->k = 0; st->m_new = m; st->status = 0; }
+    if (i == 0) { st->m_a = m; st->r = 0; st->k = 0; st->m_new = m; st->status = 0; st->spec = 0; }
     if (i < m) rows_orig[i] = (uint32_t)i;
     for (size_t t = i; t < ucount; t += (size_t)gridDim.x * blockDim.x) U[t] = 0;
 }
@@ -426,7 +462,10 @@ void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const
             uint32_t* P = Pb + slot * LW;
             uint32_t* J = Jb + slot * LW;
             uint32_t* X = Xb + (size_t)slot * LW * n;
-            // 1. tournament (reads only this panel's LW columns, already up to date)
+            // 1. tournament (reads only this panel's LW columns, already up to date), after the
+            //    search over the first SPR rows, which settles most panels alone
+            rrp_spec_kernel<Ar><<<1, RT, 0, s>>>(S, n, st, wc, ar, P, J);
+            after_launch("rrp_spec_kernel");
             rrp_chunk_kernel<Ar><<<nchunk, RT, 0, s>>>(S, n, st, wc, ar, cand[0], cnt[0]);
             after_launch("rrp_chunk_kernel");
             uint32_t nin = nchunk;

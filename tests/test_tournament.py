@@ -4,7 +4,9 @@ levels, repeated until one list is left), re-running the smallest-row rule on th
 at every merge, must give the same pivot rows, in the same pivot order, as the rule on the whole
 panel (the oracle's hxo_rref_rule).  The claim holds for any merge tree; a pairwise tree over 64-row
 leaves is checked as well.  Planted ranks, zero rows and a negative control where a chunk drops its
-candidates."""
+candidates.  The speculative first step (the rule on the first 64 rows alone, accepted only when it
+finds a pivot in every panel column) must give the rule's pivots too, with the tournament deciding
+otherwise."""
 import numpy as np
 import pytest
 
@@ -79,3 +81,41 @@ def test_tree_tournament_negative_control(O):
     want = list(zip(prow.tolist(), piv.tolist()))
     assert _tournament(A) == want
     assert _tournament(A, drop_leaf=1) != want
+
+
+def _speculative(A, spr=64, accept_partial=False):
+    """rrp_spec_kernel, then the tournament when it does not settle the panel: (pivot order, settled)."""
+    rows = list(range(min(spr, A.shape[0])))
+    order = _rule(rows, A)[1]
+    if len(order) == A.shape[1] or (accept_partial and order):
+        return order, True
+    return _tournament(A), False
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_speculative_first_rows_equal_rule(O, seed):
+    rng = np.random.default_rng(1000 + seed)
+    m, w = int(rng.integers(1, 900)), 32
+    full = seed % 2 == 0
+    rank = 32 if full else int(rng.integers(0, 33))
+    A = ((rng.integers(0, P, (m, rank)) @ rng.integers(0, P, (rank, w))) % P).astype(np.uint32)
+    if seed % 4 == 1:
+        A[: min(m, 64), int(rng.integers(0, w))] = 0       # a column whose pivot lies below row 64
+    if seed % 4 == 2 and m > 10:
+        A[1:10] = A[0]                                     # dependent rows among the first 64
+    r, piv, prow, _ = O.rref_rule(P, A, want_rref=False)
+    want = list(zip(prow.tolist(), piv.tolist()))
+    assert _speculative(A)[0] == want
+
+
+def test_speculative_settles_and_defers():
+    """The shortcut settles a dense full-rank panel and defers a panel whose column is zero in the
+    first 64 rows; accepting its partial result (a plausible mistake) changes the pivots."""
+    rng = np.random.default_rng(5)
+    A = rng.integers(0, P, (400, 32)).astype(np.uint32)
+    assert _speculative(A)[1]
+    B = A.copy()
+    B[:64, 7] = 0
+    order, settled = _speculative(B)
+    assert not settled
+    assert _speculative(B, accept_partial=True)[0] != order
