@@ -15,9 +15,9 @@
 //     rows (two threads per row); the last level writes the panel's pivot rows P (pivot order),
 //     columns J and count k.
 //  2. X = S[P, J]^{-1} S[P, :]: every workgroup owns a slab of 16 columns and performs the same
-//     row reduction on [S[P, J] | S[P, slab]] (division-free forward, one inverse per pivot,
-//     back substitution), writing X[:, slab] into the U store (row r + t, original columns) and
-//     into a contiguous copy for the next step.  X has the identity at J.
+//     row reduction on [S[P, J] | S[P, slab]] (division-free Gauss-Jordan, one barrier per pivot,
+//     then one inverse per pivot), writing X[:, slab] into the U store (row r + t, original
+//     columns) and into a contiguous copy for the next step.  X has the identity at J.
 //  3. The rows R not in P, in order, form the next Schur complement:
 //        S'[i, :] = S[R_i, W:] - S[R_i, J] X[:, W:]
 //     as GEMMs with a row gather on A and C_in, a column gather on A, and output into the other S
@@ -253,7 +253,7 @@ __global__ __launch_bounds__(PXT) void panel_x_kernel(const uint32_t* S, int64_t
                                                      uint32_t* U, int64_t ldu, uint32_t c, const uint32_t* rows_orig,
                                                      uint32_t* pivots, uint32_t* pivot_rows) {
     __shared__ uint32_t B[LW * (LW + SL)];
-    __shared__ uint32_t s_inv[LW], s_fp[LW], s_q[LW];
+    __shared__ uint32_t s_inv[LW], s_q[LW];
     const uint32_t k = st->k;
     if (k == 0) return;
     const uint32_t j0 = blockIdx.x * SL;
@@ -267,16 +267,36 @@ __global__ __launch_bounds__(PXT) void panel_x_kernel(const uint32_t* S, int64_t
     }
     if (threadIdx.x < k) s_q[threadIdx.x] = J[threadIdx.x];
     __syncthreads();
-    // forward (division-free) in pivot order: column t of the block is pivot column J[t]
-    for (uint32_t t = 0; t + 1 < k; ++t) {
-        const uint32_t wdt = bw - t - 1, cnt = (k - t - 1) * wdt;
-        const uint32_t v = B[t * bw + t];
-        for (uint32_t it = threadIdx.x; it < cnt; it += PXT) {
-            const uint32_t row = t + 1 + it / wdt, j = t + 1 + it % wdt;
-            const uint32_t f = B[row * bw + t];
-            if (f != 0) B[row * bw + j] = ar.dfree(v, B[row * bw + j], p - f, B[t * bw + j]);
+    // Division-free Gauss-Jordan in pivot order (column t of the block is pivot column J[t]):
+    // row_i <- v row_i - f row_t for every row i != t with f = B[i, t] != 0.  Every leading block of
+    // the pivot block is non-singular in this order, so the left block ends diagonal and the slab
+    // part of row i is d_i (S[P, J]^-1 S[P, slab])_i.  Step t writes, in the rows i != t, only the
+    // left columns past t, the slab and the diagonal entry (i, i) when i < t (row t is zero at
+    // column i there); the left columns below t are zero and are neither written nor read again, so
+    // no entry a step writes is read by another thread in that step: one buffer, one barrier per
+    // step.  Sixteen threads share a row.
+    {
+        const uint32_t gi = threadIdx.x >> 4, gsub = threadIdx.x & 15;
+        for (uint32_t t = 0; t < k; ++t) {
+            if (gi < k && gi != t) {
+                uint32_t* row = B + gi * bw;
+                const uint32_t* trow = B + t * bw;
+                const uint32_t f = row[t];
+                if (f != 0) {
+                    const uint32_t v = trow[t], fneg = p - f, nl = k - t - 1, ns = nl + sw;
+                    const uint32_t nslot = ns + (gi < t ? 1u : 0u);
+                    for (uint32_t sl = gsub; sl < nslot; sl += 16) {
+                        if (sl < ns) {
+                            const uint32_t col = sl < nl ? t + 1 + sl : k + (sl - nl);
+                            row[col] = ar.dfree(v, row[col], fneg, trow[col]);
+                        } else {
+                            row[gi] = ar.dfree(v, row[gi], fneg, 0u);
+                        }
+                    }
+                }
+            }
+            __syncthreads();
         }
-        __syncthreads();
     }
     if (threadIdx.x < k) {
         const uint32_t iv = invm(B[threadIdx.x * bw + threadIdx.x], p);
@@ -284,27 +304,11 @@ __global__ __launch_bounds__(PXT) void panel_x_kernel(const uint32_t* S, int64_t
         s_inv[threadIdx.x] = ar.prep(iv);
     }
     __syncthreads();
-    for (uint32_t idx = threadIdx.x; idx < k * bw; idx += PXT) {
-        const uint32_t t = idx / bw, j = idx % bw;
-        if (j > t) B[idx] = ar.mul(B[idx], s_inv[t]);
-    }
-    __syncthreads();
-    for (uint32_t t = k - 1; t >= 1; --t) {       // back substitution (block columns > t)
-        const uint32_t wdt = bw - t - 1, cnt = t * wdt;
-        if (threadIdx.x < t) s_fp[threadIdx.x] = ar.prep(B[threadIdx.x * bw + t]);
-        __syncthreads();
-        for (uint32_t it = threadIdx.x; it < cnt; it += PXT) {
-            const uint32_t row = it / wdt, j = t + 1 + it % wdt;
-            if (j < k) continue;                   // the pivot block becomes the identity below
-            const uint32_t fp = s_fp[row];
-            if (fp != 0) B[row * bw + j] = subm(B[row * bw + j], ar.mul(B[t * bw + j], fp), p);
-        }
-        __syncthreads();
-    }
-    // X[:, slab]; the slab's own pivot columns get the identity exactly (they equal it already)
+    // X[:, slab] = d^-1 (slab part); the slab's own pivot columns get the identity exactly (they
+    // equal it already)
     for (uint32_t idx = threadIdx.x; idx < k * sw; idx += PXT) {
         const uint32_t t = idx / sw, j = idx % sw;
-        uint32_t x = B[t * bw + k + j];
+        uint32_t x = ar.mul(B[t * bw + k + j], s_inv[t]);
         for (uint32_t t2 = 0; t2 < k; ++t2)
             if (s_q[t2] == j0 + j) x = (t2 == t) ? 1u : 0u;
         X[(int64_t)t * ldx + j0 + j] = x;
