@@ -38,6 +38,7 @@
 
 #include "hx_elim.h"
 #include "hx_gemm.h"
+#include "hx_limbs.h"
 
 namespace hx {
 
@@ -242,6 +243,111 @@ __global__ __launch_bounds__(RT) void rrp_merge_kernel(const uint32_t* S, int64_
     }
 }
 
+// The k <= 32 products of a panel on the INT8 matrix cores: Cout[i, j] = Cin[c_rows[i], j] -
+// sum_{q<k} A[a_rows[i], a_cols[q]] X[q, j] mod p.  A's entries are split into L balanced 8-bit
+// limbs on the fly; X's limb planes come from panel_x (plane s, column j: 32 k-bytes); the L^2
+// limb products of the one 32-deep k-step accumulate per anti-diagonal in int32 and the exact
+// recombination (hx_limbs.h) gives the residue.  One wave per 32-row block, four 32-column blocks
+// in turn; a block's Cin entries are loaded before its MFMAs.  Every output is written, also when
+// k = 0 (Cout = Cin: the trailing update doubles as the compaction into the next Schur complement).
+// No barriers: a wave past the rows returns at once.
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+constexpr int UCB = 4;        // 32-column blocks per wave
+struct UpdArgs {
+    uint32_t m, n;                           // row bound (m_dev caps it) and columns
+    const uint32_t* m_dev;
+    const uint32_t* k_dev;                   // k <= 32
+    const uint32_t* A;
+    int64_t ars;
+    const uint32_t* a_rows;                  // optional row gather
+    const uint32_t* a_cols;                  // optional column gather
+    const uint8_t* planes;                   // X's limb planes
+    uint32_t pstride, pcol0;                 // bytes per plane; first plane column
+    const uint32_t* Cin;
+    int64_t cirs;
+    const uint32_t* c_rows;                  // optional row gather on Cin
+    uint32_t* Cout;
+    int64_t cors;
+    DevMod M;
+    LimbConsts lc;
+    uint32_t half;
+};
+
+template <int L>
+__global__ __launch_bounds__(256) void upd32_kernel(UpdArgs u) {
+    constexpr int ND = 2 * L - 1;
+    constexpr uint32_t NOROW = 0xffffffffu;
+    const uint32_t m = min(*u.m_dev, u.m), k = min(*u.k_dev, 32u);
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, lr = lane & 31, lg = lane >> 5;
+    const uint32_t ncg = (u.n + 32 * UCB - 1) / (32 * UCB);
+    const uint32_t b0 = ((blockIdx.x / ncg) * 4 + wave) * 32, cg = blockIdx.x % ncg;
+    if (b0 >= m) return;                     // (k = 0 still copies Cin to Cout: the compaction)
+    const uint32_t p = u.M.p;
+    // A fragment: row b0 + lr, k-bytes 16 lg .. 16 lg + 15, all loads first
+    const uint32_t arow = b0 + lr;
+    const bool alive = arow < m;
+    const int64_t aoff = alive ? (u.a_rows ? (int64_t)u.a_rows[arow] : (int64_t)arow) * u.ars : 0;
+    uint32_t av[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const uint32_t t = 16 * lg + i;
+        av[i] = (alive && t < k) ? u.A[aoff + (u.a_cols ? (int64_t)u.a_cols[t] : (int64_t)t)] : 0u;
+    }
+    v4i fa[L];
+#pragma unroll
+    for (int s2 = 0; s2 < L; ++s2) fa[s2] = v4i{0, 0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        int32_t d[4] = {0, 0, 0, 0};
+        limbs_of(av[i], p, u.half, L, d);
+#pragma unroll
+        for (int s2 = 0; s2 < L; ++s2) fa[s2][i >> 2] |= (int)(((uint32_t)d[s2] & 0xffu) << (8 * (i & 3)));
+    }
+    // the 16 output rows of this lane (accumulator element e) and their Cin rows
+    uint32_t orow[16], crow[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        const uint32_t r = b0 + (e / 4) * 8 + lg * 4 + (e % 4);
+        orow[e] = r < m ? r : NOROW;
+        crow[e] = r < m ? (u.c_rows ? u.c_rows[r] : r) : 0u;
+    }
+    for (uint32_t cb = 0; cb < UCB; ++cb) {
+        const uint32_t jb = (cg * UCB + cb) * 32;
+        if (jb >= u.n) break;
+        const uint32_t j = jb + lr;
+        const bool jok = j < u.n;
+        uint32_t cv[16];
+#pragma unroll
+        for (int e = 0; e < 16; ++e)
+            cv[e] = (jok && orow[e] != NOROW) ? u.Cin[(int64_t)crow[e] * u.cirs + j] : 0u;
+        v4i fb[L];
+#pragma unroll
+        for (int s2 = 0; s2 < L; ++s2)
+            fb[s2] = jok ? *(const v4i*)(u.planes + (size_t)s2 * u.pstride + (size_t)(u.pcol0 + j) * 32 + 16 * lg)
+                         : v4i{0, 0, 0, 0};
+        v16i acc[ND];
+#pragma unroll
+        for (int d = 0; d < ND; ++d)
+#pragma unroll
+            for (int e = 0; e < 16; ++e) acc[d][e] = 0;
+#pragma unroll
+        for (int s2 = 0; s2 < L; ++s2)
+#pragma unroll
+            for (int t = 0; t < L; ++t)
+                acc[s2 + t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[s2], fb[t], acc[s2 + t], 0, 0, 0);
+        if (!jok) continue;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            if (orow[e] == NOROW) continue;
+            int32_t D[ND];
+#pragma unroll
+            for (int d = 0; d < ND; ++d) D[d] = acc[d][e];
+            u.Cout[(int64_t)orow[e] * u.cors + j] = subm(cv[e], limb_recombine<ND>(D, u.lc, u.M), p);
+        }
+    }
+}
+
 // X[:, slab] = S[P, J]^{-1} S[P, slab]; also the pivots / pivot rows of this panel (block 0).
 // Every workgroup reduces the same k x k pivot block beside its own SL-column slab: narrow slabs
 // spread the (multiply-latency-bound) elimination over many CUs.
@@ -251,7 +357,8 @@ template <class Ar>
 __global__ __launch_bounds__(PXT) void panel_x_kernel(const uint32_t* S, int64_t ld, uint32_t na, const uint32_t* P,
                                                      const uint32_t* J, LState* st, Ar ar, uint32_t* X, int64_t ldx,
                                                      uint32_t* U, int64_t ldu, uint32_t c, const uint32_t* rows_orig,
-                                                     uint32_t* pivots, uint32_t* pivot_rows) {
+                                                     uint32_t* pivots, uint32_t* pivot_rows, uint8_t* Xp,
+                                                     uint32_t pstride, uint32_t L, uint32_t half) {
     __shared__ uint32_t B[LW * (LW + SL)];
     __shared__ uint32_t s_inv[LW], s_q[LW];
     const uint32_t k = st->k;
@@ -313,6 +420,24 @@ __global__ __launch_bounds__(PXT) void panel_x_kernel(const uint32_t* S, int64_t
             if (s_q[t2] == j0 + j) x = (t2 == t) ? 1u : 0u;
         X[(int64_t)t * ldx + j0 + j] = x;
         U[(int64_t)(r + t) * ldu + c + j0 + j] = x;
+        B[t * bw + k + j] = x;
+    }
+    __syncthreads();
+    // X's limb planes for the matrix-core products: plane s, column j0 + j holds the 32 k-bytes
+    // (limb s of X[t, j0 + j], t = 0..31; zero past k), four k-bytes per store
+    for (uint32_t idx = threadIdx.x; idx < 8 * sw; idx += PXT) {
+        const uint32_t j = idx % sw, t4 = idx / sw;
+        uint32_t w[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const uint32_t t = 4 * t4 + e;
+            int32_t d[4] = {0, 0, 0, 0};
+            limbs_of(t < k ? B[t * bw + k + j] : 0u, p, half, (int)L, d);
+#pragma unroll
+            for (int s2 = 0; s2 < 4; ++s2) w[s2] |= ((uint32_t)d[s2] & 0xffu) << (8 * e);
+        }
+        for (uint32_t s2 = 0; s2 < L; ++s2)
+            *(uint32_t*)(Xp + (size_t)s2 * pstride + (size_t)(j0 + j) * 32 + 4 * t4) = w[s2];
     }
     if (blockIdx.x == 0 && threadIdx.x < k) {
         pivots[r + threadIdx.x] = c + s_q[threadIdx.x];
@@ -423,6 +548,7 @@ void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const
     add(mn);                      // 15 pivot rows
     add(16);                      // 16 state
     add((size_t)mn * LW);         // 17 Gauss-Jordan left operand T
+    add((size_t)2 * 32 * n);      // 18 X's limb planes (two panels: slot i & 1; 4 planes x 32 x n bytes each)
     size_t total = 0;
     std::vector<size_t> off;
     for (size_t w : sizes) { off.push_back(total); total += w; }
@@ -433,6 +559,24 @@ void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const
     uint32_t* U = at(2);
     uint32_t* Xb = at(3);
     uint32_t* T = at(17);
+    uint8_t* Xpb = (uint8_t*)at(18);
+    const uint32_t pstride = 32 * n;          // bytes per plane
+    // the limb split of the k <= 32 products (balanced digits of residues mod p)
+    const uint32_t L = ctx.p <= 255u ? 1 : (ctx.p <= 65279u ? 2 : (ctx.p <= 16711423u ? 3 : 4));
+    const uint32_t half = (ctx.p - 1) / 2;
+    const LimbConsts lc = limb_consts(ctx.p);
+    const DevMod dm = devmod(ctx);
+    auto upd = [&](const UpdArgs& u, hipStream_t st_) {
+        const uint32_t grid = (u.m + 127) / 128 * ((u.n + 32 * UCB - 1) / (32 * UCB));
+        if (grid == 0) return;
+        switch (L) {
+            case 1: upd32_kernel<1><<<grid, 256, 0, st_>>>(u); break;
+            case 2: upd32_kernel<2><<<grid, 256, 0, st_>>>(u); break;
+            case 3: upd32_kernel<3><<<grid, 256, 0, st_>>>(u); break;
+            default: upd32_kernel<4><<<grid, 256, 0, st_>>>(u); break;
+        }
+        after_launch("upd32_kernel");
+    };
     uint32_t* ro[2] = {at(4), at(5)};
     uint32_t *Rl = at(6), *flags = at(7);
     uint32_t* cand[2] = {at(8), at(9)};
@@ -453,12 +597,6 @@ void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const
         init_state_kernel<<<gsz(std::max<size_t>(m, (size_t)mn * n)), 256, 0, s>>>(st, m, ro[0], U, (size_t)mn * n);
         after_launch("init_state_kernel");
         int cur = 0;
-        // the trailing updates are k = 32 products: the automatic choice (the plain kernel)
-        const int tvar = gemm_auto_variant(ctx, m, n, LW);
-        const size_t ws_bytes = tvar == HX_GEMM_LIMB ? limb_workspace_bytes(ctx, m, LW, LW) : 0;
-        const size_t ws2_bytes = tvar == HX_GEMM_LIMB ? limb_workspace_bytes(ctx, m, n, LW) : 0;
-        void* ws = ws_bytes ? scratch(ws_bytes, 2) : nullptr;
-        void* ws2 = ws2_bytes ? scratch(ws2_bytes, 3) : nullptr;
         for (uint32_t c = 0, panel = 0; c < n; c += LW, ++panel) {
             const uint32_t na = n - c, wc = std::min<uint32_t>(LW, na);
             const uint32_t* S = Sb[cur];
@@ -466,6 +604,7 @@ void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const
             uint32_t* P = Pb + slot * LW;
             uint32_t* J = Jb + slot * LW;
             uint32_t* X = Xb + (size_t)slot * LW * n;
+            uint8_t* Xp = Xpb + (size_t)slot * 4 * pstride;
             // 1. tournament (reads only this panel's LW columns, already up to date), after the
             //    search over the first SPR rows, which settles most panels alone
             rrp_spec_kernel<Ar><<<1, RT, 0, s>>>(S, n, st, wc, ar, P, J);
@@ -489,7 +628,8 @@ void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const
                 rest_pending = false;
             }
             // 2. X = S[P,J]^{-1} S[P,:] into the U store and the contiguous copy
-            panel_x_kernel<Ar><<<(na + SL - 1) / SL, PXT, 0, s>>>(S, n, na, P, J, st, ar, X, n, U, n, c, ro[cur], piv, prow);
+            panel_x_kernel<Ar><<<(na + SL - 1) / SL, PXT, 0, s>>>(S, n, na, P, J, st, ar, X, n, U, n, c, ro[cur], piv, prow,
+                                                                  Xp, pstride, L, half);
             after_launch("panel_x_kernel");
             // 3. the next Schur complement, compacted: first the next panel's columns, then the rest
             if (na > wc) {
@@ -502,35 +642,34 @@ void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const
             HX_HIP(hipStreamWaitEvent(s2, ev_next, 0));
             if (na > wc) {
                 const uint32_t nt = na - wc, n1 = std::min<uint32_t>(LW, nt);
-                GemmArgs g;
-                g.m = m;
-                g.k = LW;
-                g.A = S;
-                g.ars = n;
-                g.acs = 1;
-                g.a_rows = Rl;
-                g.a_cols = J;
-                g.brs = n;
-                g.bcs = 1;
-                g.cirs = n;
-                g.cics = 1;
-                g.c_rows = Rl;
-                g.cors = n;
-                g.cocs = 1;
-                g.m_dev = &st->snap_m[slot];
-                g.k_dev = &st->snap_k[slot];
-                g.op = HX_OP_SUB;
-                g.n = n1;
-                g.B = X + wc;
-                g.Cin = S + wc;
-                g.Cout = Sb[cur ^ 1];
-                gemm_dispatch(ctx, g, s, tvar, ws, ws_bytes, 0);
+                // S'[i, j] = S[R_i, wc + j] - sum_t S[R_i, J_t] X[t, wc + j] on the matrix cores
+                UpdArgs u;
+                u.m = m;
+                u.m_dev = &st->snap_m[slot];
+                u.k_dev = &st->snap_k[slot];
+                u.A = S;
+                u.ars = n;
+                u.a_rows = Rl;
+                u.a_cols = J;
+                u.planes = Xp;
+                u.pstride = pstride;
+                u.Cin = S + wc;
+                u.cirs = n;
+                u.c_rows = Rl;
+                u.cors = n;
+                u.M = dm;
+                u.lc = lc;
+                u.half = half;
+                u.n = n1;
+                u.pcol0 = wc;
+                u.Cout = Sb[cur ^ 1];
+                upd(u, s);
                 if (nt > n1) {
-                    g.n = nt - n1;
-                    g.B = X + wc + n1;
-                    g.Cin = S + wc + n1;
-                    g.Cout = Sb[cur ^ 1] + n1;
-                    gemm_dispatch(ctx, g, s2, tvar, ws2, ws2_bytes, 0);
+                    u.n = nt - n1;
+                    u.pcol0 = wc + n1;
+                    u.Cin = S + wc + n1;
+                    u.Cout = Sb[cur ^ 1] + n1;
+                    upd(u, s2);
                 }
                 cur ^= 1;
             }
@@ -544,26 +683,28 @@ void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const
             HX_HIP(hipEventRecord(ev_rest, s2));
             rest_pending = true;
             if (rmax > 0) {
-                GemmArgs g;
-                g.m = rmax;
-                g.k = LW;
-                g.n = na;
-                g.A = T;
-                g.ars = LW;
-                g.acs = 1;
-                g.B = X;
-                g.brs = n;
-                g.bcs = 1;
-                g.Cin = U + c;
-                g.cirs = n;
-                g.cics = 1;
-                g.Cout = U + c;
-                g.cors = n;
-                g.cocs = 1;
-                g.m_dev = &st->snap_r[slot];
-                g.k_dev = &st->snap_k[slot];
-                g.op = HX_OP_SUB;
-                gemm_dispatch(ctx, g, s2, HX_GEMM_PLAIN, nullptr, 0, 0);
+                // U[0:r, c + j] -= sum_t T[., t] X[t, j] on the matrix cores (in place)
+                UpdArgs u;
+                u.m = rmax;
+                u.n = na;
+                u.m_dev = &st->snap_r[slot];
+                u.k_dev = &st->snap_k[slot];
+                u.A = T;
+                u.ars = LW;
+                u.a_rows = nullptr;
+                u.a_cols = nullptr;
+                u.planes = Xp;
+                u.pstride = pstride;
+                u.pcol0 = 0;
+                u.Cin = U + c;
+                u.cirs = n;
+                u.c_rows = nullptr;
+                u.Cout = U + c;
+                u.cors = n;
+                u.M = dm;
+                u.lc = lc;
+                u.half = half;
+                upd(u, s2);
             }
             advance_kernel<<<1, 64, 0, s>>>(st);
             after_launch("advance_kernel");
