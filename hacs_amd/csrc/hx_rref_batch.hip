@@ -191,12 +191,6 @@ __device__ __forceinline__ void elim_row(uint32_t* row, const uint32_t* prow, ui
     for (; q2 < wc; ++q2) row[q2] = ar.dfree(v, row[q2], fneg, prow[q2]);
 }
 
-__device__ __forceinline__ uint32_t wave_min(uint32_t x) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) x = min(x, (uint32_t)__shfl_xor((int)x, off));
-    return x;
-}
-
 // (c) X = S[P, J]^{-1} S[P, c:n] for the panel's k pivot rows P (pivot columns J = c + s_q).
 //   c1: [K | I] with K = S[P, J] into LDS (two 32 x 64 buffers at the start of the panel region,
 //       whose panel copy is dead by now);
@@ -387,10 +381,17 @@ __device__ void rref_one(const RJob& job, const Ar& ar, const RArgs& a, uint32_t
         __syncthreads();
         uint32_t k = 0;
         for (uint32_t q = 0; q < wc; ++q) {
+            // the wave's smallest candidate row: rounds of 256 rows in order, the lowest hit lane of
+            // the first round with a hit (a wave-uniform loop)
             uint32_t best = 0xffffffffu;
-            for (uint32_t u = tid; u < mu; u += RT)
-                if (!chosen[u] && pan[u * ws + q] != 0) { best = u; break; }
-            best = wave_min(best);
+            for (uint32_t u0 = 0; u0 < mu; u0 += RT) {
+                const uint32_t u = u0 + tid;
+                const uint64_t bal = __ballot(u < mu && !chosen[u] && pan[u * ws + q] != 0);
+                if (bal) {
+                    best = u0 + wave * 64 + (uint32_t)__builtin_ctzll(bal);
+                    break;
+                }
+            }
             if (lane == 0) s_cand[q & 1][wave] = best;
             __syncthreads();
             const uint32_t bu = min(min(s_cand[q & 1][0], s_cand[q & 1][1]), min(s_cand[q & 1][2], s_cand[q & 1][3]));

@@ -57,12 +57,6 @@ struct LState {               // device-side counters
     uint32_t spec;                   // 1: this panel's pivots came from its first SPR rows alone
 };
 
-__device__ __forceinline__ uint32_t wmin(uint32_t x) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) x = min(x, (uint32_t)__shfl_xor((int)x, off));
-    return x;
-}
-
 // The smallest-row rule on an LDS panel pan[u * ws + q] (rows in original order), division-free.
 // TPR threads per row: thread (u, part) owns the columns q == part (mod TPR) of row u, so a
 // column's elimination is spread over all TPR * rows threads.  The owner of column q scans it (it
@@ -83,9 +77,9 @@ __device__ uint32_t lds_rule(uint32_t* pan, uint32_t rows, uint32_t wc, uint32_t
     uint32_t k = 0;
     __syncthreads();                                   // the panel is in LDS
     for (uint32_t q = 0; q < wc; ++q) {
-        uint32_t best = 0xffffffffu;
-        if (mine && part == (q % TPR) && !done && row[q] != 0) best = u;
-        best = wmin(best);
+        // rows grow with the thread index, so the wave's smallest candidate is its lowest hit lane
+        const uint64_t bal = __ballot(mine && part == (q % TPR) && !done && row[q] != 0);
+        const uint32_t best = bal ? (wave * 64 + (uint32_t)__builtin_ctzll(bal)) / TPR : 0xffffffffu;
         if (lane == 0) s_cand[q & 1][wave] = best;
         __syncthreads();
         uint32_t bu = s_cand[q & 1][0];
@@ -145,13 +139,22 @@ __device__ __forceinline__ void block_compact(const uint8_t* flag, uint32_t rows
 // lies among the first SPR rows, below every later row.  Then P, J and k are written and the
 // tournament kernels return at once (st->spec = 1); otherwise the tournament decides (st->spec = 0).
 constexpr int SPR = 64;
+// It first moves the state past the previous panel (r += k, m_a = m_new, k = 0; a no-op before
+// the first panel).
 template <class Ar>
 __global__ __launch_bounds__(RT) void rrp_spec_kernel(const uint32_t* S, int64_t ld, LState* st, uint32_t wc, Ar ar,
                                                       uint32_t* P, uint32_t* J) {
     __shared__ uint32_t pan[SPR * (LW + 1)];
     __shared__ uint8_t chosen[SPR];
-    __shared__ uint32_t s_u[LW], s_q[LW];
-    const uint32_t ma = st->m_a, rows = min((uint32_t)SPR, ma), ws = wc | 1u;
+    __shared__ uint32_t s_u[LW], s_q[LW], s_ma;
+    if (threadIdx.x == 0) {
+        st->r += st->k;
+        st->m_a = st->m_new;
+        st->k = 0;
+        s_ma = st->m_new;
+    }
+    __syncthreads();
+    const uint32_t ma = s_ma, rows = min((uint32_t)SPR, ma), ws = wc | 1u;
     for (uint32_t idx = threadIdx.x; idx < rows * wc; idx += RT) {
         const uint32_t u = idx / wc, q = idx % wc;
         pan[u * ws + q] = S[(int64_t)u * ld + q];
@@ -358,9 +361,14 @@ __global__ __launch_bounds__(PXT) void panel_x_kernel(const uint32_t* S, int64_t
                                                      const uint32_t* J, LState* st, Ar ar, uint32_t* X, int64_t ldx,
                                                      uint32_t* U, int64_t ldu, uint32_t c, const uint32_t* rows_orig,
                                                      uint32_t* pivots, uint32_t* pivot_rows, uint8_t* Xp,
-                                                     uint32_t pstride, uint32_t L, uint32_t half) {
+                                                     uint32_t pstride, uint32_t L, uint32_t half, int slot) {
     __shared__ uint32_t B[LW * (LW + SL)];
     __shared__ uint32_t s_inv[LW], s_q[LW];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {        // the sizes of this panel's products, slot i & 1
+        st->snap_k[slot] = st->k;
+        st->snap_m[slot] = st->m_new;
+        st->snap_r[slot] = st->r;
+    }
     const uint32_t k = st->k;
     if (k == 0) return;
     const uint32_t j0 = blockIdx.x * SL;
@@ -476,14 +484,6 @@ __global__ __launch_bounds__(1024) void rest_rows_kernel(const LState* st, const
 
 // The trailing update of a panel may still run (second stream) while the next panel's
 // tournament rewrites k / m_new: it reads its own copy.
-__global__ void snapshot_kernel(LState* st, int slot) {
-    if (threadIdx.x == 0) {
-        st->snap_k[slot] = st->k;
-        st->snap_m[slot] = st->m_new;
-        st->snap_r[slot] = st->r;
-    }
-}
-
 __global__ void advance_kernel(LState* st) {
     if (threadIdx.x == 0) {
         st->r += st->k;
@@ -629,15 +629,13 @@ void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const
             }
             // 2. X = S[P,J]^{-1} S[P,:] into the U store and the contiguous copy
             panel_x_kernel<Ar><<<(na + SL - 1) / SL, PXT, 0, s>>>(S, n, na, P, J, st, ar, X, n, U, n, c, ro[cur], piv, prow,
-                                                                  Xp, pstride, L, half);
+                                                                  Xp, pstride, L, half, slot);
             after_launch("panel_x_kernel");
             // 3. the next Schur complement, compacted: first the next panel's columns, then the rest
             if (na > wc) {
                 rest_rows_kernel<<<1, 1024, 0, s>>>(st, P, flags, Rl, ro[cur], ro[cur ^ 1]);
                 after_launch("rest_rows_kernel");
             }
-            snapshot_kernel<<<1, 64, 0, s>>>(st, slot);
-            after_launch("snapshot_kernel");
             HX_HIP(hipEventRecord(ev_next, s));     // X, the row list and the sizes are ready
             HX_HIP(hipStreamWaitEvent(s2, ev_next, 0));
             if (na > wc) {
@@ -706,9 +704,9 @@ void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const
                 u.half = half;
                 upd(u, s2);
             }
-            advance_kernel<<<1, 64, 0, s>>>(st);
-            after_launch("advance_kernel");
         }
+        advance_kernel<<<1, 64, 0, s>>>(st);            // past the last panel (the spec kernel does it before the others)
+        after_launch("advance_kernel");
         // every product on s2 (the last Gauss-Jordan product included) before U is read
         HX_HIP(hipEventRecord(ev_rest, s2));
         HX_HIP(hipStreamWaitEvent(s, ev_rest, 0));
