@@ -133,6 +133,29 @@ __device__ __forceinline__ void block_compact(const uint8_t* flag, uint32_t rows
     if (f) emit(off + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull)), tid);
 }
 
+// R = the rows of S not among the panel's k pivot rows sP[0..k) (LDS, any order), in order, and
+// their original indices (the next Schur complement's rows_orig), by one workgroup of RT threads:
+// row i goes to position i - #{t : sP[t] < i}.  Called by whichever kernel settles the panel.
+__device__ void rest_rows_of(uint32_t ma, uint32_t k, const uint32_t* sP, uint32_t* R, const uint32_t* rows_orig,
+                             uint32_t* rows_orig_next) {
+    uint32_t pv[LW];
+#pragma unroll
+    for (int t = 0; t < LW; ++t) pv[t] = (uint32_t)t < k ? sP[t] : 0xffffffffu;
+    for (uint32_t i = threadIdx.x; i < ma; i += RT) {
+        uint32_t below = 0;
+        bool pivot = false;
+#pragma unroll
+        for (int t = 0; t < LW; ++t) {
+            below += pv[t] < i ? 1u : 0u;
+            pivot |= pv[t] == i;
+        }
+        if (!pivot) {
+            R[i - below] = i;
+            rows_orig_next[i - below] = rows_orig[i];
+        }
+    }
+}
+
 // The rule on the first SPR rows of S alone.  If it finds a pivot in every panel column, those
 // are the panel's pivots over all rows: by induction over the columns, the earlier pivots agree,
 // so every row is eliminated alike, and column q's pivot, the smallest row with a non-zero entry,
@@ -143,7 +166,8 @@ constexpr int SPR = 64;
 // the first panel).
 template <class Ar>
 __global__ __launch_bounds__(RT) void rrp_spec_kernel(const uint32_t* S, int64_t ld, LState* st, uint32_t wc, Ar ar,
-                                                      uint32_t* P, uint32_t* J) {
+                                                      uint32_t* P, uint32_t* J, uint32_t* R, const uint32_t* rows_orig,
+                                                      uint32_t* rows_orig_next) {
     __shared__ uint32_t pan[SPR * (LW + 1)];
     __shared__ uint8_t chosen[SPR];
     __shared__ uint32_t s_u[LW], s_q[LW], s_ma;
@@ -173,6 +197,7 @@ __global__ __launch_bounds__(RT) void rrp_spec_kernel(const uint32_t* S, int64_t
             st->m_new = ma - k;
         }
     }
+    if (ok) rest_rows_of(ma, k, s_u, R, rows_orig, rows_orig_next);
 }
 
 // Level 0: chunk `blockIdx.x` of S's rows -> its candidate rows (ascending S indices).
@@ -205,7 +230,8 @@ template <class Ar>
 __global__ __launch_bounds__(RT) void rrp_merge_kernel(const uint32_t* S, int64_t ld, uint32_t wc, Ar ar,
                                                        const uint32_t* cand_in, const uint32_t* cnt_in, uint32_t nin,
                                                        uint32_t* cand_out, uint32_t* cnt_out, int final_level,
-                                                       uint32_t* P, uint32_t* J, LState* st) {
+                                                       uint32_t* P, uint32_t* J, LState* st, uint32_t* R,
+                                                       const uint32_t* rows_orig, uint32_t* rows_orig_next) {
     __shared__ uint32_t pan[GRP * LW * (LW + 1)];
     __shared__ uint32_t rowid[GRP * LW];
     __shared__ uint8_t chosen[GRP * LW];
@@ -243,6 +269,11 @@ __global__ __launch_bounds__(RT) void rrp_merge_kernel(const uint32_t* S, int64_
             st->k = k;
             st->m_new = st->m_a - k;
         }
+        // the pivot rows as S indices, then R
+        __shared__ uint32_t s_P[LW];
+        if (threadIdx.x < k) s_P[threadIdx.x] = rowid[s_u[threadIdx.x]];
+        __syncthreads();
+        rest_rows_of(st->m_a, k, s_P, R, rows_orig, rows_orig_next);
     }
 }
 
@@ -453,35 +484,6 @@ __global__ __launch_bounds__(PXT) void panel_x_kernel(const uint32_t* S, int64_t
     }
 }
 
-// R = the rows of S not in P, in order; rows_orig of the next S; one workgroup.
-__global__ __launch_bounds__(1024) void rest_rows_kernel(const LState* st, const uint32_t* P, uint32_t* flags,
-                                                         uint32_t* R, const uint32_t* rows_orig, uint32_t* rows_orig_next) {
-    __shared__ uint32_t s_sum[1024];
-    const uint32_t ma = st->m_a, k = st->k, tid = threadIdx.x;
-    for (uint32_t i = tid; i < ma; i += 1024) flags[i] = 1;
-    __syncthreads();
-    for (uint32_t t = tid; t < k; t += 1024) flags[P[t]] = 0;
-    __syncthreads();
-    const uint32_t per = (ma + 1023) / 1024, s0 = min(tid * per, ma), s1 = min(s0 + per, ma);
-    uint32_t c = 0;
-    for (uint32_t i = s0; i < s1; ++i) c += flags[i];
-    s_sum[tid] = c;
-    __syncthreads();
-    for (uint32_t off = 1; off < 1024; off <<= 1) {      // inclusive scan (Hillis-Steele)
-        const uint32_t x = tid >= off ? s_sum[tid - off] : 0u;
-        __syncthreads();
-        s_sum[tid] += x;
-        __syncthreads();
-    }
-    uint32_t pos = s_sum[tid] - c;
-    for (uint32_t i = s0; i < s1; ++i)
-        if (flags[i]) {
-            R[pos] = i;
-            rows_orig_next[pos] = rows_orig[i];
-            ++pos;
-        }
-}
-
 // The trailing update of a panel may still run (second stream) while the next panel's
 // tournament rewrites k / m_new: it reads its own copy.
 __global__ void advance_kernel(LState* st) {
@@ -536,19 +538,18 @@ void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const
     add((size_t)2 * LW * n);      // 3 X copy (two panels: slot i & 1)
     add(m);                       // 4 rows_orig a
     add(m);                       // 5 rows_orig b
-    add(m);                       // 6 R list
-    add(m);                       // 7 flags
-    add((size_t)nchunk * LW);     // 8 cand a
-    add((size_t)nchunk * LW);     // 9 cand b
-    add(nchunk);                  // 10 cnt a
-    add(nchunk);                  // 11 cnt b
-    add(2 * LW);                  // 12 P (two panels: slot i & 1)
-    add(2 * LW);                  // 13 J
-    add(mn);                      // 14 pivots
-    add(mn);                      // 15 pivot rows
-    add(16);                      // 16 state
-    add((size_t)mn * LW);         // 17 Gauss-Jordan left operand T
-    add((size_t)2 * 32 * n);      // 18 X's limb planes (two panels: slot i & 1; 4 planes x 32 x n bytes each)
+    add((size_t)2 * m);           // 6 R list (two panels: slot i & 1)
+    add((size_t)nchunk * LW);     // 7 cand a
+    add((size_t)nchunk * LW);     // 8 cand b
+    add(nchunk);                  // 9 cnt a
+    add(nchunk);                  // 10 cnt b
+    add(2 * LW);                  // 11 P (two panels: slot i & 1)
+    add(2 * LW);                  // 12 J
+    add(mn);                      // 13 pivots
+    add(mn);                      // 14 pivot rows
+    add(16);                      // 15 state
+    add((size_t)mn * LW);         // 16 Gauss-Jordan left operand T
+    add((size_t)2 * 32 * n);      // 17 X's limb planes (two panels: slot i & 1; 4 planes x 32 x n bytes each)
     size_t total = 0;
     std::vector<size_t> off;
     for (size_t w : sizes) { off.push_back(total); total += w; }
@@ -558,8 +559,8 @@ void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const
     uint32_t* Sb[2] = {at(0), at(1)};
     uint32_t* U = at(2);
     uint32_t* Xb = at(3);
-    uint32_t* T = at(17);
-    uint8_t* Xpb = (uint8_t*)at(18);
+    uint32_t* T = at(16);
+    uint8_t* Xpb = (uint8_t*)at(17);
     const uint32_t pstride = 32 * n;          // bytes per plane
     // the limb split of the k <= 32 products (balanced digits of residues mod p)
     const uint32_t L = ctx.p <= 255u ? 1 : (ctx.p <= 65279u ? 2 : (ctx.p <= 16711423u ? 3 : 4));
@@ -578,11 +579,11 @@ void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const
         after_launch("upd32_kernel");
     };
     uint32_t* ro[2] = {at(4), at(5)};
-    uint32_t *Rl = at(6), *flags = at(7);
-    uint32_t* cand[2] = {at(8), at(9)};
-    uint32_t* cnt[2] = {at(10), at(11)};
-    uint32_t *Pb = at(12), *Jb = at(13), *piv = at(14), *prow = at(15);
-    LState* st = (LState*)at(16);
+    uint32_t* Rlb = at(6);
+    uint32_t* cand[2] = {at(7), at(8)};
+    uint32_t* cnt[2] = {at(9), at(10)};
+    uint32_t *Pb = at(11), *Jb = at(12), *piv = at(13), *prow = at(14);
+    LState* st = (LState*)at(15);
     // lookahead: the trailing update of panel i is split into the next panel's LW columns (on s)
     // and the rest (on s2, overlapping panel i+1's tournament), followed on s2 by the Gauss-Jordan
     // gather and product; panel i+1's X waits for s2 up to the gather (ev_rest).
@@ -607,7 +608,8 @@ void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const
             uint8_t* Xp = Xpb + (size_t)slot * 4 * pstride;
             // 1. tournament (reads only this panel's LW columns, already up to date), after the
             //    search over the first SPR rows, which settles most panels alone
-            rrp_spec_kernel<Ar><<<1, RT, 0, s>>>(S, n, st, wc, ar, P, J);
+            uint32_t* Rl = Rlb + (size_t)slot * m;
+            rrp_spec_kernel<Ar><<<1, RT, 0, s>>>(S, n, st, wc, ar, P, J, Rl, ro[cur], ro[cur ^ 1]);
             after_launch("rrp_spec_kernel");
             rrp_chunk_kernel<Ar><<<nchunk, RT, 0, s>>>(S, n, st, wc, ar, cand[0], cnt[0]);
             after_launch("rrp_chunk_kernel");
@@ -617,7 +619,8 @@ void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const
                 const uint32_t nout = (nin + GRP - 1) / GRP;
                 const int fin = nout == 1;
                 rrp_merge_kernel<Ar><<<nout, RT, 0, s>>>(S, n, wc, ar, cand[lv & 1], cnt[lv & 1], nin,
-                                                         cand[(lv + 1) & 1], cnt[(lv + 1) & 1], fin, P, J, st);
+                                                         cand[(lv + 1) & 1], cnt[(lv + 1) & 1], fin, P, J, st, Rl,
+                                                         ro[cur], ro[cur ^ 1]);
                 after_launch("rrp_merge_kernel");
                 nin = nout;
                 ++lv;
@@ -632,10 +635,6 @@ void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const
                                                                   Xp, pstride, L, half, slot);
             after_launch("panel_x_kernel");
             // 3. the next Schur complement, compacted: first the next panel's columns, then the rest
-            if (na > wc) {
-                rest_rows_kernel<<<1, 1024, 0, s>>>(st, P, flags, Rl, ro[cur], ro[cur ^ 1]);
-                after_launch("rest_rows_kernel");
-            }
             HX_HIP(hipEventRecord(ev_next, s));     // X, the row list and the sizes are ready
             HX_HIP(hipStreamWaitEvent(s2, ev_next, 0));
             if (na > wc) {
