@@ -16,8 +16,10 @@
 //     columns J and count k.
 //  2. X = S[P, J]^{-1} S[P, :]: every workgroup owns a slab of 16 columns and performs the same
 //     row reduction on [S[P, J] | S[P, slab]] (division-free Gauss-Jordan, one barrier per pivot,
-//     then one inverse per pivot), writing X[:, slab] into the U store (row r + t, original
-//     columns) and into a contiguous copy for the next step.  X has the identity at J.
+//     then one inverse per pivot) -- or, when the search over the first rows settled the panel,
+//     takes W = S[P, J]^{-1} from that search and computes X[:, slab] = W S[P, slab] -- writing
+//     X[:, slab] into the U store (row r + t, original columns), into a contiguous copy and into
+//     X's limb planes for the products of step 3.  X has the identity at J.
 //  3. The rows R not in P, in order, form the next Schur complement:
 //        S'[i, :] = S[R_i, W:] - S[R_i, J] X[:, W:]
 //     as GEMMs with a row gather on A and C_in, a column gather on A, and output into the other S
@@ -159,45 +161,112 @@ __device__ void rest_rows_of(uint32_t ma, uint32_t k, const uint32_t* sP, uint32
 // The rule on the first SPR rows of S alone.  If it finds a pivot in every panel column, those
 // are the panel's pivots over all rows: by induction over the columns, the earlier pivots agree,
 // so every row is eliminated alike, and column q's pivot, the smallest row with a non-zero entry,
-// lies among the first SPR rows, below every later row.  Then P, J and k are written and the
-// tournament kernels return at once (st->spec = 1); otherwise the tournament decides (st->spec = 0).
+// lies among the first SPR rows, below every later row.  Then P, J, k, the row list R and
+// W = S[P, J]^{-1} are written and the tournament kernels return at once (st->spec = 1); otherwise
+// the tournament decides (st->spec = 0).
+//
+// The elimination runs as division-free Gauss-Jordan (the chosen rows are reduced too, which
+// leaves the candidates and so the pivot choices unchanged) and every row carries its
+// coefficients: row_u = sigma_u orig_u + sum_t c_u[t] orig_{P_t} (only pivot rows are ever
+// subtracted).  A step row_u <- (v row_u - f row_bu) R^-1 maps sigma_u -> v sigma_u R^-1,
+// c_u[t'] -> (v c_u[t'] - f c_bu[t']) R^-1 and sets the new pivot's c_u[t] = -f sigma_bu R^-1.  At
+// the end row P_t is d_t e_t on the pivot columns, so M K = D with M[t][t'] = c_{P_t}[t'] (+ sigma
+// on the diagonal) and K^{-1} = D^{-1} M.  Row layout: panel columns 0..wc-1, sigma at wc, c[t] at
+// wc + 1 + t.  Step q writes every entry of the rows u != bu with f != 0 except column q (left
+// stale; the pivots so far are exactly columns 0..q-1, so a pivot row is read as zero there), so
+// no entry a step writes is read by another thread in that step; one barrier per step.  The
+// entries left of q matter only as a chosen row's own pivot entry d_t, scaled with its row.  It first moves the state past the previous panel (r += k,
+// m_a = m_new, k = 0; a no-op before the first panel).
 constexpr int SPR = 64;
-// It first moves the state past the previous panel (r += k, m_a = m_new, k = 0; a no-op before
-// the first panel).
+constexpr int SWS = 2 * LW + 1;   // spec row stride: 32 panel columns, sigma, 32 coefficients
 template <class Ar>
 __global__ __launch_bounds__(RT) void rrp_spec_kernel(const uint32_t* S, int64_t ld, LState* st, uint32_t wc, Ar ar,
                                                       uint32_t* P, uint32_t* J, uint32_t* R, const uint32_t* rows_orig,
-                                                      uint32_t* rows_orig_next) {
-    __shared__ uint32_t pan[SPR * (LW + 1)];
-    __shared__ uint8_t chosen[SPR];
-    __shared__ uint32_t s_u[LW], s_q[LW], s_ma;
-    if (threadIdx.x == 0) {
+                                                      uint32_t* rows_orig_next, uint32_t* Wg) {
+    __shared__ uint32_t pan[SPR * SWS];
+    __shared__ uint32_t s_cand[2][RT / 64];
+    __shared__ uint32_t s_u[LW], s_q[LW], s_dinv[LW], s_ma, s_ok;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, p = ar.p;
+    if (tid == 0) {
         st->r += st->k;
         st->m_a = st->m_new;
         st->k = 0;
         s_ma = st->m_new;
     }
     __syncthreads();
-    const uint32_t ma = s_ma, rows = min((uint32_t)SPR, ma), ws = wc | 1u;
-    for (uint32_t idx = threadIdx.x; idx < rows * wc; idx += RT) {
-        const uint32_t u = idx / wc, q = idx % wc;
-        pan[u * ws + q] = S[(int64_t)u * ld + q];
+    const uint32_t ma = s_ma, rows = min((uint32_t)SPR, ma), width = wc + 1 + LW;
+    for (uint32_t idx = tid; idx < rows * width; idx += RT) {
+        const uint32_t u = idx / width, c = idx % width;
+        pan[u * SWS + c] = c < wc ? S[(int64_t)u * ld + c] : (c == wc ? 1u : 0u);
     }
     static_assert(SPR * 16 == RT, "sixteen threads per row");
-    const uint32_t k = lds_rule<16>(pan, rows, wc, ws, chosen, ar, s_u, s_q);
-    const bool ok = k == wc;
-    if (ok && threadIdx.x < k) {
-        P[threadIdx.x] = s_u[threadIdx.x];
-        J[threadIdx.x] = s_q[threadIdx.x];
+    const uint32_t u = tid / 16, part = tid % 16;
+    const bool mine = u < rows;
+    uint32_t* row = pan + u * SWS;
+    bool done = false;
+    uint32_t k = 0;
+    __syncthreads();
+    for (uint32_t q = 0; q < wc; ++q) {
+        // rows grow with the thread index, so the wave's smallest candidate is its lowest hit lane
+        const uint64_t bal = __ballot(mine && part == (q % 16) && !done && row[q] != 0);
+        if (lane == 0) s_cand[q & 1][wave] = bal ? (wave * 64 + (uint32_t)__builtin_ctzll(bal)) / 16 : 0xffffffffu;
+        __syncthreads();
+        uint32_t bu = s_cand[q & 1][0];
+#pragma unroll
+        for (int w = 1; w < RT / 64; ++w) bu = min(bu, s_cand[q & 1][w]);
+        if (bu == 0xffffffffu) break;                  // a column without a pivot here: the tournament decides
+        const uint32_t* prow = pan + bu * SWS;
+        if (mine) {
+            if (u == bu) {
+                done = true;
+            } else {
+                const uint32_t f = row[q];
+                if (f != 0) {
+                    const uint32_t v = prow[q], fneg = p - f, sb = prow[wc], cnew = wc + 1 + k;
+                    for (uint32_t c = part; c <= cnew; c += 16) {
+                        if (c == q) continue;
+                        // the pivot row is zero at sigma's slot and at the earlier pivot columns c < q
+                        // (its entries there are stale, never zeroed)
+                        if (c == cnew)
+                            row[c] = ar.dfree(v, 0u, fneg, sb);
+                        else
+                            row[c] = ar.dfree(v, row[c], fneg, (c == wc || c < q) ? 0u : prow[c]);
+                    }
+                }
+            }
+        }
+        if (tid == 0) { s_u[k] = bu; s_q[k] = q; }
+        ++k;
     }
-    if (threadIdx.x == 0) {
+    const bool ok = k == wc;
+    __syncthreads();
+    if (ok && tid < k) {
+        P[tid] = s_u[tid];
+        J[tid] = s_q[tid];
+        const uint32_t iv = invm(pan[s_u[tid] * SWS + s_q[tid]], p);
+        if (iv == 0) atomicExch(&st->status, 1);
+        s_dinv[tid] = ar.prep(iv);
+    }
+    if (tid == 0) {
         st->spec = ok ? 1u : 0u;
         if (ok) {
             st->k = k;
             st->m_new = ma - k;
         }
+        s_ok = ok ? 1u : 0u;
     }
-    if (ok) rest_rows_of(ma, k, s_u, R, rows_orig, rows_orig_next);
+    __syncthreads();
+    if (s_ok) {
+        // W = D^{-1} M in Montgomery form (prep), row t, column t' (row stride LW)
+        for (uint32_t idx = tid; idx < k * k; idx += RT) {
+            const uint32_t t = idx / k, t2 = idx % k;
+            const uint32_t* pr = pan + s_u[t] * SWS;
+            uint32_t mval = pr[wc + 1 + t2];
+            if (t2 == t) mval = addm(mval, pr[wc], p);
+            Wg[t * LW + t2] = ar.prep(ar.mul(mval, s_dinv[t]));
+        }
+        rest_rows_of(ma, k, s_u, R, rows_orig, rows_orig_next);
+    }
 }
 
 // Level 0: chunk `blockIdx.x` of S's rows -> its candidate rows (ascending S indices).
@@ -392,7 +461,8 @@ __global__ __launch_bounds__(PXT) void panel_x_kernel(const uint32_t* S, int64_t
                                                      const uint32_t* J, LState* st, Ar ar, uint32_t* X, int64_t ldx,
                                                      uint32_t* U, int64_t ldu, uint32_t c, const uint32_t* rows_orig,
                                                      uint32_t* pivots, uint32_t* pivot_rows, uint8_t* Xp,
-                                                     uint32_t pstride, uint32_t L, uint32_t half, int slot) {
+                                                     uint32_t pstride, uint32_t L, uint32_t half, int slot,
+                                                     const uint32_t* Wg) {
     __shared__ uint32_t B[LW * (LW + SL)];
     __shared__ uint32_t s_inv[LW], s_q[LW];
     if (blockIdx.x == 0 && threadIdx.x == 0) {        // the sizes of this panel's products, slot i & 1
@@ -406,50 +476,64 @@ __global__ __launch_bounds__(PXT) void panel_x_kernel(const uint32_t* S, int64_t
     if (j0 >= na) return;
     const uint32_t sw = min((uint32_t)SL, na - j0), bw = k + sw, p = ar.p;
     const uint32_t r = st->r;
+    // the spec search settled the panel: W = S[P, J]^{-1} (Montgomery form) is ready, X = W S[P, slab]
+    const bool wpath = st->spec != 0;
     for (uint32_t idx = threadIdx.x; idx < k * bw; idx += PXT) {
         const uint32_t t = idx / bw, j = idx % bw;
         const int64_t row = (int64_t)P[t] * ld;
-        B[t * bw + j] = j < k ? S[row + J[j]] : S[row + j0 + (j - k)];
+        B[t * bw + j] = j < k ? (wpath ? Wg[t * LW + j] : S[row + J[j]]) : S[row + j0 + (j - k)];
     }
     if (threadIdx.x < k) s_q[threadIdx.x] = J[threadIdx.x];
     __syncthreads();
-    // Division-free Gauss-Jordan in pivot order (column t of the block is pivot column J[t]):
-    // row_i <- v row_i - f row_t for every row i != t with f = B[i, t] != 0.  Every leading block of
-    // the pivot block is non-singular in this order, so the left block ends diagonal and the slab
-    // part of row i is d_i (S[P, J]^-1 S[P, slab])_i.  Step t writes, in the rows i != t, only the
-    // left columns past t, the slab and the diagonal entry (i, i) when i < t (row t is zero at
-    // column i there); the left columns below t are zero and are neither written nor read again, so
-    // no entry a step writes is read by another thread in that step: one buffer, one barrier per
-    // step.  Sixteen threads share a row.
-    {
-        const uint32_t gi = threadIdx.x >> 4, gsub = threadIdx.x & 15;
-        for (uint32_t t = 0; t < k; ++t) {
-            if (gi < k && gi != t) {
-                uint32_t* row = B + gi * bw;
-                const uint32_t* trow = B + t * bw;
-                const uint32_t f = row[t];
-                if (f != 0) {
-                    const uint32_t v = trow[t], fneg = p - f, nl = k - t - 1, ns = nl + sw;
-                    const uint32_t nslot = ns + (gi < t ? 1u : 0u);
-                    for (uint32_t sl = gsub; sl < nslot; sl += 16) {
-                        if (sl < ns) {
-                            const uint32_t col = sl < nl ? t + 1 + sl : k + (sl - nl);
-                            row[col] = ar.dfree(v, row[col], fneg, trow[col]);
-                        } else {
-                            row[gi] = ar.dfree(v, row[gi], fneg, 0u);
+    if (wpath) {
+        static_assert(LW * SL <= PXT, "one output per thread");
+        const uint32_t t = threadIdx.x / SL, j = threadIdx.x % SL;
+        uint32_t x = 0;
+        if (t < k && j < sw)
+            for (uint32_t t2 = 0; t2 < k; ++t2) x = addm(x, ar.mul(B[t2 * bw + k + j], B[t * bw + t2]), p);
+        __syncthreads();
+        if (t < k && j < sw) B[t * bw + k + j] = x;
+        if (threadIdx.x < k) s_inv[threadIdx.x] = ar.prep(1u);
+        __syncthreads();
+    } else {
+        // Division-free Gauss-Jordan in pivot order (column t of the block is pivot column J[t]):
+        // row_i <- v row_i - f row_t for every row i != t with f = B[i, t] != 0.  Every leading block of
+        // the pivot block is non-singular in this order, so the left block ends diagonal and the slab
+        // part of row i is d_i (S[P, J]^-1 S[P, slab])_i.  Step t writes, in the rows i != t, only the
+        // left columns past t, the slab and the diagonal entry (i, i) when i < t (row t is zero at
+        // column i there); the left columns below t are zero and are neither written nor read again, so
+        // no entry a step writes is read by another thread in that step: one buffer, one barrier per
+        // step.  Sixteen threads share a row.
+        {
+            const uint32_t gi = threadIdx.x >> 4, gsub = threadIdx.x & 15;
+            for (uint32_t t = 0; t < k; ++t) {
+                if (gi < k && gi != t) {
+                    uint32_t* row = B + gi * bw;
+                    const uint32_t* trow = B + t * bw;
+                    const uint32_t f = row[t];
+                    if (f != 0) {
+                        const uint32_t v = trow[t], fneg = p - f, nl = k - t - 1, ns = nl + sw;
+                        const uint32_t nslot = ns + (gi < t ? 1u : 0u);
+                        for (uint32_t sl = gsub; sl < nslot; sl += 16) {
+                            if (sl < ns) {
+                                const uint32_t col = sl < nl ? t + 1 + sl : k + (sl - nl);
+                                row[col] = ar.dfree(v, row[col], fneg, trow[col]);
+                            } else {
+                                row[gi] = ar.dfree(v, row[gi], fneg, 0u);
+                            }
                         }
                     }
                 }
+                __syncthreads();
             }
-            __syncthreads();
         }
+        if (threadIdx.x < k) {
+            const uint32_t iv = invm(B[threadIdx.x * bw + threadIdx.x], p);
+            if (iv == 0) atomicExch(&st->status, 1);
+            s_inv[threadIdx.x] = ar.prep(iv);
+        }
+        __syncthreads();
     }
-    if (threadIdx.x < k) {
-        const uint32_t iv = invm(B[threadIdx.x * bw + threadIdx.x], p);
-        if (iv == 0) atomicExch(&st->status, 1);
-        s_inv[threadIdx.x] = ar.prep(iv);
-    }
-    __syncthreads();
     // X[:, slab] = d^-1 (slab part); the slab's own pivot columns get the identity exactly (they
     // equal it already)
     for (uint32_t idx = threadIdx.x; idx < k * sw; idx += PXT) {
@@ -550,6 +634,7 @@ void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const
     add(16);                      // 15 state
     add((size_t)mn * LW);         // 16 Gauss-Jordan left operand T
     add((size_t)2 * 32 * n);      // 17 X's limb planes (two panels: slot i & 1; 4 planes x 32 x n bytes each)
+    add((size_t)LW * LW);         // 18 W = S[P, J]^-1 from the spec search
     size_t total = 0;
     std::vector<size_t> off;
     for (size_t w : sizes) { off.push_back(total); total += w; }
@@ -561,6 +646,7 @@ void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const
     uint32_t* Xb = at(3);
     uint32_t* T = at(16);
     uint8_t* Xpb = (uint8_t*)at(17);
+    uint32_t* Wg = at(18);
     const uint32_t pstride = 32 * n;          // bytes per plane
     // the limb split of the k <= 32 products (balanced digits of residues mod p)
     const uint32_t L = ctx.p <= 255u ? 1 : (ctx.p <= 65279u ? 2 : (ctx.p <= 16711423u ? 3 : 4));
@@ -609,7 +695,7 @@ void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const
             // 1. tournament (reads only this panel's LW columns, already up to date), after the
             //    search over the first SPR rows, which settles most panels alone
             uint32_t* Rl = Rlb + (size_t)slot * m;
-            rrp_spec_kernel<Ar><<<1, RT, 0, s>>>(S, n, st, wc, ar, P, J, Rl, ro[cur], ro[cur ^ 1]);
+            rrp_spec_kernel<Ar><<<1, RT, 0, s>>>(S, n, st, wc, ar, P, J, Rl, ro[cur], ro[cur ^ 1], Wg);
             after_launch("rrp_spec_kernel");
             rrp_chunk_kernel<Ar><<<nchunk, RT, 0, s>>>(S, n, st, wc, ar, cand[0], cnt[0]);
             after_launch("rrp_chunk_kernel");
@@ -632,7 +718,7 @@ void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const
             }
             // 2. X = S[P,J]^{-1} S[P,:] into the U store and the contiguous copy
             panel_x_kernel<Ar><<<(na + SL - 1) / SL, PXT, 0, s>>>(S, n, na, P, J, st, ar, X, n, U, n, c, ro[cur], piv, prow,
-                                                                  Xp, pstride, L, half, slot);
+                                                                  Xp, pstride, L, half, slot, Wg);
             after_launch("panel_x_kernel");
             // 3. the next Schur complement, compacted: first the next panel's columns, then the rest
             HX_HIP(hipEventRecord(ev_next, s));     // X, the row list and the sizes are ready

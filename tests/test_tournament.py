@@ -119,3 +119,92 @@ def test_speculative_settles_and_defers():
     order, settled = _speculative(B)
     assert not settled
     assert _speculative(B, accept_partial=True)[0] != order
+
+
+PW = 2147483647
+_RINV = pow(1 << 32, -1, PW)
+
+
+def _dfree(v, x, fneg, y):
+    return (v * x + fneg * y) * _RINV % PW
+
+
+def _spec_w(A, stale_read=False):
+    """rrp_spec_kernel's Gauss-Jordan with coefficient tracking on the first rows of a panel, as the
+    kernel runs it (Montgomery steps, column q left stale, the pivot row read as zero left of q):
+    (pivot order, W = S[P, J]^-1) or None when a column has no pivot there."""
+    rows, wc = len(A), len(A[0])
+    pan = [list(r) for r in A]
+    sig = [1] * rows
+    co = [[0] * wc for _ in range(rows)]
+    chosen, order = [False] * rows, []
+    for q in range(wc):
+        cand = [u for u in range(rows) if not chosen[u] and pan[u][q]]
+        if not cand:
+            return None
+        bu, t = cand[0], len(order)
+        v, prow, pco, sb = pan[bu][q], list(pan[bu]), list(co[bu]), sig[bu]
+        for u in range(rows):
+            f = pan[u][q]
+            if u == bu or f == 0:
+                continue
+            fn = PW - f
+            for j in range(wc):
+                if j != q:
+                    y = prow[j] if (stale_read or j > q) else 0
+                    pan[u][j] = _dfree(v, pan[u][j], fn, y)
+            sig[u] = _dfree(v, sig[u], fn, 0)
+            for t2 in range(t):
+                co[u][t2] = _dfree(v, co[u][t2], fn, pco[t2])
+            co[u][t] = _dfree(v, 0, fn, sb)
+        chosen[bu] = True
+        order.append((bu, q))
+    W = []
+    for t, (u, q) in enumerate(order):
+        di = pow(pan[u][q], -1, PW)
+        W.append([(co[u][t2] + (sig[u] if t2 == t else 0)) * di % PW for t2 in range(len(order))])
+    return order, W
+
+
+def _inverse(K):
+    k = len(K)
+    M = [list(K[i]) + [int(i == j) for j in range(k)] for i in range(k)]
+    for c in range(k):
+        piv = next(i for i in range(c, k) if M[i][c])
+        M[c], M[piv] = M[piv], M[c]
+        iv = pow(M[c][c], -1, PW)
+        M[c] = [x * iv % PW for x in M[c]]
+        for i in range(k):
+            if i != c and M[i][c]:
+                f = M[i][c]
+                M[i] = [(a - f * b) % PW for a, b in zip(M[i], M[c])]
+    return [r[k:] for r in M]
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_spec_gauss_jordan_gives_inverse(seed):
+    """The spec search's pivots equal the smallest-row rule's and its W is exactly S[P, J]^-1; the
+    stale pivot-column read (a plausible mistake) breaks W."""
+    rng = np.random.default_rng(2000 + seed)
+    rows, wc = 24, [8, 12, 5, 8, 10, 6][seed]
+    A = rng.integers(0, PW, (rows, wc)).astype(object)
+    if seed % 2:
+        A[2:5] = (A[0] * 3 + A[1] * 7) % PW          # dependent rows among the candidates
+    A = [[int(x) for x in r] for r in A]
+    res = _spec_w(A)
+    assert res is not None
+    order, W = res
+    M = [r[:] for r in A]
+    done, want = [False] * rows, []
+    for q in range(wc):                              # the rule, plain modular arithmetic
+        b = next(i for i in range(rows) if not done[i] and M[i][q])
+        done[b] = True
+        for i in range(rows):
+            if i != b and not done[i] and M[i][q]:
+                f, iv = M[i][q], pow(M[b][q], -1, PW)
+                M[i] = [(a - f * iv * c) % PW for a, c in zip(M[i], M[b])]
+        want.append((b, q))
+    assert order == want
+    K = [[A[u][q2] for _, q2 in order] for u, _ in order]
+    assert W == _inverse(K)
+    assert _spec_w(A, stale_read=True)[1] != W
