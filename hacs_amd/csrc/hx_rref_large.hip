@@ -223,14 +223,22 @@ __global__ __launch_bounds__(RT) void rrp_spec_kernel(const uint32_t* S, int64_t
                 const uint32_t f = row[q];
                 if (f != 0) {
                     const uint32_t v = prow[q], fneg = p - f, sb = prow[wc], cnew = wc + 1 + k;
-                    for (uint32_t c = part; c <= cnew; c += 16) {
-                        if (c == q) continue;
-                        // the pivot row is zero at sigma's slot and at the earlier pivot columns c < q
-                        // (its entries there are stale, never zeroed)
-                        if (c == cnew)
-                            row[c] = ar.dfree(v, 0u, fneg, sb);
-                        else
-                            row[c] = ar.dfree(v, row[c], fneg, (c == wc || c < q) ? 0u : prow[c]);
+                    // this thread's (at most five) columns, every read issued before the writes; the
+                    // pivot row is zero at sigma's slot and at the earlier pivot columns c < q (its
+                    // entries there are stale, never zeroed), the new coefficient starts from zero
+                    static_assert(16 * 4 + 1 >= SWS - 1, "five columns per thread cover a row");
+                    uint32_t x[5], y[5];
+#pragma unroll
+                    for (int i = 0; i < 5; ++i) {
+                        const uint32_t c = part + 16 * i;
+                        const bool act = c <= cnew && c != q;
+                        x[i] = (act && c != cnew) ? row[c] : 0u;
+                        y[i] = !act ? 0u : (c == cnew ? sb : ((c == wc || c < q) ? 0u : prow[c]));
+                    }
+#pragma unroll
+                    for (int i = 0; i < 5; ++i) {
+                        const uint32_t c = part + 16 * i;
+                        if (c <= cnew && c != q) row[c] = ar.dfree(v, x[i], fneg, y[i]);
                     }
                 }
             }
