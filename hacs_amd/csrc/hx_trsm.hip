@@ -26,12 +26,12 @@ __global__ __launch_bounds__(256) void trsm_diag_kernel(const uint32_t* T, int64
     if (!upper) {
 #pragma unroll
         for (int i = 1; i < TB; ++i) {
-            if ((uint32_t)i >= nb) break;
+            if ((uint32_t)i >= nb) continue;          // (guards, not breaks: the loops unroll fully)
             uint64_t lo = 0;
             uint32_t hi = 0;
 #pragma unroll
             for (int t = 0; t < TB; ++t) {
-                if (t >= i) break;
+                if (t >= i) continue;
                 const uint64_t prod = (uint64_t)Ts[i][t] * x[t];
                 lo += prod;
                 hi += lo < prod ? 1u : 0u;
