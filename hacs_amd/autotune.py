@@ -6,7 +6,7 @@ Kinds and their choices (every choice computes the same bytes; only time differs
   howell   Howell form over Z/2^16 (dense n x n)  : one kernel (a single choice)
 
 The autotuner times every choice of every job it is given (one job per launch, HIP events on
-the job's stream, median of repeats) and is the judge: its fastest choice is the label.  The
+the job's stream, median of repeats) and sets the labels: its fastest choice is the label.  The
 selector is a 2-layer MLP (hx_mlp_* kernels; 64 hidden units, FP32) mapping job features
 (log sizes, modulus bits, density, kind) to predicted log2-times of the choices; it is trained
 on the autotuner's timings with the library's own backward and Adam kernels, and picks the

@@ -5,7 +5,8 @@ The network (64 hidden units, FP32) runs entirely in the library's own HIP kerne
 parameters in device memory.  It predicts log-time per (variant, tile) choice from job
 features; the cheapest predicted choice is taken.  Every choice computes the same bytes,
 so a misprediction only costs time.  The exhaustive autotuner (autotune.py) remains the
-fallback (unknown features, an untrained model) and the judge (it produced the labels).
+fallback (unknown features, an untrained model) and the reference it is scored against (it
+produced the labels).
 """
 from __future__ import annotations
 
