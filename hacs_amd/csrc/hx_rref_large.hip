@@ -22,8 +22,9 @@
 //     X's limb planes for the products of step 3.  X has the identity at J.
 //  3. The rows R not in P, in order, form the next Schur complement:
 //        S'[i, :] = S[R_i, W:] - S[R_i, J] X[:, W:]
-//     as GEMMs with a row gather on A and C_in, a column gather on A, and output into the other S
-//     buffer (compaction for free).  Lookahead: the next panel's W columns are computed first on
+//     as k <= 32 products on the INT8 matrix cores (upd32_kernel: A's limbs split on the fly, X's
+//     from its limb planes) with a row gather on A and C_in, a column gather on A, and output into
+//     the other S buffer (compaction for free).  Lookahead: the next panel's W columns are computed first on
 //     the caller's stream, the rest on a second stream, overlapping the next panel's tournament
 //     (which reads only those W columns); the next panel's step 2 waits for it.  P, J and the
 //     update's sizes are double-buffered by panel parity for the overlap.
