@@ -191,13 +191,46 @@ def batched_echelon(E, njobs):
         sb.run()
     torch.cuda.synchronize()
     t_single = time.perf_counter() - t0
+    # the same per-job launches dealt round-robin over 8 streams, so independent jobs can overlap
+    streams = [torch.cuda.Stream() for _ in range(8)]
+    for w_, s_ in zip(work, src):
+        w_.copy_(s_)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i, sb in enumerate(singles):
+        sb.run(stream=streams[i % 8].cuda_stream)
+    torch.cuda.synchronize()
+    t_single8 = time.perf_counter() - t0
+    del singles
     ranks = [r.rank for r in batch.results()]
+    # the other launch shape (the automatic rule picks one; both give the same bytes)
+    from hacs_amd import _lib
+    info = batch.info()
+    other = _lib.HX_RREF_BATCH_TWO_PER_CU if info["workgroups_per_cu"] == 1 else _lib.HX_RREF_BATCH_ONE_PER_CU
+    alt = E.RrefBatch(ctx, work, mode=other)
+    t_alt = []
+    for _ in range(reps):
+        for w_, s_ in zip(work, src):
+            w_.copy_(s_)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        alt.run()
+        torch.cuda.synchronize()
+        t_alt.append(time.perf_counter() - t0)
+    alt_same = [r.rank for r in alt.results()] == ranks
+    alt_info = alt.info()
+    del alt
     work_macs = float(sum((int(m) - 1) * (r * int(n) - r * (r - 1) // 2) for (m, n), r in zip(shapes, ranks)))
     return {
         "jobs": njobs,
         "jobs_per_s": njobs / min(t_batch),
         "per_job_launch_jobs_per_s": njobs / t_single,
         "speedup_vs_per_job": t_single / min(t_batch),
+        "per_job_launch_8_streams_jobs_per_s": njobs / t_single8,
+        "speedup_vs_per_job_8_streams": t_single8 / min(t_batch),
+        "launch": info,
+        "other_launch": {"launch": alt_info, "jobs_per_s": njobs / min(t_alt), "ms": min(t_alt) * 1e3,
+                         "same_ranks": alt_same},
         "graph_jobs_per_s": njobs / min(t_graph),
         "graph_equals_eager": graph_same,
         "elim_work_macs": work_macs,
@@ -205,11 +238,11 @@ def batched_echelon(E, njobs):
     }
 
 
-def howell_batch(E, njobs, cpu_budget_s=8.0):
+def howell_batch(E, njobs):
     """C5(c): Howell forms over Z/2^16 of the stream's Howell jobs (dense uniform [0, 2^16), n
     log-uniform in [32, 512]) -- jobs/s of one persistent launch vs one launch per job, and the
-    reference's ring2k::howell (u16 rung, oracle/_ref) on host cores over a sample of the same
-    jobs, whose outputs are compared with the device's."""
+    oracle's Howell on the host's cores over the same jobs, whose outputs are compared with the
+    device's."""
     import torch
     from hacs_amd import workloads as W
     k = 16
@@ -235,32 +268,22 @@ def howell_batch(E, njobs, cpu_budget_s=8.0):
     out = {"jobs": len(shapes), "k": k, "jobs_per_s": len(shapes) / best, "ms": best * 1e3,
            "per_job_launch_jobs_per_s": len(shapes) / t_single, "speedup_vs_per_job": t_single / best,
            "mean_n": float(np.mean(shapes))}
+    # the oracle's Howell (hxo_howell_fast, restating Ring2k.h's blocked echelon and Sec. 13.5 pass) on
+    # the host's threads, one job per thread, over all jobs; outputs compared with the device's
+    from concurrent.futures import ThreadPoolExecutor
     from oracle import oracle as O
-    if O.ref_available():
-        done, t_cpu, same = 0, 0.0, True
-        for j in range(len(shapes)):
-            A = E.from_device(mats[j])
-            t0 = time.perf_counter()
-            H, piv, pv = O.ref_howell(k, A, "ring2k", 16)
-            t_cpu += time.perf_counter() - t0
-            g = b.results([j])[0]
-            same = same and np.array_equal(g.H, H) and np.array_equal(g.pivots, piv) and np.array_equal(g.pivot_v, pv)
-            done += 1
-            if t_cpu > cpu_budget_s:
-                break
-        sub = E.HowellBatch(k, mats[:done])
-        sub.run()
-        torch.cuda.synchronize()
-        t_sub = 1e30
-        for _ in range(3):
-            t0 = time.perf_counter()
-            sub.run()
-            torch.cuda.synchronize()
-            t_sub = min(t_sub, time.perf_counter() - t0)
-        out["cpu_reference"] = {"jobs": done, "seconds": t_cpu, "cores": 1, "jobs_per_s": done / t_cpu,
-                                "device_jobs_per_s_same_jobs": done / t_sub,
-                                "device_equals_reference": bool(same),
-                                "routine": "ring2k::howell, u16 rung (oracle/_ref), the first jobs of the list"}
+    threads, nproc, model = host_cpu()
+    inputs = [E.from_device(M) for M in mats]
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        forms = list(ex.map(lambda A: O.howell_fast(k, A), inputs))
+    t_cpu = time.perf_counter() - t0
+    got = b.results()
+    same = all(np.array_equal(g.H, H) and np.array_equal(g.pivots, piv) and np.array_equal(g.pivot_v, pv)
+               for g, (H, piv, pv) in zip(got, forms))
+    out["cpu_port"] = {"jobs": len(shapes), "seconds": t_cpu, "cores": threads, "nproc": nproc, "cpu_model": model,
+                       "jobs_per_s": len(shapes) / t_cpu, "device_equals_port": bool(same),
+                       "routine": "hxo_howell_fast, one job per thread"}
     return out
 
 
@@ -324,15 +347,20 @@ def lu_c2(E, reps=3, cpu=True):
     work = float((m - 1) * (res.rank * n - res.rank * (res.rank - 1) // 2))
     out = {"seconds": best, "rank": res.rank, "elim_work_macs": work, "elim_macs_per_s": work / best,
            "shape": [m, n], "modulus": p}
-    from oracle import oracle as O
-    if cpu and O.ref_available():
+    if cpu:
+        from oracle import oracle as O
+        threads, nproc, model = host_cpu()
         Ah = E.from_device(A)
         t0 = time.perf_counter()
-        r_ref, piv_ref, R_ref = O.ref_rref(p, Ah)
-        t_ref = time.perf_counter() - t0
-        same = r_ref == res.rank and np.array_equal(piv_ref, res.pivots) and np.array_equal(R_ref, E.from_device(R))
-        out["cpu_reference"] = {"seconds": t_ref, "cores": 1, "routine": "elim::rref<uint32_t> (oracle/_ref)",
-                                "device_equals_reference": bool(same), "speedup": t_ref / best}
+        r_o, piv_o, prow_o, R_o = O.rref_rule(p, Ah, True, threads)
+        t_o = time.perf_counter() - t0
+        Rd = E.from_device(R)
+        same = r_o == res.rank and np.array_equal(piv_o, res.pivots) and np.array_equal(prow_o, res.pivot_rows) \
+            and np.array_equal(R_o, Rd[:r_o]) and not Rd[r_o:].any()
+        out["cpu_port"] = {"seconds": t_o, "cores": threads, "nproc": nproc, "cpu_model": model,
+                           "routine": "hxo_rref_rule (the oracle's smallest-row-rule elimination, restating "
+                                      "DenseElim.cpp's RREF conventions)",
+                           "device_equals_port": bool(same), "speedup": t_o / best}
     return out
 
 
@@ -445,23 +473,42 @@ def level1_kernels(E, n=8192, reps=20):
     return {"n": n, "kernels": out}
 
 
-def cpu_baseline(p, threads):
-    """The reference's own CPU GEMM (oracle/_ref, Kernels.cpp:1417) on a bounded sample, min of 3."""
+def host_cpu():
+    """(threads to use, nproc, CPU model): the oracle runs on at most 16 host threads (the share of one
+    GPU's host cores)."""
+    nproc = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return min(16, nproc), nproc, model
+
+
+def cpu_baseline(p):
+    """The oracle's restatement of the reference CPU GEMM (hxo_gemm_fast: Kernels.cpp:680-710's u32
+    route, threaded over row tiles as Executor.cpp:69-100) on the host's cores, on a bounded sample of
+    the headline product, min of 3."""
     from oracle import oracle as O
     from hacs_amd import workloads as W
-    m, k, n = 512, 8192, 1024
+    threads, nproc, model = host_cpu()
+    m, k, n = 256, 8192, 1024
     A = W.random_entries(m * k, p, W.seed_for(k, k, p, 0x01)).reshape(m, k)              # rows 0..m-1 of A
     B = np.ascontiguousarray(W.random_entries(k * k, p, W.seed_for(k, k, p, 0x02)).reshape(k, k)[:, :n])
-    kind = "reference" if O.ref_available() else "port"
-    fn = O.ref_gemm if kind == "reference" else O.gemm_fast
     best = 1e30
     for _ in range(3):
         t0 = time.perf_counter()
-        fn(p, A, B, None, 0, threads)
+        O.gemm_fast(p, A, B, None, 0, threads)
         best = min(best, time.perf_counter() - t0)
-    return {"value": m * k * n / best, "unit": "mod-p mult-adds/s", "cores": threads, "kind": kind,
+    return {"value": m * k * n / best, "unit": "mod-p mult-adds/s", "cores": threads, "kind": "port",
+            "nproc": nproc, "cpu_model": model, "seconds": best,
             "sample": f"C = A B mod {p}, A {m}x{k}, B {k}x{n} (the 8192-deep product of the headline "
-                      f"config on {m}x{n} outputs), min of 3"}
+                      f"config on {m}x{n} outputs, {m * k * n:.3g} mult-adds), hxo_gemm_fast on {threads} threads, "
+                      f"min of 3"}
 
 
 def main():
@@ -495,7 +542,7 @@ def main():
                "howell": lambda: howell_batch(E, args.howell_jobs),
                "ring2k": lambda: ring2k_session(E),
                "level1": lambda: level1_kernels(E),
-               "cpu": lambda: cpu_baseline(p, min(16, os.cpu_count() or 1))}[args.only]()
+               "cpu": lambda: cpu_baseline(p)}[args.only]()
         print(json.dumps({"only": args.only, **leg}), flush=True)
         return
     log(f"device {E.device_info()}; C3 n={n} p={p}")
@@ -550,7 +597,7 @@ def main():
         "limb_vs_plain": wall_p / wall_l,
         "parity": {"limb_equals_plain": same, "spot_entries_exact": spot, "limb_kernels_agree": same_alt},
         "limb_other_kernels": others,
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_INT8_TOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_INT8_TOPS, "unit": "TOP/s",
                      "frac": achieved / PEAK_INT8_TOPS, "traffic": profiled_traffic(n, ctx.limbs, kname),
                      "kernel": kname,
                      "kernel_ms": main_l,
@@ -581,8 +628,7 @@ def main():
         log(f"mixed stream: {mx}")
         res["mixed_stream"] = mx
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        res["cpu_baseline"] = cpu_baseline(p, threads)
+        res["cpu_baseline"] = cpu_baseline(p)
         log(f"cpu baseline: {res['cpu_baseline']}")
     print(json.dumps(res), flush=True)
 

@@ -22,6 +22,8 @@ HX_E_NOMEM = -4
 HX_E_NOT_INVERTIBLE = -5
 HX_E_CAPTURE = -6
 
+HX_RREF_BATCH_AUTO, HX_RREF_BATCH_ONE_PER_CU, HX_RREF_BATCH_TWO_PER_CU = 0, 1, 2
+
 HX_OP_SET, HX_OP_ADD, HX_OP_SUB = 0, 1, 2
 HX_GEMM_AUTO, HX_GEMM_PLAIN, HX_GEMM_LIMB = 0, 1, 2
 
@@ -59,6 +61,17 @@ class hx_gemm_job(C.Structure):
     _fields_ = [("m", C.c_uint32), ("n", C.c_uint32), ("k", C.c_uint32), ("op", C.c_int32),
                 ("variant", C.c_int32), ("reserved", C.c_int32), ("A", C.c_void_p), ("B", C.c_void_p), ("C", C.c_void_p),
                 ("lda", C.c_int64), ("ldb", C.c_int64), ("ldc", C.c_int64)]
+
+
+class hx_rref_batch_info(C.Structure):
+    _fields_ = [("njobs", C.c_uint32), ("workgroups_per_cu", C.c_uint32), ("grid", C.c_uint32),
+                ("region_words", C.c_uint32), ("lds_bytes", C.c_size_t), ("jobs_planes_outside_lds", C.c_uint32)]
+
+
+class hx_arena_stats(C.Structure):
+    _fields_ = [("reserved_bytes", C.c_size_t), ("live_bytes", C.c_size_t), ("peak_live_bytes", C.c_size_t),
+                ("peak_reserved_bytes", C.c_size_t), ("driver_mallocs", C.c_uint64), ("reuses", C.c_uint64),
+                ("driver_frees", C.c_uint64), ("oom_retries", C.c_uint64)]
 
 
 class hx_rref_job(C.Structure):
@@ -120,6 +133,10 @@ SIGNATURES = {
     "hx_graph_destroy": (_INT, [_V]),
     "hx_gemm_u32": (_INT, [C.POINTER(hx_modctx), hx_view, hx_view, hx_view, _INT, _V, C.POINTER(hx_gemm_opts)]),
     "hx_gemm_workspace_bytes": (_SZ, [C.POINTER(hx_modctx), _U32, _U32, _U32, _INT]),
+    "hx_gemm_limb_k_chunk": (_U32, [C.POINTER(hx_modctx), _U32, _U32, _U32, _SZ]),
+    "hx_rref_batch_info_get": (_INT, [_V, C.POINTER(hx_rref_batch_info)]),
+    "hx_arena_stats_get": (_INT, [C.POINTER(hx_arena_stats)]),
+    "hx_arena_trim": (_INT, [C.POINTER(_SZ)]),
     "hx_schur_update_u32": (_INT, [C.POINTER(hx_modctx), hx_view, hx_view, hx_view, _V, C.POINTER(hx_gemm_opts)]),
     "hx_gemm_batch_create": (_INT, [C.POINTER(hx_modctx), C.POINTER(hx_gemm_job), _U32, _INT, C.POINTER(_V)]),
     "hx_gemm_batch_run": (_INT, [_V, _V]),

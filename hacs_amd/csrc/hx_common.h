@@ -82,8 +82,16 @@ inline bool views_overlap(const hx_view& a, const hx_view& b, int eba = 4, int e
     return !(ahi <= blo || bhi <= alo);
 }
 
-// Library-owned device scratch (host side): grows, never shrinks, per device.
-void* scratch(size_t bytes, int slot);
+// Device arena (host side): hx_malloc / hx_free and every library-owned buffer go through it.
+void* arena_alloc(size_t bytes);
+void arena_free(void* p);
+// Whether s is being captured into a graph (then nothing may be allocated or freed for it).
+bool stream_capturing(hipStream_t s);
+// Library-owned scratch of at least `bytes`, private to stream s and one purpose (below); grows
+// (after waiting for s) but never shrinks; nullptr when it would have to grow inside a capture.
+enum ScratchPurpose { SCRATCH_LIMB = 0, SCRATCH_MLP = 1, SCRATCH_RREF = 2 };
+void* stream_scratch(hipStream_t s, int purpose, size_t bytes);
+void stream_scratch_release(hipStream_t s);
 
 // ---- device: exact arithmetic mod p, p < 2^32 (ModCtx.h:202-333) ----
 struct DevMod {

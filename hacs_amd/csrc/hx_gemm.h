@@ -32,12 +32,14 @@ struct GemmArgs {
 // The plain VALU kernel (any p < 2^32).
 void gemm_plain(const hx_modctx& ctx, const GemmArgs& g, hipStream_t s, int tile, hipEvent_t ev0 = nullptr,
                 hipEvent_t ev1 = nullptr);
-// The limb-MFMA route (ctx.limbs > 0). ws: workspace of >= limb_workspace_bytes() or nullptr (library scratch).
+// The limb-MFMA route (ctx.limbs > 0). ws: the limb planes' workspace (ws_bytes); K runs in the chunks it holds.
 void gemm_limb(const hx_modctx& ctx, const GemmArgs& g, hipStream_t s, void* ws, size_t ws_bytes, int tile,
                hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 size_t limb_workspace_bytes(const hx_modctx& ctx, uint32_t m, uint32_t n, uint32_t k);
 // Largest K one limb pass can absorb with exact int32 diagonals (multiple of 64).
 uint32_t limb_k_chunk(uint32_t limbs);
+// The K chunk a limb workspace of `bytes` allows for this shape (0: not one 64-deep chunk).
+uint32_t limb_k_chunk_for(const hx_modctx& ctx, uint32_t m, uint32_t n, uint32_t k, size_t bytes);
 // The variant the automatic policy picks for this shape.
 int gemm_auto_variant(const hx_modctx& ctx, uint32_t m, uint32_t n, uint32_t k);
 void gemm_dispatch(const hx_modctx& ctx, const GemmArgs& g, hipStream_t s, int variant, void* ws, size_t ws_bytes,

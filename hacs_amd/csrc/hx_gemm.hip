@@ -897,6 +897,19 @@ size_t limb_workspace_bytes(const hx_modctx& ctx, uint32_t m, uint32_t n, uint32
     return (size_t)ctx.limbs * ((size_t)round_up(std::max<uint32_t>(m, 1), LM) + round_up(std::max<uint32_t>(n, 1), LN)) * kc;
 }
 
+// The K chunk one limb pass runs in a workspace of `bytes` (imma_k_chunk's role, CudaKernels.h:100-110):
+// min(k, limb_k_chunk) when its planes fit, else the largest multiple of 64 whose planes fit; 0 when
+// not even one 64-deep chunk fits.  Chunks after the first accumulate with Add / Sub: same bytes.
+uint32_t limb_k_chunk_for(const hx_modctx& ctx, uint32_t m, uint32_t n, uint32_t k, size_t bytes) {
+    if (ctx.limbs == 0) return 0;
+    const uint32_t full = std::max<uint32_t>(1, std::min(k, limb_k_chunk(ctx.limbs)));
+    if (limb_workspace_bytes(ctx, m, n, full) <= bytes) return full;
+    const size_t per_k = (size_t)ctx.limbs * ((size_t)round_up(std::max<uint32_t>(m, 1), LM) +
+                                              round_up(std::max<uint32_t>(n, 1), LN));
+    const size_t kc = bytes / per_k / 64 * 64;
+    return (uint32_t)std::min<size_t>(kc, full);
+}
+
 void gemm_plain(const hx_modctx& ctx, const GemmArgs& g, hipStream_t s, int /*tile*/, hipEvent_t ev0, hipEvent_t ev1) {
     if (g.m == 0 || g.n == 0) return;
     const uint32_t ntm = (g.m + PM - 1) / PM, ntn = (g.n + PN - 1) / PN;
@@ -906,19 +919,16 @@ void gemm_plain(const hx_modctx& ctx, const GemmArgs& g, hipStream_t s, int /*ti
     if (ev1) HX_HIP(hipEventRecord(ev1, s));
 }
 
+// ws: the limb planes' workspace (ws_bytes of it); K runs in the chunks that fit it.  A NULL
+// workspace has been resolved by the caller (gemm_dispatch).
 void gemm_limb(const hx_modctx& ctx, const GemmArgs& g0, hipStream_t s, void* ws, size_t ws_bytes, int tile,
                hipEvent_t ev0, hipEvent_t ev1) {
     const int L = (int)ctx.limbs;
     HX_CHECK(L >= 1 && L <= 4, "the limb route needs p <= 4278124287 (ctx.limbs > 0)");
     if (g0.m == 0 || g0.n == 0) return;
-    const uint32_t kchunk = limb_k_chunk(L);
+    const uint32_t kchunk = limb_k_chunk_for(ctx, g0.m, g0.n, g0.k, ws ? ws_bytes : 0);
+    HX_CHECK(ws != nullptr && kchunk > 0, "limb workspace too small for one 64-deep K chunk of this shape");
     HX_CHECK(g0.k_dev == nullptr || g0.k <= kchunk, "device-side k needs k <= one limb chunk");
-    const size_t need = limb_workspace_bytes(ctx, g0.m, g0.n, g0.k);
-    if (ws == nullptr) {
-        ws = scratch(need, 0);
-    } else {
-        HX_CHECK(ws_bytes >= need, "limb workspace too small");
-    }
     const uint32_t Mp = round_up(g0.m, LM), Np = round_up(g0.n, LN);
     const uint32_t p = ctx.p, half = (p - 1) / 2;
     GemmArgs g = g0;
@@ -1013,7 +1023,26 @@ int gemm_auto_variant(const hx_modctx& ctx, uint32_t m, uint32_t n, uint32_t k) 
 
 void gemm_dispatch(const hx_modctx& ctx, const GemmArgs& g, hipStream_t s, int variant, void* ws, size_t ws_bytes,
                    int tile, hipEvent_t ev0, hipEvent_t ev1) {
-    if (variant == HX_GEMM_AUTO) variant = gemm_auto_variant(ctx, g.m, g.n, g.k);
+    const bool automatic = variant == HX_GEMM_AUTO;
+    if (automatic) variant = gemm_auto_variant(ctx, g.m, g.n, g.k);
+    if (variant == HX_GEMM_LIMB && g.m != 0 && g.n != 0) {
+        if (ws == nullptr) {
+            // library-owned planes private to this stream (never shared with another stream); inside a
+            // capture they can only be used when they are already large enough
+            const size_t need = limb_workspace_bytes(ctx, g.m, g.n, g.k);
+            ws = stream_scratch(s, SCRATCH_LIMB, need);
+            ws_bytes = ws ? need : 0;
+            if (ws == nullptr && !automatic)
+                throw Error{HX_E_CAPTURE, "the limb route needs a caller workspace inside a stream capture "
+                                          "(hx_gemm_opts.workspace; the library's own cannot grow there)"};
+        }
+        // the limb route runs when one 64-deep K chunk fits the workspace (the reference takes its IMMA
+        // route only with a large enough imma_workspace); otherwise AUTO runs the plain kernel
+        if (limb_k_chunk_for(ctx, g.m, g.n, g.k, ws == nullptr ? 0 : ws_bytes) == 0) {
+            HX_CHECK(automatic, "limb workspace too small for one 64-deep K chunk of this shape");
+            variant = HX_GEMM_PLAIN;
+        }
+    }
     if (variant == HX_GEMM_LIMB)
         gemm_limb(ctx, g, s, ws, ws_bytes, tile, ev0, ev1);
     else
@@ -1270,6 +1299,11 @@ HX_API int hx_gemm_u32(const hx_modctx* ctx, hx_view A, hx_view B, hx_view C, in
                       opts ? opts->workspace_bytes : 0, opts ? opts->tile : 0,
                       opts ? (hipEvent_t)opts->main_begin : nullptr, opts ? (hipEvent_t)opts->main_end : nullptr);
     });
+}
+
+HX_API uint32_t hx_gemm_limb_k_chunk(const hx_modctx* ctx, uint32_t m, uint32_t n, uint32_t k, size_t workspace_bytes) {
+    if (ctx == nullptr) return 0;
+    return limb_k_chunk_for(*ctx, m, n, k, workspace_bytes);
 }
 
 HX_API int hx_gemm_batch_create(const hx_modctx* ctx, const hx_gemm_job* jobs, uint32_t njobs, int variant,

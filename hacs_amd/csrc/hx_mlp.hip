@@ -144,7 +144,8 @@ HX_API int hx_mlp_backward(const float* X, uint32_t B, uint32_t F, const float* 
         HX_CHECK(X && W1 && W2 && hidden && y && target && dW1 && db1 && dW2 && db2 && loss, "NULL argument");
         (void)W1;
         const hipStream_t st = (hipStream_t)s;
-        float* dy = (float*)scratch(((size_t)B * O + (size_t)B * H + 1) * sizeof(float), 1);
+        float* dy = (float*)stream_scratch((hipStream_t)s, SCRATCH_MLP, ((size_t)B * O + (size_t)B * H + 1) * sizeof(float));
+        HX_CHECK(dy != nullptr, "hx_mlp_backward: its scratch cannot grow inside a stream capture (run it once before capturing)");
         float* dh = dy + (size_t)B * O;
         mlp_dy_kernel<<<1, 256, 0, st>>>(y, target, mask, B * O, dy, loss);
         after_launch("mlp_dy_kernel");

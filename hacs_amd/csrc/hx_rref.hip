@@ -5,13 +5,15 @@
 #include "hx_common.h"
 
 namespace hx {
-int rref_batch_create_impl(const hx_modctx& ctx, const hx_rref_job* jobs, uint32_t njobs, void** out);
+int rref_batch_create_impl(const hx_modctx& ctx, const hx_rref_job* jobs, uint32_t njobs, int mode, void** out);
 int rref_batch_run_impl(void* b, hipStream_t s);
 int rref_batch_status_impl(void* b, hipStream_t s);
 void rref_batch_destroy_impl(void* b);
 // The multi-kernel path for large matrices (hx_rref_large.hip); false when it does not take the shape.
 bool rref_large(const hx_modctx& ctx, uint32_t m, uint32_t n, const uint32_t* A, int64_t lda, uint32_t* R,
-                int64_t ldr, uint32_t* rank, uint32_t* pivots, uint32_t* pivot_rows, hipStream_t s);
+                int64_t ldr, uint32_t* rank, uint32_t* pivots, uint32_t* pivot_rows, hipStream_t s, bool force);
+// Whether one job of this shape fits the one-workgroup kernel's LDS (hx_rref_batch.hip).
+bool rref_batch_fits(uint32_t m, uint32_t n);
 }  // namespace hx
 
 using namespace hx;
@@ -40,7 +42,7 @@ void rref_via_batch(const hx_modctx& ctx, uint32_t m, uint32_t n, const uint32_t
         uint32_t* d_prow = d_piv + std::max<uint32_t>(mn, 1);
         uint32_t* d_rank = d_prow + std::max<uint32_t>(mn, 1);
         hx_rref_job job{m, n, dst, ldd, d_piv, d_prow, d_rank, 0};
-        rref_batch_create_impl(ctx, &job, 1, &b);
+        rref_batch_create_impl(ctx, &job, 1, HX_RREF_BATCH_AUTO, &b);
         rref_batch_run_impl(b, s);
         rref_batch_status_impl(b, s);
         HX_HIP(hipMemcpyAsync(rank, d_rank, 4, hipMemcpyDeviceToHost, s));
@@ -72,7 +74,10 @@ HX_API int hx_rref_u32(const hx_modctx* ctx, uint32_t m, uint32_t n, const uint3
         *rank = 0;
         if (m == 0 || n == 0) return;
         HX_CHECK(A != nullptr, "rref: A is NULL");
-        if (rref_large(*ctx, m, n, A, lda, R, ldr, rank, pivots, pivot_rows, (hipStream_t)s)) return;
+        // big matrices, and shapes one workgroup cannot hold (m > 4096, or rows wider than its LDS
+        // panel, e.g. 5000 x 400 or 16 x 40000), take the whole-GPU path
+        const bool force = !rref_batch_fits(m, n);
+        if (rref_large(*ctx, m, n, A, lda, R, ldr, rank, pivots, pivot_rows, (hipStream_t)s, force)) return;
         rref_via_batch(*ctx, m, n, A, lda, R, ldr, rank, pivots, pivot_rows, (hipStream_t)s);
     });
 }

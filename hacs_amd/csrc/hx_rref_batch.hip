@@ -510,6 +510,8 @@ struct RBatch {
     uint32_t region = 0, mmax = 0, nmax = 0;
     size_t lds = 0;
     uint32_t grid = 0;
+    uint32_t per_cu = 1;                // workgroups per CU the LDS size was chosen for (1 or 2)
+    uint32_t planes_out = 0;            // jobs whose first panel keeps X's limb planes outside LDS
 };
 
 constexpr size_t kLdsCap = 156 * 1024;
@@ -517,12 +519,25 @@ constexpr size_t kLdsCap = 156 * 1024;
 }  // namespace
 
 // Shared with the single-matrix entry point.
-int rref_batch_create_impl(const hx_modctx& ctx, const hx_rref_job* jobs, uint32_t njobs, void** out);
+int rref_batch_create_impl(const hx_modctx& ctx, const hx_rref_job* jobs, uint32_t njobs, int mode, void** out);
 int rref_batch_run_impl(void* b, hipStream_t s);
 void rref_batch_destroy_impl(void* b);
 
-int rref_batch_create_impl(const hx_modctx& ctx, const hx_rref_job* jobs, uint32_t njobs, void** out) {
+// The LDS test of create for a batch holding one m x n job: the row lists (fixed) and a region of
+// max(m + 1, n, 2 * 32 * 64) words within the cap.
+bool rref_batch_fits(uint32_t m, uint32_t n) {
+    if (m > 4096) return false;
+    const size_t m4 = (std::max<uint32_t>(m, 1) + 3) / 4 * 4, n4 = (std::max<uint32_t>(n, 1) + 3) / 4 * 4;
+    const size_t fixed = m4 * 4 + m4 + 64;
+    if (fixed + 4 * 64 >= kLdsCap) return false;
+    const size_t cap_words = (kLdsCap - fixed) / 4;
+    return std::max<size_t>(std::max<size_t>(m4 + 1, n4), 2 * 32 * 64) <= cap_words;
+}
+
+int rref_batch_create_impl(const hx_modctx& ctx, const hx_rref_job* jobs, uint32_t njobs, int mode, void** out) {
     HX_CHECK(ctx.p >= 2 && ctx.p < 0x80000000u, "rref: needs a prime 2 <= p < 2^31");
+    HX_CHECK(mode == HX_RREF_BATCH_AUTO || mode == HX_RREF_BATCH_ONE_PER_CU || mode == HX_RREF_BATCH_TWO_PER_CU,
+             "rref batch: mode must be HX_RREF_BATCH_AUTO, _ONE_PER_CU or _TWO_PER_CU");
     HX_CHECK(jobs != nullptr || njobs == 0, "jobs is NULL");
     RBatch* b = new RBatch();
     try {
@@ -561,13 +576,24 @@ int rref_batch_create_impl(const hx_modctx& ctx, const hx_rref_job* jobs, uint32
             wmax = std::max(wmax, w);
         }
         const size_t half_words = kLdsCap / 2 > fixed ? (kLdsCap / 2 - fixed) / 4 : 0;
-        const bool two_per_cu = (size_t)mmax * 17 <= half_words && (size_t)16 * nmax <= half_words &&
-                                half_words >= 2 * 32 * 64 && wsum >= 512.0 * wmax;
+        const bool two_fits = (size_t)mmax * 17 <= half_words && (size_t)16 * nmax <= half_words &&
+                              half_words >= 2 * 32 * 64;
+        HX_CHECK(mode != HX_RREF_BATCH_TWO_PER_CU || two_fits, "rref batch: jobs too large for two workgroups per CU");
+        const bool two_per_cu = mode == HX_RREF_BATCH_TWO_PER_CU ||
+                                (mode == HX_RREF_BATCH_AUTO && two_fits && wsum >= 512.0 * wmax);
+        b->per_cu = two_per_cu ? 2 : 1;
         const size_t cap_words = ((two_per_cu ? kLdsCap / 2 : kLdsCap) - fixed) / 4;
         b->region = (uint32_t)std::min(want, cap_words);
         HX_CHECK(b->region >= std::max<size_t>(std::max<size_t>(mmax + 1, nmax), 2 * 32 * 64),
                  "rref batch: shape exceeds the LDS panel");
         b->lds = (size_t)b->region * 4 + fixed;
+        {
+            // the kernel's own test (X's limb planes in LDS iff 8 L nb <= region, nb = n - c) at c = 0
+            const uint32_t pp = ctx.p;
+            const uint32_t L = pp <= 255u ? 1 : (pp <= 65279u ? 2 : (pp <= 16711423u ? 3 : 4));
+            for (const RJob& jj : hj)
+                if (jj.m && jj.n && (uint64_t)8 * L * jj.n > b->region) ++b->planes_out;
+        }
         HX_HIP(hipMalloc(&b->d_U, std::max<size_t>(ucount, 1) * 4));
         size_t off = 0;
         for (uint32_t i = 0; i < njobs; ++i) {
@@ -673,13 +699,25 @@ using namespace hx;
 
 extern "C" {
 
-HX_API int hx_rref_batch_create(const hx_modctx* ctx, const hx_rref_job* jobs, uint32_t njobs, int /*variant*/,
+HX_API int hx_rref_batch_create(const hx_modctx* ctx, const hx_rref_job* jobs, uint32_t njobs, int mode,
                                 hx_rref_batch* out) {
     return guard("hx_rref_batch_create", [&] {
         HX_CHECK(ctx != nullptr && out != nullptr, "NULL argument");
         void* b = nullptr;
-        rref_batch_create_impl(*ctx, jobs, njobs, &b);
+        rref_batch_create_impl(*ctx, jobs, njobs, mode, &b);
         *out = b;
+    });
+}
+HX_API int hx_rref_batch_info_get(hx_rref_batch vb, hx_rref_batch_info* out) {
+    return guard("hx_rref_batch_info_get", [&] {
+        HX_CHECK(vb != nullptr && out != nullptr, "NULL argument");
+        const RBatch* b = (const RBatch*)vb;
+        out->workgroups_per_cu = b->per_cu;
+        out->grid = b->grid;
+        out->lds_bytes = b->lds;
+        out->region_words = b->region;
+        out->jobs_planes_outside_lds = b->planes_out;
+        out->njobs = b->njobs;
     });
 }
 HX_API int hx_rref_batch_run(hx_rref_batch b, hx_stream s) {

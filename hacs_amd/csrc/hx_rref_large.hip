@@ -842,11 +842,12 @@ void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const
 
 }  // namespace
 
-// The whole-GPU path takes matrices with at least 1024 rows and m * n >= 2^21; smaller ones run
-// in one workgroup (hx_rref_batch.hip).
+// The whole-GPU path takes matrices with at least 1024 rows and m * n >= 2^21, and (force) every
+// shape the one-workgroup kernel cannot hold (more than 4096 rows, or rows too wide for its LDS
+// panel); the rest run in one workgroup (hx_rref_batch.hip).
 bool rref_large(const hx_modctx& ctx, uint32_t m, uint32_t n, const uint32_t* A, int64_t lda, uint32_t* R,
-                int64_t ldr, uint32_t* rank, uint32_t* pivots, uint32_t* pivot_rows, hipStream_t s) {
-    if (m < 1024 || (uint64_t)m * n < (1ull << 21)) return false;
+                int64_t ldr, uint32_t* rank, uint32_t* pivots, uint32_t* pivot_rows, hipStream_t s, bool force) {
+    if (!force && (m < 1024 || (uint64_t)m * n < (1ull << 21))) return false;
     if (ctx.p & 1u)
         run_large(ctx, make_mont(ctx.p), m, n, A, lda, R, ldr, rank, pivots, pivot_rows, s);
     else

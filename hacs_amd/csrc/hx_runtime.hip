@@ -5,10 +5,12 @@
 // runtime stages pageable copies itself, and pinned buffers are the caller's choice.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "hx_common.h"
@@ -59,30 +61,143 @@ const Avail& avail() {
     return a;
 }
 
+// ---- device arena (the caching allocator of Cuda/CudaCtx.h:104-168, designed afresh) ----------
+// Blocks are rounded up to a granule (512 B below 1 MiB, 2 MiB above) and cached on free in
+// exact-size free lists; a request first takes a cached block of its rounded size.  A failed
+// hipMalloc is answered by a trim (every cached block returned to the driver) and one retry.
+// Nothing here is legal inside a stream capture (hipMalloc / hipFree are not): callers allocate
+// before a capture begins.
+struct Arena {
+    std::mutex mu;
+    std::unordered_map<size_t, std::vector<void*>> free_lists;   // rounded size -> cached blocks
+    std::unordered_map<void*, size_t> live;                      // block -> rounded size
+    hx_arena_stats st{};
+};
+Arena& arena() {
+    static Arena* a = new Arena();   // never destroyed: blocks may be freed from static destructors
+    return *a;
+}
+
+size_t arena_round(size_t bytes) {
+    const size_t g = bytes < (1u << 20) ? 512 : (2u << 20);
+    return (bytes + g - 1) / g * g;
+}
+
+size_t arena_trim_locked(Arena& a) {
+    size_t freed = 0;
+    for (auto& kv : a.free_lists) {
+        for (void* p : kv.second) {
+            (void)hipFree(p);
+            freed += kv.first;
+            a.st.reserved_bytes -= kv.first;
+            ++a.st.driver_frees;
+        }
+        kv.second.clear();
+    }
+    a.free_lists.clear();
+    return freed;
+}
+
+// ---- per-stream library scratch --------------------------------------------------------------
+// One buffer per (stream, purpose): work queued on a stream only ever uses that stream's buffer,
+// so calls on different streams never share bytes.  A buffer grows by first waiting for the work
+// already queued on its stream (the only stream that reads it), then returning it to the arena.
+struct ScratchKey {
+    hipStream_t s;
+    int purpose;
+    bool operator==(const ScratchKey& o) const { return s == o.s && purpose == o.purpose; }
+};
+struct ScratchKeyHash {
+    size_t operator()(const ScratchKey& k) const { return std::hash<void*>()((void*)k.s) * 31 + (size_t)k.purpose; }
+};
 struct ScratchSlot {
     void* ptr = nullptr;
     size_t bytes = 0;
 };
 std::mutex g_scratch_mu;
-std::vector<ScratchSlot> g_scratch(16);
+std::unordered_map<ScratchKey, ScratchSlot, ScratchKeyHash> g_scratch;
 
 }  // namespace
 
-void* scratch(size_t bytes, int slot) {
-    std::lock_guard<std::mutex> lk(g_scratch_mu);
-    ScratchSlot& s = g_scratch.at(slot);
-    if (s.bytes < bytes) {
-        if (s.ptr) HX_HIP(hipFree(s.ptr));
-        s.ptr = nullptr;
-        s.bytes = 0;
-        const size_t want = bytes + bytes / 4 + 4096;
-        if (hipMalloc(&s.ptr, want) != hipSuccess) {
+void* arena_alloc(size_t bytes) {
+    if (bytes == 0) return nullptr;
+    Arena& a = arena();
+    const size_t r = arena_round(bytes);
+    std::lock_guard<std::mutex> lk(a.mu);
+    void* p = nullptr;
+    auto it = a.free_lists.find(r);
+    if (it != a.free_lists.end() && !it->second.empty()) {
+        p = it->second.back();
+        it->second.pop_back();
+        ++a.st.reuses;
+    } else {
+        if (hipMalloc(&p, r) != hipSuccess) {
             (void)hipGetLastError();
-            throw Error{HX_E_NOMEM, "library scratch allocation of " + std::to_string(want) + " bytes failed"};
+            ++a.st.oom_retries;
+            arena_trim_locked(a);
+            if (hipMalloc(&p, r) != hipSuccess) {
+                (void)hipGetLastError();
+                throw Error{HX_E_NOMEM, "device allocation of " + std::to_string(r) + " bytes failed (after a trim)"};
+            }
         }
-        s.bytes = want;
+        ++a.st.driver_mallocs;
+        a.st.reserved_bytes += r;
+        a.st.peak_reserved_bytes = std::max(a.st.peak_reserved_bytes, a.st.reserved_bytes);
     }
-    return s.ptr;
+    a.live[p] = r;
+    a.st.live_bytes += r;
+    a.st.peak_live_bytes = std::max(a.st.peak_live_bytes, a.st.live_bytes);
+    return p;
+}
+
+void arena_free(void* p) {
+    if (p == nullptr) return;
+    Arena& a = arena();
+    std::lock_guard<std::mutex> lk(a.mu);
+    auto it = a.live.find(p);
+    if (it == a.live.end()) throw Error{HX_E_ARGUMENT, "pointer was not allocated by this library (or freed twice)"};
+    const size_t r = it->second;
+    a.live.erase(it);
+    a.st.live_bytes -= r;
+    a.free_lists[r].push_back(p);
+}
+
+bool stream_capturing(hipStream_t s) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return cs != hipStreamCaptureStatusNone;
+}
+
+void* stream_scratch(hipStream_t s, int purpose, size_t bytes) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    ScratchSlot& sl = g_scratch[ScratchKey{s, purpose}];
+    if (sl.bytes >= bytes) return sl.ptr;
+    if (stream_capturing(s)) return nullptr;          // no allocation inside a capture
+    if (sl.ptr) {
+        HX_HIP(hipStreamSynchronize(s));               // the queued work that reads the old buffer is done
+        arena_free(sl.ptr);
+        sl.ptr = nullptr;
+        sl.bytes = 0;
+    }
+    const size_t want = bytes + bytes / 4 + 4096;
+    sl.ptr = arena_alloc(want);
+    sl.bytes = want;
+    return sl.ptr;
+}
+
+void stream_scratch_release(hipStream_t s) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    for (auto it = g_scratch.begin(); it != g_scratch.end();) {
+        if (it->first.s == s) {
+            arena_free(it->second.ptr);
+            it = g_scratch.erase(it);
+        } else {
+            ++it;
+        }
+    }
 }
 
 }  // namespace hx
@@ -181,7 +296,11 @@ HX_API int hx_stream_create(hx_stream* s) {
     });
 }
 HX_API int hx_stream_destroy(hx_stream s) {
-    return guard("hx_stream_destroy", [&] { HX_HIP(hipStreamDestroy((hipStream_t)s)); });
+    return guard("hx_stream_destroy", [&] {
+        HX_HIP(hipStreamSynchronize((hipStream_t)s));   // its queued work may still read its scratch
+        stream_scratch_release((hipStream_t)s);
+        HX_HIP(hipStreamDestroy((hipStream_t)s));
+    });
 }
 HX_API int hx_stream_sync(hx_stream s) {
     return guard("hx_stream_sync", [&] { HX_HIP(hipStreamSynchronize((hipStream_t)s)); });
@@ -239,19 +358,34 @@ HX_API int hx_malloc(void** ptr, size_t bytes) {
         HX_CHECK(ptr != nullptr, "ptr is NULL");
         *ptr = nullptr;
         if (bytes == 0) return;
-        if (hipMalloc(ptr, bytes) != hipSuccess) {
-            (void)hipGetLastError();
-            throw Error{HX_E_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed"};
-        }
+        *ptr = arena_alloc(bytes);
         stat_add(ST_MALLOCS, 1);
     });
 }
 HX_API int hx_free(void* ptr) {
     return guard("hx_free", [&] {
         if (ptr) {
-            HX_HIP(hipFree(ptr));
+            arena_free(ptr);
             stat_add(ST_FREES, 1);
         }
+    });
+}
+HX_API int hx_arena_stats_get(hx_arena_stats* out) {
+    return guard("hx_arena_stats_get", [&] {
+        HX_CHECK(out != nullptr, "out is NULL");
+        Arena& a = arena();
+        std::lock_guard<std::mutex> lk(a.mu);
+        *out = a.st;
+    });
+}
+HX_API int hx_arena_trim(size_t* freed) {
+    return guard("hx_arena_trim", [&] {
+        HX_CHECK(!stream_capturing(nullptr), "hx_arena_trim: not legal during a capture of the null stream");
+        Arena& a = arena();
+        std::lock_guard<std::mutex> lk(a.mu);
+        HX_HIP(hipDeviceSynchronize());
+        const size_t f = arena_trim_locked(a);
+        if (freed) *freed = f;
     });
 }
 HX_API int hx_upload(void* dev, const void* host, size_t bytes, hx_stream s) {

@@ -187,6 +187,25 @@ def reset_launch_stats() -> None:
     check(lib().hx_stats_reset())
 
 
+def arena_stats() -> dict:
+    """The device arena's counters (ArenaStats, CudaCtx.h:107-117)."""
+    st = _lib.hx_arena_stats()
+    check(lib().hx_arena_stats_get(C.byref(st)))
+    return {name: int(getattr(st, name)) for name, _ in st._fields_}
+
+
+def arena_trim() -> int:
+    """release_cached (CudaCtx.h:157-168): cached blocks back to the driver; returns the bytes freed."""
+    freed = C.c_size_t(0)
+    check(lib().hx_arena_trim(C.byref(freed)))
+    return int(freed.value)
+
+
+def limb_k_chunk(ctx: ModCtx, m: int, n: int, k: int, workspace_bytes: int) -> int:
+    """The K chunk a limb workspace of this size runs (imma_k_chunk, CudaKernels.h:100-110); 0 = none."""
+    return int(lib().hx_gemm_limb_k_chunk(C.byref(ctx.c), m, n, k, workspace_bytes))
+
+
 def schur_update(ctx: ModCtx, L, U, C_, stream=None, variant: int = HX_GEMM_AUTO) -> None:
     """C -= L * U (engine::cuda::schur_update, CudaKernels.h:130-133)."""
     gemm(ctx, L, U, C_, GemmOp.Sub, stream, variant)
@@ -268,7 +287,8 @@ class RrefBatch:
 
     The job table is uploaded once (not capturable); run() is capturable."""
 
-    def __init__(self, ctx: ModCtx, mats: list[torch.Tensor], panels: list[int] | None = None):
+    def __init__(self, ctx: ModCtx, mats: list[torch.Tensor], panels: list[int] | None = None,
+                 mode: int = _lib.HX_RREF_BATCH_AUTO):
         self.ctx = ctx
         self.mats = mats
         n = len(mats)
@@ -292,7 +312,13 @@ class RrefBatch:
             self.offsets.append((off, mn))
             off += 2 * mn + 1
         self.h = C.c_void_p()
-        check(lib().hx_rref_batch_create(C.byref(ctx.c), jobs, n, 0, C.byref(self.h)))
+        check(lib().hx_rref_batch_create(C.byref(ctx.c), jobs, n, mode, C.byref(self.h)))
+
+    def info(self) -> dict:
+        """The launch shape create chose (workgroups per CU, grid, LDS, jobs reading X's planes from HBM)."""
+        inf = _lib.hx_rref_batch_info()
+        check(lib().hx_rref_batch_info_get(self.h, C.byref(inf)))
+        return {name: int(getattr(inf, name)) for name, _ in inf._fields_}
 
     def run(self, stream=None) -> None:
         check(lib().hx_rref_batch_run(self.h, current_stream() if stream is None else stream))

@@ -82,7 +82,12 @@ typedef struct hx_gemm_opts {   /* GemmOpts, CudaKernels.h:64-79 */
                                1 = register-staged LDS copy, 2 = direct-to-LDS copy ring,
                                3 = the ring with LDS reads pipelined against the MFMAs, 4 = the ring on eight
                                waves, one 32 x 32 block each (the default) */
-    void *workspace;        /* limb route: >= hx_gemm_workspace_bytes(); NULL = library-owned (not capturable) */
+    void *workspace;        /* the limb route's planes (the reference's imma_workspace, CudaKernels.h:72-75):
+                               hx_gemm_workspace_bytes() runs K in one piece, less (at least the bytes of one
+                               64-deep K chunk, hx_gemm_limb_k_chunk) runs K in chunks, same bytes; smaller: AUTO
+                               runs the plain kernel, LIMB fails (HX_E_ARGUMENT).  NULL = library-owned planes
+                               private to the stream (grown outside a capture only: inside one, AUTO falls back
+                               to the plain kernel when they are too small and LIMB fails with HX_E_CAPTURE) */
     size_t workspace_bytes;
     hx_event main_begin;    /* optional timing events recorded on the stream right before / after the */
     hx_event main_end;      /* main product kernel (instrumentation only; NULL = none) */
@@ -119,8 +124,28 @@ HX_API int hx_event_wait(hx_stream s, hx_event e);
 HX_API int hx_event_sync(hx_event e);
 HX_API int hx_event_elapsed_ms(hx_event a, hx_event b, float *ms);
 HX_API int hx_event_destroy(hx_event e);
+/* Device memory through the library's arena (CudaCtx::dalloc / dfree / release_cached,
+ * CudaCtx.h:104-168): blocks are cached in exact-size free lists (sizes rounded up to 512 B below
+ * 1 MiB, 2 MiB above) and reused by later requests of the same rounded size; a failed device
+ * allocation is answered by a trim and one retry.  hx_free returns the block to the cache at once:
+ * no queued work may still use it (stream-ordered reuse is the caller's, as with dfree).  None of
+ * these is legal inside a stream capture. */
 HX_API int hx_malloc(void **ptr, size_t bytes);        /* dalloc */
-HX_API int hx_free(void *ptr);                          /* dfree */
+HX_API int hx_free(void *ptr);                          /* dfree: HX_E_ARGUMENT for a pointer not from hx_malloc */
+typedef struct hx_arena_stats {   /* ArenaStats, CudaCtx.h:107-117 */
+    size_t reserved_bytes;        /* held from the driver (live + cached) */
+    size_t live_bytes;            /* handed out and not yet freed (rounded sizes) */
+    size_t peak_live_bytes;
+    size_t peak_reserved_bytes;
+    uint64_t driver_mallocs;      /* device allocations made */
+    uint64_t reuses;              /* requests served from a free list */
+    uint64_t driver_frees;        /* cached blocks returned to the driver by trims */
+    uint64_t oom_retries;         /* failed allocations answered by a trim and a retry */
+} hx_arena_stats;
+HX_API int hx_arena_stats_get(hx_arena_stats *out);
+/* release_cached (CudaCtx.h:157-168): waits for the device, returns every cached (not live) block
+ * to the driver; *freed (may be NULL) receives the bytes returned. */
+HX_API int hx_arena_trim(size_t *freed);
 HX_API int hx_upload(void *dev, const void *host, size_t bytes, hx_stream s);     /* synchronous w.r.t. host memory */
 HX_API int hx_download(void *host, const void *dev, size_t bytes, hx_stream s);   /* returns when the bytes are in host memory */
 HX_API int hx_copy_d2d(void *dst, const void *src, size_t bytes, hx_stream s);
@@ -151,8 +176,12 @@ HX_API int hx_graph_destroy(hx_graph g);
  * variant PLAIN: any p; LIMB: p with ctx->limbs > 0 (p <= 4278124287), else HX_E_ARGUMENT. */
 HX_API int hx_gemm_u32(const hx_modctx *ctx, hx_view A, hx_view B, hx_view C, int op, hx_stream s,
                        const hx_gemm_opts *opts);
-/* Device workspace bytes the limb route needs for these shapes (0 for PLAIN). */
+/* Device workspace bytes the limb route needs for these shapes in one K piece (0 for PLAIN). */
 HX_API size_t hx_gemm_workspace_bytes(const hx_modctx *ctx, uint32_t m, uint32_t n, uint32_t k, int variant);
+/* The K chunk a limb workspace of workspace_bytes runs (imma_k_chunk, CudaKernels.h:100-110): min(k, the
+ * int32-exact chunk) when its planes fit, else the largest multiple of 64 that fits; 0 when not even a
+ * 64-deep chunk fits.  Host-only arithmetic (no device needed). */
+HX_API uint32_t hx_gemm_limb_k_chunk(const hx_modctx *ctx, uint32_t m, uint32_t n, uint32_t k, size_t workspace_bytes);
 /* C -= L * U (schur_update, CudaKernels.h:130-133). */
 HX_API int hx_schur_update_u32(const hx_modctx *ctx, hx_view L, hx_view U, hx_view C, hx_stream s,
                                const hx_gemm_opts *opts);
@@ -246,8 +275,23 @@ typedef struct hx_rref_job {
     uint32_t panel;         /* tuning: widest column panel (8, 16 or 32; 0 = 32); never changes a value */
 } hx_rref_job;
 typedef void *hx_rref_batch;
-HX_API int hx_rref_batch_create(const hx_modctx *ctx, const hx_rref_job *jobs, uint32_t njobs, int variant,
+/* Launch shape of the persistent batch (tuning only; never changes a value).  AUTO: two workgroups
+ * per CU, in half the LDS each, when every job fits there and the batch's total work is at least
+ * 512x its largest job's; otherwise one.  The other two force a shape (TWO_PER_CU fails with
+ * HX_E_ARGUMENT when a job does not fit half the LDS). */
+enum { HX_RREF_BATCH_AUTO = 0, HX_RREF_BATCH_ONE_PER_CU = 1, HX_RREF_BATCH_TWO_PER_CU = 2 };
+HX_API int hx_rref_batch_create(const hx_modctx *ctx, const hx_rref_job *jobs, uint32_t njobs, int mode,
                                 hx_rref_batch *out);
+/* What create chose (for tests and profiles). */
+typedef struct hx_rref_batch_info {
+    uint32_t njobs;
+    uint32_t workgroups_per_cu;       /* 1 or 2 */
+    uint32_t grid;                    /* persistent workgroups launched */
+    uint32_t region_words;            /* LDS words per workgroup for the panel / X region */
+    size_t lds_bytes;                 /* dynamic LDS per workgroup */
+    uint32_t jobs_planes_outside_lds; /* jobs whose first panel reads X's limb planes from HBM (region too small) */
+} hx_rref_batch_info;
+HX_API int hx_rref_batch_info_get(hx_rref_batch b, hx_rref_batch_info *out);
 /* One persistent launch over every job (capturable once created). */
 HX_API int hx_rref_batch_run(hx_rref_batch b, hx_stream s);
 HX_API int hx_rref_batch_destroy(hx_rref_batch b);

@@ -97,12 +97,23 @@ HXO_API uint32_t hxo_inv(uint32_t a, uint32_t p) {
 /* FNV-1a-64 over (rows, cols, entries row-major) fed as u64 words, least   */
 /* significant byte first: Fnv.h:49-55 (FnvU64) and                         */
 /* PerformanceSmokeValidation.cpp:117-125 (feed_buf).                       */
-/* Pinned by: the FNV-1a published vector for the empty input (offset       */
-/* basis) and reference-written fixtures.                                   */
+/* Pinned by: the published FNV-1a vectors (through hxo_fnv1a_bytes over   */
+/* the same bytes), the reference's engine::FnvHasher on fixed word lists   */
+/* and reference-written fixtures.                                          */
 /* ------------------------------------------------------------------------ */
 static inline uint64_t fnv_u64(uint64_t h, uint64_t v) {
     for (int i = 0; i < 8; ++i) {
         h ^= (v >> (8 * i)) & 0xffu;
+        h *= 0x100000001b3ull;
+    }
+    return h;
+}
+/* FNV-1a-64 of a byte string (the byte step of Fnv.h:58-64).  Pinned by the published FNV-1a
+ * test vectors and by the reference's FnvBytes (a fixture written with oracle/_ref). */
+HXO_API uint64_t hxo_fnv1a_bytes(const uint8_t *b, size_t n) {
+    uint64_t h = 0xcbf29ce484222325ull;
+    for (size_t i = 0; i < n; ++i) {
+        h ^= b[i];
         h *= 0x100000001b3ull;
     }
     return h;
@@ -351,6 +362,15 @@ static inline uint32_t shoup_mul(uint32_t a, uint32_t w, uint32_t ws, uint32_t p
     return r >= p ? r - p : r;
 }
 
+/* row[c:n] -= f * prow[c:n] mod p, p <= 2^31 (Shoup multiplication by the fixed f; fs =
+ * shoup_pre(f, p)).  Built for AVX-512F / AVX2 / baseline x86-64, picked at run time: the
+ * arithmetic is the same scalar formula in every clone. */
+__attribute__((target_clones("avx512f", "avx2", "default")))
+static void row_sub_shoup(uint32_t *restrict row, const uint32_t *restrict prow, uint32_t f, uint32_t fs,
+                          uint32_t p, uint32_t c, uint32_t n) {
+    for (uint32_t j = c; j < n; ++j) row[j] = msub(row[j], shoup_mul(prow[j], f, fs, p), p);
+}
+
 static void rule_worker(int tid, int T, void *va) {
     struct rule_arg *g = (struct rule_arg *)va;
     const uint32_t p = g->p, m = g->m, n = g->n;
@@ -385,8 +405,7 @@ static void rule_worker(int tid, int T, void *va) {
             if (row[c] == 0) continue;
             const uint32_t f = mmul(row[c], inv, p);
             if (shoup) {
-                const uint32_t fs = shoup_pre(f, p);
-                for (uint32_t j = c; j < n; ++j) row[j] = msub(row[j], shoup_mul(prow[j], f, fs, p), p);
+                row_sub_shoup(row, prow, f, shoup_pre(f, p), p, c, n);
             } else {
                 for (uint32_t j = c; j < n; ++j) row[j] = msub(row[j], mmul(f, prow[j], p), p);
             }
@@ -428,7 +447,10 @@ HXO_API uint32_t hxo_rref_rule(uint32_t p, uint32_t m, uint32_t n, const uint32_
                 uint32_t *rs = R + (size_t)s * ldr;
                 const uint32_t f = rs[c];
                 if (f == 0) continue;
-                for (uint32_t j = c; j < n; ++j) rs[j] = msub(rs[j], mmul(f, rt[j], p), p);
+                if (p <= 0x80000000u)
+                    row_sub_shoup(rs, rt, f, shoup_pre(f, p), p, c, n);
+                else
+                    for (uint32_t j = c; j < n; ++j) rs[j] = msub(rs[j], mmul(f, rt[j], p), p);
             }
         }
     }

@@ -64,6 +64,8 @@ def lib():
         L.hxo_reduce_u64.argtypes = [C.c_uint32, C.c_uint64]
         L.hxo_inv.restype = C.c_uint32
         L.hxo_inv.argtypes = [C.c_uint32, C.c_uint32]
+        L.hxo_fnv1a_bytes.restype = C.c_uint64
+        L.hxo_fnv1a_bytes.argtypes = [C.c_char_p, C.c_size_t]
         L.hxo_fnv_matrix.restype = C.c_uint64
         L.hxo_fnv_matrix.argtypes = [u32p, C.c_uint32, C.c_uint32, C.c_size_t]
         L.hxo_elim_work.restype = C.c_double
@@ -113,6 +115,10 @@ def ref():
         R.hxr_ring2k_howell.restype = C.c_int64
         R.hxr_ring2k_howell.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, u32p, C.c_int64, C.c_int,
                                         u32p, u32p, u32p]
+        R.hxr_fnv_words.restype = C.c_uint64
+        R.hxr_fnv_words.argtypes = [C.c_void_p, C.c_size_t]
+        R.hxr_fnv_bytes.restype = C.c_uint64
+        R.hxr_fnv_bytes.argtypes = [C.c_char_p, C.c_size_t]
         for name in ("hxr_ref_howell", "hxr_elim_howell"):
             f = getattr(R, name)
             f.restype = C.c_int64
@@ -160,6 +166,22 @@ def fnv_matrix(M: np.ndarray) -> int:
     M = np.ascontiguousarray(M, dtype=np.uint32)
     r, c = (M.shape + (1,))[:2] if M.ndim == 2 else (M.shape[0], 1)
     return lib().hxo_fnv_matrix(M, r, c, c)
+
+
+def fnv1a_bytes(b: bytes) -> int:
+    return int(lib().hxo_fnv1a_bytes(b, len(b)))
+
+
+def ref_fnv_words(words: np.ndarray) -> int:
+    """engine::FnvHasher over 64-bit words (oracle/_ref)."""
+    w = np.ascontiguousarray(words, dtype=np.uint64)
+    R = ref()
+    return int(R.hxr_fnv_words(w.ctypes.data_as(C.c_void_p), len(w)))
+
+
+def ref_fnv_bytes(b: bytes) -> int:
+    """engine::FnvBytes from the offset basis (oracle/_ref)."""
+    return int(ref().hxr_fnv_bytes(b, len(b)))
 
 
 def elim_work(rows: int, cols: int, rank: int) -> float:

@@ -16,6 +16,7 @@
 #include "DenseBuf.h"
 #include "DenseElim.h"
 #include "DenseView.h"
+#include "Fnv.h"
 #include "Kernels.h"
 #include "ModCtx.h"
 #include "RefLinAlg.h"
@@ -37,6 +38,16 @@ DenseBuf<uint32_t> to_buf(const uint32_t* A, uint32_t m, uint32_t n, int64_t lda
 }  // namespace
 
 extern "C" {
+
+// engine::FnvHasher (Fnv.h:76-83): feed() each 64-bit word in order from the offset basis.
+uint64_t hxr_fnv_words(const uint64_t* w, size_t n) {
+    engine::FnvHasher h;
+    for (size_t i = 0; i < n; ++i) h.feed(w[i]);
+    return h.value();
+}
+
+// engine::FnvBytes (Fnv.h:58-64) from the offset basis over n bytes.
+uint64_t hxr_fnv_bytes(const char* b, size_t n) { return engine::FnvBytes(engine::kFnvOffset64, std::string(b, n)); }
 
 // engine::kernels::gemm<uint32_t> (Kernels.cpp:1417-1483), split over row tiles of C on
 // `threads` host threads as the tape executor splits output tiles (Engine/Executor.cpp:69-100).

@@ -57,3 +57,36 @@ def test_workload_generators_match_oracle(O):
     nz = J[J != 0]
     assert set(np.unique(nz)) <= {1, 2, W.P_C4 - 1, W.P_C4 - 2}
     assert 0.02 < (J != 0).mean() < 0.09
+
+
+def test_limb_k_chunk_contract():
+    """hx_gemm_limb_k_chunk (host arithmetic, imma_k_chunk's role): the chosen K chunk's limb planes
+    fit the workspace, the next 64 more do not (maximality), it is a multiple of 64 unless it is the
+    whole of K, never above the int32-exact chunk, 0 exactly when one 64-deep chunk does not fit, and
+    the full workspace runs K in one piece."""
+    from hacs_amd import engine as E
+    for p in (65521, 16777213, 2147483629):
+        ctx = E.ModCtx(p)
+        L, kmax = ctx.limbs, 2147483647 // (ctx.limbs * 16384) // 64 * 64
+        for (m, n, k) in [(1, 1, 1), (300, 260, 1000), (8192, 8192, 8192), (64, 128, 40000), (5, 700, 63)]:
+            per_k = L * (-(-m // 64) * 64 + -(-n // 128) * 128)
+            full = E.gemm_workspace_bytes(ctx, m, n, k)
+            assert E.limb_k_chunk(ctx, m, n, k, full) == min(k, kmax)
+            for nbytes in [0, per_k * 64 - 1, per_k * 64, per_k * 200, per_k * 1000 + 5, full - 1, full, 1 << 40]:
+                kc = E.limb_k_chunk(ctx, m, n, k, nbytes)
+                if kc == 0:
+                    assert nbytes < per_k * 64 and nbytes < full
+                    continue
+                assert kc <= min(k, kmax)
+                assert kc == min(k, kmax) or kc % 64 == 0
+                assert per_k * (-(-kc // 64) * 64) <= nbytes, (p, m, n, k, nbytes, kc)
+                if kc < min(k, kmax):
+                    assert per_k * (kc + 64) > nbytes
+
+
+def test_arena_stats_without_device():
+    """The arena's counters are readable without a device (all zero until something is allocated)."""
+    from hacs_amd import engine as E
+    st = E.arena_stats()
+    assert set(st) >= {"reserved_bytes", "live_bytes", "driver_mallocs", "reuses", "oom_retries"}
+    assert st["live_bytes"] <= st["reserved_bytes"]

@@ -206,3 +206,33 @@ def test_level1_oracle_negative_control(O):
     planted = O.gather(p, 5, 0, None, 0, None, c)
     assert not np.array_equal(good, planted)
     assert np.array_equal(good, (np.eye(4, 4, 1, dtype=np.uint32) * 5))
+
+
+def test_fnv_published_vectors_and_word_order(O):
+    """FNV-1a-64's published vectors pin the byte step (offset basis, prime, xor-then-multiply); the
+    matrix checksum equals that byte step over (rows, cols, entries) as little-endian 64-bit words,
+    which pins the word width and byte order; FNV-1 (multiply-then-xor) and big-endian words differ."""
+    assert O.fnv1a_bytes(b"") == 0xCBF29CE484222325
+    assert O.fnv1a_bytes(b"a") == 0xAF63DC4C8601EC8C
+    assert O.fnv1a_bytes(b"foobar") == 0x85944171F73967E8
+    M = O.engine_matrix(5, 9, 2147483629, 0x07)
+    words = np.concatenate([np.array([5, 9], np.uint64), M.astype(np.uint64).ravel()])
+    assert O.fnv_matrix(M) == O.fnv1a_bytes(words.astype("<u8").tobytes())
+    assert O.fnv_matrix(M) != O.fnv1a_bytes(words.astype(">u8").tobytes())
+    h = 0xCBF29CE484222325
+    for byte in words.astype("<u8").tobytes():      # FNV-1: multiply, then xor
+        h = ((h * 0x100000001B3) & ((1 << 64) - 1)) ^ byte
+    assert O.fnv_matrix(M) != h
+
+
+def test_fnv_against_reference_hasher(O):
+    """The oracle's checksum equals the reference's own engine::FnvHasher / FnvBytes on the fixture
+    written with oracle/_ref (tests/golden/make_fnv_golden.py)."""
+    import os
+    G = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "fnv_reference.npz"),
+                allow_pickle=False)
+    M = G["matrix"]
+    assert O.fnv_matrix(M) == int(G["words_fnv"][0])
+    assert np.array_equal(G["words"][2:], M.astype(np.uint64).ravel())
+    for s, n, want in zip(G["strings"], G["string_lens"], G["strings_fnv"]):
+        assert O.fnv1a_bytes(bytes(s)[:int(n)]) == int(want)

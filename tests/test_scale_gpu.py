@@ -1,0 +1,237 @@
+"""Parity at the benchmarked sizes and launch shapes: every job of the batched workloads exactly as
+the benchmark builds them, in the modes the benchmark measures, against the CPU oracle.
+
+* the 4096-job sparse echelon batch mod 1000003 in the launch the automatic rule picks for it (one
+  workgroup per CU: its total work is 134x its largest job's, below the 512x the two-per-CU launch
+  needs), eager and after hipGraph replay, and forced into the two-per-CU launch;
+* the mixed stream's 6667 echelon jobs (n up to 1024, so the X limb planes of the widest panels
+  leave LDS) with the static panel choice and with a mix of panel widths;
+* the stream's 6666 Howell jobs over Z/2^16;
+* a chunk of the stream's GEMM jobs (n log-uniform in [64, 4096]) with per-job variants;
+* the full 8192^3 product mod 2147483629 (every limb kernel and the plain kernel) against the
+  checksum of the oracle's product, itself equal to the reference's kernels::gemm
+  (tests/golden/c3_reference.npz).
+
+The oracle runs on the host's cores (one job per thread), as the checker, never as the product."""
+import os
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+GOLD_C3 = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c3_reference.npz")
+WORKERS = 16
+
+
+def _pool():
+    return ThreadPoolExecutor(max_workers=min(WORKERS, os.cpu_count() or 1))
+
+
+def _oracle_rref(O, p, inputs):
+    """The oracle's smallest-row-rule RREF of every job (rank, pivots, pivot_rows, RREF rows)."""
+    with _pool() as ex:
+        return list(ex.map(lambda A: O.rref_rule(p, A, True, 1), inputs))
+
+
+def _compare_rref(want, outs, results, tag):
+    bad = []
+    for i, ((rank, piv, prow, rows), R, res) in enumerate(zip(want, outs, results)):
+        ok = (res.rank == rank and np.array_equal(res.pivots, piv) and np.array_equal(res.pivot_rows, prow)
+              and np.array_equal(R[:rank], rows) and not R[rank:].any())
+        if not ok:
+            bad.append(i)
+    assert not bad, f"{tag}: {len(bad)} jobs differ from the oracle, first {bad[:5]} " \
+                    f"(shapes {[outs[i].shape for i in bad[:5]]})"
+
+
+def _check_rref_jobs(O, p, inputs, outs, results, tag):
+    """Every job against the oracle: rank, pivots, pivot_rows, RREF rows and the zero rows below."""
+    _compare_rref(_oracle_rref(O, p, inputs), outs, results, tag)
+
+
+def test_c4_full_batch_eager_and_graph(hx_gpu, O):
+    """The benchmark's 4096-job batch (workloads.c4_shapes, sparse_job, p = 1000003) in the automatic
+    launch (one workgroup per CU for these jobs); every job equals the oracle, eager and after three
+    graph replays."""
+    import ctypes as C
+    from hacs_amd import engine as E
+    from hacs_amd import workloads as W
+    from hacs_amd import _lib
+    p = W.P_C4
+    ctx = E.ModCtx(p)
+    shapes = W.c4_shapes(4096)
+    inputs = [W.sparse_job(j, int(m), int(n), p) for j, (m, n) in enumerate(shapes)]
+    src = [E.to_device(A) for A in inputs]
+    work = [s.clone() for s in src]
+    b = E.RrefBatch(ctx, work)
+    info = b.info()
+    print(f"C4 batch launch: {info}", flush=True)
+    assert info["workgroups_per_cu"] == 1 and info["njobs"] == 4096, info
+    b.run()
+    b.status()
+    torch.cuda.synchronize()
+    eager = [E.from_device(w) for w in work]
+    res_eager = b.results()
+    _check_rref_jobs(O, p, inputs, eager, res_eager, "C4 eager")
+    s = torch.cuda.Stream()
+    L = _lib.lib()
+    with torch.cuda.stream(s):
+        h = s.cuda_stream
+        g = C.c_void_p()
+        _lib.check(L.hx_graph_begin(h))
+        b.run(stream=h)
+        _lib.check(L.hx_graph_end(h, C.byref(g)))
+        for rep in range(3):
+            for w_, s_ in zip(work, src):
+                w_.copy_(s_)
+            s.synchronize()
+            _lib.check(L.hx_graph_launch(g, h))
+            s.synchronize()
+            res = b.results()
+            for j in range(len(work)):
+                assert res[j].rank == res_eager[j].rank and np.array_equal(res[j].pivots, res_eager[j].pivots) and \
+                    np.array_equal(res[j].pivot_rows, res_eager[j].pivot_rows), f"replay {rep} job {j}: pivots"
+                assert np.array_equal(E.from_device(work[j]), eager[j]), f"replay {rep} job {j}: RREF"
+        _lib.check(L.hx_graph_destroy(g))
+
+
+def test_c4_batch_two_per_cu_matches(hx_gpu, O):
+    """The same 4096 jobs forced into the two-workgroups-per-CU launch (half the LDS each) equal the oracle."""
+    from hacs_amd import engine as E
+    from hacs_amd import workloads as W
+    from hacs_amd import _lib
+    p = W.P_C4
+    ctx = E.ModCtx(p)
+    shapes = W.c4_shapes(4096)
+    inputs = [W.sparse_job(j, int(m), int(n), p) for j, (m, n) in enumerate(shapes)]
+    work = [E.to_device(A) for A in inputs]
+    b = E.RrefBatch(ctx, work, mode=_lib.HX_RREF_BATCH_TWO_PER_CU)
+    assert b.info()["workgroups_per_cu"] == 2
+    b.run()
+    b.status()
+    torch.cuda.synchronize()
+    _check_rref_jobs(O, p, inputs, [E.from_device(w) for w in work], b.results(), "C4 two per CU")
+
+
+def _c5_echelon_jobs():
+    """The mixed stream's echelon jobs as the benchmark runs them (autotune.Job with salt 0x90:
+    workloads.sparse_job(j, n, n, 1000003, 0x93) generated in HBM)."""
+    from hacs_amd import workloads as W
+    stream = W.c5_stream(20000, salt=0x51)
+    return [(j, n) for j, (kind, n) in enumerate(stream) if kind == "echelon"]
+
+
+def test_c5_echelon_batch_all_jobs(hx_gpu, O):
+    """All 6667 echelon jobs of the stream in one batch: the automatic launch is two workgroups per CU
+    with the widest jobs' X limb planes read from HBM; every job equals the oracle with the static panel
+    (32) and with panels 8 / 16 / 32 mixed across the jobs."""
+    from hacs_amd import engine as E
+    from hacs_amd import workloads as W
+    p = W.P_C4
+    ctx = E.ModCtx(p)
+    jobs = _c5_echelon_jobs()
+    assert len(jobs) == 6667
+    src = [E.random_sparse(n, n, p, W.seed_for(n, n, p, 0x93) ^ j) for j, n in jobs]
+    torch.cuda.synchronize()
+    inputs = [E.from_device(s_) for s_ in src]
+    j0, n0 = jobs[0]
+    assert np.array_equal(inputs[0], W.sparse_job(j0, n0, n0, p, 0x93)), "device generator != workloads.sparse_job"
+    want = _oracle_rref(O, p, inputs)
+    for panels, tag in ((None, "static panel 32"), ([(8, 16, 32)[i % 3] for i in range(len(jobs))], "mixed panels")):
+        work = [s_.clone() for s_ in src]
+        b = E.RrefBatch(ctx, work, panels=panels)
+        info = b.info()
+        print(f"C5 echelon batch ({tag}): {info}", flush=True)
+        assert info["workgroups_per_cu"] == 2 and info["jobs_planes_outside_lds"] > 0, info
+        b.run()
+        b.status()
+        torch.cuda.synchronize()
+        _compare_rref(want, [E.from_device(w) for w in work], b.results(), f"C5 echelon ({tag})")
+        del work, b
+        torch.cuda.empty_cache()
+
+
+def test_c5_howell_batch_all_jobs(hx_gpu, O):
+    """The benchmark's 6666 Howell jobs over Z/2^16 (dense [0, 2^16), n log-uniform in [32, 512]) in
+    one batch; every form equals the oracle's."""
+    from hacs_amd import engine as E
+    from hacs_amd import workloads as W
+    k = 16
+    shapes = [n for kind, n in W.c5_stream(3 * 6666 + 3, salt=0x51) if kind == "howell"][:6666]
+    seeds = [W.seed_for(n, n, 1 << k, 0x94) ^ j for j, n in enumerate(shapes)]
+    mats = [E.random_entries(n, n, 1 << k, sd) for n, sd in zip(shapes, seeds)]
+    b = E.HowellBatch(k, mats)
+    b.run()
+    torch.cuda.synchronize()
+    got = b.results()
+    inputs = [E.from_device(M) for M in mats]
+
+    def one(i):
+        H, piv, pv = O.howell_fast(k, inputs[i])
+        g = got[i]
+        ok = np.array_equal(g.H, H) and np.array_equal(g.pivots, piv) and np.array_equal(g.pivot_v, pv)
+        return None if ok else i
+    with _pool() as ex:
+        bad = [i for i in ex.map(one, range(len(inputs))) if i is not None]
+    assert not bad, f"{len(bad)} Howell jobs differ, first {bad[:5]}"
+
+
+def test_c5_gemm_batch_chunk(hx_gpu, O):
+    """The first 240 GEMM jobs of the stream (mod 2147483629, n log-uniform in [64, 4096]) in one
+    batch with per-job variants (plain and limb alternating): jobs up to n = 768 are checked whole;
+    larger ones on 24 random rows and 24 random columns, each an exact product of its own."""
+    from hacs_amd import engine as E
+    from hacs_amd import workloads as W
+    p = W.P_C3
+    ctx = E.ModCtx(p)
+    stream = W.c5_stream(20000, salt=0x51)
+    gj = [(j, n) for j, (kind, n) in enumerate(stream) if kind == "gemm"][:240]
+    assert max(n for _, n in gj) >= 2048, "the chunk should hold large jobs"
+    jobs = []
+    for i, (j, n) in enumerate(gj):
+        A = E.random_entries(n, n, p, W.seed_for(n, n, p, 0x91) ^ j)
+        B = E.random_entries(n, n, p, W.seed_for(n, n, p, 0x92) ^ j)
+        Cm = torch.empty((n, n), dtype=torch.int32, device="cuda")
+        jobs.append((A, B, Cm, E.GemmOp.Set, (E.HX_GEMM_PLAIN, E.HX_GEMM_LIMB)[i % 2]))
+    b = E.GemmBatch(ctx, jobs)
+    b.run()
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(3)
+    for (j, n), (A, B, Cm, _, v) in zip(gj, jobs):
+        Ah, Bh, Ch = E.from_device(A), E.from_device(B), E.from_device(Cm)
+        if n <= 768:
+            assert np.array_equal(Ch, O.gemm_fast(p, Ah, Bh, None, 0, WORKERS)), f"job {j} n={n} variant {v}"
+            continue
+        rows = rng.choice(n, 24, replace=False)
+        cols = rng.choice(n, 24, replace=False)
+        assert np.array_equal(Ch[rows], O.gemm_fast(p, np.ascontiguousarray(Ah[rows]), Bh, None, 0, WORKERS)), \
+            f"job {j} n={n} variant {v}: rows"
+        assert np.array_equal(Ch[:, cols], O.gemm_fast(p, Ah, np.ascontiguousarray(Bh[:, cols]), None, 0, WORKERS)), \
+            f"job {j} n={n} variant {v}: columns"
+
+
+def test_c3_full_product_checksum(hx_gpu, O):
+    """C = A B mod 2147483629 at m = n = k = 8192 from the device generator: the FNV-1a-64 checksum of
+    every limb kernel's C and of the plain kernel's C equals the fixture (the oracle's product, equal
+    to the reference's kernels::gemm), and so do three full rows."""
+    from hacs_amd import engine as E
+    from hacs_amd import workloads as W
+    G = np.load(GOLD_C3, allow_pickle=False)
+    p, n = int(G["p"][0]), int(G["n"][0])
+    ctx = E.ModCtx(p)
+    A = E.random_entries(n, n, p, W.seed_for(n, n, p, 0x01))
+    B = E.random_entries(n, n, p, W.seed_for(n, n, p, 0x02))
+    torch.cuda.synchronize()
+    assert O.fnv_matrix(E.from_device(A)) == int(G["a_fnv"][0]) and O.fnv_matrix(E.from_device(B)) == int(G["b_fnv"][0])
+    Cm = torch.empty((n, n), dtype=torch.int32, device="cuda")
+    rows = G["rows"].astype(np.int64)
+    for variant, tile in ((E.HX_GEMM_LIMB, 1), (E.HX_GEMM_LIMB, 2), (E.HX_GEMM_LIMB, 3), (E.HX_GEMM_LIMB, 4),
+                          (E.HX_GEMM_PLAIN, 0)):
+        Cm.fill_(0)
+        E.gemm(ctx, A, B, Cm, E.GemmOp.Set, variant=variant, tile=tile)
+        torch.cuda.synchronize()
+        Ch = E.from_device(Cm)
+        assert np.array_equal(Ch[rows], G["c_rows"]), f"variant {variant} tile {tile}: rows"
+        assert O.fnv_matrix(Ch) == int(G["c_fnv"][0]), f"variant {variant} tile {tile}: checksum"
