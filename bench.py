@@ -52,9 +52,10 @@ def parse():
     ap.add_argument("--mixed-jobs", type=int, default=20000)
     ap.add_argument("--no-howell", action="store_true", help="do not time the Howell batch")
     ap.add_argument("--howell-jobs", type=int, default=6666)
-    ap.add_argument("--no-ring2k", action="store_true", help="do not time the Z/2^32 LU device session")
+    ap.add_argument("--no-ring2k", action="store_true", help="do not time the Z/2^32 LU on the device session")
+    ap.add_argument("--no-u64", action="store_true", help="do not time the 64-bit-residue GEMM")
     ap.add_argument("--no-level1", action="store_true", help="do not time the HBM-bound level-1/2 kernels")
-    ap.add_argument("--only", choices=["batch", "lu", "mixed", "cpu", "howell", "ring2k", "level1"],
+    ap.add_argument("--only", choices=["batch", "lu", "mixed", "cpu", "howell", "ring2k", "level1", "u64"],
                     help="run one secondary workload alone (for profiling) and print its record")
     ap.add_argument("--pmc-mode", action="store_true",
                     help="for counter collection: one 1 GiB device copy (the traffic calibration), then one limb "
@@ -287,44 +288,67 @@ def howell_batch(E, njobs):
     return out
 
 
-def ring2k_session(E, m=8192, panel=64):
-    """The Z/2^32 LU device session (hx_ring2k_*, the reference's ring2k::device Ops) over an m x m
-    window: one step per panel of `panel` columns with `panel` pivots -- the factored panel and L
-    uploaded from host buffers, TRSM on the pivot rows, trailing wraparound GEMM on the device.
-    The step data are synthetic (the device work does not depend on the values; the tests check the
-    session's results); the rate includes the per-step host -> device copies of the panel and L."""
+def ring2k_lu(E, m=2048, panel=64, k=32):
+    """The blocked LU over Z/2^32 (the device route of ring2k::echelon_blocked, hx_ring2k_echelon): an
+    m x m matrix of uniform words with rows scaled by powers of two, the panels factored on the host,
+    swaps / TRSM / trailing products on the device with the next panel fetched by lookahead.  Reports
+    the seconds, the trailing wraparound multiply-adds per second and the route's phase split; the
+    oracle's serial restatement of the host code times the same LU on one core for reference (its
+    result is compared entrywise)."""
     import ctypes as C
     from hacs_amd import _lib
     L = _lib.lib()
     rng = np.random.default_rng(5)
-    A = rng.integers(0, 1 << 32, (m, m), dtype=np.uint64).astype(np.uint32)
-    Pn = rng.integers(0, 1 << 32, (m, panel), dtype=np.uint64).astype(np.uint32)
-    Lh = rng.integers(0, 1 << 32, (m, panel), dtype=np.uint64).astype(np.uint32)
-    sw = np.zeros(2, np.uint32)
-    h = C.c_void_p()
-    _lib.check(L.hx_ring2k_open(4, m, m, panel, C.byref(h)))
-    try:
-        out = np.zeros((1, 1), np.uint32)
-        best = 1e30
-        for rep in range(2):
-            _lib.check(L.hx_ring2k_put(h, A.ctypes.data, m, 0, m, 0, m))
-            _lib.check(L.hx_ring2k_get(h, out.ctypes.data, 1, 0, 1, 0, 1))        # synchronises
-            t0 = time.perf_counter()
-            for r in range(0, m, panel):
-                c1 = min(r + panel, m)
-                st = _lib.hx_ring2k_step_args(m=m, r=r, np=c1 - r, pc0=r, pc1=c1, panel=Pn.ctypes.data, ldp=panel,
-                                              swaps=sw.ctypes.data, nswaps=0, L=Lh.ctypes.data, ldl=panel,
-                                              u0=c1, la0=0, la1=0, w=0)
-                _lib.check(L.hx_ring2k_step(h, C.byref(st)))
-            _lib.check(L.hx_ring2k_get(h, out.ctypes.data, 1, m - 1, 1, m - 1, 1))
-            best = min(best, time.perf_counter() - t0)
-    finally:
-        L.hx_ring2k_close(h)
-    macs = float(sum((m - min(r + panel, m)) * (m - min(r + panel, m)) * min(panel, m - r) for r in range(0, m, panel)))
-    h2d = float(sum(2 * (m - r) * panel * 4 for r in range(0, m, panel)))
-    return {"m": m, "panel": panel, "seconds": best, "steps": (m + panel - 1) // panel,
-            "trailing_wraparound_macs": macs, "macs_per_s": macs / best, "h2d_bytes": h2d,
-            "note": "Z/2^32, synthetic step data, per-step host panel + L uploads included"}
+    A0 = rng.integers(0, 1 << 32, (m, m), dtype=np.uint64).astype(np.uint32)
+    A0[::3] = (A0[::3] << np.uint32(3)).astype(np.uint32)
+    piv = np.zeros(m, np.uint32)
+    pv = np.zeros(m, np.uint32)
+    rank = C.c_uint32(0)
+    best, A = 1e30, None
+    for rep in range(2):
+        A = A0.copy()
+        _lib.check(L.hx_ring2k_stats_reset())
+        t0 = time.perf_counter()
+        _lib.check(L.hx_ring2k_echelon(4, k, m, m, A.ctypes.data_as(C.c_void_p), m, panel, m, 0, 0, C.byref(rank),
+                                       piv.ctypes.data_as(_lib._U32P), pv.ctypes.data_as(_lib._U32P)))
+        best = min(best, time.perf_counter() - t0)
+    st = _lib.hx_ring2k_stats()
+    _lib.check(L.hx_ring2k_stats_get(C.byref(st)))
+    r = rank.value
+    macs = float(sum((m - min(c + panel, m)) * (m - min(c + panel, m)) * min(panel, m - c) for c in range(0, r, panel)))
+    out = {"m": m, "n": m, "panel": panel, "k": k, "seconds": best, "rank": r, "trailing_wraparound_macs": macs,
+           "macs_per_s": macs / best, "panels": int(st.panels), "lookaheads": int(st.lookaheads),
+           "phase_s": {f: getattr(st, "ns_" + f) * 1e-9 for f in ("open", "fetch", "panel", "step", "host", "close")}}
+    from oracle import oracle as O
+    t0 = time.perf_counter()
+    R_o, r_o, piv_o, pv_o = O.r2_echelon(k, A0, panel)
+    t_o = time.perf_counter() - t0
+    out["cpu_port"] = {"seconds": t_o, "cores": 1, "routine": "hxo_r2_echelon32 (the host code's serial route)",
+                       "device_equals_port": bool(r_o == r and np.array_equal(piv_o, piv[:r]) and
+                                                  np.array_equal(pv_o, pv[:r]) and np.array_equal(R_o, A)),
+                       "speedup": t_o / best}
+    return out
+
+
+def gemm_u64_leg(E, n=4096, p=2305843009213693951, reps=3):
+    """C = A B mod 2^61 - 1 on 64-bit residues (hx_gemm_u64), n^3 mod-p multiply-adds per second."""
+    import torch
+    from hacs_amd import workloads as W
+    ctx = E.ModCtx64(p)
+    rng = np.random.default_rng(9)
+    A = E.to_device64((rng.integers(0, 1 << 62, (n, n), dtype=np.uint64) % np.uint64(p)))
+    B = E.to_device64((rng.integers(0, 1 << 62, (n, n), dtype=np.uint64) % np.uint64(p)))
+    Cm = torch.empty((n, n), dtype=torch.int64, device="cuda")
+    E.gemm_u64(ctx, A, B, Cm)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        E.gemm_u64(ctx, A, B, Cm)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    return {"n": n, "modulus": p, "ms": ms, "macs_per_s": float(n) ** 3 / (ms * 1e-3)}
 
 
 def lu_c2(E, reps=3, cpu=True):
@@ -422,7 +446,9 @@ def mixed_stream(njobs):
             jb.free()
         torch.cuda.empty_cache()
     counts = {k: kinds.count(k) for k in AT.KINDS}
-    return {"jobs": njobs, "kinds": counts, "seconds": t_run["selector"], "jobs_per_s": njobs / t_run["selector"],
+    # the selector study (moduli of 2, 3 and 4 limbs, n in [48, 640]): where the static rule errs
+    study = AT.selector_study(n_train=160, n_held=80, reps=3, steps=3000)
+    return {"jobs": njobs, "kinds": counts, "selector_study": study, "seconds": t_run["selector"], "jobs_per_s": njobs / t_run["selector"],
             "static_seconds": t_run["static"], "selector_gain_vs_static": t_run["static"] / t_run["selector"],
             "heldout_jobs": len(held), "heldout_selector_vs_best": t_sel / best,
             "heldout_static_vs_best": t_stat / best, "heldout_choice_agreement": agree,
@@ -540,7 +566,8 @@ def main():
         leg = {"batch": lambda: batched_echelon(E, args.batch_jobs), "lu": lambda: lu_c2(E, cpu=not args.no_lu_cpu),
                "mixed": lambda: mixed_stream(args.mixed_jobs),
                "howell": lambda: howell_batch(E, args.howell_jobs),
-               "ring2k": lambda: ring2k_session(E),
+               "ring2k": lambda: ring2k_lu(E),
+               "u64": lambda: gemm_u64_leg(E),
                "level1": lambda: level1_kernels(E),
                "cpu": lambda: cpu_baseline(p)}[args.only]()
         print(json.dumps({"only": args.only, **leg}), flush=True)
@@ -616,9 +643,13 @@ def main():
         log(f"howell batch: {hw}")
         res["howell_batch"] = hw
     if not args.no_ring2k:
-        r2 = ring2k_session(E)
-        log(f"ring2k session: {r2}")
-        res["ring2k_session"] = r2
+        r2 = ring2k_lu(E)
+        log(f"ring2k LU: {r2}")
+        res["ring2k_lu"] = r2
+    if not args.no_u64:
+        g64 = gemm_u64_leg(E)
+        log(f"u64 GEMM: {g64}")
+        res["gemm_u64"] = g64
     if not args.no_level1:
         l1 = level1_kernels(E)
         log(f"level-1/2 kernels: {l1}")

@@ -41,6 +41,10 @@ class hx_modctx(C.Structure):
                 ("k_max", C.c_uint64), ("two64_mod", C.c_uint64)]
 
 
+class hx_modctx64(C.Structure):
+    _fields_ = [("p", C.c_uint64), ("ninv", C.c_uint64), ("r2", C.c_uint64), ("two64", C.c_uint64)]
+
+
 class hx_view(C.Structure):
     _fields_ = [("base", C.c_void_p), ("rs", C.c_int64), ("cs", C.c_int64),
                 ("rows", C.c_uint32), ("cols", C.c_uint32)]
@@ -59,7 +63,7 @@ class hx_device_info(C.Structure):
 
 class hx_gemm_job(C.Structure):
     _fields_ = [("m", C.c_uint32), ("n", C.c_uint32), ("k", C.c_uint32), ("op", C.c_int32),
-                ("variant", C.c_int32), ("reserved", C.c_int32), ("A", C.c_void_p), ("B", C.c_void_p), ("C", C.c_void_p),
+                ("variant", C.c_int32), ("tile", C.c_int32), ("A", C.c_void_p), ("B", C.c_void_p), ("C", C.c_void_p),
                 ("lda", C.c_int64), ("ldb", C.c_int64), ("ldc", C.c_int64)]
 
 
@@ -72,6 +76,17 @@ class hx_arena_stats(C.Structure):
     _fields_ = [("reserved_bytes", C.c_size_t), ("live_bytes", C.c_size_t), ("peak_live_bytes", C.c_size_t),
                 ("peak_reserved_bytes", C.c_size_t), ("driver_mallocs", C.c_uint64), ("reuses", C.c_uint64),
                 ("driver_frees", C.c_uint64), ("oom_retries", C.c_uint64)]
+
+
+class hx_ring2k_stats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("calls", "device", "resident", "tiled", "slabs", "replays", "fallbacks",
+                                          "panels", "lookaheads", "verified", "ns_open", "ns_fetch", "ns_panel",
+                                          "ns_step", "ns_host", "ns_close")]
+
+
+class hx_staging_stats(C.Structure):
+    _fields_ = [("chunk_bytes", C.c_size_t), ("cap_bytes", C.c_size_t), ("allocated_bytes", C.c_size_t),
+                ("peak_bytes", C.c_size_t), ("acquisitions", C.c_uint64), ("waits", C.c_uint64)]
 
 
 class hx_rref_job(C.Structure):
@@ -126,6 +141,9 @@ SIGNATURES = {
     "hx_upload": (_INT, [_V, _V, _SZ, _V]),
     "hx_download": (_INT, [_V, _V, _SZ, _V]),
     "hx_copy_d2d": (_INT, [_V, _V, _SZ, _V]),
+    "hx_host_alloc": (_INT, [C.POINTER(_V), _SZ]),
+    "hx_host_free": (_INT, [_V]),
+    "hx_staging_stats_get": (_INT, [C.POINTER(hx_staging_stats)]),
     "hx_memset": (_INT, [_V, _INT, _SZ, _V]),
     "hx_graph_begin": (_INT, [_V]),
     "hx_graph_end": (_INT, [_V, C.POINTER(_V)]),
@@ -133,6 +151,8 @@ SIGNATURES = {
     "hx_graph_destroy": (_INT, [_V]),
     "hx_gemm_u32": (_INT, [C.POINTER(hx_modctx), hx_view, hx_view, hx_view, _INT, _V, C.POINTER(hx_gemm_opts)]),
     "hx_gemm_workspace_bytes": (_SZ, [C.POINTER(hx_modctx), _U32, _U32, _U32, _INT]),
+    "hx_modctx64_init": (_INT, [_U64, C.POINTER(hx_modctx64)]),
+    "hx_gemm_u64": (_INT, [C.POINTER(hx_modctx64), hx_view, hx_view, hx_view, _INT, _V]),
     "hx_gemm_limb_k_chunk": (_U32, [C.POINTER(hx_modctx), _U32, _U32, _U32, _SZ]),
     "hx_rref_batch_info_get": (_INT, [_V, C.POINTER(hx_rref_batch_info)]),
     "hx_arena_stats_get": (_INT, [C.POINTER(hx_arena_stats)]),
@@ -156,6 +176,8 @@ SIGNATURES = {
     "hx_canonicalise_u32": (_INT, [C.POINTER(hx_modctx), hx_view, _V]),
     "hx_canonicalise_acc64_u32": (_INT, [C.POINTER(hx_modctx), hx_view, hx_view, _V]),
     "hx_rref_u32": (_INT, [C.POINTER(hx_modctx), _U32, _U32, _V, _I64, _V, _I64, _U32P, _U32P, _U32P, _V]),
+    "hx_rref_workspace_bytes": (_SZ, [_U32, _U32]),
+    "hx_rref_u32_device": (_INT, [C.POINTER(hx_modctx), _U32, _U32, _V, _I64, _V, _I64, _V, _V, _V, _V, _SZ, _V]),
     "hx_rref_u32_host": (_INT, [C.POINTER(hx_modctx), _U32, _U32, _V, _V, _V, _U32, _U32P, _U32P, _U32P]),
     "hx_rref_batch_create": (_INT, [C.POINTER(hx_modctx), C.POINTER(hx_rref_job), _U32, _INT, C.POINTER(_V)]),
     "hx_rref_batch_run": (_INT, [_V, _V]),
@@ -178,6 +200,12 @@ SIGNATURES = {
     "hx_ring2k_step": (_INT, [_V, C.POINTER(hx_ring2k_step_args)]),
     "hx_ring2k_fetch": (_INT, [_V, _V, _SZ]),
     "hx_ring2k_close": (_INT, [_V]),
+    "hx_ring2k_plan": (_U32, [_U32, _U32, _U32, _U32, _U64]),
+    "hx_ring2k_store_budget": (_U64, []),
+    "hx_ring2k_verify": (_INT, []),
+    "hx_ring2k_echelon": (_INT, [_U32, _U32, _U32, _U32, _V, _SZ, _U32, _U32, _U64, _U64, _U32P, _U32P, _U32P]),
+    "hx_ring2k_stats_get": (_INT, [C.POINTER(hx_ring2k_stats)]),
+    "hx_ring2k_stats_reset": (_INT, []),
     "hx_ring2k_gemm_sub_host": (_INT, [_U32, _V, _SZ, _V, _SZ, _V, _SZ, _U32, _U32, _U32]),
 }
 

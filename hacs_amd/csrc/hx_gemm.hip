@@ -1056,23 +1056,33 @@ void gemm_dispatch(const hx_modctx& ctx, const GemmArgs& g, hipStream_t s, int v
 // ---- batched uneven GEMM -------------------------------------------------------------------
 namespace {
 
+// The limb jobs of one kernel choice (hx_gemm_job.tile 1..4): one launch each.
+struct LimbGroup {
+    uint32_t n = 0, tiles = 0;
+    LimbArgs* d_args = nullptr;
+    uint32_t* d_prefix = nullptr;         // n + 1 entries
+};
+
 struct GBatch {
     hx_modctx ctx;
     uint32_t n_plain = 0, n_limb = 0, n_prep = 0;
-    uint32_t tiles_plain = 0, tiles_limb = 0, tiles_prep = 0;
+    uint32_t tiles_plain = 0, tiles_prep = 0;
     GemmArgs* d_plain = nullptr;
-    LimbArgs* d_limb = nullptr;
     PrepArgs* d_prep = nullptr;
-    uint32_t* d_prefix = nullptr;        // [plain prefix (n_plain+1) | limb prefix (n_limb+1) | prep prefix (n_prep+1)]
+    uint32_t* d_prefix = nullptr;        // [plain prefix (n_plain+1) | prep prefix (n_prep+1)]
+    LimbGroup group[5];                  // index = kernel choice 1..4
     int8_t* d_ws = nullptr;
 };
 
 void gbatch_free(GBatch* b) {
     if (!b) return;
     (void)hipFree(b->d_plain);
-    (void)hipFree(b->d_limb);
     (void)hipFree(b->d_prep);
     (void)hipFree(b->d_prefix);
+    for (LimbGroup& g : b->group) {
+        (void)hipFree(g.d_args);
+        (void)hipFree(g.d_prefix);
+    }
     (void)hipFree(b->d_ws);
     delete b;
 }
@@ -1094,8 +1104,9 @@ void* gemm_batch_create(const hx_modctx& ctx, const hx_gemm_job* jobs, uint32_t 
         b->ctx = ctx;
         std::vector<GemmArgs> plain;
         std::vector<LimbArgs> limb;
+        std::vector<int> limb_ver;          // each limb job's kernel choice (1..4)
         std::vector<PrepArgs> prep;
-        std::vector<uint32_t> pp{0}, lp{0}, prp{0};
+        std::vector<uint32_t> pp{0}, prp{0};
         size_t ws = 0;
         const int L = (int)ctx.limbs;
         const uint32_t kchunk = L ? limb_k_chunk(L) : 0;
@@ -1143,11 +1154,9 @@ void* gemm_batch_create(const hx_modctx& ctx, const hx_gemm_job* jobs, uint32_t 
             la.Cout = j.C; la.cors = j.ldc; la.cocs = 1;
             la.op = j.op; la.M = devmod(ctx);
             limb_epilogue_consts(p, la);
+            HX_CHECK(j.tile >= 0 && j.tile <= 4, "gemm batch: tile must be 0 .. 4");
             limb.push_back(la);
-            // a job of 64+ tiles starts at a multiple of 8 blocks, so block b runs on XCD b mod 8 with
-            // the job-local XCD-aware tile order (tile_of) intact; the skipped blocks exit at once
-            if (la.ntm * la.ntn >= 64) lp.back() = (lp.back() + 7) / 8 * 8;
-            lp.push_back(lp.back() + la.ntm * la.ntn);
+            limb_ver.push_back(limb_version(j.tile));
             if (Kp > 0) {
                 PrepArgs pa{};
                 pa.src = j.A; pa.s_r = j.lda; pa.s_c = 1;
@@ -1175,15 +1184,33 @@ void* gemm_batch_create(const hx_modctx& ctx, const hx_gemm_job* jobs, uint32_t 
         b->n_limb = (uint32_t)limb.size();
         b->n_prep = (uint32_t)prep.size();
         b->tiles_plain = pp.back();
-        b->tiles_limb = lp.back();
         b->tiles_prep = prp.back();
         b->d_plain = upload_vec(plain);
-        b->d_limb = upload_vec(limb);
         b->d_prep = upload_vec(prep);
         std::vector<uint32_t> pre(pp);
-        pre.insert(pre.end(), lp.begin(), lp.end());
         pre.insert(pre.end(), prp.begin(), prp.end());
         b->d_prefix = upload_vec(pre);
+        // the limb jobs by kernel choice, largest first within a group (their order above); a job of 64+
+        // tiles starts at a multiple of 8 blocks, so block b runs on XCD b mod 8 with the job-local
+        // XCD-aware tile order (tile_of) intact; the skipped blocks exit at once
+        for (int v = 1; v <= 4; ++v) {
+            std::vector<LimbArgs> ga;
+            std::vector<uint32_t> gp{0};
+            for (size_t i = 0; i < limb.size(); ++i) {
+                if (limb_ver[i] != v) continue;
+                ga.push_back(limb[i]);
+                const uint32_t tiles = limb[i].ntm * limb[i].ntn;
+                if (tiles >= 64) gp.back() = (gp.back() + 7) / 8 * 8;
+                gp.push_back(gp.back() + tiles);
+            }
+            LimbGroup& g = b->group[v];
+            g.n = (uint32_t)ga.size();
+            g.tiles = gp.back();
+            if (g.n) {
+                g.d_args = upload_vec(ga);
+                g.d_prefix = upload_vec(gp);
+            }
+        }
         switch (L) {
             case 1: limb_attrs<1>(); break;
             case 2: limb_attrs<2>(); break;
@@ -1198,51 +1225,41 @@ void* gemm_batch_create(const hx_modctx& ctx, const hx_gemm_job* jobs, uint32_t 
     return b;
 }
 
+template <int L>
+void launch_limb_group(const LimbGroup& g, int v, hipStream_t s) {
+    if (g.tiles == 0) return;
+    switch (v) {
+        case 1: limb_batch_kernel<L><<<g.tiles, 256, limb_lds<L>(), s>>>(g.d_args, g.d_prefix, g.n); break;
+        case 2: limb2_batch_kernel<L><<<g.tiles, 256, 0, s>>>(g.d_args, g.d_prefix, g.n); break;
+        case 3: limb3_batch_kernel<L><<<g.tiles, 256, 0, s>>>(g.d_args, g.d_prefix, g.n); break;
+        default: limb4_batch_kernel<L><<<g.tiles, 512, 0, s>>>(g.d_args, g.d_prefix, g.n); break;
+    }
+    after_launch("limb_batch_kernel");
+}
+
 void gemm_batch_run(void* vb, hipStream_t s) {
     GBatch* b = (GBatch*)vb;
     const uint32_t* pre_plain = b->d_prefix;
-    const uint32_t* pre_limb = pre_plain + b->n_plain + 1;
-    const uint32_t* pre_prep = pre_limb + b->n_limb + 1;
+    const uint32_t* pre_prep = pre_plain + b->n_plain + 1;
     stat_add(ST_BATCHED, (uint64_t)b->n_plain + b->n_limb);
-    stat_add(ST_GEMMS, (b->tiles_plain ? 1 : 0) + (b->tiles_limb ? 1 : 0));
-    if (b->tiles_limb) stat_add(ST_LIMB, 1);
     if (b->tiles_prep) {
         prep_batch_kernel<<<b->tiles_prep, 256, 0, s>>>(b->d_prep, pre_prep, b->n_prep);
         after_launch("prep_batch_kernel");
     }
-    if (b->tiles_limb) {
-        if (kLimbDefault == 3) {
-            switch (b->ctx.limbs) {
-                case 1: limb3_batch_kernel<1><<<b->tiles_limb, 256, 0, s>>>(b->d_limb, pre_limb, b->n_limb); break;
-                case 2: limb3_batch_kernel<2><<<b->tiles_limb, 256, 0, s>>>(b->d_limb, pre_limb, b->n_limb); break;
-                case 3: limb3_batch_kernel<3><<<b->tiles_limb, 256, 0, s>>>(b->d_limb, pre_limb, b->n_limb); break;
-                default: limb3_batch_kernel<4><<<b->tiles_limb, 256, 0, s>>>(b->d_limb, pre_limb, b->n_limb); break;
-            }
-        } else if (kLimbDefault == 4) {
-            switch (b->ctx.limbs) {
-                case 1: limb4_batch_kernel<1><<<b->tiles_limb, 512, 0, s>>>(b->d_limb, pre_limb, b->n_limb); break;
-                case 2: limb4_batch_kernel<2><<<b->tiles_limb, 512, 0, s>>>(b->d_limb, pre_limb, b->n_limb); break;
-                case 3: limb4_batch_kernel<3><<<b->tiles_limb, 512, 0, s>>>(b->d_limb, pre_limb, b->n_limb); break;
-                default: limb4_batch_kernel<4><<<b->tiles_limb, 512, 0, s>>>(b->d_limb, pre_limb, b->n_limb); break;
-            }
-        } else if (kLimbDefault == 2) {
-            switch (b->ctx.limbs) {
-                case 1: limb2_batch_kernel<1><<<b->tiles_limb, 256, 0, s>>>(b->d_limb, pre_limb, b->n_limb); break;
-                case 2: limb2_batch_kernel<2><<<b->tiles_limb, 256, 0, s>>>(b->d_limb, pre_limb, b->n_limb); break;
-                case 3: limb2_batch_kernel<3><<<b->tiles_limb, 256, 0, s>>>(b->d_limb, pre_limb, b->n_limb); break;
-                default: limb2_batch_kernel<4><<<b->tiles_limb, 256, 0, s>>>(b->d_limb, pre_limb, b->n_limb); break;
-            }
-        } else {
-            switch (b->ctx.limbs) {
-                case 1: limb_batch_kernel<1><<<b->tiles_limb, 256, limb_lds<1>(), s>>>(b->d_limb, pre_limb, b->n_limb); break;
-                case 2: limb_batch_kernel<2><<<b->tiles_limb, 256, limb_lds<2>(), s>>>(b->d_limb, pre_limb, b->n_limb); break;
-                case 3: limb_batch_kernel<3><<<b->tiles_limb, 256, limb_lds<3>(), s>>>(b->d_limb, pre_limb, b->n_limb); break;
-                default: limb_batch_kernel<4><<<b->tiles_limb, 256, limb_lds<4>(), s>>>(b->d_limb, pre_limb, b->n_limb); break;
-            }
+    for (int v = 1; v <= 4; ++v) {
+        const LimbGroup& g = b->group[v];
+        if (g.tiles == 0) continue;
+        stat_add(ST_GEMMS, 1);
+        stat_add(ST_LIMB, 1);
+        switch (b->ctx.limbs) {
+            case 1: launch_limb_group<1>(g, v, s); break;
+            case 2: launch_limb_group<2>(g, v, s); break;
+            case 3: launch_limb_group<3>(g, v, s); break;
+            default: launch_limb_group<4>(g, v, s); break;
         }
-        after_launch("limb_batch_kernel");
     }
     if (b->tiles_plain) {
+        stat_add(ST_GEMMS, 1);
         plain_batch_kernel<<<b->tiles_plain, 256, 0, s>>>(b->d_plain, pre_plain, b->n_plain, devmod(b->ctx));
         after_launch("plain_batch_kernel");
     }

@@ -1,202 +1,376 @@
-// hx_ring2k.hip -- the device side of the Z/2^32 / Z/2^64 LU session (Cuda/CudaRing2k.h:28-46): a
-// device-resident column window [0, cw) of an m x cw matrix, driven panel by panel by a host that does
-// the pivot search (Binary/Ring2kDeviceLU.h).  Per step the device
-//   * writes the host-factored panel (rows r .. m-1, columns [pc0, pc1)) into the window,
-//   * replays the panel's row swaps (pairs, in order) on every other window column,
-//   * runs the unit-lower TRSM on the np pivot rows and the trailing C -= L U over columns [u0, w),
-//   * and, on request, copies the next panel's columns (rows r + np .. m-1 of [la0, la1)) back to
-//     pinned host memory on a second stream (lookahead; collected by fetch()).
-// All arithmetic is native wraparound in u32 / u64 (Z/2^k is the quotient, Ring2k.h:17-22): exact, so
-// the values equal the host kernels' bit for bit.
+// hx_ring2k.hip -- the device side of the blocked LU over Z/2^32 and Z/2^64 (the ring2k::device
+// Ops of Binary/Ring2kDevice.h:49-83; the reference's device session is Cuda/CudaRing2k.h:28-46).
+//
+// A session keeps a column window of the matrix resident in HBM: m rows x cw columns of native
+// words (4 or 8 bytes), row-major with row stride cw.  Wraparound arithmetic is the words' own, so
+// every entry gets the one value the host code gives it, whatever the tiling or summation order.
+//
+// One step (a panel the host has factored) is, on the session's compute stream:
+//   1. the panel's factored block lands in the window (one 2D copy, rows r..m-1 of [pc0, pc1)), its
+//      L slots and the composed row permutation land in device staging;
+//   2. the panel's row swaps reach every other window column: the swap list is composed on the host
+//      into one permutation of the rows it touches, applied per 64-column strip through LDS;
+//   3. the unit-lower TRSM on the pivot rows r..r+np-1: one wave per 64-column strip, the strip's np
+//      x 64 block in LDS (one column per lane, no barrier), L11 read from the staged L slots;
+//   4. the trailing update C[r+np.., u0..w) -= L21 U12 as a tiled wraparound product: 128 x 128
+//      output tiles, the whole K (= np <= 64) of both operands staged in LDS once, 8 x 8 outputs per
+//      thread.  The next panel's columns [la0, la1) are updated first; a second stream copies them to
+//      pinned host memory while the rest of the product runs (the lookahead fetch).
+// Host staging is pinned and double-buffered by step parity (an event guards each slot's reuse), so
+// step() returns as soon as its inputs are copied into staging.
+//
+// The host driver of the whole LU (hx_ring2k_echelon) is in hx_ring2k_lu.hip.
+#include <algorithm>
 #include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
 
 #include "hx_common.h"
 
 namespace hx {
 namespace {
 
+constexpr uint32_t kMaxPanel = 64;      // panel widths the kernels stage in LDS
+constexpr int kStrip = 64;              // columns per strip (swaps, TRSM)
+constexpr int kTile = 128;              // GEMM output tile edge
+constexpr int kTileThreads = 256;       // 16 x 16 threads, 8 x 8 outputs each
+
+// ---- kernels ------------------------------------------------------------------------------------
+
+// Rows rows[0..nt) of the window get new(rows[i]) = old(src[i]) on every column of [0, w) outside
+// [skip0, skip1).  One workgroup per 64-column strip; the touched rows' strip is staged in LDS.
 template <class E>
-__global__ void r2_swap_kernel(E* A, size_t lda, uint32_t cw, uint32_t pc0, uint32_t pc1, const uint32_t* sw,
-                               uint32_t nswaps) {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= cw || (j >= pc0 && j < pc1)) return;
-    for (uint32_t q = 0; q < nswaps; ++q) {
-        E* a = A + (size_t)sw[2 * q] * lda + j;
-        E* b = A + (size_t)sw[2 * q + 1] * lda + j;
-        const E x = *a;
-        *a = *b;
-        *b = x;
-    }
+__global__ __launch_bounds__(256) void r2_permute_kernel(E* win, uint32_t ld, uint32_t w, uint32_t skip0,
+                                                         uint32_t skip1, const uint32_t* rows, const uint32_t* src_slot,
+                                                         uint32_t nt) {
+    extern __shared__ __align__(16) unsigned char r2_lds[];
+    E* buf = (E*)r2_lds;                                  // nt x kStrip
+    const uint32_t c0 = blockIdx.x * kStrip;
+    const uint32_t lane = threadIdx.x % kStrip, grp = threadIdx.x / kStrip;   // 4 row groups
+    const uint32_t c = c0 + lane;
+    const bool live = c < w && (c < skip0 || c >= skip1);
+    for (uint32_t t = grp; t < nt; t += 4) buf[t * kStrip + lane] = live ? win[(size_t)rows[t] * ld + c] : E(0);
+    __syncthreads();
+    if (!live) return;
+    for (uint32_t t = grp; t < nt; t += 4) win[(size_t)rows[t] * ld + c] = buf[src_slot[t] * kStrip + lane];
 }
 
-// Unit-lower TRSM on the pivot rows, one column per thread, rows in order.
+// Unit-lower forward substitution on the pivot rows r .. r+np-1 of the columns [u0, w):
+// X[t] -= sum_{s<t} L11[t][s] X[s], L11 row t = L slots of pivot row r + t (columns 0 .. t-1).
+// One wave per 64-column strip; X (np x 64) in LDS, one column per lane.
 template <class E>
-__global__ void r2_trsm_kernel(E* A, size_t lda, const E* L, size_t ldl, uint32_t np, uint32_t c0, uint32_t c1) {
-    const uint32_t j = c0 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= c1) return;
+__global__ __launch_bounds__(64) void r2_trsm_kernel(E* win, uint32_t ld, uint32_t r, uint32_t np, uint32_t u0,
+                                                     uint32_t w, const E* Ls, uint32_t ldl) {
+    __shared__ E X[kMaxPanel * kStrip];
+    const uint32_t lane = threadIdx.x, c = u0 + blockIdx.x * kStrip + lane;
+    const bool live = c < w;
+    for (uint32_t t = 0; t < np; ++t) X[t * kStrip + lane] = live ? win[(size_t)(r + t) * ld + c] : E(0);
+    // each lane owns its column of X: no barrier, L11's entries are the same for every lane
     for (uint32_t t = 1; t < np; ++t) {
-        E x = A[(size_t)t * lda + j];
-        const E* lt = L + (size_t)t * ldl;
-        for (uint32_t t2 = 0; t2 < t; ++t2) x -= lt[t2] * A[(size_t)t2 * lda + j];
-        A[(size_t)t * lda + j] = x;
+        const E* lt = Ls + (size_t)t * ldl;
+        E acc = X[t * kStrip + lane];
+        for (uint32_t s = 0; s < t; ++s) acc -= lt[s] * X[s * kStrip + lane];
+        X[t * kStrip + lane] = acc;
     }
+    if (!live) return;
+    for (uint32_t t = 1; t < np; ++t) win[(size_t)(r + t) * ld + c] = X[t * kStrip + lane];
 }
 
-// C (M x N) -= L (M x K) U (K x N), wraparound; 64 x 64 tiles, 256 threads (4 x 4 each), K staged by 16.
+// C (rows [r0, r1) of the window, columns [c0, c1)) -= L21 U12 with L21 = Ls rows (i - lrow0), K = np,
+// U12 = window rows r .. r+np-1 of the same columns.  One 128 x 128 tile per workgroup; A (128 x np)
+// and B (np x 128) staged whole in LDS (K <= 64), 8 x 8 outputs per thread.
 template <class E>
-__global__ __launch_bounds__(256) void r2_gemm_sub_kernel(E* C, size_t ldc, const E* L, size_t ldl, const E* U,
-                                                         size_t ldu, uint32_t M, uint32_t N, uint32_t K) {
-    __shared__ E Ls[16][65];
-    __shared__ E Us[16][65];
-    const uint32_t n0 = blockIdx.x * 64, m0 = blockIdx.y * 64;
-    const int tid = threadIdx.x, tx = tid % 16, ty = tid / 16;
-    E acc[4][4];
+__global__ __launch_bounds__(kTileThreads) void r2_gemm_kernel(E* win, uint32_t ld, uint32_t r, uint32_t np, uint32_t r0,
+                                                               uint32_t r1, uint32_t c0, uint32_t c1, const E* Ls,
+                                                               uint32_t ldl, uint32_t lrow0, uint32_t ntc) {
+    extern __shared__ __align__(16) unsigned char r2_lds[];
+    E* As = (E*)r2_lds;                                 // np x (kTile + 1): As[k][i], transposed (padded row)
+    E* Bs = As + (size_t)np * (kTile + 1);              // np x kTile
+    const uint32_t tm = blockIdx.x / ntc, tn = blockIdx.x % ntc;
+    const uint32_t i0 = r0 + tm * kTile, j0 = c0 + tn * kTile;
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t idx = tid; idx < kTile * np; idx += kTileThreads) {
+        const uint32_t i = idx / np, k = idx % np;
+        As[k * (kTile + 1) + i] = (i0 + i < r1) ? Ls[(size_t)(i0 + i - lrow0) * ldl + k] : E(0);
+    }
+    for (uint32_t idx = tid; idx < kTile * np; idx += kTileThreads) {
+        const uint32_t k = idx / kTile, j = idx % kTile;
+        Bs[k * kTile + j] = (j0 + j < c1) ? win[(size_t)(r + k) * ld + j0 + j] : E(0);
+    }
+    __syncthreads();
+    const uint32_t tx = tid % 16, ty = tid / 16;        // columns tx + 16 q, rows ty + 16 p
+    E acc[8][8];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int a = 0; a < 8; ++a)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = 0;
-    for (uint32_t k0 = 0; k0 < K; k0 += 16) {
+        for (int b = 0; b < 8; ++b) acc[a][b] = E(0);
+    for (uint32_t k = 0; k < np; ++k) {
+        E av[8], bv[8];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const int e = tid + 256 * t;
-            const uint32_t r = e / 16, c = e % 16;
-            Ls[c][r] = (m0 + r < M && k0 + c < K) ? L[(size_t)(m0 + r) * ldl + k0 + c] : E(0);
-            const uint32_t ur = e / 64, uc = e % 64;
-            Us[ur][uc] = (k0 + ur < K && n0 + uc < N) ? U[(size_t)(k0 + ur) * ldu + n0 + uc] : E(0);
-        }
-        __syncthreads();
+        for (int a = 0; a < 8; ++a) av[a] = As[k * (kTile + 1) + ty + 16 * a];
 #pragma unroll
-        for (int kk = 0; kk < 16; ++kk) {
-            E a[4], b[4];
+        for (int b = 0; b < 8; ++b) bv[b] = Bs[k * kTile + tx + 16 * b];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = Ls[kk][ty + 16 * i];
+        for (int a = 0; a < 8; ++a)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) b[j] = Us[kk][tx + 16 * j];
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * b[j];
-        }
-        __syncthreads();
+            for (int b = 0; b < 8; ++b) acc[a][b] += av[a] * bv[b];
     }
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const uint32_t r = m0 + ty + 16 * i;
-        if (r >= M) continue;
+    for (int a = 0; a < 8; ++a) {
+        const uint32_t i = i0 + ty + 16 * a;
+        if (i >= r1) continue;
+        E* row = win + (size_t)i * ld;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const uint32_t c = n0 + tx + 16 * j;
-            if (c < N) C[(size_t)r * ldc + c] -= acc[i][j];
+        for (int b = 0; b < 8; ++b) {
+            const uint32_t j = j0 + tx + 16 * b;
+            if (j < c1) row[j] -= acc[a][b];
         }
     }
 }
 
-template <class E>
-void gemm_sub(E* C, size_t ldc, const E* L, size_t ldl, const E* U, size_t ldu, uint32_t M, uint32_t N, uint32_t K,
-              hipStream_t s) {
-    if (M == 0 || N == 0 || K == 0) return;
-    for (uint32_t m0 = 0; m0 < M; m0 += 65535u * 64) {
-        const uint32_t mm = std::min<uint32_t>(M - m0, 65535u * 64);
-        r2_gemm_sub_kernel<E><<<dim3((N + 63) / 64, (mm + 63) / 64), 256, 0, s>>>(C + (size_t)m0 * ldc, ldc,
-                                                                                 L + (size_t)m0 * ldl, ldl, U, ldu, mm,
-                                                                                 N, K);
-        after_launch("r2_gemm_sub_kernel");
-    }
-}
+// ---- session ------------------------------------------------------------------------------------
 
-struct Session {
-    uint32_t elem = 0, m = 0, cw = 0, panel = 0;
-    char* dA = nullptr;
-    char* dL = nullptr;
-    uint32_t* dSw = nullptr;
-    char* hStage = nullptr;     // panel / L / lookahead staging (pinned), m x panel elements each
-    char* hL = nullptr;
-    char* hNext = nullptr;
-    uint32_t* hSw = nullptr;
-    size_t slab = 0;
-    hipStream_t s1 = nullptr, s2 = nullptr;
-    hipEvent_t ev_la = nullptr, ev_stage = nullptr;
-    bool stage_pending = false, la_pending = false;
-    uint32_t la_rows = 0, la_cols = 0;
-    char* at(uint32_t r, uint32_t c) const { return dA + ((size_t)r * cw + c) * elem; }
-    void release() noexcept {
-        if (s1) (void)hipStreamSynchronize(s1);
-        if (s2) (void)hipStreamSynchronize(s2);
-        (void)hipFree(dA);
-        (void)hipFree(dL);
-        (void)hipFree(dSw);
-        (void)hipHostFree(hStage);
-        (void)hipHostFree(hL);
-        (void)hipHostFree(hNext);
-        (void)hipHostFree(hSw);
-        if (ev_la) (void)hipEventDestroy(ev_la);
-        if (ev_stage) (void)hipEventDestroy(ev_stage);
-        if (s1) (void)hipStreamDestroy(s1);
-        if (s2) (void)hipStreamDestroy(s2);
-        (void)hipGetLastError();
-    }
+struct HostSlot {
+    void* buf = nullptr;          // pinned: panel block | L block (m x panel words each) | permutation (4 x 64 u32)
+    hipEvent_t done = nullptr;    // the copies out of this slot have completed
+    bool used = false;
 };
 
-void copy_rows(char* dst, size_t ldd_bytes, const char* src, size_t lds_bytes, size_t row_bytes, uint32_t rows) {
-    if (ldd_bytes == row_bytes && lds_bytes == row_bytes) {
-        std::memcpy(dst, src, row_bytes * rows);
-        return;
+struct Session {
+    uint32_t elem = 4, m = 0, cw = 0, panel = 0;
+    void* win = nullptr;          // m x cw words
+    void* d_L = nullptr;          // m x panel words (L slots of the step, rows r .. m-1)
+    uint32_t* d_perm = nullptr;   // 4 x panel: touched rows | their source slots
+    void* h_la = nullptr;         // pinned lookahead block, m x panel words
+    HostSlot slot[2];
+    int parity = 0;
+    hipStream_t s = nullptr, s_copy = nullptr;
+    hipEvent_t ev_la_src = nullptr, ev_la_done = nullptr;
+    bool la_pending = false;
+    uint32_t la_rows = 0, la_cols = 0;
+    size_t slot_bytes = 0;
+};
+
+size_t words_bytes(uint32_t elem, size_t words) { return words * elem; }
+
+void session_free(Session* S) {
+    if (!S) return;
+    if (S->s) (void)hipStreamSynchronize(S->s);
+    if (S->s_copy) (void)hipStreamSynchronize(S->s_copy);
+    for (HostSlot& h : S->slot) {
+        if (h.done) (void)hipEventDestroy(h.done);
+        if (h.buf) (void)hipHostFree(h.buf);
     }
-    for (uint32_t i = 0; i < rows; ++i) std::memcpy(dst + ldd_bytes * i, src + lds_bytes * i, row_bytes);
+    if (S->h_la) (void)hipHostFree(S->h_la);
+    if (S->ev_la_src) (void)hipEventDestroy(S->ev_la_src);
+    if (S->ev_la_done) (void)hipEventDestroy(S->ev_la_done);
+    if (S->win) arena_free(S->win);
+    if (S->d_L) arena_free(S->d_L);
+    if (S->d_perm) arena_free(S->d_perm);
+    if (S->s) (void)hipStreamDestroy(S->s);
+    if (S->s_copy) (void)hipStreamDestroy(S->s_copy);
+    delete S;
+}
+
+Session* session_open(uint32_t elem, uint32_t m, uint32_t cw, uint32_t panel) {
+    HX_CHECK(elem == 4 || elem == 8, "ring2k: elem must be 4 or 8");
+    HX_CHECK(m > 0 && cw > 0, "ring2k: empty window");
+    HX_CHECK(panel >= 1 && panel <= kMaxPanel, "ring2k: panel must be 1 .. 64");
+    Session* S = new Session();
+    try {
+        S->elem = elem;
+        S->m = m;
+        S->cw = cw;
+        S->panel = panel;
+        S->win = arena_alloc(words_bytes(elem, (size_t)m * cw));
+        S->d_L = arena_alloc(words_bytes(elem, (size_t)m * panel));
+        S->d_perm = (uint32_t*)arena_alloc(sizeof(uint32_t) * 4 * kMaxPanel);
+        S->slot_bytes = 2 * words_bytes(elem, (size_t)m * panel) + sizeof(uint32_t) * 4 * kMaxPanel;
+        for (HostSlot& h : S->slot) {
+            HX_HIP(hipHostMalloc(&h.buf, S->slot_bytes, hipHostMallocDefault));
+            HX_HIP(hipEventCreateWithFlags(&h.done, hipEventDisableTiming));
+        }
+        HX_HIP(hipHostMalloc(&S->h_la, words_bytes(elem, (size_t)m * panel), hipHostMallocDefault));
+        HX_HIP(hipStreamCreateWithFlags(&S->s, hipStreamNonBlocking));
+        HX_HIP(hipStreamCreateWithFlags(&S->s_copy, hipStreamNonBlocking));
+        HX_HIP(hipEventCreateWithFlags(&S->ev_la_src, hipEventDisableTiming));
+        HX_HIP(hipEventCreateWithFlags(&S->ev_la_done, hipEventDisableTiming));
+    } catch (...) {
+        session_free(S);
+        throw;
+    }
+    return S;
+}
+
+void copy_block(Session* S, void* host, size_t ldh, uint32_t r0, uint32_t rows, uint32_t c0, uint32_t cols, bool to_dev) {
+    HX_CHECK(r0 + (uint64_t)rows <= S->m && c0 + (uint64_t)cols <= S->cw, "ring2k: block outside the window");
+    HX_CHECK(rows == 0 || cols == 0 || (host != nullptr && ldh >= cols), "ring2k: bad host block");
+    if (rows == 0 || cols == 0) return;
+    const size_t e = S->elem;
+    char* d = (char*)S->win + ((size_t)r0 * S->cw + c0) * e;
+    // the lookahead copy (second stream) may still read the window
+    HX_HIP(hipStreamWaitEvent(S->s, S->ev_la_done, 0));
+    if (to_dev)
+        HX_HIP(hipMemcpy2DAsync(d, (size_t)S->cw * e, host, ldh * e, cols * e, rows, hipMemcpyHostToDevice, S->s));
+    else
+        HX_HIP(hipMemcpy2DAsync(host, ldh * e, d, (size_t)S->cw * e, cols * e, rows, hipMemcpyDeviceToHost, S->s));
+    HX_HIP(hipStreamSynchronize(S->s));
+    stat_add(to_dev ? ST_H2D_BYTES : ST_D2H_BYTES, (uint64_t)rows * cols * e);
+    stat_add(to_dev ? ST_H2D_CALLS : ST_D2H_CALLS, 1);
 }
 
 template <class E>
-void step_impl(Session& S, const hx_ring2k_step_args& a) {
-    const uint32_t rows = a.m - a.r, rn = a.r + a.np, W = a.w ? a.w : S.cw;
-    const size_t eb = sizeof(E);
-    if (S.stage_pending) {   // the previous step's staging buffers are free once its copies ran
-        HX_HIP(hipEventSynchronize(S.ev_stage));
-        S.stage_pending = false;
+void launch_gemm(Session* S, uint32_t r, uint32_t np, uint32_t r0, uint32_t r1, uint32_t c0, uint32_t c1, const E* Ls,
+                 uint32_t ldl, uint32_t lrow0, hipStream_t st) {
+    if (r1 <= r0 || c1 <= c0 || np == 0) return;
+    const uint32_t ntm = (r1 - r0 + kTile - 1) / kTile, ntn = (c1 - c0 + kTile - 1) / kTile;
+    const size_t lds = sizeof(E) * ((size_t)np * (kTile + 1) + (size_t)np * kTile);
+    r2_gemm_kernel<E><<<ntm * ntn, kTileThreads, lds, st>>>((E*)S->win, S->cw, r, np, r0, r1, c0, c1, Ls, ldl, lrow0,
+                                                             ntn);
+    after_launch("r2_gemm_kernel");
+}
+
+template <class E>
+void gemm_attrs() {
+    static std::once_flag once;
+    std::call_once(once, [] {
+        const int most = (int)(sizeof(E) * ((size_t)kMaxPanel * (kTile + 1) + (size_t)kMaxPanel * kTile));
+        (void)hipFuncSetAttribute((const void*)r2_gemm_kernel<E>, hipFuncAttributeMaxDynamicSharedMemorySize, most);
+        (void)hipFuncSetAttribute((const void*)r2_permute_kernel<E>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)(sizeof(E) * 2 * kMaxPanel * kStrip));
+    });
+}
+
+template <class E>
+void step_impl(Session* S, const hx_ring2k_step_args& a) {
+    const uint32_t m = S->m, w = a.w ? a.w : S->cw;
+    HX_CHECK(a.m == m, "ring2k step: m differs from the session's");
+    HX_CHECK(w <= S->cw, "ring2k step: w beyond the window");
+    HX_CHECK(a.r <= m && a.np <= S->panel && a.r + (uint64_t)a.np <= m, "ring2k step: bad pivot rows");
+    HX_CHECK(a.nswaps <= S->panel, "ring2k step: more swaps than panel columns");
+    HX_CHECK(a.pc0 <= a.pc1 && a.pc1 <= w, "ring2k step: bad panel columns");
+    HX_CHECK(a.la0 <= a.la1 && a.la1 <= w && a.la1 - a.la0 <= S->panel, "ring2k step: bad lookahead columns");
+    gemm_attrs<E>();
+    const uint32_t rows = m - a.r;
+    const bool update = a.np > 0 && a.u0 < w;
+    // ---- host staging: the L block and the composed permutation ----
+    HostSlot& hs = S->slot[S->parity];
+    S->parity ^= 1;
+    if (hs.used) HX_HIP(hipEventSynchronize(hs.done));
+    E* hP = (E*)hs.buf;
+    E* hL = hP + (size_t)m * S->panel;
+    uint32_t* hperm = (uint32_t*)(hL + (size_t)m * S->panel);
+    if (update) {
+        HX_CHECK(a.L != nullptr && a.ldl >= a.np, "ring2k step: L missing");
+        const E* L = (const E*)a.L;
+        for (uint32_t i = 0; i < rows; ++i) std::memcpy(hL + (size_t)i * a.np, L + (size_t)i * a.ldl, sizeof(E) * a.np);
     }
-    if (a.panel != nullptr && rows > 0) {
+    // the swaps, in order, composed into one permutation of the rows they touch: new(row) = old(src(row))
+    std::vector<uint32_t> touched, srcrow;
+    if (a.nswaps) {
+        HX_CHECK(a.swaps != nullptr, "ring2k step: swaps missing");
+        std::vector<std::pair<uint32_t, uint32_t>> cur;   // (row, the original row whose entries it now holds)
+        auto find = [&](uint32_t row) -> size_t {
+            for (size_t i = 0; i < cur.size(); ++i)
+                if (cur[i].first == row) return i;
+            cur.push_back({row, row});
+            return cur.size() - 1;
+        };
+        for (uint32_t t = 0; t < a.nswaps; ++t) {
+            const uint32_t x = a.swaps[2 * t], y = a.swaps[2 * t + 1];
+            HX_CHECK(x < m && y < m, "ring2k step: swap row out of range");
+            if (x == y) continue;
+            const size_t ix = find(x), iy = find(y);
+            std::swap(cur[ix].second, cur[iy].second);
+        }
+        std::sort(cur.begin(), cur.end());
+        for (auto& pr : cur) touched.push_back(pr.first);
+        for (auto& pr : cur)
+            srcrow.push_back((uint32_t)(std::lower_bound(touched.begin(), touched.end(), pr.second) - touched.begin()));
+    }
+    const uint32_t nt = (uint32_t)touched.size();
+    HX_CHECK(nt <= 2 * kMaxPanel, "ring2k step: the swaps touch more than 128 rows");
+    std::memcpy(hperm, touched.data(), sizeof(uint32_t) * nt);
+    std::memcpy(hperm + 2 * kMaxPanel, srcrow.data(), sizeof(uint32_t) * nt);
+
+    hipStream_t s = S->s;
+    // the previous lookahead copy reads the window: the swaps and updates below come after it
+    HX_HIP(hipStreamWaitEvent(s, S->ev_la_done, 0));
+    // 1. the factored panel into the window (rows r .. m-1 of its columns), through the slot
+    if (a.panel && a.pc1 > a.pc0 && rows > 0) {
         const uint32_t pw = a.pc1 - a.pc0;
-        copy_rows(S.hStage, pw * eb, (const char*)a.panel, a.ldp * eb, pw * eb, rows);
-        HX_HIP(hipMemcpy2DAsync(S.at(a.r, a.pc0), S.cw * eb, S.hStage, pw * eb, pw * eb, rows, hipMemcpyHostToDevice, S.s1));
+        HX_CHECK(pw <= S->panel && a.ldp >= pw, "ring2k step: panel wider than the session's, or ldp < width");
+        const size_t e = sizeof(E);
+        for (uint32_t i = 0; i < rows; ++i)
+            std::memcpy(hP + (size_t)i * pw, (const E*)a.panel + (size_t)i * a.ldp, e * pw);
+        HX_HIP(hipMemcpy2DAsync((char*)S->win + ((size_t)a.r * S->cw + a.pc0) * e, (size_t)S->cw * e, hP, pw * e,
+                                pw * e, rows, hipMemcpyHostToDevice, s));
+        stat_add(ST_H2D_BYTES, (uint64_t)rows * pw * e);
     }
-    if (a.nswaps > 0) {
-        std::memcpy(S.hSw, a.swaps, sizeof(uint32_t) * 2 * a.nswaps);
-        HX_HIP(hipMemcpyAsync(S.dSw, S.hSw, sizeof(uint32_t) * 2 * a.nswaps, hipMemcpyHostToDevice, S.s1));
-        r2_swap_kernel<E><<<(W + 255) / 256, 256, 0, S.s1>>>((E*)S.dA, S.cw, W, a.pc0, a.pc1, S.dSw, a.nswaps);
-        after_launch("r2_swap_kernel");
+    if (nt) HX_HIP(hipMemcpyAsync(S->d_perm, hperm, sizeof(uint32_t) * 4 * kMaxPanel, hipMemcpyHostToDevice, s));
+    if (update) HX_HIP(hipMemcpyAsync(S->d_L, hL, sizeof(E) * (size_t)rows * a.np, hipMemcpyHostToDevice, s));
+    HX_HIP(hipEventRecord(hs.done, s));
+    hs.used = true;
+    // 2. the swaps on every other window column
+    if (nt) {
+        r2_permute_kernel<E><<<(w + kStrip - 1) / kStrip, 256, sizeof(E) * nt * kStrip, s>>>(
+            (E*)S->win, S->cw, w, a.pc0, a.pc1, S->d_perm, S->d_perm + 2 * kMaxPanel, nt);
+        after_launch("r2_permute_kernel");
     }
-    const bool update = a.np > 0 && a.u0 < W;
+    // 3. TRSM on the pivot rows, 4. the trailing product (lookahead columns first)
+    S->la_pending = false;
     if (update) {
-        copy_rows(S.hL, a.np * eb, (const char*)a.L, a.ldl * eb, a.np * eb, rows);
-        HX_HIP(hipMemcpy2DAsync(S.dL, S.panel * eb, S.hL, a.np * eb, a.np * eb, rows, hipMemcpyHostToDevice, S.s1));
-    }
-    HX_HIP(hipEventRecord(S.ev_stage, S.s1));
-    S.stage_pending = true;
-    E* A = (E*)S.dA;
-    const E* Ld = (const E*)S.dL;
-    const bool la = a.la1 > a.la0 && rn < a.m;
-    auto gemm_cols = [&](uint32_t c0, uint32_t c1) {
-        if (c1 > c0 && rn < a.m)
-            gemm_sub<E>(A + (size_t)rn * S.cw + c0, S.cw, Ld + (size_t)a.np * S.panel, S.panel,
-                        A + (size_t)a.r * S.cw + c0, S.cw, a.m - rn, c1 - c0, a.np, S.s1);
-    };
-    if (update) {
-        if (a.np >= 2) {
-            r2_trsm_kernel<E><<<(W - a.u0 + 255) / 256, 256, 0, S.s1>>>(A + (size_t)a.r * S.cw, S.cw, Ld, S.panel, a.np,
-                                                                        a.u0, W);
+        const E* Ls = (const E*)S->d_L;          // row i of the block = global row r + i
+        if (a.np > 1) {
+            r2_trsm_kernel<E><<<(w - a.u0 + kStrip - 1) / kStrip, 64, 0, s>>>((E*)S->win, S->cw, a.r, a.np, a.u0, w, Ls,
+                                                                             a.np);
             after_launch("r2_trsm_kernel");
         }
-        gemm_cols(a.u0, (la && a.la0 == a.u0) ? a.la1 : W);   // the lookahead columns first
+        const uint32_t rb = a.r + a.np;
+        const bool la = a.la1 > a.la0 && rb < m;
+        if (la) {
+            HX_CHECK(a.la0 >= a.u0, "ring2k step: lookahead columns left of the update");
+            launch_gemm<E>(S, a.r, a.np, rb, m, a.la0, a.la1, Ls + (size_t)a.np * a.np, a.np, rb, s);
+            HX_HIP(hipEventRecord(S->ev_la_src, s));
+            HX_HIP(hipStreamWaitEvent(S->s_copy, S->ev_la_src, 0));
+            const size_t e = sizeof(E);
+            S->la_rows = m - rb;
+            S->la_cols = a.la1 - a.la0;
+            HX_HIP(hipMemcpy2DAsync(S->h_la, (size_t)S->la_cols * e, (char*)S->win + ((size_t)rb * S->cw + a.la0) * e,
+                                    (size_t)S->cw * e, (size_t)S->la_cols * e, S->la_rows, hipMemcpyDeviceToHost,
+                                    S->s_copy));
+            HX_HIP(hipEventRecord(S->ev_la_done, S->s_copy));
+            stat_add(ST_D2H_BYTES, (uint64_t)S->la_rows * S->la_cols * e);
+            S->la_pending = true;
+            launch_gemm<E>(S, a.r, a.np, rb, m, a.u0, a.la0, Ls + (size_t)a.np * a.np, a.np, rb, s);
+            launch_gemm<E>(S, a.r, a.np, rb, m, a.la1, w, Ls + (size_t)a.np * a.np, a.np, rb, s);
+        } else {
+            launch_gemm<E>(S, a.r, a.np, rb, m, a.u0, w, Ls + (size_t)a.np * a.np, a.np, rb, s);
+        }
+    } else if (a.la1 > a.la0 && a.r + a.np < m) {
+        // no update, but the caller still wants the next panel's columns back
+        HX_HIP(hipEventRecord(S->ev_la_src, s));
+        HX_HIP(hipStreamWaitEvent(S->s_copy, S->ev_la_src, 0));
+        const size_t e = sizeof(E);
+        const uint32_t rb = a.r + a.np;
+        S->la_rows = m - rb;
+        S->la_cols = a.la1 - a.la0;
+        HX_HIP(hipMemcpy2DAsync(S->h_la, (size_t)S->la_cols * e, (char*)S->win + ((size_t)rb * S->cw + a.la0) * e,
+                                (size_t)S->cw * e, (size_t)S->la_cols * e, S->la_rows, hipMemcpyDeviceToHost, S->s_copy));
+        HX_HIP(hipEventRecord(S->ev_la_done, S->s_copy));
+        S->la_pending = true;
     }
-    if (la) {
-        HX_HIP(hipEventRecord(S.ev_la, S.s1));
-        HX_HIP(hipStreamWaitEvent(S.s2, S.ev_la, 0));
-        const uint32_t lw = a.la1 - a.la0, lr = a.m - rn;
-        HX_HIP(hipMemcpy2DAsync(S.hNext, lw * eb, S.at(rn, a.la0), S.cw * eb, lw * eb, lr, hipMemcpyDeviceToHost, S.s2));
-        S.la_pending = true;
-        S.la_rows = lr;
-        S.la_cols = lw;
-        if (update && a.la0 == a.u0) gemm_cols(a.la1, W);
-    }
+}
+
+void fetch_impl(Session* S, void* host, size_t ldh) {
+    HX_CHECK(S->la_pending, "ring2k fetch: the last step had no lookahead");
+    HX_CHECK(host != nullptr && ldh >= S->la_cols, "ring2k fetch: bad host block");
+    HX_HIP(hipEventSynchronize(S->ev_la_done));
+    const size_t e = S->elem;
+    for (uint32_t i = 0; i < S->la_rows; ++i)
+        std::memcpy((char*)host + (size_t)i * ldh * e, (const char*)S->h_la + (size_t)i * S->la_cols * e, S->la_cols * e);
+    S->la_pending = false;
 }
 
 }  // namespace
@@ -207,41 +381,17 @@ using namespace hx;
 extern "C" {
 
 HX_API size_t hx_ring2k_session_bytes(uint32_t elem, uint32_t m, uint32_t cw, uint32_t panel) {
-    return (size_t)m * cw * elem + (size_t)m * panel * elem + sizeof(uint32_t) * 2 * (panel + 1);
+    return (size_t)elem * ((size_t)m * cw + (size_t)m * panel) + sizeof(uint32_t) * 4 * kMaxPanel;
 }
+
 HX_API size_t hx_ring2k_staging_bytes(uint32_t elem, uint32_t m, uint32_t panel) {
-    return 3 * (size_t)m * panel * elem + sizeof(uint32_t) * 2 * (panel + 1);
+    return 3 * (size_t)elem * m * panel + 2 * sizeof(uint32_t) * 4 * kMaxPanel;
 }
 
 HX_API int hx_ring2k_open(uint32_t elem, uint32_t m, uint32_t cw, uint32_t panel, hx_ring2k* out) {
     return guard("hx_ring2k_open", [&] {
         HX_CHECK(out != nullptr, "out is NULL");
-        HX_CHECK(elem == 4 || elem == 8, "ring2k session: elem must be 4 or 8");
-        HX_CHECK(m > 0 && cw > 0 && panel > 0, "ring2k session: empty session");
-        Session* S = new Session();
-        try {
-            S->elem = elem;
-            S->m = m;
-            S->cw = cw;
-            S->panel = panel;
-            S->slab = (size_t)m * panel * elem;
-            HX_HIP(hipStreamCreateWithFlags(&S->s1, hipStreamNonBlocking));
-            HX_HIP(hipStreamCreateWithFlags(&S->s2, hipStreamNonBlocking));
-            HX_HIP(hipEventCreateWithFlags(&S->ev_la, hipEventDisableTiming));
-            HX_HIP(hipEventCreateWithFlags(&S->ev_stage, hipEventDisableTiming));
-            HX_HIP(hipMalloc(&S->dA, (size_t)m * cw * elem));
-            HX_HIP(hipMalloc(&S->dL, S->slab));
-            HX_HIP(hipMalloc(&S->dSw, sizeof(uint32_t) * 2 * (panel + 1)));
-            HX_HIP(hipHostMalloc(&S->hStage, S->slab, hipHostMallocDefault));
-            HX_HIP(hipHostMalloc(&S->hL, S->slab, hipHostMallocDefault));
-            HX_HIP(hipHostMalloc(&S->hNext, S->slab, hipHostMallocDefault));
-            HX_HIP(hipHostMalloc(&S->hSw, sizeof(uint32_t) * 2 * (panel + 1), hipHostMallocDefault));
-        } catch (...) {
-            S->release();
-            delete S;
-            throw;
-        }
-        *out = S;
+        *out = session_open(elem, m, cw, panel);
     });
 }
 
@@ -249,44 +399,22 @@ HX_API int hx_ring2k_put(hx_ring2k s, const void* host, size_t ldh, uint32_t r0,
                          uint32_t cols) {
     return guard("hx_ring2k_put", [&] {
         HX_CHECK(s != nullptr, "NULL session");
-        Session& S = *(Session*)s;
-        HX_CHECK((uint64_t)r0 + rows <= S.m && (uint64_t)c0 + cols <= S.cw, "ring2k put: block outside the window");
-        if (rows == 0 || cols == 0) return;
-        const size_t eb = S.elem;
-        HX_HIP(hipMemcpy2DAsync(S.at(r0, c0), S.cw * eb, host, ldh * eb, cols * eb, rows, hipMemcpyHostToDevice, S.s1));
-        HX_HIP(hipStreamSynchronize(S.s1));
+        copy_block((Session*)s, (void*)host, ldh, r0, rows, c0, cols, true);
     });
 }
 
 HX_API int hx_ring2k_get(hx_ring2k s, void* host, size_t ldh, uint32_t r0, uint32_t rows, uint32_t c0, uint32_t cols) {
     return guard("hx_ring2k_get", [&] {
         HX_CHECK(s != nullptr, "NULL session");
-        Session& S = *(Session*)s;
-        HX_CHECK((uint64_t)r0 + rows <= S.m && (uint64_t)c0 + cols <= S.cw, "ring2k get: block outside the window");
-        HX_HIP(hipStreamSynchronize(S.s2));
-        S.la_pending = false;
-        if (rows == 0 || cols == 0) {
-            HX_HIP(hipStreamSynchronize(S.s1));
-            return;
-        }
-        const size_t eb = S.elem;
-        HX_HIP(hipMemcpy2DAsync(host, ldh * eb, S.at(r0, c0), S.cw * eb, cols * eb, rows, hipMemcpyDeviceToHost, S.s1));
-        HX_HIP(hipStreamSynchronize(S.s1));
+        copy_block((Session*)s, host, ldh, r0, rows, c0, cols, false);
     });
 }
 
 HX_API int hx_ring2k_step(hx_ring2k s, const hx_ring2k_step_args* a) {
     return guard("hx_ring2k_step", [&] {
         HX_CHECK(s != nullptr && a != nullptr, "NULL argument");
-        Session& S = *(Session*)s;
-        HX_CHECK(!S.la_pending, "ring2k step: the previous lookahead was not fetched");
-        HX_CHECK(a->m == S.m && a->r <= a->m && a->np <= S.panel && a->nswaps <= S.panel && a->pc0 <= a->pc1,
-                 "ring2k step: arguments outside the session");
-        HX_CHECK(a->pc1 - a->pc0 <= S.panel && (a->panel == nullptr || a->pc1 <= S.cw), "ring2k step: bad panel");
-        HX_CHECK(a->w <= S.cw && a->la1 <= (a->w ? a->w : S.cw) && (a->la1 <= a->la0 || a->la1 - a->la0 <= S.panel),
-                 "ring2k step: bad window / lookahead");
-        for (uint32_t q = 0; q < 2 * a->nswaps; ++q) HX_CHECK(a->swaps[q] < S.m, "ring2k step: swap row outside");
-        if (S.elem == 4)
+        Session* S = (Session*)s;
+        if (S->elem == 4)
             step_impl<uint32_t>(S, *a);
         else
             step_impl<uint64_t>(S, *a);
@@ -296,19 +424,15 @@ HX_API int hx_ring2k_step(hx_ring2k s, const hx_ring2k_step_args* a) {
 HX_API int hx_ring2k_fetch(hx_ring2k s, void* host, size_t ldh) {
     return guard("hx_ring2k_fetch", [&] {
         HX_CHECK(s != nullptr, "NULL session");
-        Session& S = *(Session*)s;
-        HX_CHECK(S.la_pending, "ring2k fetch: no lookahead pending");
-        HX_HIP(hipStreamSynchronize(S.s2));
-        S.la_pending = false;
-        copy_rows((char*)host, ldh * S.elem, S.hNext, (size_t)S.la_cols * S.elem, (size_t)S.la_cols * S.elem, S.la_rows);
+        fetch_impl((Session*)s, host, ldh);
     });
 }
 
 HX_API int hx_ring2k_close(hx_ring2k s) {
-    Session* S = (Session*)s;
-    if (!S) return HX_OK;
-    S->release();
-    delete S;
+    try {
+        session_free((Session*)s);
+    } catch (...) {
+    }
     return HX_OK;
 }
 
@@ -316,37 +440,36 @@ HX_API int hx_ring2k_gemm_sub_host(uint32_t elem, void* C, size_t ldc, const voi
                                    size_t ldu, uint32_t m, uint32_t n, uint32_t k) {
     return guard("hx_ring2k_gemm_sub_host", [&] {
         HX_CHECK(elem == 4 || elem == 8, "elem must be 4 or 8");
-        if (m == 0 || n == 0) return;
-        const size_t eb = elem;
-        char *dC = nullptr, *dL = nullptr, *dU = nullptr;
-        hipStream_t st = nullptr;
+        if (m == 0 || n == 0 || k == 0) return;
+        HX_CHECK(C && L && U && ldc >= n && ldl >= k && ldu >= n, "bad operands");
+        // a session whose window holds U (rows 0 .. k-1) over C (rows k .. k+m-1); K in slices of <= 64
+        const uint32_t kc_max = std::min<uint32_t>(k, kMaxPanel);
+        Session* S = session_open(elem, k + m, n, kc_max);
         try {
-            HX_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-            HX_HIP(hipMalloc(&dC, (size_t)m * n * eb));
-            HX_HIP(hipMalloc(&dL, (size_t)m * std::max<uint32_t>(k, 1) * eb));
-            HX_HIP(hipMalloc(&dU, (size_t)std::max<uint32_t>(k, 1) * n * eb));
-            HX_HIP(hipMemcpy2DAsync(dC, n * eb, C, ldc * eb, n * eb, m, hipMemcpyHostToDevice, st));
-            if (k) {
-                HX_HIP(hipMemcpy2DAsync(dL, k * eb, L, ldl * eb, k * eb, m, hipMemcpyHostToDevice, st));
-                HX_HIP(hipMemcpy2DAsync(dU, n * eb, U, ldu * eb, n * eb, k, hipMemcpyHostToDevice, st));
+            const size_t e = elem;
+            copy_block(S, (void*)U, ldu, 0, k, 0, n, true);
+            copy_block(S, C, ldc, k, m, 0, n, true);
+            std::vector<char> Lb((size_t)m * kc_max * e);
+            for (uint32_t k0 = 0; k0 < k; k0 += kMaxPanel) {
+                const uint32_t kc = std::min<uint32_t>(kMaxPanel, k - k0);
+                for (uint32_t i = 0; i < m; ++i)
+                    std::memcpy(Lb.data() + (size_t)i * kc * e, (const char*)L + ((size_t)i * ldl + k0) * e, kc * e);
+                HX_HIP(hipMemcpyAsync(S->d_L, Lb.data(), (size_t)m * kc * e, hipMemcpyHostToDevice, S->s));
+                if (elem == 4) {
+                    gemm_attrs<uint32_t>();
+                    launch_gemm<uint32_t>(S, k0, kc, k, k + m, 0, n, (const uint32_t*)S->d_L, kc, k, S->s);
+                } else {
+                    gemm_attrs<uint64_t>();
+                    launch_gemm<uint64_t>(S, k0, kc, k, k + m, 0, n, (const uint64_t*)S->d_L, kc, k, S->s);
+                }
+                HX_HIP(hipStreamSynchronize(S->s));     // Lb is refilled by the next slice
             }
-            if (elem == 4)
-                gemm_sub<uint32_t>((uint32_t*)dC, n, (const uint32_t*)dL, k, (const uint32_t*)dU, n, m, n, k, st);
-            else
-                gemm_sub<uint64_t>((uint64_t*)dC, n, (const uint64_t*)dL, k, (const uint64_t*)dU, n, m, n, k, st);
-            HX_HIP(hipMemcpy2DAsync(C, ldc * eb, dC, n * eb, n * eb, m, hipMemcpyDeviceToHost, st));
-            HX_HIP(hipStreamSynchronize(st));
+            copy_block(S, C, ldc, k, m, 0, n, false);
         } catch (...) {
-            (void)hipFree(dC);
-            (void)hipFree(dL);
-            (void)hipFree(dU);
-            if (st) (void)hipStreamDestroy(st);
+            session_free(S);
             throw;
         }
-        HX_HIP(hipFree(dC));
-        HX_HIP(hipFree(dL));
-        HX_HIP(hipFree(dU));
-        HX_HIP(hipStreamDestroy(st));
+        session_free(S);
     });
 }
 

@@ -14,11 +14,23 @@ bool rref_large(const hx_modctx& ctx, uint32_t m, uint32_t n, const uint32_t* A,
                 int64_t ldr, uint32_t* rank, uint32_t* pivots, uint32_t* pivot_rows, hipStream_t s, bool force);
 // Whether one job of this shape fits the one-workgroup kernel's LDS (hx_rref_batch.hip).
 bool rref_batch_fits(uint32_t m, uint32_t n);
+size_t rref_large_workspace_bytes(uint32_t m, uint32_t n);
+void rref_large_enqueue(const hx_modctx& ctx, uint32_t m, uint32_t n, const uint32_t* A, int64_t lda, uint32_t* R,
+                        int64_t ldr, uint32_t* d_rank, uint32_t* d_piv, uint32_t* d_prow, void* ws, size_t ws_bytes,
+                        hipStream_t s);
 }  // namespace hx
 
 using namespace hx;
 
 namespace {
+
+__global__ void rref_zero_rank_kernel(uint32_t* d_rank) {
+    if (threadIdx.x == 0) *d_rank = 0;
+}
+void rref_zero_rank_kernel_launch(uint32_t* d_rank, hipStream_t s) {
+    rref_zero_rank_kernel<<<1, 64, 0, s>>>(d_rank);
+    after_launch("rref_zero_rank_kernel");
+}
 
 // One job through the persistent workgroup kernel: R <- A, then R is eliminated in place.
 void rref_via_batch(const hx_modctx& ctx, uint32_t m, uint32_t n, const uint32_t* A, int64_t lda, uint32_t* R,
@@ -74,11 +86,32 @@ HX_API int hx_rref_u32(const hx_modctx* ctx, uint32_t m, uint32_t n, const uint3
         *rank = 0;
         if (m == 0 || n == 0) return;
         HX_CHECK(A != nullptr, "rref: A is NULL");
-        // big matrices, and shapes one workgroup cannot hold (m > 4096, or rows wider than its LDS
-        // panel, e.g. 5000 x 400 or 16 x 40000), take the whole-GPU path
+        // big matrices, and shapes one workgroup cannot hold well (m > 4096, or rows too wide for a
+        // 16-column LDS panel, e.g. 5000 x 400, 16 x 40000, 600 x 37000), take the whole-GPU path
         const bool force = !rref_batch_fits(m, n);
         if (rref_large(*ctx, m, n, A, lda, R, ldr, rank, pivots, pivot_rows, (hipStream_t)s, force)) return;
         rref_via_batch(*ctx, m, n, A, lda, R, ldr, rank, pivots, pivot_rows, (hipStream_t)s);
+    });
+}
+
+HX_API size_t hx_rref_workspace_bytes(uint32_t m, uint32_t n) {
+    return (m == 0 || n == 0) ? 0 : rref_large_workspace_bytes(m, n);
+}
+
+HX_API int hx_rref_u32_device(const hx_modctx* ctx, uint32_t m, uint32_t n, const uint32_t* A, int64_t lda, uint32_t* R,
+                              int64_t ldr, uint32_t* d_rank, uint32_t* d_pivots, uint32_t* d_pivot_rows, void* workspace,
+                              size_t workspace_bytes, hx_stream s) {
+    return guard("hx_rref_u32_device", [&] {
+        HX_CHECK(ctx != nullptr && d_rank != nullptr, "NULL argument");
+        HX_CHECK(ctx->p >= 2 && ctx->p < 0x80000000u, "rref: needs a prime 2 <= p < 2^31");
+        HX_CHECK(lda >= (int64_t)n && (R == nullptr || ldr >= (int64_t)n), "rref: leading dimension < n");
+        HX_CHECK(m == 0 || n == 0 || A != nullptr, "rref: A is NULL");
+        if (m == 0 || n == 0) {
+            rref_zero_rank_kernel_launch(d_rank, (hipStream_t)s);
+            return;
+        }
+        rref_large_enqueue(*ctx, m, n, A, lda, R, ldr, d_rank, d_pivots, d_pivot_rows, workspace, workspace_bytes,
+                           (hipStream_t)s);
     });
 }
 

@@ -523,15 +523,18 @@ int rref_batch_create_impl(const hx_modctx& ctx, const hx_rref_job* jobs, uint32
 int rref_batch_run_impl(void* b, hipStream_t s);
 void rref_batch_destroy_impl(void* b);
 
-// The LDS test of create for a batch holding one m x n job: the row lists (fixed) and a region of
-// max(m + 1, n, 2 * 32 * 64) words within the cap.
+// Whether one m x n job runs well in one workgroup: at most 4096 rows, and an LDS region (after the
+// row lists) that holds a panel of at least 16 columns -- the kernel halves its panel width until
+// m (w | 1) and w n fit the region, so a wider row would run in panels of a few columns (a
+// 600 x 37000 job: one column per panel, 37000 panels on one workgroup).  Such shapes take the
+// whole-GPU path instead.
 bool rref_batch_fits(uint32_t m, uint32_t n) {
     if (m > 4096) return false;
     const size_t m4 = (std::max<uint32_t>(m, 1) + 3) / 4 * 4, n4 = (std::max<uint32_t>(n, 1) + 3) / 4 * 4;
     const size_t fixed = m4 * 4 + m4 + 64;
     if (fixed + 4 * 64 >= kLdsCap) return false;
     const size_t cap_words = (kLdsCap - fixed) / 4;
-    return std::max<size_t>(std::max<size_t>(m4 + 1, n4), 2 * 32 * 64) <= cap_words;
+    return m4 * 17 <= cap_words && 16 * n4 <= cap_words && (size_t)2 * 32 * 64 <= cap_words;
 }
 
 int rref_batch_create_impl(const hx_modctx& ctx, const hx_rref_job* jobs, uint32_t njobs, int mode, void** out) {

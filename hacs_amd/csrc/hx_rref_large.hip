@@ -37,6 +37,8 @@
 //  All sizes (m_a, k, r) live in device memory; the host only knows upper bounds, so a panel is a
 //  fixed sequence of launches.
 #include <algorithm>
+#include <mutex>
+#include <unordered_map>
 #include <vector>
 
 #include "hx_elim.h"
@@ -58,6 +60,7 @@ struct LState {               // device-side counters
     uint32_t snap_k[2], snap_m[2];   // the sizes of panel i's trailing update, slot i & 1
     uint32_t snap_r[2];              // the pivot rows stored before panel i (its Gauss-Jordan rows)
     uint32_t spec;                   // 1: this panel's pivots came from its first SPR rows alone
+    uint32_t arrive;                 // blocks of the pivot search done (the last one merges when needed)
 };
 
 // The smallest-row rule on an LDS panel pan[u * ws + q] (rows in original order), division-free.
@@ -136,15 +139,16 @@ __device__ __forceinline__ void block_compact(const uint8_t* flag, uint32_t rows
     if (f) emit(off + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull)), tid);
 }
 
-// R = the rows of S not among the panel's k pivot rows sP[0..k) (LDS, any order), in order, and
-// their original indices (the next Schur complement's rows_orig), by one workgroup of RT threads:
-// row i goes to position i - #{t : sP[t] < i}.  Called by whichever kernel settles the panel.
+// R = the rows of S not among the panel's k pivot rows sP[0..k) (any order), in order, and their
+// original indices (the next Schur complement's rows_orig), by NT threads (thread ids 0 .. NT-1):
+// row i goes to position i - #{t : sP[t] < i}.  Called by whichever block settles the panel.
+template <int NT>
 __device__ void rest_rows_of(uint32_t ma, uint32_t k, const uint32_t* sP, uint32_t* R, const uint32_t* rows_orig,
                              uint32_t* rows_orig_next) {
     uint32_t pv[LW];
 #pragma unroll
     for (int t = 0; t < LW; ++t) pv[t] = (uint32_t)t < k ? sP[t] : 0xffffffffu;
-    for (uint32_t i = threadIdx.x; i < ma; i += RT) {
+    for (uint32_t i = threadIdx.x; i < ma; i += NT) {
         uint32_t below = 0;
         bool pivot = false;
 #pragma unroll
@@ -159,136 +163,123 @@ __device__ void rest_rows_of(uint32_t ma, uint32_t k, const uint32_t* sP, uint32
     }
 }
 
-// The rule on the first SPR rows of S alone.  If it finds a pivot in every panel column, those
-// are the panel's pivots over all rows: by induction over the columns, the earlier pivots agree,
-// so every row is eliminated alike, and column q's pivot, the smallest row with a non-zero entry,
-// lies among the first SPR rows, below every later row.  Then P, J, k, the row list R and
-// W = S[P, J]^{-1} are written and the tournament kernels return at once (st->spec = 1); otherwise
-// the tournament decides (st->spec = 0).
+// ---- the panel's pivot search: one launch --------------------------------------------------------
+// Block 0, wave 0: the rule on the first SPR = 64 rows of S alone ("spec").  If it finds a pivot in
+// every panel column, those are the panel's pivots over all rows: by induction over the columns the
+// earlier pivots agree, so every row is eliminated alike, and column q's pivot, the smallest row with
+// a non-zero entry, lies among the first 64 rows, below every later row.  Then P, J, k, the row list
+// R and W = S[P, J]^{-1} are written and st->spec = 1.
+// Blocks 1 .. nchunk: tournament level 0 on 256-row chunks, run beside the spec search whatever it
+// finds (their lists are used only when it fails).
+// The last block to finish merges the lists (16 per group, level by level) when the spec search
+// failed, and writes P, J, k and R.  So a panel the spec search settles costs one launch, and one
+// it does not settle costs no extra launch either.
 //
-// The elimination runs as division-free Gauss-Jordan (the chosen rows are reduced too, which
-// leaves the candidates and so the pivot choices unchanged) and every row carries its
-// coefficients: row_u = sigma_u orig_u + sum_t c_u[t] orig_{P_t} (only pivot rows are ever
-// subtracted).  A step row_u <- (v row_u - f row_bu) R^-1 maps sigma_u -> v sigma_u R^-1,
-// c_u[t'] -> (v c_u[t'] - f c_bu[t']) R^-1 and sets the new pivot's c_u[t] = -f sigma_bu R^-1.  At
-// the end row P_t is d_t e_t on the pivot columns, so M K = D with M[t][t'] = c_{P_t}[t'] (+ sigma
-// on the diagonal) and K^{-1} = D^{-1} M.  Row layout: panel columns 0..wc-1, sigma at wc, c[t] at
-// wc + 1 + t.  Step q writes every entry of the rows u != bu with f != 0 except column q (left
-// stale; the pivots so far are exactly columns 0..q-1, so a pivot row is read as zero there), so
-// no entry a step writes is read by another thread in that step; one barrier per step.  The
-// entries left of q matter only as a chosen row's own pivot entry d_t, scaled with its row.  It first moves the state past the previous panel (r += k,
-// m_a = m_new, k = 0; a no-op before the first panel).
+// The spec search, one lane per row, the row in registers: panel columns x[0..31], sigma and the
+// coefficients c[0..31] on the pivot rows so far, row_u = sigma_u orig_u + sum_t c_u[t] orig_{P_t}.
+// Per column q the pivot is the lowest lane with a non-zero (one ballot), its entries are broadcast
+// by readlane (uniform values), and every other row with f = x[q] != 0 takes the division-free
+// Gauss-Jordan step row_u <- (v row_u - f row_b) R^{-1}: x[q'] for q' > q, sigma, c[t] for t < k and
+// the new c[k] = -f sigma_b R^{-1}; x[q] becomes exactly 0 and every other column left of q is zero
+// in the pivot row, so a row's own pivot entry d is the only one left of q that changes (scaled by
+// v R^{-1}).  At the end row P_t is d_t e_t on the pivot columns, so M K = D with M[t][t'] =
+// c_{P_t}[t'] (+ sigma on the diagonal) and K^{-1} = D^{-1} M.  No barrier, no LDS round trip.
 constexpr int SPR = 64;
-constexpr int SWS = 2 * LW + 1;   // spec row stride: 32 panel columns, sigma, 32 coefficients
+constexpr int CH_LDS = CH * (LW + 1);                    // chunk panel words
+constexpr int MG_LDS = GRP * LW * (LW + 1);              // merge panel words
+
 template <class Ar>
-__global__ __launch_bounds__(RT) void rrp_spec_kernel(const uint32_t* S, int64_t ld, LState* st, uint32_t wc, Ar ar,
-                                                      uint32_t* P, uint32_t* J, uint32_t* R, const uint32_t* rows_orig,
-                                                      uint32_t* rows_orig_next, uint32_t* Wg) {
-    __shared__ uint32_t pan[SPR * SWS];
-    __shared__ uint32_t s_cand[2][RT / 64];
-    __shared__ uint32_t s_u[LW], s_q[LW], s_dinv[LW], s_ma, s_ok;
-    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, p = ar.p;
-    if (tid == 0) {
-        st->r += st->k;
-        st->m_a = st->m_new;
-        st->k = 0;
-        s_ma = st->m_new;
-    }
-    __syncthreads();
-    const uint32_t ma = s_ma, rows = min((uint32_t)SPR, ma), width = wc + 1 + LW;
-    for (uint32_t idx = tid; idx < rows * width; idx += RT) {
-        const uint32_t u = idx / width, c = idx % width;
-        pan[u * SWS + c] = c < wc ? S[(int64_t)u * ld + c] : (c == wc ? 1u : 0u);
-    }
-    static_assert(SPR * 16 == RT, "sixteen threads per row");
-    const uint32_t u = tid / 16, part = tid % 16;
-    const bool mine = u < rows;
-    uint32_t* row = pan + u * SWS;
-    bool done = false;
-    uint32_t k = 0;
-    __syncthreads();
-    for (uint32_t q = 0; q < wc; ++q) {
-        // rows grow with the thread index, so the wave's smallest candidate is its lowest hit lane
-        const uint64_t bal = __ballot(mine && part == (q % 16) && !done && row[q] != 0);
-        if (lane == 0) s_cand[q & 1][wave] = bal ? (wave * 64 + (uint32_t)__builtin_ctzll(bal)) / 16 : 0xffffffffu;
-        __syncthreads();
-        uint32_t bu = s_cand[q & 1][0];
+__device__ void spec_wave(const uint32_t* S, int64_t ld, LState* st, uint32_t wc, const Ar& ar, uint32_t* P,
+                          uint32_t* J, uint32_t* Wg, uint32_t* s_u, uint32_t* s_q, uint32_t* s_res) {
+    const uint32_t lane = threadIdx.x, p = ar.p;
+    const uint32_t ma = st->m_new;                           // the rows active before this panel's advance
+    const uint32_t rows = min((uint32_t)SPR, ma);
+    const bool mine = lane < rows;
+    uint32_t x[LW], c[LW];
 #pragma unroll
-        for (int w = 1; w < RT / 64; ++w) bu = min(bu, s_cand[q & 1][w]);
-        if (bu == 0xffffffffu) break;                  // a column without a pivot here: the tournament decides
-        const uint32_t* prow = pan + bu * SWS;
-        if (mine) {
-            if (u == bu) {
-                done = true;
+    for (int q = 0; q < LW; ++q) {
+        x[q] = (mine && (uint32_t)q < wc) ? S[(int64_t)lane * ld + q] : 0u;
+        c[q] = 0u;
+    }
+    uint32_t sig = 1u, d = 0u, myt = 0xffffffffu, k = 0;
+    bool done = false, ok = true;
+#pragma unroll
+    for (int q = 0; q < LW; ++q) {
+        if ((uint32_t)q < wc && ok) {
+            const uint64_t bal = __ballot(mine && !done && x[q] != 0u);
+            if (bal == 0) {
+                ok = false;                                  // a column without a pivot here: the tournament decides
             } else {
-                const uint32_t f = row[q];
-                if (f != 0) {
-                    const uint32_t v = prow[q], fneg = p - f, sb = prow[wc], cnew = wc + 1 + k;
-                    // this thread's (at most five) columns, every read issued before the writes; the
-                    // pivot row is zero at sigma's slot and at the earlier pivot columns c < q (its
-                    // entries there are stale, never zeroed), the new coefficient starts from zero
-                    static_assert(16 * 4 + 1 >= SWS - 1, "five columns per thread cover a row");
-                    uint32_t x[5], y[5];
+                const uint32_t bu = (uint32_t)__builtin_ctzll(bal);
+                const uint32_t v = __builtin_amdgcn_readlane(x[q], bu);
+                const uint32_t ys = __builtin_amdgcn_readlane(sig, bu);
+                uint32_t y[LW], yc[LW];
 #pragma unroll
-                    for (int i = 0; i < 5; ++i) {
-                        const uint32_t c = part + 16 * i;
-                        const bool act = c <= cnew && c != q;
-                        x[i] = (act && c != cnew) ? row[c] : 0u;
-                        y[i] = !act ? 0u : (c == cnew ? sb : ((c == wc || c < q) ? 0u : prow[c]));
-                    }
+                for (int q2 = q + 1; q2 < LW; ++q2) y[q2] = __builtin_amdgcn_readlane(x[q2], bu);
 #pragma unroll
-                    for (int i = 0; i < 5; ++i) {
-                        const uint32_t c = part + 16 * i;
-                        if (c <= cnew && c != q) row[c] = ar.dfree(v, x[i], fneg, y[i]);
+                for (int t = 0; t < LW; ++t) yc[t] = (uint32_t)t < k ? __builtin_amdgcn_readlane(c[t], bu) : 0u;
+                if (lane == bu) {
+                    done = true;
+                    d = v;
+                    myt = k;
+                } else if (mine && x[q] != 0u) {
+                    const uint32_t fneg = p - x[q];
+#pragma unroll
+                    for (int q2 = q + 1; q2 < LW; ++q2) x[q2] = ar.dfree(v, x[q2], fneg, y[q2]);
+                    x[q] = 0u;
+                    sig = ar.dfree(v, sig, fneg, 0u);
+                    if (done) d = ar.dfree(v, d, fneg, 0u);
+#pragma unroll
+                    for (int t = 0; t < LW; ++t) {
+                        if ((uint32_t)t < k) c[t] = ar.dfree(v, c[t], fneg, yc[t]);
+                        else if ((uint32_t)t == k) c[t] = ar.dfree(v, 0u, fneg, ys);
                     }
                 }
+                if (lane == 0) {
+                    s_u[k] = bu;
+                    s_q[k] = (uint32_t)q;
+                }
+                ++k;
             }
         }
-        if (tid == 0) { s_u[k] = bu; s_q[k] = q; }
-        ++k;
     }
-    const bool ok = k == wc;
-    __syncthreads();
-    if (ok && tid < k) {
-        P[tid] = s_u[tid];
-        J[tid] = s_q[tid];
-        const uint32_t iv = invm(pan[s_u[tid] * SWS + s_q[tid]], p);
+    ok = ok && k == wc;
+    if (lane == 0) {
+        s_res[0] = ok ? 1u : 0u;
+        s_res[1] = k;
+        s_res[2] = ma;
+    }
+    if (!ok) return;
+    __builtin_amdgcn_wave_barrier();
+    // settled: the pivots and W = D^{-1} M (Montgomery form, prep)
+    if (lane < k) {
+        P[lane] = s_u[lane];
+        J[lane] = s_q[lane];
+    }
+    if (done) {
+        const uint32_t iv = invm(d, p);
         if (iv == 0) atomicExch(&st->status, 1);
-        s_dinv[tid] = ar.prep(iv);
-    }
-    if (tid == 0) {
-        st->spec = ok ? 1u : 0u;
-        if (ok) {
-            st->k = k;
-            st->m_new = ma - k;
+        const uint32_t dinv = ar.prep(iv);
+#pragma unroll
+        for (int t = 0; t < LW; ++t) {
+            if ((uint32_t)t < k) {
+                const uint32_t mval = (uint32_t)t == myt ? addm(c[t], sig, p) : c[t];
+                Wg[myt * LW + t] = ar.prep(ar.mul(mval, dinv));
+            }
         }
-        s_ok = ok ? 1u : 0u;
-    }
-    __syncthreads();
-    if (s_ok) {
-        // W = D^{-1} M in Montgomery form (prep), row t, column t' (row stride LW)
-        for (uint32_t idx = tid; idx < k * k; idx += RT) {
-            const uint32_t t = idx / k, t2 = idx % k;
-            const uint32_t* pr = pan + s_u[t] * SWS;
-            uint32_t mval = pr[wc + 1 + t2];
-            if (t2 == t) mval = addm(mval, pr[wc], p);
-            Wg[t * LW + t2] = ar.prep(ar.mul(mval, s_dinv[t]));
-        }
-        rest_rows_of(ma, k, s_u, R, rows_orig, rows_orig_next);
     }
 }
 
-// Level 0: chunk `blockIdx.x` of S's rows -> its candidate rows (ascending S indices).
+// Level 0: chunk ch of S's rows (ma rows active) -> its candidate rows (ascending S indices).
 template <class Ar>
-__global__ __launch_bounds__(RT) void rrp_chunk_kernel(const uint32_t* S, int64_t ld, const LState* st, uint32_t wc,
-                                                       Ar ar, uint32_t* cand, uint32_t* cnt) {
-    __shared__ uint32_t pan[CH * (LW + 1)];
+__device__ void chunk_block(const uint32_t* S, int64_t ld, uint32_t ma, uint32_t wc, const Ar& ar, uint32_t ch,
+                            uint32_t* pan, uint32_t* cand, uint32_t* cnt) {
     __shared__ uint8_t chosen[CH];
     __shared__ uint32_t s_u[LW], s_q[LW];
-    if (st->spec) return;                              // the first rows settled the panel
-    const uint32_t ma = st->m_a, r0 = blockIdx.x * CH;
+    __shared__ uint32_t s_wcnt[RT / 64];
+    const uint32_t r0 = ch * CH;
     if (r0 >= ma) {
-        if (threadIdx.x == 0) cnt[blockIdx.x] = 0;
+        if (threadIdx.x == 0) cnt[ch] = 0;
         return;
     }
     const uint32_t rows = min((uint32_t)CH, ma - r0), ws = wc | 1u;
@@ -298,24 +289,22 @@ __global__ __launch_bounds__(RT) void rrp_chunk_kernel(const uint32_t* S, int64_
     }
     static_assert(CH * 4 <= RT, "four threads per chunk row");
     const uint32_t k = lds_rule<4>(pan, rows, wc, ws, chosen, ar, s_u, s_q);
-    __shared__ uint32_t s_wcnt[RT / 64];
-    block_compact(chosen, rows, s_wcnt, [&](uint32_t pos, uint32_t u) { cand[blockIdx.x * LW + pos] = r0 + u; });
-    if (threadIdx.x == 0) cnt[blockIdx.x] = k;
+    block_compact(chosen, rows, s_wcnt, [&](uint32_t pos, uint32_t u) { cand[ch * LW + pos] = r0 + u; });
+    if (threadIdx.x == 0) cnt[ch] = k;
 }
 
-// Merge: group `blockIdx.x` of up to GRP candidate lists -> one list; the final level writes P, J, k.
+// Merge: group g of up to GRP candidate lists -> one list; the final level writes P, J, k, R and the
+// state (ma rows active before the panel).
 template <class Ar>
-__global__ __launch_bounds__(RT) void rrp_merge_kernel(const uint32_t* S, int64_t ld, uint32_t wc, Ar ar,
-                                                       const uint32_t* cand_in, const uint32_t* cnt_in, uint32_t nin,
-                                                       uint32_t* cand_out, uint32_t* cnt_out, int final_level,
-                                                       uint32_t* P, uint32_t* J, LState* st, uint32_t* R,
-                                                       const uint32_t* rows_orig, uint32_t* rows_orig_next) {
-    __shared__ uint32_t pan[GRP * LW * (LW + 1)];
+__device__ void merge_group(const uint32_t* S, int64_t ld, uint32_t wc, const Ar& ar, const uint32_t* cand_in,
+                            const uint32_t* cnt_in, uint32_t nin, uint32_t g, uint32_t* cand_out, uint32_t* cnt_out,
+                            bool final_level, uint32_t ma, uint32_t* P, uint32_t* J, LState* st, uint32_t* R,
+                            const uint32_t* rows_orig, uint32_t* rows_orig_next, uint32_t* pan) {
     __shared__ uint32_t rowid[GRP * LW];
     __shared__ uint8_t chosen[GRP * LW];
-    __shared__ uint32_t s_u[LW], s_q[LW], s_off[GRP + 1];
-    if (st->spec) return;                              // the first rows settled the panel
-    const uint32_t g0 = blockIdx.x * GRP, ng = min((uint32_t)GRP, nin - g0);
+    __shared__ uint32_t s_u[LW], s_q[LW], s_off[GRP + 1], s_P[LW];
+    __shared__ uint32_t s_wcnt[RT / 64];
+    const uint32_t g0 = g * GRP, ng = min((uint32_t)GRP, nin - g0);
     if (threadIdx.x < ng) s_off[threadIdx.x + 1] = cnt_in[g0 + threadIdx.x];   // the list lengths, in parallel
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -334,38 +323,101 @@ __global__ __launch_bounds__(RT) void rrp_merge_kernel(const uint32_t* S, int64_
     }
     static_assert(GRP * LW * 2 <= RT, "two threads per candidate row");
     const uint32_t k = lds_rule<2>(pan, rows, wc, ws, chosen, ar, s_u, s_q);
-    __shared__ uint32_t s_wcnt[RT / 64];
-    block_compact(chosen, rows, s_wcnt,
-                  [&](uint32_t pos, uint32_t u) { cand_out[blockIdx.x * LW + pos] = rowid[u]; });
-    if (threadIdx.x == 0) cnt_out[blockIdx.x] = k;
+    block_compact(chosen, rows, s_wcnt, [&](uint32_t pos, uint32_t u) { cand_out[g * LW + pos] = rowid[u]; });
+    if (threadIdx.x == 0) cnt_out[g] = k;
     if (final_level) {
         if (threadIdx.x < k) {
             P[threadIdx.x] = rowid[s_u[threadIdx.x]];
             J[threadIdx.x] = s_q[threadIdx.x];
+            s_P[threadIdx.x] = rowid[s_u[threadIdx.x]];
         }
+        __syncthreads();
+        rest_rows_of<RT>(ma, k, s_P, R, rows_orig, rows_orig_next);
         if (threadIdx.x == 0) {
             st->k = k;
-            st->m_new = st->m_a - k;
+            st->m_new = ma - k;
         }
-        // the pivot rows as S indices, then R
-        __shared__ uint32_t s_P[LW];
-        if (threadIdx.x < k) s_P[threadIdx.x] = rowid[s_u[threadIdx.x]];
-        __syncthreads();
-        rest_rows_of(st->m_a, k, s_P, R, rows_orig, rows_orig_next);
     }
+    __syncthreads();
+}
+
+template <class Ar>
+__global__ __launch_bounds__(RT) void rrp_pivot_kernel(const uint32_t* S, int64_t ld, LState* st, uint32_t wc, Ar ar,
+                                                       uint32_t* P, uint32_t* J, uint32_t* R, const uint32_t* rows_orig,
+                                                       uint32_t* rows_orig_next, uint32_t* Wg, uint32_t* cand0,
+                                                       uint32_t* cnt0, uint32_t* cand1, uint32_t* cnt1, uint32_t nchunk) {
+    __shared__ uint32_t pool[MG_LDS > CH_LDS ? MG_LDS : CH_LDS];
+    __shared__ uint32_t s_u[LW], s_q[LW], s_res[3];
+    __shared__ uint32_t s_last;
+    if (blockIdx.x == 0) {
+        if (threadIdx.x < 64) spec_wave(S, ld, st, wc, ar, P, J, Wg, s_u, s_q, s_res);
+        __syncthreads();
+        if (s_res[0]) {
+            // settled: the row list by the whole block, then the state moves past the previous panel
+            const uint32_t k = s_res[1], ma = s_res[2];
+            rest_rows_of<RT>(ma, k, s_u, R, rows_orig, rows_orig_next);
+            if (threadIdx.x == 0) {
+                st->r += st->k;
+                st->m_a = ma;
+                st->k = k;
+                st->m_new = ma - k;
+                st->spec = 1u;
+            }
+        } else if (threadIdx.x == 0) {
+            st->spec = 0u;
+        }
+    } else {
+        // the rows active before this panel's advance (st->m_new may be rewritten by a settled spec search
+        // meanwhile; then these lists are not used)
+        const uint32_t ma = __hip_atomic_load(&st->m_new, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        chunk_block(S, ld, ma, wc, ar, blockIdx.x - 1, pool, cand0, cnt0);
+    }
+    // arrival: every thread's writes reach the device before its block counts itself
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t n = __hip_atomic_fetch_add(&st->arrive, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = n + 1 == gridDim.x ? 1u : 0u;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    if (threadIdx.x == 0) st->arrive = 0u;                   // for the next panel (no other block reads it now)
+    const uint32_t spec = __hip_atomic_load(&st->spec, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    if (spec) return;
+    // the tournament decides: the advance past the previous panel, then the merge levels
+    const uint32_t ma = __hip_atomic_load(&st->m_new, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        st->r += st->k;
+        st->m_a = ma;
+        st->k = 0;
+    }
+    uint32_t nin = nchunk;
+    int lv = 0;
+    do {
+        const uint32_t nout = (nin + GRP - 1) / GRP;
+        const bool odd = (lv & 1) != 0;
+        for (uint32_t g = 0; g < nout; ++g)
+            merge_group(S, ld, wc, ar, odd ? cand1 : cand0, odd ? cnt1 : cnt0, nin, g, odd ? cand0 : cand1,
+                        odd ? cnt0 : cnt1, nout == 1, ma, P, J, st, R, rows_orig, rows_orig_next, pool);
+        __threadfence_block();
+        __syncthreads();
+        nin = nout;
+        ++lv;
+    } while (nin > 1);
 }
 
 // The k <= 32 products of a panel on the INT8 matrix cores: Cout[i, j] = Cin[c_rows[i], j] -
 // sum_{q<k} A[a_rows[i], a_cols[q]] X[q, j] mod p.  A's entries are split into L balanced 8-bit
 // limbs on the fly; X's limb planes come from panel_x (plane s, column j: 32 k-bytes); the L^2
 // limb products of the one 32-deep k-step accumulate per anti-diagonal in int32 and the exact
-// recombination (hx_limbs.h) gives the residue.  One wave per 32-row block, four 32-column blocks
-// in turn; a block's Cin entries are loaded before its MFMAs.  Every output is written, also when
+// recombination (hx_limbs.h) gives the residue.  One wave per 32 x 32 output block (UCB = 1: many
+// waves in flight hide the gathered loads); a block's Cin entries are loaded before its MFMAs.  Every output is written, also when
 // k = 0 (Cout = Cin: the trailing update doubles as the compaction into the next Schur complement).
 // No barriers: a wave past the rows returns at once.
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
-constexpr int UCB = 4;        // 32-column blocks per wave
+constexpr int UCB = 1;        // 32-column blocks per wave (one: more waves in flight hide the C loads)
 struct UpdArgs {
     uint32_t m, n;                           // row bound (m_dev caps it) and columns
     const uint32_t* m_dev;
@@ -589,7 +641,7 @@ __global__ void advance_kernel(LState* st) {
 
 __global__ void init_state_kernel(LState* st, uint32_t m, uint32_t* rows_orig, uint32_t* U, size_t ucount) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) { st->m_a = m; st->r = 0; st->k = 0; st->m_new = m; st->status = 0; st->spec = 0; }
+    if (i == 0) { st->m_a = m; st->r = 0; st->k = 0; st->m_new = m; st->status = 0; st->spec = 0; st->arrive = 0; }
     if (i < m) rows_orig[i] = (uint32_t)i;
     for (size_t t = i; t < ucount; t += (size_t)gridDim.x * blockDim.x) U[t] = 0;
 }
@@ -605,8 +657,41 @@ __global__ void gj_gather_kernel(const uint32_t* U, int64_t ldu, uint32_t c, con
     }
 }
 
-// R = the r rank rows of U, then zero rows (the RREF with its zero rows last).
-__global__ void rref_out_kernel(const uint32_t* U, uint32_t n, uint32_t r, uint32_t m, uint32_t* R, int64_t ldr) {
+inline uint32_t gsz(uint64_t n) { return (uint32_t)std::min<uint64_t>((n + 255) / 256, 65536); }
+
+// A second stream and two events for the lookahead, per calling stream (created once, reused by
+// every elimination issued on that stream, so a call allocates nothing and can be captured).
+struct SideStream {
+    hipStream_t s2 = nullptr;
+    hipEvent_t ev_next = nullptr, ev_rest = nullptr;
+};
+SideStream side_for(hipStream_t s) {
+    static std::mutex mu;
+    static std::unordered_map<hipStream_t, SideStream> map;
+    std::lock_guard<std::mutex> lk(mu);
+    SideStream& ss = map[s];
+    if (ss.s2 == nullptr) {
+        HX_HIP(hipStreamCreateWithFlags(&ss.s2, hipStreamNonBlocking));
+        HX_HIP(hipEventCreateWithFlags(&ss.ev_next, hipEventDisableTiming));
+        HX_HIP(hipEventCreateWithFlags(&ss.ev_rest, hipEventDisableTiming));
+    }
+    return ss;
+}
+
+// The results on the device: rank = r (0xffffffff when a pivot had no inverse), pivots, pivot rows.
+__global__ void finish_kernel(const LState* st, const uint32_t* piv, const uint32_t* prow, uint32_t mn, uint32_t* d_rank,
+                              uint32_t* d_piv, uint32_t* d_prow) {
+    const uint32_t r = st->r, i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *d_rank = st->status ? 0xffffffffu : r;
+    if (i < r && i < mn) {
+        if (d_piv) d_piv[i] = piv[i];
+        if (d_prow) d_prow[i] = prow[i];
+    }
+}
+
+// R = the rank rows of U (the rank read on the device), then zero rows.
+__global__ void rref_out_dev_kernel(const uint32_t* U, uint32_t n, const LState* st, uint32_t m, uint32_t* R, int64_t ldr) {
+    const uint32_t r = st->r;
     const uint64_t total = (uint64_t)m * n;
     for (uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
          idx += (uint64_t)gridDim.x * blockDim.x) {
@@ -615,14 +700,23 @@ __global__ void rref_out_kernel(const uint32_t* U, uint32_t n, uint32_t r, uint3
     }
 }
 
-inline uint32_t gsz(uint64_t n) { return (uint32_t)std::min<uint64_t>((n + 255) / 256, 65536); }
+// Workspace words of one m x n elimination (the buffers of enqueue_large, each rounded to 64 words).
+size_t large_workspace_words(uint32_t m, uint32_t n) {
+    const uint32_t mn = std::min(m, n), nchunk = (m + CH - 1) / CH;
+    const size_t w[] = {(size_t)m * n, (size_t)m * n, (size_t)mn * n, (size_t)2 * LW * n, m, m, (size_t)2 * m,
+                        (size_t)nchunk * LW, (size_t)nchunk * LW, nchunk, nchunk, 2 * LW, 2 * LW, mn, mn, 16,
+                        (size_t)mn * LW, (size_t)2 * 32 * n, (size_t)LW * LW};
+    size_t total = 0;
+    for (size_t x : w) total += (x + 63) / 64 * 64;
+    return total;
+}
 
 template <class Ar>
-void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const uint32_t* A, int64_t lda,
-               uint32_t* Rout, int64_t ldr, uint32_t* rank, uint32_t* pivots_h, uint32_t* pivot_rows_h, hipStream_t s) {
+void enqueue_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const uint32_t* A, int64_t lda,
+                   uint32_t* Rout, int64_t ldr, uint32_t* d_rank, uint32_t* d_piv, uint32_t* d_prow, uint32_t* base,
+                   hipStream_t s, const SideStream& side) {
     const uint32_t mn = std::min(m, n);
     const uint32_t nchunk = (m + CH - 1) / CH;
-    // device buffers (one allocation)
     std::vector<size_t> sizes;
     auto add = [&](size_t words) { sizes.push_back((words + 63) / 64 * 64); };
     add((size_t)m * n);           // 0 S0
@@ -647,8 +741,6 @@ void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const
     size_t total = 0;
     std::vector<size_t> off;
     for (size_t w : sizes) { off.push_back(total); total += w; }
-    uint32_t* base = nullptr;
-    HX_HIP(hipMalloc(&base, total * 4));
     auto at = [&](int i) { return base + off[i]; };
     uint32_t* Sb[2] = {at(0), at(1)};
     uint32_t* U = at(2);
@@ -682,13 +774,10 @@ void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const
     // lookahead: the trailing update of panel i is split into the next panel's LW columns (on s)
     // and the rest (on s2, overlapping panel i+1's tournament), followed on s2 by the Gauss-Jordan
     // gather and product; panel i+1's X waits for s2 up to the gather (ev_rest).
-    hipStream_t s2 = nullptr;
-    hipEvent_t ev_next = nullptr, ev_rest = nullptr;
-    HX_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
-    HX_HIP(hipEventCreateWithFlags(&ev_next, hipEventDisableTiming));
-    HX_HIP(hipEventCreateWithFlags(&ev_rest, hipEventDisableTiming));
+    hipStream_t s2 = side.s2;
+    hipEvent_t ev_next = side.ev_next, ev_rest = side.ev_rest;
     bool rest_pending = false;
-    try {
+    {
         HX_HIP(hipMemcpy2DAsync(Sb[0], (size_t)n * 4, A, (size_t)lda * 4, (size_t)n * 4, m, hipMemcpyDeviceToDevice, s));
         init_state_kernel<<<gsz(std::max<size_t>(m, (size_t)mn * n)), 256, 0, s>>>(st, m, ro[0], U, (size_t)mn * n);
         after_launch("init_state_kernel");
@@ -704,22 +793,9 @@ void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const
             // 1. tournament (reads only this panel's LW columns, already up to date), after the
             //    search over the first SPR rows, which settles most panels alone
             uint32_t* Rl = Rlb + (size_t)slot * m;
-            rrp_spec_kernel<Ar><<<1, RT, 0, s>>>(S, n, st, wc, ar, P, J, Rl, ro[cur], ro[cur ^ 1], Wg);
-            after_launch("rrp_spec_kernel");
-            rrp_chunk_kernel<Ar><<<nchunk, RT, 0, s>>>(S, n, st, wc, ar, cand[0], cnt[0]);
-            after_launch("rrp_chunk_kernel");
-            uint32_t nin = nchunk;
-            int lv = 0;
-            do {
-                const uint32_t nout = (nin + GRP - 1) / GRP;
-                const int fin = nout == 1;
-                rrp_merge_kernel<Ar><<<nout, RT, 0, s>>>(S, n, wc, ar, cand[lv & 1], cnt[lv & 1], nin,
-                                                         cand[(lv + 1) & 1], cnt[(lv + 1) & 1], fin, P, J, st, Rl,
-                                                         ro[cur], ro[cur ^ 1]);
-                after_launch("rrp_merge_kernel");
-                nin = nout;
-                ++lv;
-            } while (nin > 1);
+            rrp_pivot_kernel<Ar><<<1 + nchunk, RT, 0, s>>>(S, n, st, wc, ar, P, J, Rl, ro[cur], ro[cur ^ 1], Wg, cand[0],
+                                                           cnt[0], cand[1], cnt[1], nchunk);
+            after_launch("rrp_pivot_kernel");
             // the previous panel's update must be complete before its rows are read whole
             if (rest_pending) {
                 HX_HIP(hipStreamWaitEvent(s, ev_rest, 0));
@@ -805,39 +881,14 @@ void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const
         HX_HIP(hipEventRecord(ev_rest, s2));
         HX_HIP(hipStreamWaitEvent(s, ev_rest, 0));
         rest_pending = false;
-        LState h;
-        HX_HIP(hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, s));
-        HX_HIP(hipStreamSynchronize(s));
-        if (h.status != 0) throw Error{HX_E_NOT_INVERTIBLE, "a pivot had no inverse mod p (is p prime?)"};
-        const uint32_t r = h.r;
-        *rank = r;
-        std::vector<uint32_t> hp(std::max<uint32_t>(r, 1)), hr(std::max<uint32_t>(r, 1));
-        if (r) {
-            HX_HIP(hipMemcpyAsync(hp.data(), piv, (size_t)r * 4, hipMemcpyDeviceToHost, s));
-            HX_HIP(hipMemcpyAsync(hr.data(), prow, (size_t)r * 4, hipMemcpyDeviceToHost, s));
-            HX_HIP(hipStreamSynchronize(s));
-            if (pivots_h) std::copy(hp.begin(), hp.begin() + r, pivots_h);
-            if (pivot_rows_h) std::copy(hr.begin(), hr.begin() + r, pivot_rows_h);
-        }
+        // the results, on the device: rank (0xffffffff when a pivot had no inverse), pivots, pivot rows, R
+        finish_kernel<<<gsz(std::max<uint32_t>(mn, 1)), 256, 0, s>>>(st, piv, prow, mn, d_rank, d_piv, d_prow);
+        after_launch("finish_kernel");
         if (Rout) {
-            rref_out_kernel<<<gsz((uint64_t)m * n), 256, 0, s>>>(U, n, r, m, Rout, ldr);
-            after_launch("rref_out_kernel");
-            HX_HIP(hipStreamSynchronize(s));
+            rref_out_dev_kernel<<<gsz((uint64_t)m * n), 256, 0, s>>>(U, n, st, m, Rout, ldr);
+            after_launch("rref_out_dev_kernel");
         }
-    } catch (...) {
-        (void)hipStreamSynchronize(s);
-        (void)hipStreamSynchronize(s2);
-        (void)hipEventDestroy(ev_next);
-        (void)hipEventDestroy(ev_rest);
-        (void)hipStreamDestroy(s2);
-        (void)hipFree(base);
-        throw;
     }
-    HX_HIP(hipStreamSynchronize(s2));
-    HX_HIP(hipEventDestroy(ev_next));
-    HX_HIP(hipEventDestroy(ev_rest));
-    HX_HIP(hipStreamDestroy(s2));
-    HX_HIP(hipFree(base));
 }
 
 }  // namespace
@@ -845,13 +896,50 @@ void run_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, const
 // The whole-GPU path takes matrices with at least 1024 rows and m * n >= 2^21, and (force) every
 // shape the one-workgroup kernel cannot hold (more than 4096 rows, or rows too wide for its LDS
 // panel); the rest run in one workgroup (hx_rref_batch.hip).
+size_t rref_large_workspace_bytes(uint32_t m, uint32_t n) { return large_workspace_words(m, n) * 4; }
+
+// Enqueues the elimination on s: no allocation, no host synchronisation (capturable).
+void rref_large_enqueue(const hx_modctx& ctx, uint32_t m, uint32_t n, const uint32_t* A, int64_t lda, uint32_t* R,
+                        int64_t ldr, uint32_t* d_rank, uint32_t* d_piv, uint32_t* d_prow, void* ws, size_t ws_bytes,
+                        hipStream_t s) {
+    HX_CHECK(ws != nullptr && ws_bytes >= rref_large_workspace_bytes(m, n), "rref: workspace too small");
+    HX_CHECK(d_rank != nullptr, "rref: rank is NULL");
+    const SideStream side = side_for(s);
+    if (ctx.p & 1u)
+        enqueue_large(ctx, make_mont(ctx.p), m, n, A, lda, R, ldr, d_rank, d_piv, d_prow, (uint32_t*)ws, s, side);
+    else
+        enqueue_large(ctx, make_barrett(ctx.p), m, n, A, lda, R, ldr, d_rank, d_piv, d_prow, (uint32_t*)ws, s, side);
+}
+
 bool rref_large(const hx_modctx& ctx, uint32_t m, uint32_t n, const uint32_t* A, int64_t lda, uint32_t* R,
                 int64_t ldr, uint32_t* rank, uint32_t* pivots, uint32_t* pivot_rows, hipStream_t s, bool force) {
     if (!force && (m < 1024 || (uint64_t)m * n < (1ull << 21))) return false;
-    if (ctx.p & 1u)
-        run_large(ctx, make_mont(ctx.p), m, n, A, lda, R, ldr, rank, pivots, pivot_rows, s);
-    else
-        run_large(ctx, make_barrett(ctx.p), m, n, A, lda, R, ldr, rank, pivots, pivot_rows, s);
+    // the workspace (and three result arrays) from the arena: after the first call of a shape the block
+    // is reused, no device allocation
+    const uint32_t mn = std::min(m, n);
+    const size_t wsb = rref_large_workspace_bytes(m, n), res_words = 2 * (size_t)std::max<uint32_t>(mn, 1) + 64;
+    uint32_t* block = (uint32_t*)arena_alloc(wsb + res_words * 4);
+    try {
+        uint32_t* d_res = (uint32_t*)((char*)block + wsb);
+        uint32_t* d_piv = d_res + 64;
+        uint32_t* d_prow = d_piv + std::max<uint32_t>(mn, 1);
+        rref_large_enqueue(ctx, m, n, A, lda, R, ldr, d_res, d_piv, d_prow, block, wsb, s);
+        uint32_t h_rank = 0;
+        HX_HIP(hipMemcpyAsync(&h_rank, d_res, 4, hipMemcpyDeviceToHost, s));
+        HX_HIP(hipStreamSynchronize(s));
+        if (h_rank == 0xffffffffu) throw Error{HX_E_NOT_INVERTIBLE, "a pivot had no inverse mod p (is p prime?)"};
+        *rank = h_rank;
+        if (h_rank) {
+            if (pivots) HX_HIP(hipMemcpyAsync(pivots, d_piv, (size_t)h_rank * 4, hipMemcpyDeviceToHost, s));
+            if (pivot_rows) HX_HIP(hipMemcpyAsync(pivot_rows, d_prow, (size_t)h_rank * 4, hipMemcpyDeviceToHost, s));
+            HX_HIP(hipStreamSynchronize(s));
+        }
+    } catch (...) {
+        (void)hipStreamSynchronize(s);
+        arena_free(block);
+        throw;
+    }
+    arena_free(block);
     return true;
 }
 

@@ -1,8 +1,8 @@
 // hx_runtime.hip -- the runtime part of the C-ABI: availability, errors, the modulus,
 // streams, events, memory, transfers, graph capture and device input generation.
 // Replaces engine::cuda::CudaCtx / GraphCapture (Cuda/CudaCtx.h:51-253) for one
-// process on one GPU.  Transfers go straight through hipMemcpyAsync: on MI355X the
-// runtime stages pageable copies itself, and pinned buffers are the caller's choice.
+// process on one GPU: a caching device arena, per-stream library scratch, and pageable
+// transfers staged through two pinned chunks (pinned host buffers go straight to the DMA).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -116,6 +116,51 @@ struct ScratchSlot {
 };
 std::mutex g_scratch_mu;
 std::unordered_map<ScratchKey, ScratchSlot, ScratchKeyHash> g_scratch;
+
+// ---- pinned staging for pageable transfers (the CopyPool role, Cuda/CudaCtx.cpp:158-230) -------------
+// Two pinned chunks of 16 MiB, allocated on first use: a transfer moves through them alternately, so
+// the host memcpy of one chunk overlaps the DMA of the other (an event per chunk says when its DMA is
+// done).  Host buffers that are already pinned skip the staging.  One transfer at a time (a mutex).
+struct Staging {
+    std::mutex mu;
+    void* chunk[2] = {nullptr, nullptr};
+    hipEvent_t done[2] = {nullptr, nullptr};
+    bool busy[2] = {false, false};
+    hx_staging_stats st{};
+};
+constexpr size_t kStageChunk = (size_t)16 << 20;
+Staging& staging() {
+    static Staging* s = new Staging();
+    return *s;
+}
+void staging_init_locked(Staging& S) {
+    if (S.chunk[0]) return;
+    for (int i = 0; i < 2; ++i) {
+        HX_HIP(hipHostMalloc(&S.chunk[i], kStageChunk, hipHostMallocDefault));
+        HX_HIP(hipEventCreateWithFlags(&S.done[i], hipEventDisableTiming));
+    }
+    S.st.chunk_bytes = kStageChunk;
+    S.st.cap_bytes = 2 * kStageChunk;
+    S.st.allocated_bytes = 2 * kStageChunk;
+    S.st.peak_bytes = 2 * kStageChunk;
+}
+bool host_is_pinned(const void* p) {
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type == hipMemoryTypeHost;
+}
+// Waits for chunk i's previous DMA (counted as a wait when it had not finished).
+void chunk_acquire(Staging& S, int i) {
+    ++S.st.acquisitions;
+    if (!S.busy[i]) return;
+    if (hipEventQuery(S.done[i]) == hipErrorNotReady) ++S.st.waits;
+    (void)hipGetLastError();
+    HX_HIP(hipEventSynchronize(S.done[i]));
+    S.busy[i] = false;
+}
 
 }  // namespace
 
@@ -391,8 +436,28 @@ HX_API int hx_arena_trim(size_t* freed) {
 HX_API int hx_upload(void* dev, const void* host, size_t bytes, hx_stream s) {
     return guard("hx_upload", [&] {
         if (bytes == 0) return;
-        HX_HIP(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, (hipStream_t)s));
-        HX_HIP(hipStreamSynchronize((hipStream_t)s));
+        HX_CHECK(dev != nullptr && host != nullptr, "NULL pointer");
+        HX_CHECK(!stream_capturing((hipStream_t)s), "hx_upload is not legal inside a stream capture");
+        const hipStream_t st = (hipStream_t)s;
+        if (host_is_pinned(host)) {
+            HX_HIP(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, st));
+        } else {
+            Staging& S = staging();
+            std::lock_guard<std::mutex> lk(S.mu);
+            staging_init_locked(S);
+            int i = 0;
+            for (size_t off = 0; off < bytes; off += kStageChunk, i ^= 1) {
+                const size_t n = std::min(kStageChunk, bytes - off);
+                chunk_acquire(S, i);
+                std::memcpy(S.chunk[i], (const char*)host + off, n);
+                HX_HIP(hipMemcpyAsync((char*)dev + off, S.chunk[i], n, hipMemcpyHostToDevice, st));
+                HX_HIP(hipEventRecord(S.done[i], st));
+                S.busy[i] = true;
+            }
+        }
+        // the host buffer is free again when this returns; the device side is stream-ordered (as the
+        // reference's upload), so a later kernel on s reads the bytes
+        HX_HIP(hipStreamSynchronize(st));
         stat_add(ST_H2D_BYTES, bytes);
         stat_add(ST_H2D_CALLS, 1);
     });
@@ -400,10 +465,63 @@ HX_API int hx_upload(void* dev, const void* host, size_t bytes, hx_stream s) {
 HX_API int hx_download(void* host, const void* dev, size_t bytes, hx_stream s) {
     return guard("hx_download", [&] {
         if (bytes == 0) return;
-        HX_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, (hipStream_t)s));
-        HX_HIP(hipStreamSynchronize((hipStream_t)s));
+        HX_CHECK(dev != nullptr && host != nullptr, "NULL pointer");
+        HX_CHECK(!stream_capturing((hipStream_t)s), "hx_download is not legal inside a stream capture");
+        const hipStream_t st = (hipStream_t)s;
+        if (host_is_pinned(host)) {
+            HX_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, st));
+            HX_HIP(hipStreamSynchronize(st));
+        } else {
+            Staging& S = staging();
+            std::lock_guard<std::mutex> lk(S.mu);
+            staging_init_locked(S);
+            // chunk c's DMA is queued before chunk c-1 is copied out of its pinned buffer
+            const size_t nchunks = (bytes + kStageChunk - 1) / kStageChunk;
+            for (size_t c = 0; c <= nchunks; ++c) {
+                if (c < nchunks) {
+                    const int i = (int)(c & 1);
+                    chunk_acquire(S, i);
+                    const size_t off = c * kStageChunk, n = std::min(kStageChunk, bytes - off);
+                    HX_HIP(hipMemcpyAsync(S.chunk[i], (const char*)dev + off, n, hipMemcpyDeviceToHost, st));
+                    HX_HIP(hipEventRecord(S.done[i], st));
+                    S.busy[i] = true;
+                }
+                if (c > 0) {
+                    const int j = (int)((c - 1) & 1);
+                    const size_t off = (c - 1) * kStageChunk, n = std::min(kStageChunk, bytes - off);
+                    HX_HIP(hipEventSynchronize(S.done[j]));
+                    S.busy[j] = false;
+                    std::memcpy((char*)host + off, S.chunk[j], n);
+                }
+            }
+        }
         stat_add(ST_D2H_BYTES, bytes);
         stat_add(ST_D2H_CALLS, 1);
+    });
+}
+HX_API int hx_host_alloc(void** ptr, size_t bytes) {
+    return guard("hx_host_alloc", [&] {
+        HX_CHECK(ptr != nullptr, "ptr is NULL");
+        *ptr = nullptr;
+        if (bytes == 0) return;
+        HX_HIP(hipHostMalloc(ptr, bytes, hipHostMallocDefault));
+    });
+}
+HX_API int hx_host_free(void* ptr) {
+    return guard("hx_host_free", [&] {
+        if (ptr) HX_HIP(hipHostFree(ptr));
+    });
+}
+HX_API int hx_staging_stats_get(hx_staging_stats* out) {
+    return guard("hx_staging_stats_get", [&] {
+        HX_CHECK(out != nullptr, "out is NULL");
+        Staging& S = staging();
+        std::lock_guard<std::mutex> lk(S.mu);
+        *out = S.st;
+        if (S.st.chunk_bytes == 0) {      // not yet allocated: report the configuration
+            out->chunk_bytes = kStageChunk;
+            out->cap_bytes = 2 * kStageChunk;
+        }
     });
 }
 HX_API int hx_copy_d2d(void* dst, const void* src, size_t bytes, hx_stream s) {

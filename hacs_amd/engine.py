@@ -53,6 +53,35 @@ class ModCtx:
         return self.c.barrett_m
 
 
+class ModCtx64:
+    """The device-relevant part of engine::ModCtx<uint64_t> (ModCtx.h): p < 2^64, Montgomery constants."""
+
+    def __init__(self, p: int):
+        self.c = _lib.hx_modctx64()
+        check(lib().hx_modctx64_init(p, C.byref(self.c)))
+
+    @property
+    def p(self) -> int:
+        return self.c.p
+
+
+def gemm_u64(ctx: ModCtx64, A: torch.Tensor, B: torch.Tensor, C_: torch.Tensor, op: int = HX_OP_SET,
+             stream=None) -> None:
+    """C op= A * B mod p on 64-bit residues (int64 / uint64 device tensors; the u64 kernel of
+    Cuda/cuda_kernels.cu:104-164, CPU path kernels::gemm<uint64_t>)."""
+    check(lib().hx_gemm_u64(C.byref(ctx.c), view64(A), view64(B), view64(C_), op,
+                            current_stream() if stream is None else stream))
+
+
+def to_device64(a: np.ndarray, device: str = "cuda") -> torch.Tensor:
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    return torch.from_numpy(a.view(np.int64)).to(device)
+
+
+def from_device64(t: torch.Tensor) -> np.ndarray:
+    return t.detach().cpu().contiguous().numpy().view(np.uint64)
+
+
 def current_stream() -> int:
     return torch.cuda.current_stream().cuda_stream
 
@@ -282,6 +311,38 @@ def rref(ctx: ModCtx, A: torch.Tensor, R: torch.Tensor | None = None, stream=Non
     return RrefResult(r, piv[:r].copy(), prow[:r].copy())
 
 
+def rref_workspace_bytes(m: int, n: int) -> int:
+    return int(lib().hx_rref_workspace_bytes(m, n))
+
+
+def rref_device(ctx: ModCtx, A: torch.Tensor, R: torch.Tensor | None, d_out: torch.Tensor, workspace: torch.Tensor,
+                stream=None) -> None:
+    """The elimination entirely on the device (hx_rref_u32_device): nothing allocated, nothing read back,
+    so it can be captured in a graph.  d_out (int32, >= 1 + 2 min(m, n) entries) receives rank, then the
+    pivots, then the pivot rows; workspace: >= rref_workspace_bytes(m, n) bytes."""
+    _u32_tensor(A)
+    m, n = A.shape
+    mn = max(1, min(m, n))
+    if d_out.numel() < 1 + 2 * mn:
+        raise ValueError("d_out too small")
+    base = d_out.data_ptr()
+    rp, ldr = (None, 0) if R is None else (C.c_void_p(R.data_ptr()), R.stride(0))
+    check(lib().hx_rref_u32_device(C.byref(ctx.c), m, n, C.c_void_p(A.data_ptr()), A.stride(0), rp, ldr,
+                                   C.c_void_p(base), C.c_void_p(base + 4), C.c_void_p(base + 4 + 4 * mn),
+                                   C.c_void_p(workspace.data_ptr()), workspace.numel() * workspace.element_size(),
+                                   current_stream() if stream is None else stream))
+
+
+def rref_device_result(d_out: torch.Tensor, m: int, n: int) -> RrefResult:
+    """rank, pivots and pivot rows from rref_device's output array (after a synchronisation)."""
+    mn = max(1, min(m, n))
+    h = from_device(d_out)
+    r = int(h[0])
+    if r == 0xFFFFFFFF:
+        raise HxError(_lib.HX_E_NOT_INVERTIBLE, "a pivot had no inverse mod p")
+    return RrefResult(r, h[1:1 + r].copy(), h[1 + mn:1 + mn + r].copy())
+
+
 class RrefBatch:
     """Many uneven RREF jobs in one persistent launch; each job's matrix is overwritten by its RREF.
 
@@ -413,7 +474,7 @@ class HowellBatch:
 class GemmBatch:
     """Uneven GEMM jobs in one launch per kernel variant (generalises GemmBatch<Elt>,
     Cuda/CudaKernels.h:135-185, which needs one shared shape).  jobs: (A, B, C, op) row-major
-    device tensors, or (A, B, C, op, variant).  Created once (job table and limb workspace
+    device tensors, or (A, B, C, op, variant), or (A, B, C, op, variant, limb tile 1..4).  Created once (job table and limb workspace
     uploaded: not capturable); run() is capturable."""
 
     def __init__(self, ctx: ModCtx, jobs, variant: int = HX_GEMM_AUTO):
@@ -430,6 +491,7 @@ class GemmBatch:
             arr[i].n = B.shape[1]
             arr[i].op = op
             arr[i].variant = jb[4] if len(jb) > 4 else HX_GEMM_AUTO
+            arr[i].tile = jb[5] if len(jb) > 5 else 0
             arr[i].A, arr[i].B, arr[i].C = C.c_void_p(A.data_ptr()), C.c_void_p(B.data_ptr()), C.c_void_p(C_.data_ptr())
             ld = [t.stride(0) if t.numel() and t.shape[0] > 1 else t.shape[1] for t in (A, B, C_)]
             arr[i].lda, arr[i].ldb, arr[i].ldc = ld

@@ -146,8 +146,21 @@ HX_API int hx_arena_stats_get(hx_arena_stats *out);
 /* release_cached (CudaCtx.h:157-168): waits for the device, returns every cached (not live) block
  * to the driver; *freed (may be NULL) receives the bytes returned. */
 HX_API int hx_arena_trim(size_t *freed);
-HX_API int hx_upload(void *dev, const void *host, size_t bytes, hx_stream s);     /* synchronous w.r.t. host memory */
-HX_API int hx_download(void *host, const void *dev, size_t bytes, hx_stream s);   /* returns when the bytes are in host memory */
+/* upload / download (CudaCtx::upload / download, CudaCtx.h:171-177): pageable host memory moves through two
+ * pinned 16 MiB staging chunks (the memcpy of one overlapping the DMA of the other); a pinned host buffer
+ * (hx_host_alloc) goes straight to the DMA.  Both return when the host buffer may be reused / holds the
+ * bytes.  Not legal inside a stream capture (HX_E_ARGUMENT). */
+HX_API int hx_upload(void *dev, const void *host, size_t bytes, hx_stream s);
+HX_API int hx_download(void *host, const void *dev, size_t bytes, hx_stream s);
+/* Pinned host memory (CudaCtx::pinned_alloc / pinned_free, CudaCtx.h:182-184). */
+HX_API int hx_host_alloc(void **ptr, size_t bytes);
+HX_API int hx_host_free(void *ptr);
+typedef struct hx_staging_stats {   /* PinnedPoolStats, CudaCtx.h:82-91 */
+    size_t chunk_bytes, cap_bytes, allocated_bytes, peak_bytes;
+    uint64_t acquisitions;           /* chunks taken */
+    uint64_t waits;                  /* takes that had to wait for the chunk's previous DMA */
+} hx_staging_stats;
+HX_API int hx_staging_stats_get(hx_staging_stats *out);
 HX_API int hx_copy_d2d(void *dst, const void *src, size_t bytes, hx_stream s);
 HX_API int hx_memset(void *dst, int value, size_t bytes, hx_stream s);
 
@@ -186,13 +199,26 @@ HX_API uint32_t hx_gemm_limb_k_chunk(const hx_modctx *ctx, uint32_t m, uint32_t 
 HX_API int hx_schur_update_u32(const hx_modctx *ctx, hx_view L, hx_view U, hx_view C, hx_stream s,
                                const hx_gemm_opts *opts);
 
+/* ---- GEMM mod p for 64-bit residues (Cuda/cuda_kernels.cu:104-164, the u64 kernel; CPU path of record
+ *      kernels::gemm<uint64_t>, ModArithmetic/Kernels.h:184) ----------------------------------------- */
+typedef struct hx_modctx64 {    /* ModCtx<uint64_t>'s device-relevant constants */
+    uint64_t p;                 /* modulus, 2 <= p < 2^64 */
+    uint64_t ninv;              /* -p^-1 mod 2^64 (odd p; 0 for even p) */
+    uint64_t r2;                /* 2^128 mod p */
+    uint64_t two64;             /* 2^64 mod p */
+} hx_modctx64;
+HX_API int hx_modctx64_init(uint64_t p, hx_modctx64 *out);
+/* C op= A * B mod p; views of 8-byte words (strides in elements), operands canonical (< p); k < 2^32.
+ * Exact 160-bit sums, one reduction per output (Montgomery for odd p, long division for even p). */
+HX_API int hx_gemm_u64(const hx_modctx64 *ctx, hx_view A, hx_view B, hx_view C, int op, hx_stream s);
+
 /* ---- batched GEMM of uneven shapes (generalises GemmBatch<Elt>, CudaKernels.h:135-185, whose
  *      instances must share one shape) ------------------------------------------------------ */
 typedef struct hx_gemm_job {
     uint32_t m, n, k;
     int32_t op;                 /* HX_OP_* */
     int32_t variant;            /* HX_GEMM_*: AUTO = the batch's variant */
-    int32_t reserved;
+    int32_t tile;               /* the limb kernel of this job (hx_gemm_opts.tile: 0 = default, 1 .. 4); tuning only */
     const uint32_t *A;          /* row-major, leading dimension lda (device) */
     const uint32_t *B;
     uint32_t *C;
@@ -256,6 +282,16 @@ HX_API int hx_canonicalise_acc64_u32(const hx_modctx *ctx, hx_view acc, hx_view 
  * Synchronous: returns when rank and pivots are on the host. */
 HX_API int hx_rref_u32(const hx_modctx *ctx, uint32_t m, uint32_t n, const uint32_t *A, int64_t lda, uint32_t *R,
                        int64_t ldr, uint32_t *rank, uint32_t *pivots, uint32_t *pivot_rows, hx_stream s);
+/* The same elimination entirely on the device, for callers that capture it in a graph (no counterpart in
+ * the reference, whose imma_rref synchronises): nothing is allocated and nothing is read back.  d_rank
+ * (device, one entry) receives the rank, or 0xffffffff when a pivot had no inverse (p not prime);
+ * d_pivots / d_pivot_rows (device, min(m, n) entries, may be NULL) and R (device, may be NULL) as above.
+ * workspace: >= hx_rref_workspace_bytes(m, n) bytes of device memory, caller-owned.  Every shape runs the
+ * whole-GPU path.  Two calls on one stream may share a workspace; calls on different streams need their own. */
+HX_API size_t hx_rref_workspace_bytes(uint32_t m, uint32_t n);
+HX_API int hx_rref_u32_device(const hx_modctx *ctx, uint32_t m, uint32_t n, const uint32_t *A, int64_t lda, uint32_t *R,
+                              int64_t ldr, uint32_t *d_rank, uint32_t *d_pivots, uint32_t *d_pivot_rows,
+                              void *workspace, size_t workspace_bytes, hx_stream s);
 /* The reference's row-slab form (imma_rref, CudaKernels.h:276-288): src(user, r0, nr) returns a
  * host pointer to rows [r0, r0 + nr) (row-contiguous, row stride = the returned ld); sink(user, rank,
  * &ld) returns where the rank x n RREF rows go (NULL: no rows wanted). */
@@ -319,8 +355,10 @@ HX_API int hx_howell_batch_destroy(hx_howell_batch b);
 /* ---- the Z/2^32 / Z/2^64 LU session (Cuda/CudaRing2k.h:28-46; installed through the
  *      ring2k::device::Ops table, Binary/Ring2kDevice.h:64-83; host driver Binary/Ring2kDeviceLU.h) ----
  * A device-resident column window [0, cw) of an m x cw matrix of elem-byte entries (4 or 8); the host
- * does the pivot search and hands each step its factored panel, row swaps and L slots.  Wraparound
- * arithmetic: exact.  put / get / fetch are synchronous; step is asynchronous (two streams). */
+ * does the pivot search and hands each step its factored panel, row swaps and L slots (panel <= 64 columns,
+ * at most one swap per panel column).  Wraparound arithmetic: exact.  put / get / fetch are synchronous;
+ * step copies its host inputs into pinned staging and returns, the device work runs on the session's
+ * streams (the lookahead copy on a second one). */
 typedef struct hx_ring2k_step_args {   /* ring2k::device::StepArgs, Binary/Ring2kDevice.h:49-62 */
     uint32_t m;                   /* rows of A */
     uint32_t r, np;               /* the panel's pivot rows r .. r + np - 1 */
@@ -345,6 +383,31 @@ HX_API int hx_ring2k_get(hx_ring2k s, void *host, size_t ldh, uint32_t r0, uint3
 HX_API int hx_ring2k_step(hx_ring2k s, const hx_ring2k_step_args *a);
 HX_API int hx_ring2k_fetch(hx_ring2k s, void *host, size_t ldh);
 HX_API int hx_ring2k_close(hx_ring2k s);   /* never fails; frees everything */
+/* Ops::plan (Binary/Ring2kDevice.h:65-67): the column window for an m x n LU of elem-byte words with this
+ * panel width under a device memory budget (0 = 90 % of the free device memory): n (resident), a multiple
+ * of panel below n (tiled), or 0 (declined; the host code runs). */
+HX_API uint32_t hx_ring2k_plan(uint32_t elem, uint32_t m, uint32_t n, uint32_t panel, uint64_t budget_bytes);
+/* Ops::store_budget (Ring2kDevice.h:80-81): host bytes one phase's L record may hold; from the environment
+ * variable HX_RING2K_HOST_MB (MiB), 0 = unlimited. */
+HX_API uint64_t hx_ring2k_store_budget(void);
+/* Ops::verify (Ring2kDevice.h:78-79): 1 when HX_VERIFY_DEVICE is set (not "0"): the caller re-runs each device
+ * LU on the host and compares. */
+HX_API int hx_ring2k_verify(void);
+/* The device route of ring2k::echelon_blocked (Binary/Ring2kDeviceLU.h; the MinValuation rule) on the session
+ * above: A (m x n host words of elem bytes, row stride lda) in place, entries understood mod 2^k.  rank, and
+ * pivots / pivot_v (host, min(m, n) entries) exactly as the host code leaves them.  col_limit: pivots are
+ * sought in columns [0, col_limit) only.  budget_bytes: as hx_ring2k_plan (a small budget forces the tiled
+ * route); store_bytes: the phase record's cap (0 = unlimited).  HX_E_NOMEM when not one panel-wide window
+ * fits (nothing was written). */
+HX_API int hx_ring2k_echelon(uint32_t elem, uint32_t k, uint32_t m, uint32_t n, void *A, size_t lda, uint32_t panel,
+                             uint32_t col_limit, uint64_t budget_bytes, uint64_t store_bytes, uint32_t *rank,
+                             uint32_t *pivots, uint32_t *pivot_v);
+typedef struct hx_ring2k_stats {   /* ring2k::device::Stats, Binary/Ring2kDevice.h:86-101 */
+    uint64_t calls, device, resident, tiled, slabs, replays, fallbacks, panels, lookaheads, verified;
+    uint64_t ns_open, ns_fetch, ns_panel, ns_step, ns_host, ns_close;   /* wall ns on the calling thread */
+} hx_ring2k_stats;
+HX_API int hx_ring2k_stats_get(hx_ring2k_stats *out);
+HX_API int hx_ring2k_stats_reset(void);
 /* One trailing GEMM C (m x n) -= L (m x k) U (k x n) on the device, all host row-major (gemm_sub_host). */
 HX_API int hx_ring2k_gemm_sub_host(uint32_t elem, void *C, size_t ldc, const void *L, size_t ldl, const void *U,
                                    size_t ldu, uint32_t m, uint32_t n, uint32_t k);

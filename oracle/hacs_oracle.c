@@ -659,3 +659,147 @@ HXO_API uint32_t hxo_howell_fast(uint32_t k, uint32_t m, uint32_t n, const uint3
     free(rows);
     return h;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Blocked LU over Z/2^k on the native u32 / u64 rungs                      */
+/* ------------------------------------------------------------------------ */
+/* Restates ring2k::echelon_blocked's serial route (Binary/Ring2k.h:538-612 */
+/* and :667-683, the MinValuation rule) with NativeTraits' arithmetic      */
+/* (:160-238): entries are native wraparound words understood mod 2^k;      */
+/* per panel [c, c1) and column j the pivot is the row of least valuation  */
+/* mod 2^k from row r + np down (the first such row; a unit stops the       */
+/* search), swapped into place with its L row; every row below it gets      */
+/* slot f = (R[j] >> v) u^-1 (u the pivot's odd part; slot 0 when R[j] is   */
+/* 0 mod 2^k) and R[j, c1) -= f P[j, c1); then the unit-lower TRSM on the   */
+/* panel's pivot rows and C -= L U on the rows below, over columns [c1, n). */
+/* The device session's host driver must leave A, rank, pivots and          */
+/* pivot_v exactly so (Ring2kDeviceLU.h's contract).  Returns the rank;     */
+/* pivots / pivot_v receive min(m, n) entries at most.  Pinned by the       */
+/* reference's own echelon_blocked (fixture written with oracle/_ref), and  */
+/* by the invariant that the Howell form of A mod 2^k is unchanged.          */
+#define HXO_R2_ECHELON(NAME, E, BITS)                                                                         \
+    static inline E NAME##_inv_odd(E u) {                                                                     \
+        E x = u;                                                                                              \
+        for (uint32_t b = 3; b < BITS; b *= 2) x = (E)(x * (E)((E)2 - (E)(u * x)));                          \
+        return x;                                                                                             \
+    }                                                                                                         \
+    static inline uint32_t NAME##_val(E a, uint32_t k) {                                                      \
+        const E mk = k >= BITS ? (E)~(E)0 : (E)(((E)1 << k) - 1);                                             \
+        const E x = a & mk;                                                                                   \
+        if (x == 0) return k;                                                                                 \
+        const uint32_t c = (uint32_t)__builtin_ctzll((uint64_t)x);                                           \
+        return c < k ? c : k;                                                                                 \
+    }                                                                                                         \
+    HXO_API uint32_t NAME(uint32_t k, uint32_t m, uint32_t n, E *A, int64_t lda, uint32_t panel, uint32_t climit, \
+                          uint32_t *pivots, uint32_t *pivot_v) {                                              \
+        if (panel == 0) panel = 1;                                                                            \
+        if (m == 0 || n == 0) return 0;                                                                       \
+        if (climit > n) climit = n;                                                                           \
+        E *L = (E *)calloc((size_t)m * panel, sizeof(E));                                                     \
+        uint32_t r = 0, c = 0, npiv = 0;                                                                      \
+        while (c < climit && r < m) {                                                                         \
+            const uint32_t c1 = c + panel < climit ? c + panel : climit;                                      \
+            uint32_t np = 0;                                                                                  \
+            for (uint32_t j = c; j < c1 && r + np < m; ++j) {                                                 \
+                const uint32_t prow = r + np;                                                                 \
+                uint32_t best = m, bv = k;                                                                    \
+                for (uint32_t i = prow; i < m; ++i) {                                                         \
+                    const uint32_t v = NAME##_val(A[(int64_t)i * lda + j], k);                                \
+                    if (v < bv) { bv = v; best = i; if (v == 0) break; }                                      \
+                }                                                                                             \
+                if (best == m) continue;                                                                      \
+                if (best != prow) {                                                                           \
+                    for (uint32_t q = 0; q < n; ++q) {                                                        \
+                        E t = A[(int64_t)best * lda + q];                                                     \
+                        A[(int64_t)best * lda + q] = A[(int64_t)prow * lda + q];                              \
+                        A[(int64_t)prow * lda + q] = t;                                                       \
+                    }                                                                                         \
+                    for (uint32_t t = 0; t < panel; ++t) {                                                    \
+                        E x = L[(size_t)best * panel + t];                                                    \
+                        L[(size_t)best * panel + t] = L[(size_t)prow * panel + t];                            \
+                        L[(size_t)prow * panel + t] = x;                                                      \
+                    }                                                                                         \
+                }                                                                                             \
+                const E *P = A + (int64_t)prow * lda;                                                         \
+                const E ui = NAME##_inv_odd((E)(P[j] >> bv));                                                 \
+                for (uint32_t i = prow + 1; i < m; ++i) {                                                     \
+                    E *R = A + (int64_t)i * lda;                                                              \
+                    E *slot = L + (size_t)i * panel + np;                                                     \
+                    if (NAME##_val(R[j], k) == k) { *slot = 0; continue; }                                    \
+                    const E f = (E)((E)(R[j] >> bv) * ui);                                                    \
+                    *slot = f;                                                                                \
+                    for (uint32_t q = j; q < c1; ++q) R[q] = (E)(R[q] - (E)(f * P[q]));                       \
+                }                                                                                             \
+                pivots[npiv] = j;                                                                             \
+                pivot_v[npiv] = bv;                                                                           \
+                ++npiv;                                                                                       \
+                ++np;                                                                                         \
+            }                                                                                                 \
+            if (np > 0 && c1 < n) {                                                                           \
+                for (uint32_t t = 1; t < np; ++t) {                                                           \
+                    const E *lt = L + (size_t)(r + t) * panel;                                                \
+                    E *Rt = A + (int64_t)(r + t) * lda;                                                       \
+                    for (uint32_t t2 = 0; t2 < t; ++t2) {                                                     \
+                        const E f = lt[t2];                                                                   \
+                        if (f == 0) continue;                                                                 \
+                        const E *Rs = A + (int64_t)(r + t2) * lda;                                            \
+                        for (uint32_t q = c1; q < n; ++q) Rt[q] = (E)(Rt[q] - (E)(f * Rs[q]));               \
+                    }                                                                                         \
+                }                                                                                             \
+                for (uint32_t i = r + np; i < m; ++i) {                                                       \
+                    E *Ci = A + (int64_t)i * lda;                                                             \
+                    const E *li = L + (size_t)i * panel;                                                      \
+                    for (uint32_t t = 0; t < np; ++t) {                                                       \
+                        const E f = li[t];                                                                    \
+                        if (f == 0) continue;                                                                 \
+                        const E *Ut = A + (int64_t)(r + t) * lda;                                             \
+                        for (uint32_t q = c1; q < n; ++q) Ci[q] = (E)(Ci[q] - (E)(f * Ut[q]));               \
+                    }                                                                                         \
+                }                                                                                             \
+            }                                                                                                 \
+            r += np;                                                                                          \
+            c = c1;                                                                                           \
+        }                                                                                                     \
+        free(L);                                                                                              \
+        return r;                                                                                             \
+    }
+HXO_R2_ECHELON(hxo_r2_echelon32, uint32_t, 32u)
+HXO_R2_ECHELON(hxo_r2_echelon64, uint64_t, 64u)
+
+/* ------------------------------------------------------------------------ */
+/* GEMM mod p on 64-bit residues, 2 <= p < 2^64                             */
+/* ------------------------------------------------------------------------ */
+/* C op= A B mod p with every product and sum exact in unsigned __int128     */
+/* (each product reduced before the sum, as RefLinAlg.cpp:94-106 does for   */
+/* the u32 naive GEMM).  Threaded over rows.  Pinned by exact Python        */
+/* integers, the rank-1 x all-(p-1) closed form, and the reference's own    */
+/* kernels::gemm<uint64_t> (fixture written with oracle/_ref).               */
+struct gemm64_arg {
+    uint64_t p;
+    uint32_t m, n, k;
+    const uint64_t *A, *B;
+    uint64_t *C;
+    int op;
+};
+static void gemm64_worker(int tid, int T, void *va) {
+    const struct gemm64_arg *g = (const struct gemm64_arg *)va;
+    const uint64_t p = g->p;
+    for (uint32_t i = (uint32_t)tid; i < g->m; i += (uint32_t)T)
+        for (uint32_t j = 0; j < g->n; ++j) {
+            uint64_t s = 0;
+            for (uint32_t t = 0; t < g->k; ++t) {
+                const uint64_t x = (uint64_t)(((unsigned __int128)g->A[(size_t)i * g->k + t] * g->B[(size_t)t * g->n + j]) % p);
+                s = (uint64_t)(((unsigned __int128)s + x) % p);
+            }
+            uint64_t *c = g->C + (size_t)i * g->n + j;
+            if (g->op == 0) *c = s;
+            else if (g->op == 1) *c = (uint64_t)(((unsigned __int128)*c + s) % p);
+            else *c = (uint64_t)(((unsigned __int128)*c + (p - s)) % p);
+        }
+}
+HXO_API void hxo_gemm_u64(uint64_t p, uint32_t m, uint32_t n, uint32_t k, const uint64_t *A, const uint64_t *B,
+                          uint64_t *C, int op, int nthreads) {
+    struct gemm64_arg g = {p, m, n, k, A, B, C, op};
+    if (m == 0 || n == 0) return;
+    team_run(nthreads, gemm64_worker, &g);
+}

@@ -86,6 +86,14 @@ def lib():
         L.hxo_rref_rule.restype = C.c_uint32
         L.hxo_rref_rule.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, u32p, C.c_int64, u32p, u32p,
                                     _u32p_or_null, C.c_int64, C.c_int]
+        u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+        L.hxo_gemm_u64.restype = None
+        L.hxo_gemm_u64.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, u64p, u64p, u64p, C.c_int, C.c_int]
+        for name, E in (("hxo_r2_echelon32", C.c_uint32), ("hxo_r2_echelon64", C.c_uint64)):
+            f = getattr(L, name)
+            f.restype = C.c_uint32
+            f.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_int64, C.c_uint32, C.c_uint32,
+                          u32p, u32p]
         for name in ("hxo_howell_naive", "hxo_howell_fast"):
             f = getattr(L, name)
             f.restype = C.c_uint32
@@ -115,6 +123,12 @@ def ref():
         R.hxr_ring2k_howell.restype = C.c_int64
         R.hxr_ring2k_howell.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, u32p, C.c_int64, C.c_int,
                                         u32p, u32p, u32p]
+        R.hxr_gemm_u64.restype = C.c_int
+        R.hxr_gemm_u64.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.c_int]
+        R.hxr_r2_echelon.restype = C.c_int64
+        R.hxr_r2_echelon.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_int64, C.c_uint32,
+                                     C.c_uint32, u32p, u32p]
         R.hxr_fnv_words.restype = C.c_uint64
         R.hxr_fnv_words.argtypes = [C.c_void_p, C.c_size_t]
         R.hxr_fnv_bytes.restype = C.c_uint64
@@ -166,6 +180,55 @@ def fnv_matrix(M: np.ndarray) -> int:
     M = np.ascontiguousarray(M, dtype=np.uint32)
     r, c = (M.shape + (1,))[:2] if M.ndim == 2 else (M.shape[0], 1)
     return lib().hxo_fnv_matrix(M, r, c, c)
+
+
+def gemm_u64(p: int, A: np.ndarray, B: np.ndarray, C0: np.ndarray | None = None, op: int = 0, threads: int = 8):
+    """C op= A B mod p on 64-bit residues, exact (unsigned __int128)."""
+    A = np.ascontiguousarray(A, dtype=np.uint64)
+    B = np.ascontiguousarray(B, dtype=np.uint64)
+    m, k = A.shape
+    n = B.shape[1]
+    Cm = np.zeros((m, n), np.uint64) if C0 is None else np.ascontiguousarray(C0, dtype=np.uint64).copy()
+    lib().hxo_gemm_u64(p, m, n, k, A, B, Cm, op, threads)
+    return Cm
+
+
+def ref_gemm_u64(p: int, A: np.ndarray, B: np.ndarray, C0: np.ndarray | None = None, op: int = 0):
+    """The reference's own kernels::gemm<uint64_t> (oracle/_ref)."""
+    A = np.ascontiguousarray(A, dtype=np.uint64)
+    B = np.ascontiguousarray(B, dtype=np.uint64)
+    m, k = A.shape
+    n = B.shape[1]
+    Cm = np.zeros((m, n), np.uint64) if C0 is None else np.ascontiguousarray(C0, dtype=np.uint64).copy()
+    if ref().hxr_gemm_u64(p, m, n, k, A.ctypes.data_as(C.c_void_p), B.ctypes.data_as(C.c_void_p),
+                          Cm.ctypes.data_as(C.c_void_p), op) != 0:
+        raise RuntimeError("reference kernels::gemm<uint64_t> failed")
+    return Cm
+
+
+def r2_echelon(k: int, A: np.ndarray, panel: int, climit: int | None = None):
+    """ring2k::echelon_blocked (MinValuation) on the native rung of A's dtype (uint32: Z/2^32 words,
+    uint64: Z/2^64 words), entries understood mod 2^k: (A after the LU, rank, pivots, pivot_v)."""
+    A = np.ascontiguousarray(A).copy()
+    m, n = A.shape
+    fn = lib().hxo_r2_echelon32 if A.dtype == np.uint32 else lib().hxo_r2_echelon64
+    piv = np.zeros(max(1, min(m, n)), np.uint32)
+    pv = np.zeros(max(1, min(m, n)), np.uint32)
+    r = fn(k, m, n, A.ctypes.data_as(C.c_void_p), n, panel, n if climit is None else climit, piv, pv)
+    return A, int(r), piv[:r].copy(), pv[:r].copy()
+
+
+def ref_r2_echelon(k: int, A: np.ndarray, panel: int, climit: int | None = None):
+    """The reference's own echelon_blocked (oracle/_ref) on the same rung: (A, rank, pivots, pivot_v)."""
+    A = np.ascontiguousarray(A).copy()
+    m, n = A.shape
+    piv = np.zeros(max(1, min(m, n)), np.uint32)
+    pv = np.zeros(max(1, min(m, n)), np.uint32)
+    r = ref().hxr_r2_echelon(A.dtype.itemsize, k, m, n, A.ctypes.data_as(C.c_void_p), n, panel,
+                             n if climit is None else climit, piv, pv)
+    if r < 0:
+        raise RuntimeError("reference echelon_blocked failed")
+    return A, int(r), piv[:r].copy(), pv[:r].copy()
 
 
 def fnv1a_bytes(b: bytes) -> int:

@@ -52,6 +52,28 @@ uint64_t hxr_fnv_bytes(const char* b, size_t n) { return engine::FnvBytes(engine
 // engine::kernels::gemm<uint32_t> (Kernels.cpp:1417-1483), split over row tiles of C on
 // `threads` host threads as the tape executor splits output tiles (Engine/Executor.cpp:69-100).
 // A, B, C row-major with leading dimensions; op 0 Set, 1 Add, 2 Sub. Returns 0 or -1 on an exception.
+// engine::kernels::gemm<uint64_t> (Kernels.h:184) on row-major arrays, one thread.
+int hxr_gemm_u64(uint64_t p, uint32_t m, uint32_t n, uint32_t k, const uint64_t* A, const uint64_t* B, uint64_t* C,
+                 int op) {
+    try {
+        const ModCtx<uint64_t> ctx(p);
+        DenseBuf<uint64_t> a(m, k), b(k, n), c(m, n);
+        for (uint32_t i = 0; i < m; ++i)
+            for (uint32_t t = 0; t < k; ++t) a.at(i, t) = A[static_cast<size_t>(i) * k + t];
+        for (uint32_t t = 0; t < k; ++t)
+            for (uint32_t j = 0; j < n; ++j) b.at(t, j) = B[static_cast<size_t>(t) * n + j];
+        for (uint32_t i = 0; i < m; ++i)
+            for (uint32_t j = 0; j < n; ++j) c.at(i, j) = C[static_cast<size_t>(i) * n + j];
+        engine::kernels::gemm<uint64_t>(ctx, DenseView<const uint64_t>(a), DenseView<const uint64_t>(b),
+                                        DenseView<uint64_t>(c), static_cast<engine::kernels::GemmOp>(op));
+        for (uint32_t i = 0; i < m; ++i)
+            for (uint32_t j = 0; j < n; ++j) C[static_cast<size_t>(i) * n + j] = c.at(i, j);
+        return 0;
+    } catch (...) {
+        return -1;
+    }
+}
+
 int hxr_gemm_u32(uint32_t p, uint32_t m, uint32_t n, uint32_t k, const uint32_t* A, int64_t lda, const uint32_t* B,
                  int64_t ldb, uint32_t* C, int64_t ldc, int op, int threads) {
     try {
@@ -151,6 +173,34 @@ int64_t hxr_ring2k_howell(uint32_t k, uint32_t m, uint32_t n, const uint32_t* A,
             return hr;
         };
         if (rung == 16) return run(uint16_t{});
+        return run(uint32_t{});
+    } catch (...) {
+        return -1;
+    }
+}
+
+// engine::ring2k::echelon_blocked<Traits<E>> (Binary/Ring2k.h:538-694), MinValuation, on the u32 (elem 4) or u64
+// (elem 8) rung: A (m x n words of elem bytes, row stride lda) in place; returns the rank, pivots / pivot_v filled.
+int64_t hxr_r2_echelon(uint32_t elem, uint32_t k, uint32_t m, uint32_t n, void* A, int64_t lda, uint32_t panel,
+                       uint32_t climit, uint32_t* pivots, uint32_t* pivot_v) {
+    try {
+        auto run = [&](auto tag) -> int64_t {
+            using E = decltype(tag);
+            E* src = static_cast<E*>(A);
+            engine::ring2k::Mat<E> a(m, n);
+            for (uint32_t i = 0; i < m; ++i)
+                for (uint32_t j = 0; j < n; ++j) a.at(i, j) = src[static_cast<int64_t>(i) * lda + j];
+            const engine::ring2k::EchelonInfo e = engine::ring2k::echelon_blocked<engine::ring2k::Traits<E>>(
+                a, k, engine::ring2k::PivotRule::MinValuation, panel, climit);
+            for (uint32_t i = 0; i < m; ++i)
+                for (uint32_t j = 0; j < n; ++j) src[static_cast<int64_t>(i) * lda + j] = a.at(i, j);
+            for (size_t t = 0; t < e.pivots.size(); ++t) {
+                pivots[t] = e.pivots[t];
+                pivot_v[t] = e.pivot_v[t];
+            }
+            return e.rank;
+        };
+        if (elem == 8) return run(uint64_t{});
         return run(uint32_t{});
     } catch (...) {
         return -1;

@@ -13,9 +13,12 @@ def make(rng: np.random.Generator, m: int, n: int, p: int, kind: str, rank: int 
         M = rng.integers(0, p, (m, n), dtype=np.uint64)
     elif kind == "planted":
         r = max(1, min(rank or max(1, min(m, n) // 2), m, n))
-        L = rng.integers(0, p, (m, r), dtype=np.uint64).astype(object)
-        R = rng.integers(0, p, (r, n), dtype=np.uint64).astype(object)
-        M = (L.dot(R) % p).astype(np.uint64)
+        L = rng.integers(0, p, (m, r), dtype=np.uint64).astype(np.uint32)
+        R = rng.integers(0, p, (r, n), dtype=np.uint64).astype(np.uint32)
+        # the exact product mod p by the oracle's GEMM (itself pinned against exact integers and the
+        # reference's kernels::gemm); numpy object arithmetic would take hours at the large shapes
+        from oracle import oracle as O
+        M = O.gemm_fast(p, L, R, None, 0, 16).astype(np.uint64)
     elif kind == "sparse":
         z = rng.integers(0, 1000, (m, n))
         vals = rng.integers(0, p, (m, n), dtype=np.uint64)

@@ -236,3 +236,68 @@ def test_fnv_against_reference_hasher(O):
     assert np.array_equal(G["words"][2:], M.astype(np.uint64).ravel())
     for s, n, want in zip(G["strings"], G["string_lens"], G["strings_fnv"]):
         assert O.fnv1a_bytes(bytes(s)[:int(n)]) == int(want)
+
+
+def test_r2_echelon_vs_reference(O):
+    """The oracle's blocked LU over Z/2^k (u32 / u64 rungs) equals the reference's own echelon_blocked
+    entrywise on the fixture (matrix, rank, pivots, valuations); the Howell form mod 2^k of the result
+    equals the input's (row operations only, all unimodular)."""
+    import os
+    G = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "r2_reference.npz"),
+                allow_pickle=False)
+    for i in range(int(G["count"][0])):
+        elem, k, m, n, panel, climit, r = (int(x) for x in G[f"case{i}_meta"])
+        A = G[f"case{i}_A"]
+        R, rank, piv, pv = O.r2_echelon(k, A, panel, climit)
+        assert rank == r and np.array_equal(piv, G[f"case{i}_piv"]) and np.array_equal(pv, G[f"case{i}_pv"]), i
+        assert np.array_equal(R, G[f"case{i}_R"]), f"case {i}: matrix"
+        if k <= 32:
+            mk = np.uint64((1 << k) - 1)
+            Ain = (A.astype(np.uint64) & mk).astype(np.uint32)
+            Rout = (R.astype(np.uint64) & mk).astype(np.uint32)
+            H0, p0, v0 = O.howell_fast(k, Ain)
+            H1, p1, v1 = O.howell_fast(k, Rout)
+            assert np.array_equal(H0, H1) and np.array_equal(p0, p1) and np.array_equal(v0, v1), f"case {i}: rowspan"
+
+
+def test_r2_echelon_negative_control(O):
+    """The comparison is entrywise on the native words, not only mod 2^k: adding 2^k to one input entry
+    leaves the pivots and valuations alone (they read values mod 2^k) but must change the matrix."""
+    import os
+    G = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "r2_reference.npz"),
+                allow_pickle=False)
+    elem, k, m, n, panel, climit, r = (int(x) for x in G["case0_meta"])
+    assert elem == 4 and k < 32
+    A = G["case0_A"].copy()
+    A[20, 5] = np.uint32((int(A[20, 5]) + (1 << k)) & 0xFFFFFFFF)
+    R, rank, piv, pv = O.r2_echelon(k, A, panel, climit)
+    assert rank == r and np.array_equal(piv, G["case0_piv"]) and np.array_equal(pv, G["case0_pv"])
+    assert not np.array_equal(R, G["case0_R"])
+
+
+def test_gemm_u64_exact_and_reference(O):
+    """The u64 GEMM oracle: exact Python integers (every op, ragged shapes, k = 0, even and odd moduli
+    up to 2^64 - 2), the all-(p-1) closed form K (p-1)^2 = K mod p, and the reference's own
+    kernels::gemm<uint64_t> on the committed fixture."""
+    import os
+    rng = np.random.default_rng(64)
+    for p in (2305843009213693951, 18446744073709551557, (1 << 64) - 2, 4294967311, 7, 1 << 40):
+        for (m, n, k) in ((5, 7, 9), (17, 3, 200), (3, 4, 0)):
+            def draw(shape):
+                return ((rng.integers(0, 1 << 32, shape, dtype=np.uint64) << np.uint64(32)) |
+                        rng.integers(0, 1 << 32, shape, dtype=np.uint64)) % np.uint64(p)
+            A, B, C0 = draw((m, k)), draw((k, n)), draw((m, n))
+            ex = A.astype(object).dot(B.astype(object)) if k else np.zeros((m, n), dtype=object)
+            for op in (0, 1, 2):
+                want = (ex if op == 0 else (C0.astype(object) + ex if op == 1 else C0.astype(object) - ex)) % p
+                assert (O.gemm_u64(p, A, B, C0, op).astype(object) == want).all(), (p, m, n, k, op)
+    p, K = 18446744073709551557, 3000
+    A = np.full((2, K), p - 1, np.uint64)
+    assert (O.gemm_u64(p, A, A.T.copy()) == np.uint64(K % p)).all()
+    G = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "u64_reference.npz"),
+                allow_pickle=False)
+    for i in range(int(G["count"][0])):
+        p = int(G[f"case{i}_p"][0])
+        A, B, C0 = G[f"case{i}_A"], G[f"case{i}_B"], G[f"case{i}_C0"]
+        assert np.array_equal(O.gemm_u64(p, A, B), G[f"case{i}_set"]), i
+        assert np.array_equal(O.gemm_u64(p, A, B, C0, 2), G[f"case{i}_sub"]), i

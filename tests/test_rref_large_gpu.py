@@ -70,3 +70,85 @@ def test_trsm_unit_matches_oracle(hx_gpu, O):
         Tu = np.triu(T, 1) if upper else np.tril(T, -1)
         Tu = Tu + np.eye(n, dtype=np.uint32)
         assert np.array_equal(O.gemm_fast(p, Tu, X), B), f"T X != B for n={n} upper={upper}"
+
+
+def test_rref_device_entry_captured_and_replayed(hx_gpu, O):
+    """hx_rref_u32_device (no allocation, no read-back) captured in a hipGraph: three replays with eager
+    work and synchronisations between them, each equal to the oracle; and the synchronous entry on
+    the same matrix allocates nothing from the driver after its first call (the arena reuses its block)."""
+    import ctypes as C
+    from hacs_amd import engine as E
+    from hacs_amd import _lib
+    p = 2147483647
+    ctx = E.ModCtx(p)
+    rng = np.random.default_rng(77)
+    m, n = 1200, 1400
+    A = make(rng, m, n, p, "planted", rank=900)
+    want = O.rref_rule(p, A, threads=16)
+    dA = E.to_device(A)
+    R = torch.zeros((m, n), dtype=torch.int32, device="cuda")
+    out = torch.zeros(1 + 2 * min(m, n), dtype=torch.int32, device="cuda")
+    ws = torch.empty(E.rref_workspace_bytes(m, n), dtype=torch.uint8, device="cuda")
+    s = torch.cuda.Stream()
+    L = _lib.lib()
+    with torch.cuda.stream(s):
+        h = s.cuda_stream
+        E.rref_device(ctx, dA, R, out, ws, stream=h)          # eager once (creates the side stream)
+        s.synchronize()
+        res = E.rref_device_result(out, m, n)
+        assert res.rank == want[0]
+        g = C.c_void_p()
+        _lib.check(L.hx_graph_begin(h))
+        E.rref_device(ctx, dA, R, out, ws, stream=h)
+        _lib.check(L.hx_graph_end(h, C.byref(g)))
+        for rep in range(3):
+            R.fill_(7)
+            out.fill_(0)
+            s.synchronize()
+            _lib.check(L.hx_graph_launch(g, h))
+            s.synchronize()
+            res = E.rref_device_result(out, m, n)
+            assert res.rank == want[0] and np.array_equal(res.pivots, want[1]) and np.array_equal(res.pivot_rows, want[2])
+            got = E.from_device(R)
+            assert np.array_equal(got[:res.rank], want[3]) and not got[res.rank:].any(), f"replay {rep}"
+            other = torch.empty((m, n), dtype=torch.int32, device="cuda")
+            E.copy(dA, other, stream=h)                       # eager work between replays
+            s.synchronize()
+        _lib.check(L.hx_graph_destroy(g))
+    res1 = E.rref(ctx, dA, R)
+    torch.cuda.synchronize()
+    st0 = E.arena_stats()
+    res2 = E.rref(ctx, dA, R)
+    torch.cuda.synchronize()
+    st1 = E.arena_stats()
+    assert res1.rank == res2.rank == want[0]
+    assert st1["driver_mallocs"] == st0["driver_mallocs"], "the second call must reuse the arena block"
+
+
+def test_staged_transfers(hx_gpu):
+    """Pageable uploads / downloads through the pinned chunks (several chunks, a ragged tail) and a pinned
+    host buffer straight to the DMA give the bytes back unchanged; the staging counters move."""
+    import ctypes as C
+    from hacs_amd import _lib
+    L = _lib.lib()
+    nbytes = (40 << 20) + 12345
+    host = np.random.default_rng(1).integers(0, 256, nbytes, dtype=np.uint8)
+    dev = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    st0 = _lib.hx_staging_stats()
+    _lib.check(L.hx_staging_stats_get(C.byref(st0)))
+    _lib.check(L.hx_upload(C.c_void_p(dev.data_ptr()), host.ctypes.data, nbytes, None))
+    back = np.zeros_like(host)
+    _lib.check(L.hx_download(back.ctypes.data, C.c_void_p(dev.data_ptr()), nbytes, None))
+    assert np.array_equal(back, host)
+    st1 = _lib.hx_staging_stats()
+    _lib.check(L.hx_staging_stats_get(C.byref(st1)))
+    assert st1.acquisitions >= st0.acquisitions + 6 and st1.chunk_bytes == 16 << 20
+    pin = C.c_void_p()
+    _lib.check(L.hx_host_alloc(C.byref(pin), nbytes))
+    try:
+        C.memmove(pin, host.ctypes.data, nbytes)
+        dev.zero_()
+        _lib.check(L.hx_upload(C.c_void_p(dev.data_ptr()), pin, nbytes, None))
+        assert np.array_equal(dev.cpu().numpy(), host)
+    finally:
+        _lib.check(L.hx_host_free(pin))

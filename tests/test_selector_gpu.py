@@ -55,13 +55,22 @@ def test_selector_trains_and_selects(hx_gpu):
     loss = sel.fit([j.kind for j in train], [j.n for j in train], Tt, steps=1500)
     assert np.isfinite(loss)
     slots = sel.choose([j.kind for j in held], [j.n for j in held])
-    stat = [AT.static_choice(j.kind, j.n) for j in held]
     best = float(np.nanmin(np.where(np.isfinite(Th), Th, np.nan), axis=1).sum())
     t_sel = float(sum(Th[i, s] for i, s in enumerate(slots)))
-    t_stat = float(sum(Th[i, s] for i, s in enumerate(stat)))
-    print(f"held-out: best {best * 1e3:.3f} ms, selector {t_sel * 1e3:.3f} ms ({t_sel / best:.3f}x), "
-          f"static {t_stat * 1e3:.3f} ms ({t_stat / best:.3f}x)")
+    print(f"held-out: best {best * 1e3:.3f} ms, selector {t_sel * 1e3:.3f} ms ({t_sel / best:.3f}x)")
     assert t_sel <= 1.25 * best
+
+
+def test_selector_gains_over_static_rule(hx_gpu):
+    """The study the selector is for: GEMM and echelon jobs with n in [48, 640] and moduli of 2, 3 and 4
+    limbs, where the library's static rule (a size threshold that ignores the limb count, the default
+    limb kernel, the widest panel) is wrong on part of the jobs.  Held out, the trained network stays
+    within 5 % of the exhaustive autotuner's best and costs less than the static rule."""
+    from hacs_amd import autotune as AT
+    r = AT.selector_study(n_train=120, n_held=60, reps=3, steps=2500)
+    print("selector study:", r)
+    assert r["all"]["selector_vs_best"] <= 1.05, r
+    assert r["all"]["static_vs_selector"] > 1.0, r
 
 
 def test_device_generators_match_workloads(hx_gpu):
