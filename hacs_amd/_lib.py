@@ -157,6 +157,8 @@ SIGNATURES = {
     "hx_rref_batch_info_get": (_INT, [_V, C.POINTER(hx_rref_batch_info)]),
     "hx_arena_stats_get": (_INT, [C.POINTER(hx_arena_stats)]),
     "hx_arena_trim": (_INT, [C.POINTER(_SZ)]),
+    "hx_gemm_u32_noalloc": (_INT, [C.POINTER(hx_modctx), hx_view, hx_view, hx_view, _INT, _V, C.POINTER(hx_gemm_opts)]),
+    "hx_gemm_noalloc_scratch_bytes": (_SZ, []),
     "hx_schur_update_u32": (_INT, [C.POINTER(hx_modctx), hx_view, hx_view, hx_view, _V, C.POINTER(hx_gemm_opts)]),
     "hx_gemm_batch_create": (_INT, [C.POINTER(hx_modctx), C.POINTER(hx_gemm_job), _U32, _INT, C.POINTER(_V)]),
     "hx_gemm_batch_run": (_INT, [_V, _V]),

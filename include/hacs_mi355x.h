@@ -195,6 +195,16 @@ HX_API size_t hx_gemm_workspace_bytes(const hx_modctx *ctx, uint32_t m, uint32_t
  * int32-exact chunk) when its planes fit, else the largest multiple of 64 that fits; 0 when not even a
  * 64-deep chunk fits.  Host-only arithmetic (no device needed). */
 HX_API uint32_t hx_gemm_limb_k_chunk(const hx_modctx *ctx, uint32_t m, uint32_t n, uint32_t k, size_t workspace_bytes);
+/* hx_gemm_u32 without any device allocation.  No counterpart in the reference: for callers that cannot
+ * allocate (code under graph capture, tight serving loops, harnesses that forbid it).  The limb route's
+ * planes live in a fixed 64 MiB device array per GPU (hx_gemm_noalloc_scratch_bytes), K running in the
+ * chunks that fit it; opts->workspace is ignored.  Limit: limbs * (round_up(m, 64) + round_up(n, 128)) * 64
+ * bytes (one 64-deep K chunk) must fit the array, else LIMB fails with HX_E_ARGUMENT and AUTO runs the plain
+ * kernel (which needs no scratch).  Every call on one GPU shares the array: calls that overlap on a GPU
+ * (two streams, two host threads) race and must be serialised by the caller. */
+HX_API int hx_gemm_u32_noalloc(const hx_modctx *ctx, hx_view A, hx_view B, hx_view C, int op, hx_stream s,
+                               const hx_gemm_opts *opts);
+HX_API size_t hx_gemm_noalloc_scratch_bytes(void);
 /* C -= L * U (schur_update, CudaKernels.h:130-133). */
 HX_API int hx_schur_update_u32(const hx_modctx *ctx, hx_view L, hx_view U, hx_view C, hx_stream s,
                                const hx_gemm_opts *opts);
