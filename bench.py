@@ -55,7 +55,10 @@ def parse():
     ap.add_argument("--no-ring2k", action="store_true", help="do not time the Z/2^32 LU on the device session")
     ap.add_argument("--no-u64", action="store_true", help="do not time the 64-bit-residue GEMM")
     ap.add_argument("--no-level1", action="store_true", help="do not time the HBM-bound level-1/2 kernels")
-    ap.add_argument("--only", choices=["batch", "lu", "mixed", "cpu", "howell", "ring2k", "level1", "u64"],
+    ap.add_argument("--no-f2", action="store_true", help="do not time the F_2 RREF and product")
+    ap.add_argument("--no-splitk", action="store_true", help="do not time the split-K product")
+    ap.add_argument("--only", choices=["batch", "lu", "mixed", "cpu", "howell", "ring2k", "level1", "u64", "f2",
+                                       "splitk"],
                     help="run one secondary workload alone (for profiling) and print its record")
     ap.add_argument("--pmc-mode", action="store_true",
                     help="for counter collection: one 1 GiB device copy (the traffic calibration), then one limb "
@@ -351,6 +354,73 @@ def gemm_u64_leg(E, n=4096, p=2305843009213693951, reps=3):
     return {"n": n, "modulus": p, "ms": ms, "macs_per_s": float(n) ** 3 / (ms * 1e-3)}
 
 
+def f2_leg(E, n=8192, rank=6000, reps=3):
+    """F_2 on bit-packed rows: the in-place RREF of an n x n matrix of planted rank (rank random rows, the
+    others sums of two of them, shuffled) and the product C = A B at n^3 bits; ms and bit
+    multiply-adds per second (n^3 / t for the product)."""
+    import torch
+    rng = np.random.default_rng(82)
+    A = np.empty((n, n), np.uint32)
+    A[:rank] = rng.integers(0, 2, (rank, n))
+    pairs = rng.integers(0, rank, (n - rank, 2))
+    A[rank:] = A[pairs[:, 0]] ^ A[pairs[:, 1]]
+    A = A[rng.permutation(n)]
+    W = torch.from_numpy(E.pack_bits(A).view(np.int64).copy()).cuda()
+    work = torch.empty_like(W)
+    times = []
+    for _ in range(reps + 1):
+        work.copy_(W)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        r, _ = E.f2_rref(work, n)
+        times.append(time.perf_counter() - t)
+    Bw = torch.from_numpy(E.pack_bits(rng.integers(0, 2, (n, n)).astype(np.uint32)).view(np.int64).copy()).cuda()
+    Cw = torch.empty_like(W)
+    E.f2_gemm(W, Bw, Cw, n, n)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        E.f2_gemm(W, Bw, Cw, n, n)
+    e1.record()
+    torch.cuda.synchronize()
+    gms = e0.elapsed_time(e1) / reps
+    rref_s = min(times[1:])
+    return {"n": n, "rank": int(r), "planted_rank": rank, "rref_ms": rref_s * 1e3,
+            "rref_bit_ops_per_s": float(n) * n * rank / rref_s, "gemm_ms": gms,
+            "gemm_bit_macs_per_s": float(n) ** 3 / (gms * 1e-3)}
+
+
+def splitk_leg(E, m=256, n=256, k=65536, reps=5):
+    """A thin deep product mod 2147483629 (m x n x k = 256 x 256 x 65536): the plain kernel unsplit, the
+    plain kernel split K ways by the automatic policy, and the limb route; ms each, same bytes."""
+    import torch
+    from hacs_amd import workloads as W
+    p = W.P_C3
+    ctx = E.ModCtx(p)
+    A = E.random_entries(m, k, p, 0x51)
+    B = E.random_entries(k, n, p, 0x52)
+    out = {}
+    ref = None
+    for name, kw in [("plain", {"variant": E.HX_GEMM_PLAIN, "splits": 1}),
+                     ("plain_split", {"variant": E.HX_GEMM_PLAIN, "splits": 0}),
+                     ("limb", {"variant": E.HX_GEMM_LIMB})]:
+        Cm = torch.empty((m, n), dtype=torch.int32, device="cuda")
+        E.gemm(ctx, A, B, Cm, E.GemmOp.Set, **kw)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            E.gemm(ctx, A, B, Cm, E.GemmOp.Set, **kw)
+        e1.record()
+        torch.cuda.synchronize()
+        out[name + "_ms"] = e0.elapsed_time(e1) / reps
+        ref = Cm if ref is None else ref
+        out[name + "_same_bytes"] = bool(torch.equal(Cm, ref))
+    out.update({"shape": [m, n, k], "modulus": p, "splits": E.suggest_splits(m, n, k)})
+    return out
+
+
 def lu_c2(E, reps=3, cpu=True):
     """C2: RREF of the 4096 x 4096 rank-3000 matrix mod 2^31-1 (rank, pivots, pivot_rows, RREF)."""
     import torch
@@ -568,6 +638,8 @@ def main():
                "howell": lambda: howell_batch(E, args.howell_jobs),
                "ring2k": lambda: ring2k_lu(E),
                "u64": lambda: gemm_u64_leg(E),
+               "f2": lambda: f2_leg(E),
+               "splitk": lambda: splitk_leg(E),
                "level1": lambda: level1_kernels(E),
                "cpu": lambda: cpu_baseline(p)}[args.only]()
         print(json.dumps({"only": args.only, **leg}), flush=True)
@@ -650,6 +722,14 @@ def main():
         g64 = gemm_u64_leg(E)
         log(f"u64 GEMM: {g64}")
         res["gemm_u64"] = g64
+    if not args.no_f2:
+        f2 = f2_leg(E)
+        log(f"F_2: {f2}")
+        res["f2"] = f2
+    if not args.no_splitk:
+        sk = splitk_leg(E)
+        log(f"split-K: {sk}")
+        res["splitk"] = sk
     if not args.no_level1:
         l1 = level1_kernels(E)
         log(f"level-1/2 kernels: {l1}")

@@ -52,7 +52,8 @@ class hx_view(C.Structure):
 
 class hx_gemm_opts(C.Structure):
     _fields_ = [("variant", C.c_int), ("tile", C.c_int), ("workspace", C.c_void_p),
-                ("workspace_bytes", C.c_size_t), ("main_begin", C.c_void_p), ("main_end", C.c_void_p)]
+                ("workspace_bytes", C.c_size_t), ("main_begin", C.c_void_p), ("main_end", C.c_void_p),
+                ("splits", C.c_uint32), ("split_workspace", C.c_void_p), ("split_workspace_bytes", C.c_size_t)]
 
 
 class hx_device_info(C.Structure):
@@ -104,7 +105,14 @@ class hx_stats(C.Structure):
     _fields_ = [("kernel_launches", C.c_uint64), ("gemm_launches", C.c_uint64), ("batched_instances", C.c_uint64),
                 ("limb_launches", C.c_uint64), ("h2d_bytes", C.c_uint64), ("d2h_bytes", C.c_uint64),
                 ("d2d_bytes", C.c_uint64), ("h2d_calls", C.c_uint64), ("d2h_calls", C.c_uint64),
-                ("mallocs", C.c_uint64), ("frees", C.c_uint64)]
+                ("mallocs", C.c_uint64), ("frees", C.c_uint64), ("splitk_launches", C.c_uint64)]
+
+
+class hx_elim_stats(C.Structure):
+    _fields_ = [("calls", C.c_uint64), ("panels", C.c_uint64), ("panel_columns", C.c_uint64),
+                ("kernel_launches", C.c_uint64), ("gemms", C.c_uint64), ("h2d_bytes", C.c_uint64),
+                ("d2h_bytes", C.c_uint64), ("peak_device_bytes", C.c_size_t), ("seconds", C.c_double),
+                ("spec_panels", C.c_uint64)]
 
 
 class hx_ring2k_step_args(C.Structure):
@@ -157,6 +165,13 @@ SIGNATURES = {
     "hx_rref_batch_info_get": (_INT, [_V, C.POINTER(hx_rref_batch_info)]),
     "hx_arena_stats_get": (_INT, [C.POINTER(hx_arena_stats)]),
     "hx_arena_trim": (_INT, [C.POINTER(_SZ)]),
+    "hx_f2_rref": (_INT, [_U32, _U32, _V, _U32, _U32P, _U32P, _V]),
+    "hx_f2_rref_workspace_bytes": (_SZ, [_U32, _U32, _U32]),
+    "hx_f2_gemm": (_INT, [_U32, _U32, _U32, _V, _U32, _V, _U32, _V, _U32, _INT, _V]),
+    "hx_elim_stats_get": (_INT, [C.POINTER(hx_elim_stats)]),
+    "hx_elim_stats_reset": (_INT, []),
+    "hx_gemm_suggest_splits": (_U32, [_U32, _U32, _U32]),
+    "hx_gemm_split_workspace_bytes": (_SZ, [_U32, _U32, _U32, _U32]),
     "hx_gemm_u32_noalloc": (_INT, [C.POINTER(hx_modctx), hx_view, hx_view, hx_view, _INT, _V, C.POINTER(hx_gemm_opts)]),
     "hx_gemm_noalloc_scratch_bytes": (_SZ, []),
     "hx_schur_update_u32": (_INT, [C.POINTER(hx_modctx), hx_view, hx_view, hx_view, _V, C.POINTER(hx_gemm_opts)]),

@@ -58,12 +58,15 @@ hx_view hv(const DeviceView<E>& v) {
     return hx_view{(void*)v.base, (int64_t)v.rs, (int64_t)v.cs, v.rows, v.cols};
 }
 // The reference's limb-route workspace (GemmOpts::imma_workspace) is the library's limb workspace;
-// the split-K workspace (GemmOpts::workspace) has no use here (the library never splits K).
+// its split-K count and workspace (GemmOpts::splits / workspace) are the plain kernel's split-K.
 hx_gemm_opts gopts(const GemmOpts& o) {
     hx_gemm_opts h{};
     h.variant = o.imma == 0 ? HX_GEMM_PLAIN : HX_GEMM_AUTO;
     h.workspace = o.imma_workspace;
     h.workspace_bytes = o.imma_workspace_bytes;
+    h.splits = o.splits == 0 ? 1u : o.splits;       // the reference's default (1) never splits
+    h.split_workspace = o.workspace;
+    h.split_workspace_bytes = o.workspace_bytes;
     return h;
 }
 
@@ -268,8 +271,11 @@ bool GraphCapture::capturing() const { return impl_->open; }
 bool capturing_now() { return t_capture_depth > 0; }
 
 // ---- GEMM (CudaKernels.h:62-133) ---------------------------------------------------------------
-size_t gemm_workspace_bytes(uint32_t, uint32_t, uint32_t, size_t) { return 0; }   // no split-K workspace needed
-uint32_t suggest_splits(uint32_t, uint32_t, uint32_t) { return 1; }
+size_t gemm_workspace_bytes(uint32_t m, uint32_t n, uint32_t splits, size_t elt_bytes) {
+    // the partials are u32 words whatever elt_bytes; k is not known here, so no clamp to k
+    return elt_bytes == 4 ? hx_gemm_split_workspace_bytes(m, n, 0xffffffffu, splits) : 0;
+}
+uint32_t suggest_splits(uint32_t m, uint32_t n, uint32_t k) { return hx_gemm_suggest_splits(m, n, k); }
 bool imma_supported() { return available(); }
 bool imma_policy() { return true; }
 

@@ -51,9 +51,16 @@ int guard(const char* where, F&& body) {
 // Cuda/CudaCtx.h:94-101).  Relaxed atomics: counts only, never synchronisation.
 enum StatId {
     ST_KERNELS, ST_GEMMS, ST_BATCHED, ST_LIMB, ST_H2D_BYTES, ST_D2H_BYTES, ST_D2D_BYTES, ST_H2D_CALLS,
-    ST_D2H_CALLS, ST_MALLOCS, ST_FREES, ST_COUNT
+    ST_D2H_CALLS, ST_MALLOCS, ST_FREES, ST_SPLITK, ST_COUNT
 };
 void stat_add(StatId id, uint64_t n);
+uint64_t stat_value(StatId id);
+
+// Elimination counters (ImmaElimStats, Cuda/CudaKernels.h:289-298), diagnostics only.
+enum ElimStatId { EL_CALLS, EL_PANELS, EL_PANEL_COLUMNS, EL_LAUNCHES, EL_GEMMS, EL_H2D, EL_D2H, EL_SPEC_PANELS, EL_COUNT };
+void elim_stat_add(ElimStatId id, uint64_t n);
+void elim_peak_bytes(size_t bytes);          // keeps the largest single call's device footprint
+void elim_seconds_add(double seconds);
 
 // Checks the launch just issued (and counts it).
 inline void after_launch(const char* what) {
@@ -89,7 +96,7 @@ void arena_free(void* p);
 bool stream_capturing(hipStream_t s);
 // Library-owned scratch of at least `bytes`, private to stream s and one purpose (below); grows
 // (after waiting for s) but never shrinks; nullptr when it would have to grow inside a capture.
-enum ScratchPurpose { SCRATCH_LIMB = 0, SCRATCH_MLP = 1, SCRATCH_RREF = 2 };
+enum ScratchPurpose { SCRATCH_LIMB = 0, SCRATCH_MLP = 1, SCRATCH_RREF = 2, SCRATCH_SPLIT = 3 };
 void* stream_scratch(hipStream_t s, int purpose, size_t bytes);
 void stream_scratch_release(hipStream_t s);
 

@@ -23,6 +23,19 @@ struct ArMont {
     }
     __device__ __forceinline__ uint32_t prep(uint32_t f) const { return redc((uint64_t)f * r2); }   // f R
     __device__ __forceinline__ uint32_t mul(uint32_t y, uint32_t fprep) const { return redc((uint64_t)y * fprep); }
+    // a^{-1} mod p, 0 when a has none: a^(p-2) by Montgomery square-and-multiply (the same steps in every
+    // lane, no data-dependent loop), checked; a failed check (p not prime) falls back to Euclid.
+    __device__ uint32_t inv(uint32_t a) const {
+        if (a == 0) return 0;
+        uint32_t e = p - 2, base = prep(a), acc = prep(1u);
+        while (e) {
+            if (e & 1u) acc = mul(acc, base);
+            base = mul(base, base);
+            e >>= 1;
+        }
+        const uint32_t r = mul(acc, 1u);
+        return (uint64_t)r * a % p == 1u ? r : invm(a, p);
+    }
 };
 
 // Barrett arithmetic (any p < 2^31, used for even p, i.e. p = 2).
@@ -40,6 +53,7 @@ struct ArBarrett {
     }
     __device__ __forceinline__ uint32_t prep(uint32_t f) const { return f; }
     __device__ __forceinline__ uint32_t mul(uint32_t y, uint32_t fprep) const { return red((uint64_t)y * fprep); }
+    __device__ uint32_t inv(uint32_t a) const { return invm(a, p); }
 };
 
 inline ArMont make_mont(uint32_t p) {

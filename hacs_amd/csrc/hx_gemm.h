@@ -29,9 +29,14 @@ struct GemmArgs {
     int op = HX_OP_SET;
 };
 
-// The plain VALU kernel (any p < 2^32).
+// The plain VALU kernel (any p < 2^32); splits > 1: split-K in ws (plain_split_workspace_bytes).
 void gemm_plain(const hx_modctx& ctx, const GemmArgs& g, hipStream_t s, int tile, hipEvent_t ev0 = nullptr,
-                hipEvent_t ev1 = nullptr);
+                hipEvent_t ev1 = nullptr, uint32_t splits = 1, void* ws = nullptr, size_t ws_bytes = 0);
+// Split-K of the plain kernel: the split count actually run (no empty split, at most HX_MAX_SPLITS),
+// the workspace it needs, and the automatic policy's count for a shape.
+uint32_t plain_splits_clamp(uint32_t k, uint32_t splits);
+size_t plain_split_workspace_bytes(uint32_t m, uint32_t n, uint32_t splits);
+uint32_t plain_suggest_splits(uint32_t m, uint32_t n, uint32_t k);
 // The limb-MFMA route (ctx.limbs > 0). ws: the limb planes' workspace (ws_bytes); K runs in the chunks it holds.
 void gemm_limb(const hx_modctx& ctx, const GemmArgs& g, hipStream_t s, void* ws, size_t ws_bytes, int tile,
                hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
@@ -42,7 +47,10 @@ uint32_t limb_k_chunk(uint32_t limbs);
 uint32_t limb_k_chunk_for(const hx_modctx& ctx, uint32_t m, uint32_t n, uint32_t k, size_t bytes);
 // The variant the automatic policy picks for this shape.
 int gemm_auto_variant(const hx_modctx& ctx, uint32_t m, uint32_t n, uint32_t k);
+// splits: 1 = never split K, 0 = the automatic policy, > 1 = that many (plain kernel only); split_ws NULL =
+// the stream's own partials.
 void gemm_dispatch(const hx_modctx& ctx, const GemmArgs& g, hipStream_t s, int variant, void* ws, size_t ws_bytes,
-                   int tile, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+                   int tile, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr, uint32_t splits = 1,
+                   void* split_ws = nullptr, size_t split_ws_bytes = 0);
 
 }  // namespace hx

@@ -67,11 +67,26 @@ __device__ __forceinline__ void tile_of(uint32_t b, uint32_t ntm, uint32_t ntn, 
 // ------------------------------------------------------------------------------------------
 constexpr int PM = 64, PN = 64, PK = 32;
 
-__device__ __forceinline__ void plain_tile(const GemmArgs& g, const DevMod& M, uint32_t tm, uint32_t tn) {
+// Split-K (GemmOpts::splits, CudaKernels.h:64-66): workgroup (tile, z) sums the K range
+// [z kc, min(k, (z + 1) kc)) exactly and writes its canonical partial to part[(tile splits + z)
+// PM PN + cell]; the tile's last split to arrive (an agent-scope counter per tile, zero at launch)
+// adds the partials in the order z = 0, 1, ... and applies op to C -- no second launch.  The
+// bytes equal the unsplit product's (every partial is exact, the sum is reduced mod p once more).
+struct SplitK {
+    uint32_t* part;
+    uint32_t* arrive;
+    uint32_t splits, kc;
+};
+
+template <bool SPLIT>
+__device__ __forceinline__ void plain_tile(const GemmArgs& g, const DevMod& M, uint32_t tm, uint32_t tn,
+                                           const SplitK& sk = SplitK{}, uint32_t tile = 0, uint32_t z = 0) {
     __shared__ uint32_t As[2][PK][PM + 1];
     __shared__ uint32_t Bs[2][PK][PN + 1];
     const uint32_t m = g.m_dev ? min(*g.m_dev, g.m) : g.m;
-    const uint32_t k = g.k_dev ? min(*g.k_dev, g.k) : g.k;
+    const uint32_t kall = g.k_dev ? min(*g.k_dev, g.k) : g.k;
+    const uint32_t kb = SPLIT ? z * sk.kc : 0u;
+    const uint32_t k = SPLIT ? min(kall, kb + sk.kc) : kall;      // this workgroup's K range: [kb, k)
     const uint32_t n = g.n;
     const uint32_t m0 = tm * PM, n0 = tn * PN;
     if (m0 >= m) return;
@@ -117,15 +132,15 @@ __device__ __forceinline__ void plain_tile(const GemmArgs& g, const DevMod& M, u
             Bs[buf][e / PN][e % PN] = rb[t];
         }
     };
-    const uint32_t KT = (k + PK - 1) / PK;
+    const uint32_t KT = k > kb ? (k - kb + PK - 1) / PK : 0u;
     if (KT > 0) {
-        fetch(0);
+        fetch(kb);
         stash(0);
     }
     __syncthreads();
     for (uint32_t kt = 0; kt < KT; ++kt) {
         const int buf = kt & 1;
-        if (kt + 1 < KT) fetch((kt + 1) * PK);
+        if (kt + 1 < KT) fetch(kb + (kt + 1) * PK);
 #pragma unroll 4
         for (int kk = 0; kk < PK; ++kk) {
             uint32_t a[4], b[4];
@@ -146,6 +161,45 @@ __device__ __forceinline__ void plain_tile(const GemmArgs& g, const DevMod& M, u
         if (kt + 1 < KT) stash(buf ^ 1);
         __syncthreads();
     }
+    if (SPLIT) {
+        // this split's partial, then the arrival; the last split of the tile combines
+        uint32_t* const mine = sk.part + ((size_t)tile * sk.splits + z) * (PM * PN);
+        uint32_t r[4][4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                r[i][j] = red96(hi[i][j], lo[i][j], M);
+                mine[(ty + 16 * i) * PN + tx + 16 * j] = r[i][j];
+            }
+        __shared__ uint32_t s_last;
+        __threadfence();
+        __syncthreads();
+        if (tid == 0)
+            s_last = __hip_atomic_fetch_add(&sk.arrive[tile], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1 ==
+                     sk.splits;
+        __syncthreads();
+        if (!s_last) return;
+        __threadfence();                                   // every split's partial is visible here
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const uint32_t row = m0 + ty + 16 * i;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t col = n0 + tx + 16 * j;
+                if (row >= m || col >= n) continue;
+                const uint32_t cell = (ty + 16 * i) * PN + tx + 16 * j;
+                uint64_t sum = 0;                          // < 64 p < 2^38
+                for (uint32_t zz = 0; zz < sk.splits; ++zz)
+                    sum += zz == z ? r[i][j] : sk.part[((size_t)tile * sk.splits + zz) * (PM * PN) + cell];
+                const uint32_t v = red64(sum, M);
+                const uint32_t c = g.op == HX_OP_SET ? 0u : ld_cin(g, row, col);
+                g.Cout[(int64_t)row * g.cors + (int64_t)col * g.cocs] = apply_op(c, v, g.op, M.p);
+            }
+        }
+        if (tid == 0) sk.arrive[tile] = 0u;               // (the host clears the counters before each launch too)
+        return;
+    }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const uint32_t row = m0 + ty + 16 * i;
@@ -165,7 +219,21 @@ __global__ __launch_bounds__(256) void plain_kernel(GemmArgs g, DevMod M) {
     const uint32_t ntm = (g.m + PM - 1) / PM, ntn = (g.n + PN - 1) / PN;
     uint32_t tm, tn;
     tile_of(blockIdx.x, ntm, ntn, tm, tn);
-    plain_tile(g, M, tm, tn);
+    plain_tile<false>(g, M, tm, tn);
+}
+
+// Split-K launch: blockIdx = z * tiles + b (every tile's split z before any split z + 1).
+__global__ __launch_bounds__(256) void plain_split_kernel(GemmArgs g, DevMod M, SplitK sk) {
+    const uint32_t ntm = (g.m + PM - 1) / PM, ntn = (g.n + PN - 1) / PN, tiles = ntm * ntn;
+    const uint32_t z = blockIdx.x / tiles;
+    uint32_t tm, tn;
+    tile_of(blockIdx.x % tiles, ntm, ntn, tm, tn);
+    plain_tile<true>(g, M, tm, tn, sk, tm * ntn + tn, z);
+}
+
+__global__ void clear_words_kernel(uint32_t* w, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) w[i] = 0u;
 }
 
 // The workgroup's job in a batch: prefix[j] = first flat tile of job j (prefix[njobs] = total).
@@ -183,7 +251,7 @@ __global__ __launch_bounds__(256) void plain_batch_kernel(const GemmArgs* jobs, 
     const uint32_t j = job_of(prefix, njobs, blockIdx.x);
     const GemmArgs g = jobs[j];
     const uint32_t local = blockIdx.x - prefix[j], ntn = (g.n + PN - 1) / PN;
-    plain_tile(g, M, local / ntn, local % ntn);
+    plain_tile<false>(g, M, local / ntn, local % ntn);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -910,9 +978,59 @@ uint32_t limb_k_chunk_for(const hx_modctx& ctx, uint32_t m, uint32_t n, uint32_t
     return (uint32_t)std::min<size_t>(kc, full);
 }
 
-void gemm_plain(const hx_modctx& ctx, const GemmArgs& g, hipStream_t s, int /*tile*/, hipEvent_t ev0, hipEvent_t ev1) {
+// The K slice of one split: a multiple of the kernel's K step, so every split but the last is full.
+static uint32_t split_kc(uint32_t k, uint32_t splits) {
+    const uint64_t per = ((uint64_t)k + splits - 1) / splits;
+    return (uint32_t)((per + PK - 1) / PK * PK);
+}
+
+uint32_t plain_splits_clamp(uint32_t k, uint32_t splits) {
+    if (splits <= 1 || k == 0) return 1;
+    splits = std::min<uint32_t>(splits, HX_MAX_SPLITS);
+    const uint64_t kc = split_kc(k, splits);
+    return (uint32_t)(((uint64_t)k + kc - 1) / kc);                  // no empty split
+}
+
+size_t plain_split_workspace_bytes(uint32_t m, uint32_t n, uint32_t splits) {
+    if (splits <= 1 || m == 0 || n == 0) return 0;
+    const size_t tiles = (size_t)((m + PM - 1) / PM) * ((n + PN - 1) / PN);
+    return (tiles * 4 + 255) / 256 * 256 + tiles * splits * (size_t)(PM * PN) * 4;
+}
+
+// The split count the automatic policy takes for the plain kernel (suggest_splits' role,
+// cuda_kernels.cu:770-777, sized for gfx950): when the output tiles alone give fewer than 256
+// workgroups (one per CU) and K is deep, enough K slices of at least 256 for about 1024 workgroups.
+uint32_t plain_suggest_splits(uint32_t m, uint32_t n, uint32_t k) {
+    const uint64_t tiles = (uint64_t)((m + PM - 1) / PM) * ((n + PN - 1) / PN);
+    if (tiles == 0 || tiles >= 256 || k < 1024) return 1;
+    uint64_t sp = (1024 + tiles - 1) / tiles;
+    sp = std::min<uint64_t>(sp, k / 256);
+    return plain_splits_clamp(k, (uint32_t)std::max<uint64_t>(1, sp));
+}
+
+void gemm_plain(const hx_modctx& ctx, const GemmArgs& g, hipStream_t s, int /*tile*/, hipEvent_t ev0, hipEvent_t ev1,
+                uint32_t splits, void* ws, size_t ws_bytes) {
     if (g.m == 0 || g.n == 0) return;
     const uint32_t ntm = (g.m + PM - 1) / PM, ntn = (g.n + PN - 1) / PN;
+    splits = plain_splits_clamp(g.k, splits);
+    if (splits > 1) {
+        HX_CHECK(g.m_dev == nullptr && g.k_dev == nullptr, "split-K needs host-side sizes");
+        HX_CHECK(ws != nullptr && ws_bytes >= plain_split_workspace_bytes(g.m, g.n, splits),
+                 "split-K workspace too small (hx_gemm_split_workspace_bytes)");
+        const uint32_t tiles = ntm * ntn;
+        SplitK sk;
+        sk.arrive = (uint32_t*)ws;
+        sk.part = (uint32_t*)((char*)ws + ((size_t)tiles * 4 + 255) / 256 * 256);
+        sk.splits = splits;
+        sk.kc = split_kc(g.k, splits);
+        clear_words_kernel<<<(tiles + 255) / 256, 256, 0, s>>>(sk.arrive, tiles);
+        after_launch("clear_words_kernel");
+        if (ev0) HX_HIP(hipEventRecord(ev0, s));
+        plain_split_kernel<<<tiles * splits, 256, 0, s>>>(g, devmod(ctx), sk);
+        after_launch("plain_split_kernel");
+        if (ev1) HX_HIP(hipEventRecord(ev1, s));
+        return;
+    }
     if (ev0) HX_HIP(hipEventRecord(ev0, s));
     plain_kernel<<<ntm * ntn, 256, 0, s>>>(g, devmod(ctx));
     after_launch("plain_kernel");
@@ -1022,7 +1140,7 @@ int gemm_auto_variant(const hx_modctx& ctx, uint32_t m, uint32_t n, uint32_t k) 
 }
 
 void gemm_dispatch(const hx_modctx& ctx, const GemmArgs& g, hipStream_t s, int variant, void* ws, size_t ws_bytes,
-                   int tile, hipEvent_t ev0, hipEvent_t ev1) {
+                   int tile, hipEvent_t ev0, hipEvent_t ev1, uint32_t splits, void* split_ws, size_t split_ws_bytes) {
     const bool automatic = variant == HX_GEMM_AUTO;
     if (automatic) variant = gemm_auto_variant(ctx, g.m, g.n, g.k);
     if (variant == HX_GEMM_LIMB && g.m != 0 && g.n != 0) {
@@ -1043,13 +1161,35 @@ void gemm_dispatch(const hx_modctx& ctx, const GemmArgs& g, hipStream_t s, int v
             variant = HX_GEMM_PLAIN;
         }
     }
+    // split-K (plain kernel only, host-side sizes only): splits == 0 is the automatic policy
+    uint32_t sp = 1;
+    if (variant != HX_GEMM_LIMB && g.m != 0 && g.n != 0 && g.m_dev == nullptr && g.k_dev == nullptr) {
+        const bool auto_split = splits == 0;
+        sp = auto_split ? plain_suggest_splits(g.m, g.n, g.k) : plain_splits_clamp(g.k, splits);
+        if (sp > 1) {
+            const size_t need = plain_split_workspace_bytes(g.m, g.n, sp);
+            if (split_ws == nullptr) {
+                // library-owned partials private to this stream; inside a capture only when already large enough
+                split_ws = stream_scratch(s, SCRATCH_SPLIT, need);
+                split_ws_bytes = split_ws ? need : 0;
+                if (split_ws == nullptr && !auto_split)
+                    throw Error{HX_E_CAPTURE, "split-K needs a caller workspace inside a stream capture "
+                                              "(hx_gemm_opts.split_workspace; the library's own cannot grow there)"};
+            } else {
+                HX_CHECK(auto_split || split_ws_bytes >= need,
+                         "split-K workspace too small (hx_gemm_split_workspace_bytes)");
+            }
+            if (split_ws == nullptr || split_ws_bytes < need) sp = 1;      // automatic: no split
+        }
+    }
     if (variant == HX_GEMM_LIMB)
         gemm_limb(ctx, g, s, ws, ws_bytes, tile, ev0, ev1);
     else
-        gemm_plain(ctx, g, s, tile, ev0, ev1);
+        gemm_plain(ctx, g, s, tile, ev0, ev1, sp, split_ws, split_ws_bytes);
     if (g.m != 0 && g.n != 0) {
         stat_add(ST_GEMMS, 1);
         if (variant == HX_GEMM_LIMB) stat_add(ST_LIMB, 1);
+        if (sp > 1) stat_add(ST_SPLITK, 1);
     }
 }
 
@@ -1314,8 +1454,16 @@ HX_API int hx_gemm_u32(const hx_modctx* ctx, hx_view A, hx_view B, hx_view C, in
         HX_CHECK(variant != HX_GEMM_LIMB || ctx->limbs > 0, "gemm: the limb route needs p <= 4278124287");
         gemm_dispatch(*ctx, args_of(A, B, C, op), (hipStream_t)s, variant, opts ? opts->workspace : nullptr,
                       opts ? opts->workspace_bytes : 0, opts ? opts->tile : 0,
-                      opts ? (hipEvent_t)opts->main_begin : nullptr, opts ? (hipEvent_t)opts->main_end : nullptr);
+                      opts ? (hipEvent_t)opts->main_begin : nullptr, opts ? (hipEvent_t)opts->main_end : nullptr,
+                      opts ? opts->splits : 0u, opts ? opts->split_workspace : nullptr,
+                      opts ? opts->split_workspace_bytes : 0);
     });
+}
+
+HX_API uint32_t hx_gemm_suggest_splits(uint32_t m, uint32_t n, uint32_t k) { return plain_suggest_splits(m, n, k); }
+
+HX_API size_t hx_gemm_split_workspace_bytes(uint32_t m, uint32_t n, uint32_t k, uint32_t splits) {
+    return plain_split_workspace_bytes(m, n, plain_splits_clamp(k, splits));
 }
 
 HX_API uint32_t hx_gemm_limb_k_chunk(const hx_modctx* ctx, uint32_t m, uint32_t n, uint32_t k, size_t workspace_bytes) {

@@ -47,6 +47,11 @@ inline LimbConsts limb_consts(uint32_t p) {
     return c;
 }
 
+// The weight of anti-diagonal d in the recombination: 2^(8d) for d <= 3, w_d (balanced) above.
+__device__ __forceinline__ int64_t limb_weight(int d, const LimbConsts& c) {
+    return d <= 3 ? (int64_t)(1u << (8 * d)) : (int64_t)c.w[d - 4];
+}
+
 // sum_d 256^d D_d mod p from the ND = 2L - 1 anti-diagonal sums of one output.
 template <int ND>
 __device__ __forceinline__ uint32_t limb_recombine(const int32_t (&D)[ND], const LimbConsts& c, const DevMod& M) {

@@ -1,5 +1,6 @@
 // hx_rref.hip -- the single-matrix RREF entry points (imma_rref's role, Cuda/CudaKernels.h:249-298,
 // generalised from p <= 127 to every prime p < 2^31).
+#include <chrono>
 #include <vector>
 
 #include "hx_common.h"
@@ -122,13 +123,15 @@ HX_API int hx_rref_u32_host(const hx_modctx* ctx, uint32_t m, uint32_t n, hx_row
         *rank = 0;
         if (m == 0 || n == 0) return;
         if (slab_rows == 0) slab_rows = (uint32_t)std::max<uint64_t>(1, (64ull << 20) / (4ull * n));
+        const auto t0 = std::chrono::steady_clock::now();
         uint32_t* dA = nullptr;
         uint32_t* dR = nullptr;
         hipStream_t s;
         HX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         try {
-            HX_HIP(hipMalloc(&dA, (size_t)m * n * 4));
-            if (sink) HX_HIP(hipMalloc(&dR, (size_t)m * n * 4));
+            dA = (uint32_t*)arena_alloc((size_t)m * n * 4);
+            if (sink) dR = (uint32_t*)arena_alloc((size_t)m * n * 4);
+            elim_peak_bytes((size_t)m * n * 4 * (sink ? 2 : 1) + rref_large_workspace_bytes(m, n));
             for (uint32_t r0 = 0; r0 < m; r0 += slab_rows) {
                 const uint32_t nr = std::min(slab_rows, m - r0);
                 int64_t ld = n;
@@ -137,6 +140,7 @@ HX_API int hx_rref_u32_host(const hx_modctx* ctx, uint32_t m, uint32_t n, hx_row
                 HX_HIP(hipMemcpy2DAsync(dA + (size_t)r0 * n, (size_t)n * 4, rows, (size_t)ld * 4, (size_t)n * 4, nr,
                                         hipMemcpyHostToDevice, s));
                 HX_HIP(hipStreamSynchronize(s));   // the source view is valid until the next call
+                elim_stat_add(EL_H2D, (uint64_t)nr * n * 4);
             }
             const int rc = hx_rref_u32(ctx, m, n, dA, n, dR, n, rank, pivots, pivot_rows, (hx_stream)s);
             if (rc != HX_OK) throw Error{rc, hx_last_error()};
@@ -148,16 +152,19 @@ HX_API int hx_rref_u32_host(const hx_modctx* ctx, uint32_t m, uint32_t n, hx_row
                     HX_HIP(hipMemcpy2DAsync(out, (size_t)ld * 4, dR, (size_t)n * 4, (size_t)n * 4, *rank,
                                             hipMemcpyDeviceToHost, s));
                     HX_HIP(hipStreamSynchronize(s));
+                    elim_stat_add(EL_D2H, (uint64_t)*rank * n * 4);
                 }
             }
         } catch (...) {
-            (void)hipFree(dA);
-            (void)hipFree(dR);
+            (void)hipStreamSynchronize(s);
+            arena_free(dA);
+            arena_free(dR);
             (void)hipStreamDestroy(s);
             throw;
         }
-        HX_HIP(hipFree(dA));
-        if (dR) HX_HIP(hipFree(dR));
+        arena_free(dA);
+        arena_free(dR);
+        elim_seconds_add(std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
         HX_HIP(hipStreamDestroy(s));
     });
 }

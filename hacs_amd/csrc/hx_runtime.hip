@@ -22,6 +22,22 @@ static thread_local std::string g_last_error;
 static std::atomic<uint64_t> g_stat[ST_COUNT];
 
 void stat_add(StatId id, uint64_t n) { g_stat[id].fetch_add(n, std::memory_order_relaxed); }
+uint64_t stat_value(StatId id) { return g_stat[id].load(std::memory_order_relaxed); }
+
+static std::atomic<uint64_t> g_elim[EL_COUNT];
+static std::atomic<uint64_t> g_elim_peak{0};
+static std::mutex g_elim_mu;
+static double g_elim_seconds = 0.0;
+void elim_stat_add(ElimStatId id, uint64_t n) { g_elim[id].fetch_add(n, std::memory_order_relaxed); }
+void elim_peak_bytes(size_t bytes) {
+    uint64_t cur = g_elim_peak.load(std::memory_order_relaxed);
+    while (bytes > cur && !g_elim_peak.compare_exchange_weak(cur, bytes, std::memory_order_relaxed)) {
+    }
+}
+void elim_seconds_add(double seconds) {
+    std::lock_guard<std::mutex> lk(g_elim_mu);
+    g_elim_seconds += seconds;
+}
 
 void set_error(const std::string& msg) { g_last_error = msg; }
 int fail(int code, const std::string& msg) {
@@ -391,10 +407,36 @@ HX_API int hx_stats_get(hx_stats* out) {
         out->d2h_calls = g(ST_D2H_CALLS);
         out->mallocs = g(ST_MALLOCS);
         out->frees = g(ST_FREES);
+        out->splitk_launches = g(ST_SPLITK);
     });
 }
 HX_API int hx_stats_reset(void) {
     for (auto& c : g_stat) c.store(0, std::memory_order_relaxed);
+    return HX_OK;
+}
+
+HX_API int hx_elim_stats_get(hx_elim_stats* out) {
+    return guard("hx_elim_stats_get", [&] {
+        HX_CHECK(out != nullptr, "out is NULL");
+        auto g = [](ElimStatId id) { return g_elim[id].load(std::memory_order_relaxed); };
+        out->calls = g(EL_CALLS);
+        out->panels = g(EL_PANELS);
+        out->panel_columns = g(EL_PANEL_COLUMNS);
+        out->kernel_launches = g(EL_LAUNCHES);
+        out->gemms = g(EL_GEMMS);
+        out->h2d_bytes = g(EL_H2D);
+        out->d2h_bytes = g(EL_D2H);
+        out->spec_panels = g(EL_SPEC_PANELS);
+        out->peak_device_bytes = (size_t)g_elim_peak.load(std::memory_order_relaxed);
+        std::lock_guard<std::mutex> lk(g_elim_mu);
+        out->seconds = g_elim_seconds;
+    });
+}
+HX_API int hx_elim_stats_reset(void) {
+    for (auto& c : g_elim) c.store(0, std::memory_order_relaxed);
+    g_elim_peak.store(0, std::memory_order_relaxed);
+    std::lock_guard<std::mutex> lk(g_elim_mu);
+    g_elim_seconds = 0.0;
     return HX_OK;
 }
 

@@ -82,6 +82,49 @@ def from_device64(t: torch.Tensor) -> np.ndarray:
     return t.detach().cpu().contiguous().numpy().view(np.uint64)
 
 
+# ---- F_2 on bit-packed rows (binary::BitMatrix, Binary/BitMatrix.h) -------------------------------------
+def pack_bits(a: np.ndarray, ldw: int | None = None) -> np.ndarray:
+    """A 0/1 matrix (m x n) as bit-packed rows: ldw uint64 words per row (default ceil(n / 64)), bit j
+    of the row is bit j % 64 of word j / 64, padding zero (the BitMatrix layout)."""
+    a = np.asarray(a)
+    m, n = a.shape
+    w = (n + 63) // 64 if ldw is None else ldw
+    bits = np.zeros((m, w * 64), np.uint8)
+    bits[:, :n] = a & 1
+    by = np.packbits(bits.reshape(m, w * 8, 8)[:, :, ::-1], axis=2).reshape(m, w * 8)
+    return np.ascontiguousarray(by).view(np.uint64).reshape(m, w)
+
+
+def unpack_bits(words: np.ndarray, n: int) -> np.ndarray:
+    """The inverse of pack_bits: an m x n uint32 matrix of 0 / 1."""
+    w = np.ascontiguousarray(words, dtype=np.uint64)
+    m = w.shape[0]
+    by = w.view(np.uint8).reshape(m, -1, 1)
+    bits = np.unpackbits(by, axis=2)[:, :, ::-1].reshape(m, -1)
+    return bits[:, :n].astype(np.uint32)
+
+
+def f2_rref(A_words: torch.Tensor, n: int, stream=None) -> tuple[int, np.ndarray]:
+    """In-place RREF over F_2 of a bit-packed device matrix (int64 tensor m x ldw); returns (rank, pivot
+    columns) -- rref(BitMatrix&), Binary/BitMatrix.h:101-105."""
+    m, ldw = A_words.shape
+    assert A_words.is_contiguous()
+    piv = np.zeros(max(1, min(m, n)), np.uint32)
+    rank = C.c_uint32(0)
+    check(lib().hx_f2_rref(m, n, C.c_void_p(A_words.data_ptr()), ldw, C.byref(rank),
+                           piv.ctypes.data_as(_lib._U32P), current_stream() if stream is None else stream))
+    return rank.value, piv[:rank.value].copy()
+
+
+def f2_gemm(A_words: torch.Tensor, B_words: torch.Tensor, C_words: torch.Tensor, k: int, n: int,
+            accumulate: bool = False, stream=None) -> None:
+    """C = A B (or C += A B) over F_2 on bit-packed device matrices (multiply_m4rm, BitMatrix.h:129)."""
+    m = A_words.shape[0]
+    check(lib().hx_f2_gemm(m, n, k, C.c_void_p(A_words.data_ptr()), A_words.stride(0),
+                           C.c_void_p(B_words.data_ptr()), B_words.stride(0), C.c_void_p(C_words.data_ptr()),
+                           C_words.stride(0), 1 if accumulate else 0, current_stream() if stream is None else stream))
+
+
 def current_stream() -> int:
     return torch.cuda.current_stream().cuda_stream
 
@@ -110,20 +153,27 @@ def from_device(t: torch.Tensor) -> np.ndarray:
     return t.detach().cpu().contiguous().numpy().view(np.uint32)
 
 
-def _opts(variant: int, tile: int = 0, workspace: torch.Tensor | None = None) -> _lib.hx_gemm_opts:
+def _opts(variant: int, tile: int = 0, workspace: torch.Tensor | None = None, splits: int = 0,
+          split_workspace: torch.Tensor | None = None) -> _lib.hx_gemm_opts:
     o = _lib.hx_gemm_opts()
     o.variant = variant
     o.tile = tile
     if workspace is not None:
         o.workspace = C.c_void_p(workspace.data_ptr())
         o.workspace_bytes = workspace.numel() * workspace.element_size()
+    o.splits = splits
+    if split_workspace is not None:
+        o.split_workspace = C.c_void_p(split_workspace.data_ptr())
+        o.split_workspace_bytes = split_workspace.numel() * split_workspace.element_size()
     return o
 
 
 def gemm(ctx: ModCtx, A: torch.Tensor, B: torch.Tensor, C_: torch.Tensor, op: int = GemmOp.Set, stream=None,
-         variant: int = HX_GEMM_AUTO, tile: int = 0, workspace: torch.Tensor | None = None) -> None:
-    """C op= A * B mod p  (engine::cuda::gemm, Cuda/CudaKernels.h:120-123)."""
-    o = _opts(variant, tile, workspace)
+         variant: int = HX_GEMM_AUTO, tile: int = 0, workspace: torch.Tensor | None = None, splits: int = 0,
+         split_workspace: torch.Tensor | None = None) -> None:
+    """C op= A * B mod p  (engine::cuda::gemm, Cuda/CudaKernels.h:120-123).  splits: split-K of the plain
+    kernel (0 = automatic, 1 = never, more = that many K slices; GemmOpts::splits)."""
+    o = _opts(variant, tile, workspace, splits, split_workspace)
     check(lib().hx_gemm_u32(C.byref(ctx.c), view(A), view(B), view(C_), op,
                             current_stream() if stream is None else stream, C.byref(o)))
 
@@ -133,6 +183,16 @@ def gemm_nt(ctx: ModCtx, A, B, C_, op: int = GemmOp.Set, stream=None, variant: i
     o = _opts(variant)
     check(lib().hx_gemm_nt_u32(C.byref(ctx.c), view(A), view(B), view(C_), op,
                                current_stream() if stream is None else stream, C.byref(o)))
+
+
+def suggest_splits(m: int, n: int, k: int) -> int:
+    """The automatic split-K count of an m x n x k plain product (suggest_splits, CudaKernels.h:79)."""
+    return int(lib().hx_gemm_suggest_splits(m, n, k))
+
+
+def split_workspace_bytes(m: int, n: int, k: int, splits: int) -> int:
+    """Workspace bytes of a split-K product (gemm_workspace_bytes, CudaKernels.h:78)."""
+    return int(lib().hx_gemm_split_workspace_bytes(m, n, k, splits))
 
 
 def _s(stream):

@@ -91,7 +91,17 @@ typedef struct hx_gemm_opts {   /* GemmOpts, CudaKernels.h:64-79 */
     size_t workspace_bytes;
     hx_event main_begin;    /* optional timing events recorded on the stream right before / after the */
     hx_event main_end;      /* main product kernel (instrumentation only; NULL = none) */
+    uint32_t splits;        /* split-K of the plain kernel (GemmOpts::splits, CudaKernels.h:64-66): 0 = automatic
+                               (hx_gemm_suggest_splits), 1 = never, 2 .. HX_MAX_SPLITS = that many K slices (fewer
+                               when k is short).  The limb route never splits.  Same bytes either way. */
+    void *split_workspace;  /* the partial sums (GemmOpts::workspace): >= hx_gemm_split_workspace_bytes(); NULL =
+                               library-owned partials private to the stream (grown outside a capture only: inside
+                               one, automatic splitting is skipped when they are too small and an explicit
+                               count fails with HX_E_CAPTURE) */
+    size_t split_workspace_bytes;
 } hx_gemm_opts;
+
+#define HX_MAX_SPLITS 64
 
 typedef struct hx_device_info {   /* CudaCtx.h:55-64 DeviceInfo */
     char name[128];
@@ -174,9 +184,27 @@ typedef struct hx_stats {
     uint64_t h2d_bytes, d2h_bytes, d2d_bytes;   /* hx_upload / hx_download / hx_copy_d2d */
     uint64_t h2d_calls, d2h_calls;
     uint64_t mallocs, frees;     /* hx_malloc / hx_free */
+    uint64_t splitk_launches;    /* plain GEMMs run split-K */
 } hx_stats;
 HX_API int hx_stats_get(hx_stats *out);
 HX_API int hx_stats_reset(void);
+
+/* Elimination counters (ImmaElimStats / imma_elim_stats, CudaKernels.h:289-298), over the whole-GPU RREF
+ * (hx_rref_u32 on large shapes, hx_rref_u32_device, hx_rref_u32_host). */
+typedef struct hx_elim_stats {
+    uint64_t calls;              /* eliminations enqueued */
+    uint64_t panels;             /* column panels (the reference's bases) */
+    uint64_t panel_columns;      /* their columns */
+    uint64_t kernel_launches;    /* kernels launched while an elimination was enqueued (process-wide difference) */
+    uint64_t gemms;              /* the panels' update products (matrix-core launches) */
+    uint64_t h2d_bytes, d2h_bytes;   /* the host-slab form's row transfers */
+    size_t peak_device_bytes;    /* the largest single call's device footprint */
+    double seconds;              /* wall time inside the synchronous calls, summed */
+    uint64_t spec_panels;        /* panels whose pivots the search over their first 64 rows settled (synchronous
+                                    calls only; the rest ran the full tournament) */
+} hx_elim_stats;
+HX_API int hx_elim_stats_get(hx_elim_stats *out);
+HX_API int hx_elim_stats_reset(void);
 
 /* ---- graph capture (CudaCtx.h:223-253 GraphCapture) ---------------------- */
 HX_API int hx_graph_begin(hx_stream s);
@@ -195,6 +223,11 @@ HX_API size_t hx_gemm_workspace_bytes(const hx_modctx *ctx, uint32_t m, uint32_t
  * int32-exact chunk) when its planes fit, else the largest multiple of 64 that fits; 0 when not even a
  * 64-deep chunk fits.  Host-only arithmetic (no device needed). */
 HX_API uint32_t hx_gemm_limb_k_chunk(const hx_modctx *ctx, uint32_t m, uint32_t n, uint32_t k, size_t workspace_bytes);
+/* Split-K (suggest_splits / gemm_workspace_bytes, CudaKernels.h:80-81): the count the automatic policy takes
+ * for an m x n x k plain product (1 when the output tiles alone fill the GPU or k < 1024), and the workspace
+ * bytes of a split product (0 for splits <= 1).  Host-only arithmetic. */
+HX_API uint32_t hx_gemm_suggest_splits(uint32_t m, uint32_t n, uint32_t k);
+HX_API size_t hx_gemm_split_workspace_bytes(uint32_t m, uint32_t n, uint32_t k, uint32_t splits);
 /* hx_gemm_u32 without any device allocation.  No counterpart in the reference: for callers that cannot
  * allocate (code under graph capture, tight serving loops, harnesses that forbid it).  The limb route's
  * planes live in a fixed 64 MiB device array per GPU (hx_gemm_noalloc_scratch_bytes), K running in the
@@ -208,6 +241,21 @@ HX_API size_t hx_gemm_noalloc_scratch_bytes(void);
 /* C -= L * U (schur_update, CudaKernels.h:130-133). */
 HX_API int hx_schur_update_u32(const hx_modctx *ctx, hx_view L, hx_view U, hx_view C, hx_stream s,
                                const hx_gemm_opts *opts);
+
+/* ---- F_2 on bit-packed rows (binary::BitMatrix, Binary/BitMatrix.h:49-140) ---------------------------
+ * A bit matrix in device memory: row i is `ldw` 64-bit words, bit j of the row is bit j % 64 of word
+ * j / 64 (ldw >= ceil(n / 64)); padding bits (columns >= n) are zero and stay zero.  Over F_2 the RREF,
+ * its pivot columns and the rank are unique, so the results equal the reference's bit for bit. */
+/* In-place reduced row echelon form (rref(BitMatrix&), BitMatrix.h:101-105: leading 1 in each pivot
+ * column, zero above and below, zero rows last); rank and the pivot columns ascending (pivots: room for
+ * min(m, n), or NULL).  Synchronous on s; its workspace comes from the library's arena
+ * (hx_f2_rref_workspace_bytes).  m <= 16384 rows (HX_E_ARGUMENT beyond). */
+HX_API int hx_f2_rref(uint32_t m, uint32_t n, uint64_t *A, uint32_t ldw, uint32_t *rank, uint32_t *pivots, hx_stream s);
+HX_API size_t hx_f2_rref_workspace_bytes(uint32_t m, uint32_t n, uint32_t ldw);
+/* C = A B (accumulate = 0) or C += A B (accumulate = 1) over F_2 (multiply_m4rm, BitMatrix.h:129):
+ * A m x k, B k x n, C m x n, all bit-packed with the given row strides in words. */
+HX_API int hx_f2_gemm(uint32_t m, uint32_t n, uint32_t k, const uint64_t *A, uint32_t lda, const uint64_t *B,
+                      uint32_t ldb, uint64_t *C, uint32_t ldc, int accumulate, hx_stream s);
 
 /* ---- GEMM mod p for 64-bit residues (Cuda/cuda_kernels.cu:104-164, the u64 kernel; CPU path of record
  *      kernels::gemm<uint64_t>, ModArithmetic/Kernels.h:184) ----------------------------------------- */

@@ -103,13 +103,17 @@ private:
 
 struct GemmOpts {                 // GemmOpts (CudaKernels.h:64-79)
     int variant = HX_GEMM_AUTO;   // HX_GEMM_PLAIN / HX_GEMM_LIMB / AUTO
-    void* workspace = nullptr;    // allocate before a capture
+    void* workspace = nullptr;    // the limb planes; allocate before a capture
     size_t workspace_bytes = 0;
+    uint32_t splits = 0;          // split-K of the plain kernel: 0 = automatic, 1 = never
+    void* split_workspace = nullptr;
+    size_t split_workspace_bytes = 0;
 };
 
 inline void gemm(const ModCtx32& ctx, DeviceView<const uint32_t> A, DeviceView<const uint32_t> B,
                  DeviceView<uint32_t> C, GemmOp op, Stream s, const GemmOpts& o = GemmOpts{}) {
-    hx_gemm_opts co{o.variant, 0, o.workspace, o.workspace_bytes, nullptr, nullptr};
+    hx_gemm_opts co{o.variant, 0, o.workspace, o.workspace_bytes, nullptr, nullptr,
+                    o.splits, o.split_workspace, o.split_workspace_bytes};
     check(hx_gemm_u32(&ctx.c, hx_view{(void*)A.base, A.rs, A.cs, A.rows, A.cols},
                       hx_view{(void*)B.base, B.rs, B.cs, B.rows, B.cols}, C.c_view(), (int)op, s.h, &co));
 }

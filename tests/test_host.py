@@ -90,3 +90,27 @@ def test_arena_stats_without_device():
     st = E.arena_stats()
     assert set(st) >= {"reserved_bytes", "live_bytes", "driver_mallocs", "reuses", "oom_retries"}
     assert st["live_bytes"] <= st["reserved_bytes"]
+
+
+def test_split_k_policy():
+    """The split-K count the automatic policy takes (host arithmetic): 1 while the output tiles alone
+    fill the GPU (256 of 64 x 64) or k < 1024, else enough K slices of >= 256 for about 1024 workgroups,
+    at most 64, never an empty slice; and the workspace a split product needs (tile counters, then
+    one 64 x 64 block of u32 partials per tile and slice)."""
+    L = _lib.lib()
+    assert L.hx_gemm_suggest_splits(8192, 8192, 8192) == 1
+    assert L.hx_gemm_suggest_splits(1024, 1024, 100000) == 1          # 256 tiles
+    assert L.hx_gemm_suggest_splits(64, 64, 1000) == 1                # k < 1024
+    assert L.hx_gemm_suggest_splits(256, 256, 65536) == 64            # 16 tiles: min(64, 65536 / 256)
+    assert L.hx_gemm_suggest_splits(500, 500, 4096) == 16             # 64 tiles: min(16, 4096 / 256)
+    assert L.hx_gemm_suggest_splits(64, 64, 1100) == 4                # 1 tile: min(1024, 4) slices of 288
+    assert L.hx_gemm_suggest_splits(0, 64, 5000) == 1
+    for (m, n, k, s) in [(256, 256, 65536, 64), (100, 70, 3333, 7), (1, 1, 40000, 64)]:
+        tiles = -(-m // 64) * -(-n // 64)
+        kc = -(-(-(-k // s)) // 32) * 32
+        eff = -(-k // kc)
+        want = -(-tiles * 4 // 256) * 256 + tiles * eff * 64 * 64 * 4
+        assert L.hx_gemm_split_workspace_bytes(m, n, k, s) == want, (m, n, k, s)
+    # a count above what k allows shrinks (no empty slice): k = 40 in slices of 32 is 2 slices
+    assert L.hx_gemm_split_workspace_bytes(64, 64, 40, 64) == 256 + 2 * 64 * 64 * 4
+    assert L.hx_gemm_split_workspace_bytes(64, 64, 5000, 1) == 0

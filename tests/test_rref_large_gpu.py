@@ -55,6 +55,30 @@ def test_rref_c2_full_size(hx_gpu, O):
     assert not Rh[res.rank:].any()
 
 
+def test_elimination_counters(hx_gpu, O):
+    """The elimination counters (ImmaElimStats, Cuda/CudaKernels.h:289-298) over one whole-GPU RREF: one
+    call, ceil(n / 32) panels over n columns, at least one update product per panel but the last, the
+    call's workspace as the peak footprint and a positive wall time; reset clears them."""
+    import ctypes as C
+    from hacs_amd import _lib
+    from hacs_amd import engine as E
+    L = _lib.lib()
+    p, m, n = 2147483647, 1500, 1400
+    A = make(np.random.default_rng(9), m, n, p, "planted", rank=1000)
+    _lib.check(L.hx_elim_stats_reset())
+    res = E.rref(E.ModCtx(p), E.to_device(A))
+    st = _lib.hx_elim_stats()
+    _lib.check(L.hx_elim_stats_get(C.byref(st)))
+    assert res.rank == O.rref_rule(p, A, threads=16)[0]
+    assert st.calls == 1 and st.panels == -(-n // 32) and st.panel_columns == n
+    assert st.gemms >= st.panels - 1 and st.kernel_launches >= st.panels + st.gemms
+    assert st.peak_device_bytes >= L.hx_rref_workspace_bytes(m, n) and st.seconds > 0.0
+    assert st.h2d_bytes == 0 and st.d2h_bytes == 0
+    _lib.check(L.hx_elim_stats_reset())
+    _lib.check(L.hx_elim_stats_get(C.byref(st)))
+    assert st.calls == 0 and st.peak_device_bytes == 0 and st.seconds == 0.0
+
+
 def test_trsm_unit_matches_oracle(hx_gpu, O):
     from hacs_amd import engine as E
     p = 2147483629
