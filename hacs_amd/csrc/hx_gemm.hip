@@ -1143,6 +1143,12 @@ void gemm_dispatch(const hx_modctx& ctx, const GemmArgs& g, hipStream_t s, int v
                    int tile, hipEvent_t ev0, hipEvent_t ev1, uint32_t splits, void* split_ws, size_t split_ws_bytes) {
     const bool automatic = variant == HX_GEMM_AUTO;
     if (automatic) variant = gemm_auto_variant(ctx, g.m, g.n, g.k);
+    // a thin deep product whose limb tiles cannot fill the GPU runs faster on the plain kernel split
+    // over K (256 x 256 x 65536: plain split 64 ways 0.87 ms, limb route 1.50 ms, plain unsplit 11.7 ms)
+    if (automatic && variant == HX_GEMM_LIMB && splits != 1 && g.m_dev == nullptr && g.k_dev == nullptr) {
+        const uint64_t limb_tiles = (uint64_t)((g.m + LM - 1) / LM) * ((g.n + LN - 1) / LN);
+        if (limb_tiles < 32 && plain_suggest_splits(g.m, g.n, g.k) > 1) variant = HX_GEMM_PLAIN;
+    }
     if (variant == HX_GEMM_LIMB && g.m != 0 && g.n != 0) {
         if (ws == nullptr) {
             // library-owned planes private to this stream (never shared with another stream); inside a

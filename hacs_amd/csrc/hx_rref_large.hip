@@ -799,16 +799,14 @@ inline uint32_t gsz(uint64_t n) { return (uint32_t)std::min<uint64_t>((n + 255) 
 
 // A second stream and two events for the lookahead, per calling stream (created once, reused by
 // every elimination issued on that stream, so a call allocates nothing and can be captured).
-// The streams beside the caller's: s2 takes the wide trailing updates, s3 the Gauss-Jordan gathers
-// and products, sc the panel chain when the caller's stream is the legacy null stream (which every
-// blocking stream would otherwise synchronise with).  Outside a capture s2 is restricted to all but
-// 16 compute units (a CU-masked queue, on the 256-CU part), so the chain's kernels -- the latency-critical
-// pivot search above all, whose 1024-thread blocks do not fit beside the update's waves -- always
-// find free CUs; a masked stream is a blocking one, so a capture (and the null-stream case, through
-// sc) never makes the chain wait for it implicitly.  A capture uses the unmasked s2n (a graph does not
-// keep queue masks anyway).  (Masking s3, the Gauss-Jordan products, as well was measured slower.)
+// The streams beside the caller's (non-blocking, created once per caller stream): s2 takes the wide
+// trailing updates, s3 the Gauss-Jordan gathers and products, sc the panel chain when the caller's
+// stream is the legacy null stream.  (A CU-masked queue for s2, which leaves CUs free for the chain,
+// was measured: 3 % faster alone, but a masked stream is a blocking one, so every launch on it waits
+// for the null stream, and when the null stream shares a hardware queue with another side stream the
+// updates serialise behind it -- 16.3 ms instead of 10 for C2 once other streams existed.)
 struct SideStream {
-    hipStream_t s2 = nullptr, s2n = nullptr, s3 = nullptr, sc = nullptr;
+    hipStream_t s2 = nullptr, s3 = nullptr, sc = nullptr;
     hipEvent_t ev_x = nullptr, ev_rest = nullptr, ev_gj[2] = {nullptr, nullptr}, ev_fork = nullptr,
                ev_join = nullptr;
 };
@@ -818,30 +816,9 @@ SideStream side_for(hipStream_t s) {
     std::lock_guard<std::mutex> lk(mu);
     SideStream& ss = map[s];
     if (ss.sc == nullptr) {
-        HX_HIP(hipStreamCreateWithFlags(&ss.s2n, hipStreamNonBlocking));
+        HX_HIP(hipStreamCreateWithFlags(&ss.s2, hipStreamNonBlocking));
         HX_HIP(hipStreamCreateWithFlags(&ss.s3, hipStreamNonBlocking));
         HX_HIP(hipStreamCreateWithFlags(&ss.sc, hipStreamNonBlocking));
-        int dev = 0, ncu = 0;
-        HX_HIP(hipGetDevice(&dev));
-        HX_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-        ss.s2 = ss.s2n;
-        if (ncu == 256) {
-            // two CUs of each of the 8 XCDs left out, whether the mask's bits run XCD by XCD (bit i on
-            // XCD i / 32) or round-robin over the XCDs (bit i on XCD i % 8): the bits 32x + x and
-            // 32x + 16 + (x + 4) % 8 for x = 0..7 (an unbalanced mask slows every kernel on the
-            // stream to its most reduced XCD's pace)
-            std::vector<uint32_t> mask(8, 0xffffffffu);
-            for (int x = 0; x < 8; ++x) {
-                const int a = 32 * x + x, b = 32 * x + 16 + (x + 4) % 8;
-                mask[a / 32] &= ~(1u << (a % 32));
-                mask[b / 32] &= ~(1u << (b % 32));
-            }
-            hipStream_t masked = nullptr;
-            if (hipExtStreamCreateWithCUMask(&masked, (uint32_t)mask.size(), mask.data()) == hipSuccess)
-                ss.s2 = masked;
-            else
-                (void)hipGetLastError();
-        }
         for (hipEvent_t* e : {&ss.ev_x, &ss.ev_rest, &ss.ev_gj[0], &ss.ev_gj[1], &ss.ev_fork, &ss.ev_join})
             HX_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
     }
@@ -959,9 +936,8 @@ void enqueue_large(const hx_modctx& ctx, const Ar& ar, uint32_t m, uint32_t n, c
     // chain's stream) and the rest (on s2, overlapping panel i+1's pivot search); the Gauss-Jordan
     // gather and product run on s3.  Panel i+1's X waits for s2 (its rows are read whole); panel
     // i+2, which reuses panel i's slot (P, J, X, its limb planes, the sizes), waits for s3.
-    const bool capturing = stream_capturing(s);
     hipStream_t sc = s == nullptr ? side.sc : s;            // the panel chain
-    hipStream_t s2 = capturing ? side.s2n : side.s2, s3 = side.s3;
+    hipStream_t s2 = side.s2, s3 = side.s3;
     if (sc != s) {
         HX_HIP(hipEventRecord(side.ev_fork, s));
         HX_HIP(hipStreamWaitEvent(sc, side.ev_fork, 0));

@@ -803,3 +803,57 @@ HXO_API void hxo_gemm_u64(uint64_t p, uint32_t m, uint32_t n, uint32_t k, const 
     if (m == 0 || n == 0) return;
     team_run(nthreads, gemm64_worker, &g);
 }
+
+/* ------------------------------------------------------------------------- */
+/* Smith valuations over Z/2^k by global minimal-valuation pivoting: the     */
+/* reference's smith_naive (Binary/Ring2k.h:1004-1035, "the prototype's      */
+/* smith mode").  A: m x n words (row-major, consumed), values read mod 2^k   */
+/* (k <= 64; the reference's u32 / u64 rungs agree mod 2^k).  sigma: the     */
+/* valuations in pivot order (ascending).  Returns their count.               */
+static uint64_t sm_inv_odd(uint64_t u) { /* u^-1 mod 2^64, Newton from 3 bits */
+    uint64_t x = u;
+    for (int i = 0; i < 6; ++i) x *= 2u - u * x;
+    return x;
+}
+HXO_API int64_t hxo_smith_naive(uint32_t k, uint32_t m, uint32_t n, uint64_t *A, uint32_t *sigma) {
+    const uint64_t mask = k >= 64 ? ~(uint64_t)0 : (((uint64_t)1 << k) - 1);
+#define SM_AT(i, j) A[(size_t)(i) * n + (j)]
+    int64_t cnt = 0;
+    for (uint32_t r = 0; r < m && r < n; ++r) {
+        uint32_t bi = m, bj = n, bv = k;
+        for (uint32_t i = r; i < m && bv > 0; ++i)
+            for (uint32_t j = r; j < n; ++j) {
+                const uint64_t x = SM_AT(i, j) & mask;
+                const uint32_t v = x == 0 ? k : (uint32_t)__builtin_ctzll(x);
+                if (v < bv) {
+                    bv = v;
+                    bi = i;
+                    bj = j;
+                    if (v == 0) break;
+                }
+            }
+        if (bi == m) break;
+        if (bi != r)
+            for (uint32_t j = 0; j < n; ++j) {
+                const uint64_t t = SM_AT(bi, j);
+                SM_AT(bi, j) = SM_AT(r, j);
+                SM_AT(r, j) = t;
+            }
+        if (bj != r)
+            for (uint32_t i = 0; i < m; ++i) {
+                const uint64_t t = SM_AT(i, bj);
+                SM_AT(i, bj) = SM_AT(i, r);
+                SM_AT(i, r) = t;
+            }
+        const uint64_t ui = sm_inv_odd((SM_AT(r, r) & mask) >> bv);
+        for (uint32_t i = r + 1; i < m; ++i) {
+            const uint64_t x = SM_AT(i, r) & mask;
+            if (x == 0) continue;
+            const uint64_t f = (x >> bv) * ui;
+            for (uint32_t j = r; j < n; ++j) SM_AT(i, j) -= f * SM_AT(r, j);
+        }
+        sigma[cnt++] = bv;
+    }
+#undef SM_AT
+    return cnt;
+}

@@ -231,6 +231,38 @@ def ref_r2_echelon(k: int, A: np.ndarray, panel: int, climit: int | None = None)
     return A, int(r), piv[:r].copy(), pv[:r].copy()
 
 
+def smith(k: int, A: np.ndarray) -> np.ndarray:
+    """The Smith valuations of A over Z/2^k (ascending), by the oracle's restatement of smith_naive
+    (global minimal-valuation pivoting); entries read mod 2^k, k <= 64."""
+    M = np.ascontiguousarray(A, dtype=np.uint64).copy()
+    m, n = M.shape
+    sig = np.zeros(max(1, min(m, n)), np.uint32)
+    L = lib()
+    L.hxo_smith_naive.restype = C.c_int64
+    L.hxo_smith_naive.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, u32p]
+    r = L.hxo_smith_naive(k, m, n, M.ctypes.data_as(C.c_void_p), sig)
+    return sig[:r].copy()
+
+
+def ref_smith(k: int, A: np.ndarray, naive: bool = False):
+    """The reference's own ring2k::smith (levels) or smith_naive (oracle/_ref) on A's rung (uint32 / uint64):
+    (sigma, level_ranks)."""
+    A = np.ascontiguousarray(A)
+    m, n = A.shape
+    sig = np.zeros(max(1, min(m, n)), np.uint32)
+    lev = np.zeros(max(1, k), np.uint32)
+    nl = C.c_uint32(0)
+    R = ref()
+    R.hxr_smith.restype = C.c_int64
+    R.hxr_smith.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_int64, C.c_int,
+                            u32p, u32p, C.POINTER(C.c_uint32)]
+    r = R.hxr_smith(A.dtype.itemsize, k, m, n, A.ctypes.data_as(C.c_void_p), n, 1 if naive else 0, sig, lev,
+                    C.byref(nl))
+    if r < 0:
+        raise RuntimeError("reference smith failed")
+    return sig[:r].copy(), lev[:nl.value].copy()
+
+
 def fnv1a_bytes(b: bytes) -> int:
     return int(lib().hxo_fnv1a_bytes(b, len(b)))
 

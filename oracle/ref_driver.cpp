@@ -207,6 +207,34 @@ int64_t hxr_r2_echelon(uint32_t elem, uint32_t k, uint32_t m, uint32_t n, void* 
     }
 }
 
+// engine::ring2k::smith (valuation levels, Binary/Ring2k.h:701) or smith_naive (global minimal-valuation
+// pivoting) on a copy of A over Z/2^k (elem 4: k <= 32, elem 8: k <= 64): sigma ascending (room for
+// min(m, n)) and, for smith, level_ranks (room for k; *nlevels set).  Returns the number of invariants.
+int64_t hxr_smith(uint32_t elem, uint32_t k, uint32_t m, uint32_t n, const void* A, int64_t lda, int naive,
+                  uint32_t* sigma, uint32_t* level_ranks, uint32_t* nlevels) {
+    try {
+        auto run = [&](auto tag) -> int64_t {
+            using E = decltype(tag);
+            const E* src = static_cast<const E*>(A);
+            engine::ring2k::Mat<E> a(m, n);
+            for (uint32_t i = 0; i < m; ++i)
+                for (uint32_t j = 0; j < n; ++j) a.at(i, j) = src[static_cast<int64_t>(i) * lda + j];
+            const engine::ring2k::SmithResult r =
+                naive ? engine::ring2k::smith_naive<engine::ring2k::Traits<E>>(std::move(a), k)
+                      : engine::ring2k::smith<engine::ring2k::Traits<E>>(std::move(a), k);
+            for (size_t t = 0; t < r.sigma.size(); ++t) sigma[t] = r.sigma[t];
+            if (nlevels) *nlevels = static_cast<uint32_t>(r.level_ranks.size());
+            if (level_ranks)
+                for (size_t t = 0; t < r.level_ranks.size(); ++t) level_ranks[t] = r.level_ranks[t];
+            return static_cast<int64_t>(r.sigma.size());
+        };
+        if (elem == 8) return run(uint64_t{});
+        return run(uint32_t{});
+    } catch (...) {
+        return -1;
+    }
+}
+
 // engine::ref::howell<uint32_t>(2, k, M) (RefLinAlg.cpp:231-290), the naive Sec. 13.5 pass; k <= 31.
 int64_t hxr_ref_howell(uint32_t k, uint32_t m, uint32_t n, const uint32_t* A, int64_t lda, uint32_t* H,
                        uint32_t* pivots, uint32_t* pivot_v) {

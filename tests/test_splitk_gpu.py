@@ -67,7 +67,8 @@ def test_split_k_strided_views(hx_gpu, O):
 
 def test_split_k_automatic_policy(hx_gpu, O):
     """A 256 x 256 x 16384 plain product: the policy splits it (16 tiles, 64 slices of 256), the bytes are
-    the oracle's and the unsplit kernel's; an 8192-row product is not split."""
+    the oracle's and the unsplit kernel's; an 8192-row product is not split; AUTO takes the split plain
+    kernel for this shape rather than the limb route."""
     from hacs_amd import engine as E
     p = 2147483629
     ctx = E.ModCtx(p)
@@ -89,6 +90,14 @@ def test_split_k_automatic_policy(hx_gpu, O):
     assert _stats().splitk_launches == before + 1
     assert np.array_equal(E.from_device(dC), want)
     assert E.suggest_splits(8192, 8192, 8192) == 1
+    # AUTO: the limb route's 8 tiles cannot fill the GPU, so the product runs on the split plain kernel
+    st0 = _stats()
+    dC.fill_(0)
+    E.gemm(ctx, dA, dB, dC, E.GemmOp.Set, variant=E.HX_GEMM_AUTO)
+    torch.cuda.synchronize()
+    st1 = _stats()
+    assert st1.splitk_launches == st0.splitk_launches + 1 and st1.limb_launches == st0.limb_launches
+    assert np.array_equal(E.from_device(dC), want)
 
 
 def test_split_k_workspace_too_small(hx_gpu, O):
