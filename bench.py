@@ -57,8 +57,10 @@ def parse():
     ap.add_argument("--no-level1", action="store_true", help="do not time the HBM-bound level-1/2 kernels")
     ap.add_argument("--no-f2", action="store_true", help="do not time the F_2 RREF and product")
     ap.add_argument("--no-splitk", action="store_true", help="do not time the split-K product")
+    ap.add_argument("--no-smith", action="store_true", help="do not time the Smith valuations over Z/2^32")
+    ap.add_argument("--no-smith-cpu", action="store_true", help="do not time the reference's Smith on the host")
     ap.add_argument("--only", choices=["batch", "lu", "mixed", "cpu", "howell", "ring2k", "level1", "u64", "f2",
-                                       "splitk"],
+                                       "splitk", "smith"],
                     help="run one secondary workload alone (for profiling) and print its record")
     ap.add_argument("--pmc-mode", action="store_true",
                     help="for counter collection: one 1 GiB device copy (the traffic calibration), then one limb "
@@ -330,6 +332,35 @@ def ring2k_lu(E, m=2048, panel=64, k=32):
                        "device_equals_port": bool(r_o == r and np.array_equal(piv_o, piv[:r]) and
                                                   np.array_equal(pv_o, pv[:r]) and np.array_equal(R_o, A)),
                        "speedup": t_o / best}
+    return out
+
+
+def smith_leg(E, n=4096, k=32, cpu=True):
+    """The Smith valuations over Z/2^32 of an n x n matrix of uniform words (the workload of the reference's
+    ring2k timing rung, Ring2kLadderValidation.cpp:1024-1033) on the device (hx_ring2k_smith: F_2 rank profile,
+    odd-pivot LU on the session, halved Schur complement per level; host buffer in, PCIe included), min of 2;
+    beside it the reference's own ring2k::smith (oracle/_ref, one host core) on the same matrix, whose sigma
+    is compared."""
+    rng = np.random.default_rng(0x7173 + n + k)
+    A = rng.integers(0, 1 << 32, (n, n), dtype=np.uint64).astype(np.uint32)
+    best, res = 1e30, None
+    for _ in range(2):
+        t0 = time.perf_counter()
+        res = E.ring2k_smith(A, k)
+        best = min(best, time.perf_counter() - t0)
+    out = {"n": n, "k": k, "seconds": best, "rank": int(len(res.sigma)), "levels": int(len(res.level_ranks)),
+           "sigma_tail": [int(x) for x in res.sigma[-8:]]}
+    if cpu:
+        from oracle import oracle as O
+        if O.ref_available():
+            t0 = time.perf_counter()
+            sig, lev = O.ref_smith(k, A)
+            t_r = time.perf_counter() - t0
+            out["cpu_reference"] = {"seconds": t_r, "cores": 1, "kind": "reference",
+                                    "routine": "ring2k::smith (the reference's sources, compiled in oracle/_ref)",
+                                    "device_equals_reference": bool(np.array_equal(sig, res.sigma) and
+                                                                    np.array_equal(lev, res.level_ranks)),
+                                    "speedup": t_r / best}
     return out
 
 
@@ -640,6 +671,7 @@ def main():
                "u64": lambda: gemm_u64_leg(E),
                "f2": lambda: f2_leg(E),
                "splitk": lambda: splitk_leg(E),
+               "smith": lambda: smith_leg(E, cpu=not args.no_smith_cpu),
                "level1": lambda: level1_kernels(E),
                "cpu": lambda: cpu_baseline(p)}[args.only]()
         print(json.dumps({"only": args.only, **leg}), flush=True)
@@ -718,6 +750,10 @@ def main():
         r2 = ring2k_lu(E)
         log(f"ring2k LU: {r2}")
         res["ring2k_lu"] = r2
+    if not args.no_smith:
+        sm = smith_leg(E, cpu=not args.no_smith_cpu)
+        log(f"ring2k smith: {sm}")
+        res["ring2k_smith"] = sm
     if not args.no_u64:
         g64 = gemm_u64_leg(E)
         log(f"u64 GEMM: {g64}")

@@ -221,6 +221,7 @@ SIGNATURES = {
     "hx_ring2k_store_budget": (_U64, []),
     "hx_ring2k_verify": (_INT, []),
     "hx_ring2k_echelon": (_INT, [_U32, _U32, _U32, _U32, _V, _SZ, _U32, _U32, _U64, _U64, _U32P, _U32P, _U32P]),
+    "hx_ring2k_smith": (_INT, [_U32, _U32, _U32, _U32, _V, _SZ, _U32, _U32P, _U32P, _U32P, _U32P]),
     "hx_ring2k_stats_get": (_INT, [C.POINTER(hx_ring2k_stats)]),
     "hx_ring2k_stats_reset": (_INT, []),
     "hx_ring2k_gemm_sub_host": (_INT, [_U32, _V, _SZ, _V, _SZ, _V, _SZ, _U32, _U32, _U32]),

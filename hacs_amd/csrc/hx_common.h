@@ -100,6 +100,16 @@ enum ScratchPurpose { SCRATCH_LIMB = 0, SCRATCH_MLP = 1, SCRATCH_RREF = 2, SCRAT
 void* stream_scratch(hipStream_t s, int purpose, size_t bytes);
 void stream_scratch_release(hipStream_t s);
 
+// A ring2k session's resident window (m x cw words, row stride cw) and its compute stream, for the
+// drivers that run their own kernels between steps (the Smith levels).  The stream has been made to
+// wait for any lookahead copy still reading the window.
+struct Ring2kWindow {
+    void* win;
+    uint32_t m, cw;
+    hipStream_t s;
+};
+Ring2kWindow ring2k_window(hx_ring2k sess);
+
 // ---- device: exact arithmetic mod p, p < 2^32 (ModCtx.h:202-333) ----
 struct DevMod {
     uint32_t p;

@@ -211,6 +211,13 @@ void copy_block(Session* S, void* host, size_t ldh, uint32_t r0, uint32_t rows, 
     char* d = (char*)S->win + ((size_t)r0 * S->cw + c0) * e;
     // the lookahead copy (second stream) may still read the window
     HX_HIP(hipStreamWaitEvent(S->s, S->ev_la_done, 0));
+    if (cols == S->cw && ldh == cols) {
+        // whole rows on both sides: one contiguous block through the runtime's pinned staging
+        const size_t bytes = (size_t)rows * cols * e;
+        const int rc = to_dev ? hx_upload(d, host, bytes, (hx_stream)S->s) : hx_download(host, d, bytes, (hx_stream)S->s);
+        if (rc != HX_OK) throw Error{rc, hx_last_error()};
+        return;
+    }
     if (to_dev)
         HX_HIP(hipMemcpy2DAsync(d, (size_t)S->cw * e, host, ldh * e, cols * e, rows, hipMemcpyHostToDevice, S->s));
     else
@@ -374,6 +381,14 @@ void fetch_impl(Session* S, void* host, size_t ldh) {
 }
 
 }  // namespace
+
+Ring2kWindow ring2k_window(hx_ring2k sess) {
+    HX_CHECK(sess != nullptr, "NULL session");
+    Session* S = (Session*)sess;
+    HX_HIP(hipStreamWaitEvent(S->s, S->ev_la_done, 0));
+    return Ring2kWindow{S->win, S->m, S->cw, S->s};
+}
+
 }  // namespace hx
 
 using namespace hx;

@@ -64,10 +64,60 @@ struct PanelRecord {
     std::vector<E> L;
 };
 
+// The host factor of one panel: P holds the panel's pw columns (first column c) on the rows r .. r+rows-1,
+// eliminated column by column with the MinValuation rule mod 2^k; the swaps (global row pairs) go to
+// `swaps`, the L slots to the rows' slots of L (m x panel, swapped with their rows), each pivot column and
+// valuation to pivots / pivot_v at npiv (advanced).  Returns the panel's pivot count.
+template <class E>
+uint32_t factor_panel(uint32_t k, std::vector<E>& P, uint32_t rows, uint32_t pw, std::vector<E>& L, uint32_t r,
+                      uint32_t panel, std::vector<uint32_t>& swaps, uint32_t c, uint32_t* pivots, uint32_t* pivot_v,
+                      uint32_t& npiv) {
+    using W = Wrap<E>;
+    uint32_t np = 0;
+    swaps.clear();
+    for (uint32_t j = 0; j < pw && np < rows; ++j) {
+        uint32_t best = rows, bv = k;
+        for (uint32_t i = np; i < rows; ++i) {
+            const uint32_t v = W::val(P[(size_t)i * pw + j], k);
+            if (v < bv) {
+                bv = v;
+                best = i;
+                if (v == 0) break;
+            }
+        }
+        if (best == rows) continue;
+        if (best != np) {
+            std::swap_ranges(P.begin() + (size_t)best * pw, P.begin() + (size_t)(best + 1) * pw,
+                             P.begin() + (size_t)np * pw);
+            std::swap_ranges(L.begin() + (size_t)(r + best) * panel, L.begin() + (size_t)(r + best + 1) * panel,
+                             L.begin() + (size_t)(r + np) * panel);
+            swaps.push_back(r + best);
+            swaps.push_back(r + np);
+        }
+        const E* prow = P.data() + (size_t)np * pw;
+        const E ui = W::inv_odd((E)(prow[j] >> bv));
+        for (uint32_t i = np + 1; i < rows; ++i) {
+            E* row = P.data() + (size_t)i * pw;
+            E& slot = L[(size_t)(r + i) * panel + np];
+            if (W::val(row[j], k) == k) {
+                slot = (E)0;
+                continue;
+            }
+            const E f = (E)((E)(row[j] >> bv) * ui);
+            slot = f;
+            for (uint32_t q = j; q < pw; ++q) row[q] = (E)(row[q] - (E)(f * prow[q]));
+        }
+        pivots[npiv] = c + j;
+        pivot_v[npiv] = bv;
+        ++npiv;
+        ++np;
+    }
+    return np;
+}
+
 template <class E>
 void echelon_run(uint32_t k, uint32_t m, uint32_t n, E* A, size_t lda, uint32_t panel, uint32_t climit, uint32_t cw,
                  uint64_t store_cap, uint32_t* rank_out, uint32_t* pivots, uint32_t* pivot_v) {
-    using W = Wrap<E>;
     const bool tiled = cw < n;
     using clk = std::chrono::steady_clock;
     auto tick = clk::now();
@@ -105,46 +155,7 @@ void echelon_run(uint32_t k, uint32_t m, uint32_t n, E* A, size_t lda, uint32_t 
                     ok(hx_ring2k_get(sess, P.data(), pw, r, rows, c - s0, pw));
                 }
                 lap(R2_NS_FETCH);
-                // the panel, eliminated column by column (rows of P are global rows r + i)
-                uint32_t np = 0;
-                swaps.clear();
-                for (uint32_t j = 0; j < pw && np < rows; ++j) {
-                    uint32_t best = rows, bv = k;
-                    for (uint32_t i = np; i < rows; ++i) {
-                        const uint32_t v = W::val(P[(size_t)i * pw + j], k);
-                        if (v < bv) {
-                            bv = v;
-                            best = i;
-                            if (v == 0) break;
-                        }
-                    }
-                    if (best == rows) continue;
-                    if (best != np) {
-                        std::swap_ranges(P.begin() + (size_t)best * pw, P.begin() + (size_t)(best + 1) * pw,
-                                         P.begin() + (size_t)np * pw);
-                        std::swap_ranges(L.begin() + (size_t)(r + best) * panel, L.begin() + (size_t)(r + best + 1) * panel,
-                                         L.begin() + (size_t)(r + np) * panel);
-                        swaps.push_back(r + best);
-                        swaps.push_back(r + np);
-                    }
-                    const E* prow = P.data() + (size_t)np * pw;
-                    const E ui = W::inv_odd((E)(prow[j] >> bv));
-                    for (uint32_t i = np + 1; i < rows; ++i) {
-                        E* row = P.data() + (size_t)i * pw;
-                        E& slot = L[(size_t)(r + i) * panel + np];
-                        if (W::val(row[j], k) == k) {
-                            slot = (E)0;
-                            continue;
-                        }
-                        const E f = (E)((E)(row[j] >> bv) * ui);
-                        slot = f;
-                        for (uint32_t q = j; q < pw; ++q) row[q] = (E)(row[q] - (E)(f * prow[q]));
-                    }
-                    pivots[npiv] = c + j;
-                    pivot_v[npiv] = bv;
-                    ++npiv;
-                    ++np;
-                }
+                const uint32_t np = factor_panel<E>(k, P, rows, pw, L, r, panel, swaps, c, pivots, pivot_v, npiv);
                 lap(R2_NS_PANEL);
                 const uint32_t rn = r + np;
                 const uint64_t lbytes = (uint64_t)rows * np * sizeof(E);
@@ -265,6 +276,232 @@ uint32_t plan_window(uint32_t elem, uint32_t m, uint32_t n, uint32_t panel, uint
     return cw >= panel ? (uint32_t)cw : 0;
 }
 
+// ---- the Smith valuations over Z/2^k by levels (ring2k::smith, Binary/Ring2k.h:701-841) ----------------
+// The matrix stays resident in one session window for every level; level s works on the block of rows and
+// columns [R, m) x [R, n) (R = the pivots of the levels before), read mod 2^(k - s):
+//   1. one wave per 64 columns of a row packs the block's low bits into a bit matrix (a ballot per wave)
+//      and flags whether any entry is non-zero mod 2^(k - s) (none: the valuations are complete);
+//   2. the F_2 RREF of the bit matrix (hx_f2_rref) gives the level's rank r0 and its pivot columns;
+//   3. those columns move to the front of the block (a gather through a row-chunk buffer);
+//   4. the session's LU over the first r0 columns of the block: the host-factored panels find an odd pivot
+//      in every one of them (MinValuation and FirstOdd choose the same row there: the first odd one), and
+//      the device updates the rest of the block;
+//   5. the Schur complement, rows and columns [R + r0, ..), is even: it is halved in place (a kernel that
+//      also flags an odd entry, which would be an internal error) and becomes the next level's block.
+// The pivots of level s contribute r0 valuations s.  The Smith form of A^T is that of A, so a matrix with
+// more rows than columns is transposed on the device first (fewer rows for the F_2 search and the panels).
+
+template <class E>
+__global__ __launch_bounds__(256) void smith_pack_kernel(const E* __restrict__ win, uint32_t ld, uint32_t R,
+                                                         uint32_t rows, uint32_t cols, uint32_t ldw, E kmask,
+                                                         uint64_t* __restrict__ bits, uint32_t* __restrict__ flags) {
+    const uint64_t wv = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t lane = threadIdx.x & 63;
+    if (wv >= (uint64_t)rows * ldw) return;               // the wave index is uniform: whole waves leave
+    const uint32_t i = (uint32_t)(wv / ldw), g = (uint32_t)(wv % ldw), j = g * 64 + lane;
+    const E x = j < cols ? win[(size_t)(R + i) * ld + R + j] : (E)0;
+    const uint64_t odd = __ballot((int)(x & (E)1));
+    const uint64_t nz = __ballot((int)((x & kmask) != 0));
+    if (lane == 0) {
+        bits[(size_t)i * ldw + g] = odd;
+        if (nz) flags[0] = 1u;
+    }
+}
+
+// tmp row i = window row R + i0 + i, columns R + order[j] (j < cols)
+template <class E>
+__global__ __launch_bounds__(256) void smith_gather_kernel(const E* __restrict__ win, uint32_t ld, uint32_t R, uint32_t i0,
+                                                           uint32_t cols, const uint32_t* __restrict__ order,
+                                                           E* __restrict__ tmp) {
+    const uint32_t j = blockIdx.x * 256 + threadIdx.x, i = blockIdx.y;
+    if (j >= cols) return;
+    tmp[(size_t)i * cols + j] = win[(size_t)(R + i0 + i) * ld + R + order[j]];
+}
+
+// window rows R + blockIdx.y, columns [R, R + cols): x -> x >> 1, flags[1] = 1 on an odd entry
+template <class E>
+__global__ __launch_bounds__(256) void smith_halve_kernel(E* __restrict__ win, uint32_t ld, uint32_t R, uint32_t cols,
+                                                          uint32_t* __restrict__ flags) {
+    const uint32_t j = blockIdx.x * 256 + threadIdx.x, i = blockIdx.y;
+    if (j >= cols) return;
+    E* x = win + (size_t)(R + i) * ld + R + j;
+    const E v = *x;
+    if (v & (E)1) flags[1] = 1u;
+    *x = v >> 1;
+}
+
+// dst (cols x rows, row stride ldd) = src^T, src rows x cols (row stride cols); 64 x 64 tiles through LDS
+template <class E>
+__global__ __launch_bounds__(256) void smith_transpose_kernel(const E* __restrict__ src, uint32_t rows, uint32_t cols,
+                                                              E* __restrict__ dst, uint32_t ldd) {
+    __shared__ E t[64][65];
+    const uint32_t r0 = blockIdx.x * 64, c0 = blockIdx.y * 64;     // x: row tiles (the long side)
+    const uint32_t tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (uint32_t y = ty; y < 64; y += 4) {
+        const uint32_t r = r0 + y, c = c0 + tx;
+        if (r < rows && c < cols) t[y][tx] = src[(size_t)r * cols + c];
+    }
+    __syncthreads();
+    for (uint32_t y = ty; y < 64; y += 4) {
+        const uint32_t c = c0 + y, r = r0 + tx;
+        if (r < rows && c < cols) dst[(size_t)c * ldd + r] = t[tx][y];
+    }
+}
+
+// A device buffer from the arena, returned once its stream has drained (also when an error unwinds).
+struct ArenaBuf {
+    void* p = nullptr;
+    hipStream_t s = nullptr;
+    ArenaBuf(size_t bytes, hipStream_t st) : p(arena_alloc(bytes)), s(st) {}
+    ~ArenaBuf() {
+        if (!p) return;
+        (void)hipStreamSynchronize(s);
+        arena_free(p);
+    }
+    ArenaBuf(const ArenaBuf&) = delete;
+    ArenaBuf& operator=(const ArenaBuf&) = delete;
+};
+
+// A: am x an host words (row stride lda).  The device block is m x n with m <= n (A or A^T).
+template <class E>
+void smith_run(uint32_t k, uint32_t am, uint32_t an, const E* A, size_t lda, uint32_t panel, uint32_t* sigma,
+               uint32_t* nsigma, uint32_t* level_ranks, uint32_t* nlevels) {
+    const bool tr = am > an;
+    const uint32_t m = tr ? an : am, n = tr ? am : an;
+    HX_CHECK(m <= 16384, "ring2k smith: min(m, n) > 16384 is not supported (the F_2 rank profile's row limit)");
+    HX_CHECK(plan_window(sizeof(E), m, n, panel, 0) == n, "ring2k smith: the matrix does not fit in device memory");
+    hx_ring2k sess = nullptr;
+    ok(hx_ring2k_open(sizeof(E), m, n, panel, &sess));
+    try {
+        const Ring2kWindow W = ring2k_window(sess);
+        const hipStream_t st = W.s;
+        E* win = (E*)W.win;
+        if (!tr) {
+            ok(hx_ring2k_put(sess, A, lda, 0, m, 0, n));
+        } else {
+            ArenaBuf src(sizeof(E) * (size_t)am * an, st);
+            if (lda == an) {
+                ok(hx_upload(src.p, A, sizeof(E) * (size_t)am * an, (hx_stream)st));
+            } else {
+                HX_HIP(hipMemcpy2DAsync(src.p, sizeof(E) * an, A, sizeof(E) * lda, sizeof(E) * an, am,
+                                        hipMemcpyHostToDevice, st));
+                stat_add(ST_H2D_BYTES, sizeof(E) * (uint64_t)am * an);
+            }
+            smith_transpose_kernel<E><<<dim3((am + 63) / 64, (an + 63) / 64), 256, 0, st>>>((const E*)src.p, am, an, win, n);
+            after_launch("smith_transpose_kernel");
+            HX_HIP(hipStreamSynchronize(st));
+        }
+        const uint32_t ldw0 = (n + 63) / 64;
+        ArenaBuf bits(sizeof(uint64_t) * (size_t)m * ldw0, st), flags(256, st), d_order(sizeof(uint32_t) * n, st);
+        const size_t chunk = std::max<size_t>(1, std::min<size_t>(m, ((size_t)64 << 20) / (sizeof(E) * n)));
+        ArenaBuf tmp(sizeof(E) * chunk * n, st);
+        uint32_t* fl = (uint32_t*)flags.p;
+        std::vector<E> L((size_t)m * panel, (E)0), P;
+        std::vector<uint32_t> swaps, piv(m), order(n), lpiv(panel), lpv(panel);
+        uint32_t R = 0, ns = 0, nl = 0;
+        for (uint32_t s = 0; s < k && R < m && R < n; ++s) {
+            const uint32_t kk = k - s, rows = m - R, cols = n - R, ldw = (cols + 63) / 64;
+            const E kmask = kk >= 8 * sizeof(E) ? (E)~(E)0 : (E)(((E)1 << kk) - 1);
+            // 1. the low bits and the zero test
+            HX_HIP(hipMemsetAsync(fl, 0, 2 * sizeof(uint32_t), st));
+            const uint64_t waves = (uint64_t)rows * ldw;
+            smith_pack_kernel<E><<<(uint32_t)((waves + 3) / 4), 256, 0, st>>>(win, n, R, rows, cols, ldw, kmask,
+                                                                              (uint64_t*)bits.p, fl);
+            after_launch("smith_pack_kernel");
+            uint32_t hf[2] = {0, 0};
+            HX_HIP(hipMemcpyAsync(hf, fl, sizeof(hf), hipMemcpyDeviceToHost, st));
+            HX_HIP(hipStreamSynchronize(st));
+            if (!hf[0]) break;
+            // 2. the F_2 rank profile
+            uint32_t r0 = 0;
+            ok(hx_f2_rref(rows, cols, (uint64_t*)bits.p, ldw, &r0, piv.data(), (hx_stream)st));
+            level_ranks[s] = r0;
+            nl = s + 1;
+            for (uint32_t t = 0; t < r0; ++t) sigma[ns++] = s;
+            if (r0 > 0) {
+                // 3. the pivot columns to the front (pivots ascend: none moves when the last is r0 - 1)
+                if (piv[r0 - 1] != r0 - 1) {
+                    for (uint32_t j = 0, t = 0, q = r0; j < cols; ++j) {
+                        if (t < r0 && piv[t] == j)
+                            order[t++] = j;
+                        else
+                            order[q++] = j;
+                    }
+                    HX_HIP(hipMemcpyAsync(d_order.p, order.data(), sizeof(uint32_t) * cols, hipMemcpyHostToDevice, st));
+                    for (uint32_t i0 = 0; i0 < rows; i0 += (uint32_t)chunk) {
+                        const uint32_t nr = std::min<uint32_t>((uint32_t)chunk, rows - i0);
+                        smith_gather_kernel<E><<<dim3((cols + 255) / 256, nr), 256, 0, st>>>(
+                            win, n, R, i0, cols, (const uint32_t*)d_order.p, (E*)tmp.p);
+                        after_launch("smith_gather_kernel");
+                        HX_HIP(hipMemcpy2DAsync(win + (size_t)(R + i0) * n + R, sizeof(E) * n, tmp.p, sizeof(E) * cols,
+                                                sizeof(E) * cols, nr, hipMemcpyDeviceToDevice, st));
+                    }
+                }
+                // 4. the odd-pivot LU over the block's first r0 columns
+                const uint32_t climit = R + r0;
+                uint32_t r = R, c = R;
+                bool have_next = false;
+                while (c < climit) {
+                    const uint32_t c1 = std::min(c + panel, climit), pw = c1 - c, prow = m - r;
+                    P.assign((size_t)prow * pw, (E)0);
+                    if (have_next) {
+                        ok(hx_ring2k_fetch(sess, P.data(), pw));
+                        note(R2_LOOKAHEADS);
+                    } else {
+                        ok(hx_ring2k_get(sess, P.data(), pw, r, prow, c, pw));
+                    }
+                    uint32_t cnt = 0;
+                    const uint32_t np = factor_panel<E>(kk, P, prow, pw, L, r, panel, swaps, c, lpiv.data(), lpv.data(), cnt);
+                    bool agree = np == pw;
+                    for (uint32_t t = 0; agree && t < np; ++t) agree = lpiv[t] == c + t && lpv[t] == 0;
+                    if (!agree) throw Error{HX_E_RUNTIME, "ring2k smith: the odd-pivot LU disagrees with the F_2 rank profile"};
+                    const uint32_t nc1 = std::min(c1 + panel, climit);
+                    const bool next = c1 < climit;
+                    hx_ring2k_step_args a{};
+                    a.m = m;
+                    a.r = r;
+                    a.np = np;
+                    a.pc0 = c;
+                    a.pc1 = c1;
+                    a.panel = P.data();
+                    a.ldp = pw;
+                    a.swaps = swaps.data();
+                    a.nswaps = (uint32_t)(swaps.size() / 2);
+                    a.L = L.data() + (size_t)r * panel;
+                    a.ldl = panel;
+                    a.u0 = c1;
+                    a.la0 = next ? c1 : 0;
+                    a.la1 = next ? nc1 : 0;
+                    a.w = n;
+                    ok(hx_ring2k_step(sess, &a));
+                    note(R2_PANELS);
+                    have_next = next;
+                    r += np;
+                    c = c1;
+                }
+            }
+            // 5. the even Schur complement, halved in place
+            const uint32_t Rn = R + r0;
+            if (Rn < m && Rn < n) {
+                (void)ring2k_window(sess);              // after the last step's lookahead copy, if any
+                smith_halve_kernel<E><<<dim3((n - Rn + 255) / 256, m - Rn), 256, 0, st>>>(win, n, Rn, n - Rn, fl);
+                after_launch("smith_halve_kernel");
+                HX_HIP(hipMemcpyAsync(hf, fl, sizeof(hf), hipMemcpyDeviceToHost, st));
+                HX_HIP(hipStreamSynchronize(st));
+                if (hf[1]) throw Error{HX_E_RUNTIME, "ring2k smith: the Schur complement is not even"};
+            }
+            R = Rn;
+        }
+        HX_HIP(hipStreamSynchronize(st));
+        *nsigma = ns;
+        *nlevels = nl;
+    } catch (...) {
+        hx_ring2k_close(sess);
+        throw;
+    }
+    hx_ring2k_close(sess);
+}
+
 }  // namespace
 }  // namespace hx
 
@@ -310,6 +547,25 @@ HX_API int hx_ring2k_echelon(uint32_t elem, uint32_t k, uint32_t m, uint32_t n, 
             echelon_run<uint32_t>(k, m, n, (uint32_t*)A, lda, panel, climit, cw, store_bytes, rank, pivots, pivot_v);
         else
             echelon_run<uint64_t>(k, m, n, (uint64_t*)A, lda, panel, climit, cw, store_bytes, rank, pivots, pivot_v);
+    });
+}
+
+HX_API int hx_ring2k_smith(uint32_t elem, uint32_t k, uint32_t m, uint32_t n, const void* A, size_t lda,
+                           uint32_t panel, uint32_t* sigma, uint32_t* nsigma, uint32_t* level_ranks, uint32_t* nlevels) {
+    return guard("hx_ring2k_smith", [&] {
+        HX_CHECK(elem == 4 || elem == 8, "elem must be 4 or 8");
+        HX_CHECK(k >= 1 && k <= 8 * elem, "need 1 <= k <= the word's bits");
+        HX_CHECK(nsigma != nullptr && nlevels != nullptr, "NULL argument");
+        *nsigma = 0;
+        *nlevels = 0;
+        if (m == 0 || n == 0) return;
+        HX_CHECK(A != nullptr && lda >= n && sigma != nullptr && level_ranks != nullptr, "NULL argument or lda < n");
+        if (panel == 0) panel = 64;
+        HX_CHECK(panel <= 64, "panel must be 0 (64) or 1 .. 64");
+        if (elem == 4)
+            smith_run<uint32_t>(k, m, n, (const uint32_t*)A, lda, panel, sigma, nsigma, level_ranks, nlevels);
+        else
+            smith_run<uint64_t>(k, m, n, (const uint64_t*)A, lda, panel, sigma, nsigma, level_ranks, nlevels);
     });
 }
 

@@ -125,6 +125,33 @@ def f2_gemm(A_words: torch.Tensor, B_words: torch.Tensor, C_words: torch.Tensor,
                            C_words.stride(0), 1 if accumulate else 0, current_stream() if stream is None else stream))
 
 
+class SmithResult:
+    """ring2k::SmithResult (Binary/Ring2k.h:408-413): sigma ascending (the non-zero Smith diagonal entries
+    2^sigma_i over Z/2^k), level_ranks[s] = #{sigma == s} for each level run."""
+
+    def __init__(self, k: int, sigma: np.ndarray, level_ranks: np.ndarray):
+        self.k, self.sigma, self.level_ranks = k, sigma, level_ranks
+
+    def length(self) -> int:
+        return int(sum(self.k - int(x) for x in self.sigma))
+
+
+def ring2k_smith(A: np.ndarray, k: int, panel: int = 0) -> SmithResult:
+    """The Smith valuations of the host matrix A (uint32 or uint64 words, entries mod 2^k) on the device --
+    ring2k::smith (Binary/Ring2k.h:701-841).  A is not modified."""
+    A = np.ascontiguousarray(A)
+    if A.dtype not in (np.uint32, np.uint64):
+        raise TypeError("ring2k_smith: A must be uint32 or uint64")
+    m, n = A.shape
+    sigma = np.zeros(max(1, min(m, n)), np.uint32)
+    lev = np.zeros(max(1, k), np.uint32)
+    ns, nl = C.c_uint32(0), C.c_uint32(0)
+    check(lib().hx_ring2k_smith(A.dtype.itemsize, k, m, n, A.ctypes.data_as(C.c_void_p), n, panel,
+                                sigma.ctypes.data_as(_lib._U32P), C.byref(ns), lev.ctypes.data_as(_lib._U32P),
+                                C.byref(nl)))
+    return SmithResult(k, sigma[:ns.value].copy(), lev[:nl.value].copy())
+
+
 def current_stream() -> int:
     return torch.cuda.current_stream().cuda_stream
 

@@ -460,6 +460,16 @@ HX_API int hx_ring2k_verify(void);
 HX_API int hx_ring2k_echelon(uint32_t elem, uint32_t k, uint32_t m, uint32_t n, void *A, size_t lda, uint32_t panel,
                              uint32_t col_limit, uint64_t budget_bytes, uint64_t store_bytes, uint32_t *rank,
                              uint32_t *pivots, uint32_t *pivot_v);
+/* ring2k::smith (Binary/Ring2k.h:701-841; SmithResult, Ring2k.h:408-413): the Smith valuations of A (m x n host
+ * words of elem bytes, row stride lda, entries understood mod 2^k; A is not modified) by levels, the matrix
+ * resident on the device: per level the F_2 rank profile of the low bits (hx_f2_rref), the odd-pivot LU of the
+ * session above on those columns, the even Schur complement halved.  sigma (min(m, n) entries) receives the
+ * valuations in ascending order, *nsigma their count (the non-zero Smith diagonal entries 2^sigma_i);
+ * level_ranks (k entries) receives #{sigma == s} for each level run, *nlevels their count.  panel: 0 (64) or
+ * 1 .. 64.  min(m, n) <= 16384 (a taller matrix is transposed on the device), and the matrix must fit in
+ * device memory (HX_E_ARGUMENT otherwise). */
+HX_API int hx_ring2k_smith(uint32_t elem, uint32_t k, uint32_t m, uint32_t n, const void *A, size_t lda, uint32_t panel,
+                           uint32_t *sigma, uint32_t *nsigma, uint32_t *level_ranks, uint32_t *nlevels);
 typedef struct hx_ring2k_stats {   /* ring2k::device::Stats, Binary/Ring2kDevice.h:86-101 */
     uint64_t calls, device, resident, tiled, slabs, replays, fallbacks, panels, lookaheads, verified;
     uint64_t ns_open, ns_fetch, ns_panel, ns_step, ns_host, ns_close;   /* wall ns on the calling thread */

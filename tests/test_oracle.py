@@ -301,3 +301,82 @@ def test_gemm_u64_exact_and_reference(O):
         A, B, C0 = G[f"case{i}_A"], G[f"case{i}_B"], G[f"case{i}_C0"]
         assert np.array_equal(O.gemm_u64(p, A, B), G[f"case{i}_set"]), i
         assert np.array_equal(O.gemm_u64(p, A, B, C0, 2), G[f"case{i}_sub"]), i
+
+
+def _smith_levels(sigma, nlev):
+    return np.bincount(np.asarray(sigma, np.int64), minlength=nlev)[:nlev] if nlev else np.zeros(0, np.int64)
+
+
+def test_smith_oracle_vs_reference_fixture(O):
+    """The oracle's Smith valuations (a restatement of smith_naive) equal the reference's own ring2k::smith
+    by levels on the committed fixture (sigma; level_ranks = the count of each valuation over the levels run)."""
+    import os
+    G = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "smith_reference.npz"),
+                allow_pickle=False)
+    for i in range(int(G["count"][0])):
+        k = int(G[f"case{i}_k"][0])
+        sig, lev = G[f"case{i}_sigma"], G[f"case{i}_levels"]
+        assert np.array_equal(O.smith(k, G[f"case{i}_A"]), sig), f"case {i}"
+        assert np.array_equal(_smith_levels(sig, len(lev)), lev), f"case {i}"
+
+
+def test_smith_oracle_closed_forms(O):
+    """What the mathematics fixes: a diagonal 2^v u (u odd) has the sorted v < k as valuations, also under
+    permutations and unit-triangular row / column mixing (unimodular); the zero matrix has none; at k = 1 the
+    count is the F_2 rank; multiplying A by 2^t adds t to every valuation and drops those reaching k."""
+    rng = np.random.default_rng(0x5311)
+    for (m, n, k, dt) in ((30, 40, 32, np.uint32), (45, 25, 61, np.uint64), (20, 20, 7, np.uint32)):
+        r = min(m, n) - 3
+        vals = np.sort(rng.integers(0, k, r))
+        D = np.zeros((m, n), dt)
+        for i, v in enumerate(vals):
+            D[i, i] = dt(((int(rng.integers(0, 1 << 30)) | 1) << int(v)) & int(np.iinfo(dt).max))
+        assert np.array_equal(O.smith(k, D), vals)
+        A = D[rng.permutation(m)][:, rng.permutation(n)]
+        with np.errstate(over="ignore"):
+            for d in (1, 2, 4, 7):
+                A[d:] += rng.integers(0, 1 << 31, (m - d, 1)).astype(dt) * A[:-d]
+                A[:, d:] += A[:, :-d] * rng.integers(0, 1 << 31, (1, n - d)).astype(dt)
+        assert np.array_equal(O.smith(k, A), vals)
+        t = 3
+        with np.errstate(over="ignore"):
+            A2 = (A << dt(t)).astype(dt)
+        assert np.array_equal(O.smith(k, A2), (vals + t)[vals + t < k])
+    assert len(O.smith(40, np.zeros((6, 9), np.uint64))) == 0
+    B = rng.integers(0, 2, (37, 53)).astype(np.uint32)
+    B[5] = B[3] ^ B[4]
+    assert len(O.smith(1, B)) == O.rref_rule(2, B)[0]
+
+
+def test_smith_oracle_brute_force_tiny(O):
+    """On tiny matrices over Z/4 and Z/8 the valuations follow from the column span itself: im A = (+) Z/2^(k - v_i),
+    so the number of its elements killed by 2^t is 2^(sum_i min(t, k - v_i)); counted by enumerating every x."""
+    rng = np.random.default_rng(0x5312)
+    for k in (2, 3):
+        q = 1 << k
+        for (m, n) in ((2, 2), (2, 3), (3, 2), (3, 3)):
+            for _ in range(12):
+                A = rng.integers(0, q, (m, n)).astype(np.int64)
+                if rng.random() < 0.3:
+                    A[rng.integers(0, m)] *= 2
+                xs = np.array(list(itertools.product(range(q), repeat=n)), np.int64)
+                img = np.unique((xs @ A.T) % q, axis=0)
+                c = [0]
+                for t in range(1, k + 1):
+                    killed = int(((img * (1 << t)) % q == 0).all(axis=1).sum())
+                    c.append(int(round(np.log2(killed))))
+                # #{i : k - v_i >= t} = c_t - c_{t-1}
+                want = []
+                for t in range(k, 0, -1):
+                    cnt = (c[t] - c[t - 1]) - (sum(1 for v in want if k - v >= t + 1) if t < k else 0)
+                    want += [k - t] * cnt
+                want.sort()
+                assert np.array_equal(O.smith(k, A.astype(np.uint32)), np.array(want, np.uint32)), (k, A.tolist())
+
+
+def test_smith_oracle_negative_control(O):
+    """A planted mistake (one entry of a unit diagonal made even) must change the valuations."""
+    D = np.eye(6, dtype=np.uint32) * 5
+    assert np.array_equal(O.smith(8, D), np.zeros(6, np.uint32))
+    D[2, 2] = 6
+    assert not np.array_equal(O.smith(8, D), np.zeros(6, np.uint32))
