@@ -58,7 +58,7 @@ def parse():
     ap.add_argument("--no-f2", action="store_true", help="do not time the F_2 RREF and product")
     ap.add_argument("--no-splitk", action="store_true", help="do not time the split-K product")
     ap.add_argument("--no-smith", action="store_true", help="do not time the Smith valuations over Z/2^32")
-    ap.add_argument("--no-smith-cpu", action="store_true", help="do not time the reference's Smith on the host")
+    ap.add_argument("--no-smith-cpu", action="store_true", help="do not time the oracle's Smith sample on the host")
     ap.add_argument("--only", choices=["batch", "lu", "mixed", "cpu", "howell", "ring2k", "level1", "u64", "f2",
                                        "splitk", "smith"],
                     help="run one secondary workload alone (for profiling) and print its record")
@@ -335,32 +335,37 @@ def ring2k_lu(E, m=2048, panel=64, k=32):
     return out
 
 
-def smith_leg(E, n=4096, k=32, cpu=True):
+def smith_leg(E, n=4096, k=32, cpu=True, n_cpu=2048):
     """The Smith valuations over Z/2^32 of an n x n matrix of uniform words (the workload of the reference's
     ring2k timing rung, Ring2kLadderValidation.cpp:1024-1033) on the device (hx_ring2k_smith: F_2 rank profile,
-    odd-pivot LU on the session, halved Schur complement per level; host buffer in, PCIe included), min of 2;
-    beside it the reference's own ring2k::smith (oracle/_ref, one host core) on the same matrix, whose sigma
-    is compared."""
-    rng = np.random.default_rng(0x7173 + n + k)
-    A = rng.integers(0, 1 << 32, (n, n), dtype=np.uint64).astype(np.uint32)
-    best, res = 1e30, None
-    for _ in range(2):
-        t0 = time.perf_counter()
-        res = E.ring2k_smith(A, k)
-        best = min(best, time.perf_counter() - t0)
+    odd-pivot LU on the session, halved Schur complement per level; host buffer in, PCIe included), min of 2.
+    Beside it a bounded sample: the same on an n_cpu x n_cpu matrix, on the device and by the oracle's
+    restatement of smith_naive on one host core, whose sigma is compared."""
+    def matrix(size):
+        rng = np.random.default_rng(0x7173 + size + k)
+        return rng.integers(0, 1 << 32, (size, size), dtype=np.uint64).astype(np.uint32)
+
+    def timed(A):
+        best, res = 1e30, None
+        for _ in range(2):
+            t0 = time.perf_counter()
+            res = E.ring2k_smith(A, k)
+            best = min(best, time.perf_counter() - t0)
+        return best, res
+
+    best, res = timed(matrix(n))
     out = {"n": n, "k": k, "seconds": best, "rank": int(len(res.sigma)), "levels": int(len(res.level_ranks)),
            "sigma_tail": [int(x) for x in res.sigma[-8:]]}
     if cpu:
         from oracle import oracle as O
-        if O.ref_available():
-            t0 = time.perf_counter()
-            sig, lev = O.ref_smith(k, A)
-            t_r = time.perf_counter() - t0
-            out["cpu_reference"] = {"seconds": t_r, "cores": 1, "kind": "reference",
-                                    "routine": "ring2k::smith (the reference's sources, compiled in oracle/_ref)",
-                                    "device_equals_reference": bool(np.array_equal(sig, res.sigma) and
-                                                                    np.array_equal(lev, res.level_ranks)),
-                                    "speedup": t_r / best}
+        A = matrix(n_cpu)
+        t_dev, r_dev = timed(A)
+        t0 = time.perf_counter()
+        sig = O.smith(k, A)
+        t_o = time.perf_counter() - t0
+        out["cpu_port"] = {"n": n_cpu, "seconds": t_o, "device_seconds": t_dev, "cores": 1, "kind": "port",
+                           "routine": "hxo_smith_naive (global minimal-valuation pivoting, one core)",
+                           "device_equals_port": bool(np.array_equal(sig, r_dev.sigma)), "speedup": t_o / t_dev}
     return out
 
 
