@@ -29,7 +29,8 @@ sys.path.insert(0, ROOT)
 N_C3 = 8192
 PEAK_INT8_TOPS = 5000.0      # MI355X dense INT8 MFMA, tera-ops/s
 LIMB_DEFAULT = 4             # the limb kernel hx_gemm_u32 runs when opts.tile is 0
-LIMB_KERNELS = {1: "limb_kernel<%d>", 2: "limb2_kernel<%d>", 3: "limb3_kernel<%d>", 4: "limb4_kernel<%d>"}
+LIMB_KERNELS = {1: "limb_kernel<%d>", 2: "limb2_kernel<%d>", 3: "limb3_kernel<%d>", 4: "limb4_kernel<%d>",
+                5: "limb5_kernel<%d>"}
 PEAK_HBM_GBS = 8000.0
 
 
@@ -42,7 +43,7 @@ def parse():
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--tile", type=int, default=0,
                     help="limb kernel: 0 default, 1 register-staged LDS copy, 2 direct-to-LDS ring, "
-                         "3 ring with pipelined LDS reads, 4 the same on eight waves")
+                         "3 ring with pipelined LDS reads, 4 the same on eight waves, 5 96 x 128 tiles on twelve waves")
     ap.add_argument("--no-cpu", action="store_true", help="do not time the CPU baseline")
     ap.add_argument("--no-batch", action="store_true", help="do not time the batched echelon workload")
     ap.add_argument("--batch-jobs", type=int, default=4096)
@@ -65,6 +66,8 @@ def parse():
     ap.add_argument("--pmc-mode", action="store_true",
                     help="for counter collection: one 1 GiB device copy (the traffic calibration), then one limb "
                          "GEMM of the headline shape; no timing, no JSON")
+    ap.add_argument("--pmc-reps", type=int, default=1, help="--pmc-mode: the number of limb GEMMs (later ones run "
+                                                            "at the clock of a warm GPU)")
     return ap.parse_args()
 
 
@@ -664,7 +667,8 @@ def main():
         A = E.random_entries(n, n, p, W.seed_for(n, n, p, 0x01))
         B = E.random_entries(n, n, p, W.seed_for(n, n, p, 0x02))
         Cm = torch.empty((n, n), dtype=torch.int32, device="cuda")
-        E.gemm(ctx, A, B, Cm, E.GemmOp.Set, variant=E.HX_GEMM_LIMB, tile=args.tile)
+        for _ in range(args.pmc_reps):
+            E.gemm(ctx, A, B, Cm, E.GemmOp.Set, variant=E.HX_GEMM_LIMB, tile=args.tile)
         torch.cuda.synchronize()
         log("pmc mode done")
         return

@@ -828,6 +828,108 @@ __device__ __forceinline__ void limb_tile4(const LimbArgs& a, uint32_t tm, uint3
     limb_store<L, 1>(a, acc, m, m0 + wm * 32, n0 + wn * 32, lane);
 }
 
+// A 96 x 128 tile on twelve waves (three per SIMD), each wave one 32 x 32 block (wm = wave % 3: rows,
+// wn = wave / 3: columns).  Per MAC the tile streams (96 + 128) / (96 * 128) bytes of planes from L2
+// against (64 + 128) / (64 * 128) for the 64 x 128 tile: 22 % less, and each XCD's 32 concurrent tiles
+// cover 1.5 times the outputs.  The accumulators (2L-1 x 16 registers) plus one k-step's fragments fit
+// the third of a SIMD's registers a wave gets, so the fragments are not double-buffered: the other two
+// waves of the SIMD cover a wave's LDS reads.  Two stages of 64 k-bytes (112 KB at L = 4): the copy of
+// stage kt + 1 runs while stage kt is consumed.  The 14L copy blocks of a stage are dealt round-robin over
+// the twelve waves; A rows past the plane's last row (the last row tile of an m that is not a multiple
+// of 96) are read from the plane's last row and never stored.
+constexpr int LM5 = 96;
+
+template <int L>
+__device__ __forceinline__ void stage_issue12(const LimbArgs& a, const int8_t* Abase, const int8_t* Bbase, int wave,
+                                              int br, uint32_t m0, uint32_t mp, uint32_t kt, uint8_t* st) {
+    constexpr int SA = L * LM5 * 64, NBLK = 14 * L, PER = (NBLK + 11) / 12;
+    const size_t k0 = (size_t)kt * LK;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int blk = wave + 12 * i;
+        if (blk >= NBLK) break;
+        const int8_t* src;
+        uint8_t* dst;
+        if (blk < 6 * L) {
+            const int s = blk / 6, g = blk % 6;
+            const uint32_t row = min(m0 + (uint32_t)(16 * g + br), mp - 1) - m0 - (uint32_t)br;
+            src = Abase + s * a.a_plane + (size_t)row * a.Kp + k0;
+            dst = st + s * (LM5 * 64) + g * 1024;
+        } else {
+            const int s = (blk - 6 * L) >> 3, g = (blk - 6 * L) & 7;
+            src = Bbase + s * a.b_plane + (size_t)(16 * g) * a.Kp + k0;
+            dst = st + SA + s * (LN * 64) + g * 1024;
+        }
+        __builtin_amdgcn_global_load_lds((const void*)src, (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
+    }
+}
+
+template <int L>
+__device__ __forceinline__ void stage_frag12(const uint8_t* st, int ks, int wm, int wn, int lane, LFrag1<L>& f) {
+    constexpr int SA = L * LM5 * 64;
+    const int lr = lane & 31, q = ks * 2 + (lane >> 5);
+    const int ra = wm * 32 + lr, rb = wn * 32 + lr;
+#pragma unroll
+    for (int s = 0; s < L; ++s) f.a[s] = *(const v4i*)(st + s * (LM5 * 64) + ra * 64 + 16 * (q ^ swz(ra)));
+#pragma unroll
+    for (int s = 0; s < L; ++s) f.b[s] = *(const v4i*)(st + SA + s * (LN * 64) + rb * 64 + 16 * (q ^ swz(rb)));
+}
+
+template <int L>
+__device__ __forceinline__ void limb_tile5(const LimbArgs& a, uint32_t tm, uint32_t tn) {
+    constexpr int SA = L * LM5 * 64, SB = L * LN * 64, STAGE = SA + SB;
+    constexpr int ND = 2 * L - 1;
+    __shared__ __attribute__((aligned(1024))) uint8_t s0[STAGE];
+    __shared__ __attribute__((aligned(1024))) uint8_t s1[STAGE];
+    const uint32_t m = a.m_dev ? min(*a.m_dev, a.m) : a.m;
+    const uint32_t m0 = tm * LM5, n0 = tn * LN;
+    if (m0 >= m) return;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm = wave % 3, wn = wave / 3;
+
+    v16i acc[1][ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[0][d][r] = 0;
+
+    const int br = lane >> 2, bq = (lane & 3) ^ swz(br);
+    const int8_t* Abase = a.A8 + (size_t)(m0 + br) * a.Kp + 16 * bq;
+    const int8_t* Bbase = a.B8 + (size_t)(n0 + br) * a.Kp + 16 * bq;
+    const uint32_t KT = a.Kp / LK;
+    const uint32_t mp = (uint32_t)(a.a_plane / a.Kp);    // rows per A plane
+    auto body = [&](uint32_t kt, const uint8_t* cur, uint8_t* nxt) {
+        __builtin_amdgcn_s_waitcnt(wait_imm(0, 15));      // this wave's copies of stage kt have landed
+        wg_barrier();                                     // everyone's; and stage kt - 1 is read out
+        if (kt + 1 < KT) stage_issue12<L>(a, Abase, Bbase, wave, br, m0, mp, kt + 1, nxt);
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+            LFrag1<L> f;
+            stage_frag12<L>(cur, ks, wm, wn, lane, f);
+            __builtin_amdgcn_s_waitcnt(wait_imm(63, 0));
+            __builtin_amdgcn_sched_barrier(0);
+            frag_mfma1<L>(f, acc);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    };
+    if (KT > 0) {
+        stage_issue12<L>(a, Abase, Bbase, wave, br, m0, mp, 0, s0);
+        for (uint32_t kt = 0; kt < KT; kt += 2) {
+            body(kt, s0, s1);
+            if (kt + 1 < KT) body(kt + 1, s1, s0);
+        }
+    }
+    limb_store<L, 1>(a, acc, m, m0 + wm * 32, n0 + wn * 32, lane);
+}
+
+template <int L>
+__global__ __launch_bounds__(768, 1) void limb5_kernel(LimbArgs a) {
+    uint32_t tm, tn;
+    tile_of(blockIdx.x, a.ntm, a.ntn, tm, tn);
+    limb_tile5<L>(a, tm, tn);
+}
+
 template <int L>
 __global__ __launch_bounds__(512, 1) void limb4_kernel(LimbArgs a) {
     uint32_t tm, tn;
@@ -923,9 +1025,9 @@ void limb_attrs() {
 // Limb kernel choice (hx_gemm_opts.tile for the limb variant): 1 = register-staged copy
 // (limb_kernel), 2 = direct-to-LDS copy ring (limb2_kernel), 3 = the ring with the LDS reads
 // pipelined against the MFMAs (limb3_kernel), 4 = limb3's pipeline on eight waves
-// (limb4_kernel); 0 = the default.
+// (limb4_kernel), 5 = the 96 x 128 tile on twelve waves (limb5_kernel); 0 = the default.
 constexpr int kLimbDefault = 4;
-inline int limb_version(int tile) { return tile >= 1 && tile <= 4 ? tile : kLimbDefault; }
+inline int limb_version(int tile) { return tile >= 1 && tile <= 5 ? tile : kLimbDefault; }
 
 template <int L>
 void launch_limb(const LimbArgs& la, hipStream_t s, int ver) {
@@ -943,6 +1045,13 @@ void launch_limb(const LimbArgs& la, hipStream_t s, int ver) {
     if (ver == 4) {
         limb4_kernel<L><<<blocks, 512, 0, s>>>(la);
         after_launch("limb4_kernel");
+        return;
+    }
+    if (ver == 5) {
+        LimbArgs l5 = la;
+        l5.ntm = (la.m + LM5 - 1) / LM5;
+        limb5_kernel<L><<<l5.ntm * l5.ntn, 768, 0, s>>>(l5);
+        after_launch("limb5_kernel");
         return;
     }
     limb_attrs<L>();

@@ -81,7 +81,8 @@ typedef struct hx_gemm_opts {   /* GemmOpts, CudaKernels.h:64-79 */
     int tile;               /* 0 = automatic; a variant-specific choice (tuning only, same bytes): for LIMB,
                                1 = register-staged LDS copy, 2 = direct-to-LDS copy ring,
                                3 = the ring with LDS reads pipelined against the MFMAs, 4 = the ring on eight
-                               waves, one 32 x 32 block each (the default) */
+                               waves, one 32 x 32 block each (the default), 5 = 96 x 128 workgroup tiles on
+                               twelve waves */
     void *workspace;        /* the limb route's planes (the reference's imma_workspace, CudaKernels.h:72-75):
                                hx_gemm_workspace_bytes() runs K in one piece, less (at least the bytes of one
                                64-deep K chunk, hx_gemm_limb_k_chunk) runs K in chunks, same bytes; smaller: AUTO
@@ -276,7 +277,8 @@ typedef struct hx_gemm_job {
     uint32_t m, n, k;
     int32_t op;                 /* HX_OP_* */
     int32_t variant;            /* HX_GEMM_*: AUTO = the batch's variant */
-    int32_t tile;               /* the limb kernel of this job (hx_gemm_opts.tile: 0 = default, 1 .. 4); tuning only */
+    int32_t tile;               /* the limb kernel of this job (hx_gemm_opts.tile: 0 = default, 1 .. 4; 5 runs
+                                   as 4 in a batch); tuning only */
     const uint32_t *A;          /* row-major, leading dimension lda (device) */
     const uint32_t *B;
     uint32_t *C;

@@ -148,19 +148,21 @@ def test_gemm_batch_uneven(hx_gpu, O, variant):
             assert np.array_equal(E.from_device(jb[2]), want), f"rep {rep} {A.shape} x {B.shape} op {op}"
 
 
-@pytest.mark.parametrize("tile", [1, 2, 3, 4])
+@pytest.mark.parametrize("tile", [1, 2, 3, 4, 5])
 @pytest.mark.parametrize("p", [251, 65521, 16777213, 2147483629])
 def test_limb_kernels_all_copy_paths(hx_gpu, O, p, tile):
     """Every limb kernel (tile 1: register-staged LDS copy; tile 2: direct-to-LDS three-stage
     ring; tile 3: the ring with the LDS reads pipelined against the MFMAs; tile 4: tile 3's
-    pipeline on eight waves, one 32 x 32 block each) for every limb count,
+    pipeline on eight waves, one 32 x 32 block each; tile 5: 96 x 128 tiles on twelve waves, whose
+    last row tile reads past the plane's rows when m is not a multiple of 96) for every limb count,
     with k covering 1..5 LDS stages (the rings' unroll by three), ragged m / n, every op, and a k
     above one limb chunk (two accumulation passes)."""
     from hacs_amd import engine as E
     ctx = E.ModCtx(p)
     assert ctx.limbs in (1, 2, 3, 4)
     rng = np.random.default_rng(p + tile)
-    shapes = [(64, 128, 64), (65, 129, 128), (200, 130, 192), (31, 300, 256), (129, 257, 320), (96, 96, 1000)]
+    shapes = [(64, 128, 64), (65, 129, 128), (200, 130, 192), (31, 300, 256), (129, 257, 320), (96, 96, 1000),
+              (193, 64, 448), (288, 200, 64)]
     for (m, n, k) in shapes:
         A = rng.integers(0, p, (m, k), dtype=np.uint64).astype(np.uint32)
         B = rng.integers(0, p, (k, n), dtype=np.uint64).astype(np.uint32)
