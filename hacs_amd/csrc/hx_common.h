@@ -99,6 +99,14 @@ bool stream_capturing(hipStream_t s);
 enum ScratchPurpose { SCRATCH_LIMB = 0, SCRATCH_MLP = 1, SCRATCH_RREF = 2, SCRATCH_SPLIT = 3 };
 void* stream_scratch(hipStream_t s, int purpose, size_t bytes);
 void stream_scratch_release(hipStream_t s);
+// A rows x width-byte block between pageable host memory and the device (either direction, pitches in
+// bytes) through the runtime's pinned staging chunks, ordered on st; returns once it has landed.
+void staged_copy_2d(void* dst, size_t dpitch, const void* src, size_t spitch, size_t width, size_t rows,
+                    bool to_dev, hipStream_t st);
+// hx_upload / hx_download's status turned into an Error.
+inline void host_status(int rc) {
+    if (rc != HX_OK) throw Error{rc, hx_last_error()};
+}
 
 // A ring2k session's resident window (m x cw words, row stride cw) and its compute stream, for the
 // drivers that run their own kernels between steps (the Smith levels).  The stream has been made to

@@ -322,13 +322,9 @@ HX_API int hx_f2_rref(uint32_t m, uint32_t n, uint64_t* A, uint32_t ldw, uint32_
             f2_place_kernel<<<g, 256, 0, st_>>>(A, ldw, m, state, T);
             after_launch("f2_place_kernel");
             F2State h{};
-            HX_HIP(hipMemcpyAsync(&h, state, sizeof(h), hipMemcpyDeviceToHost, st_));
-            HX_HIP(hipStreamSynchronize(st_));
+            host_status(hx_download(&h, state, sizeof(h), s));
             *rank = h.rank;
-            if (pivots && h.rank) {
-                HX_HIP(hipMemcpyAsync(pivots, pcol, (size_t)h.rank * 4, hipMemcpyDeviceToHost, st_));
-                HX_HIP(hipStreamSynchronize(st_));
-            }
+            if (pivots && h.rank) host_status(hx_download(pivots, pcol, (size_t)h.rank * 4, s));
         } catch (...) {
             (void)hipStreamSynchronize(st_);
             arena_free(ws);

@@ -214,17 +214,15 @@ void copy_block(Session* S, void* host, size_t ldh, uint32_t r0, uint32_t rows, 
     if (cols == S->cw && ldh == cols) {
         // whole rows on both sides: one contiguous block through the runtime's pinned staging
         const size_t bytes = (size_t)rows * cols * e;
-        const int rc = to_dev ? hx_upload(d, host, bytes, (hx_stream)S->s) : hx_download(host, d, bytes, (hx_stream)S->s);
-        if (rc != HX_OK) throw Error{rc, hx_last_error()};
+        host_status(to_dev ? hx_upload(d, host, bytes, (hx_stream)S->s) : hx_download(host, d, bytes, (hx_stream)S->s));
         return;
     }
+    // a strided block through the runtime's pinned staging (a 2D copy straight into pageable memory has
+    // returned from its stream synchronisation before its bytes had landed)
     if (to_dev)
-        HX_HIP(hipMemcpy2DAsync(d, (size_t)S->cw * e, host, ldh * e, cols * e, rows, hipMemcpyHostToDevice, S->s));
+        staged_copy_2d(d, (size_t)S->cw * e, host, ldh * e, (size_t)cols * e, rows, true, S->s);
     else
-        HX_HIP(hipMemcpy2DAsync(host, ldh * e, d, (size_t)S->cw * e, cols * e, rows, hipMemcpyDeviceToHost, S->s));
-    HX_HIP(hipStreamSynchronize(S->s));
-    stat_add(to_dev ? ST_H2D_BYTES : ST_D2H_BYTES, (uint64_t)rows * cols * e);
-    stat_add(to_dev ? ST_H2D_CALLS : ST_D2H_CALLS, 1);
+        staged_copy_2d(host, ldh * e, d, (size_t)S->cw * e, (size_t)cols * e, rows, false, S->s);
 }
 
 template <class E>
@@ -469,7 +467,7 @@ HX_API int hx_ring2k_gemm_sub_host(uint32_t elem, void* C, size_t ldc, const voi
                 const uint32_t kc = std::min<uint32_t>(kMaxPanel, k - k0);
                 for (uint32_t i = 0; i < m; ++i)
                     std::memcpy(Lb.data() + (size_t)i * kc * e, (const char*)L + ((size_t)i * ldl + k0) * e, kc * e);
-                HX_HIP(hipMemcpyAsync(S->d_L, Lb.data(), (size_t)m * kc * e, hipMemcpyHostToDevice, S->s));
+                host_status(hx_upload(S->d_L, Lb.data(), (size_t)m * kc * e, (hx_stream)S->s));
                 if (elem == 4) {
                     gemm_attrs<uint32_t>();
                     launch_gemm<uint32_t>(S, k0, kc, k, k + m, 0, n, (const uint32_t*)S->d_L, kc, k, S->s);

@@ -380,13 +380,10 @@ void smith_run(uint32_t k, uint32_t am, uint32_t an, const E* A, size_t lda, uin
             ok(hx_ring2k_put(sess, A, lda, 0, m, 0, n));
         } else {
             ArenaBuf src(sizeof(E) * (size_t)am * an, st);
-            if (lda == an) {
+            if (lda == an)
                 ok(hx_upload(src.p, A, sizeof(E) * (size_t)am * an, (hx_stream)st));
-            } else {
-                HX_HIP(hipMemcpy2DAsync(src.p, sizeof(E) * an, A, sizeof(E) * lda, sizeof(E) * an, am,
-                                        hipMemcpyHostToDevice, st));
-                stat_add(ST_H2D_BYTES, sizeof(E) * (uint64_t)am * an);
-            }
+            else
+                staged_copy_2d(src.p, sizeof(E) * an, A, sizeof(E) * lda, sizeof(E) * an, am, true, st);
             smith_transpose_kernel<E><<<dim3((am + 63) / 64, (an + 63) / 64), 256, 0, st>>>((const E*)src.p, am, an, win, n);
             after_launch("smith_transpose_kernel");
             HX_HIP(hipStreamSynchronize(st));
@@ -409,8 +406,7 @@ void smith_run(uint32_t k, uint32_t am, uint32_t an, const E* A, size_t lda, uin
                                                                               (uint64_t*)bits.p, fl);
             after_launch("smith_pack_kernel");
             uint32_t hf[2] = {0, 0};
-            HX_HIP(hipMemcpyAsync(hf, fl, sizeof(hf), hipMemcpyDeviceToHost, st));
-            HX_HIP(hipStreamSynchronize(st));
+            ok(hx_download(hf, fl, sizeof(hf), (hx_stream)st));
             if (!hf[0]) break;
             // 2. the F_2 rank profile
             uint32_t r0 = 0;
@@ -427,7 +423,7 @@ void smith_run(uint32_t k, uint32_t am, uint32_t an, const E* A, size_t lda, uin
                         else
                             order[q++] = j;
                     }
-                    HX_HIP(hipMemcpyAsync(d_order.p, order.data(), sizeof(uint32_t) * cols, hipMemcpyHostToDevice, st));
+                    ok(hx_upload(d_order.p, order.data(), sizeof(uint32_t) * cols, (hx_stream)st));
                     for (uint32_t i0 = 0; i0 < rows; i0 += (uint32_t)chunk) {
                         const uint32_t nr = std::min<uint32_t>((uint32_t)chunk, rows - i0);
                         smith_gather_kernel<E><<<dim3((cols + 255) / 256, nr), 256, 0, st>>>(
@@ -454,7 +450,16 @@ void smith_run(uint32_t k, uint32_t am, uint32_t an, const E* A, size_t lda, uin
                     const uint32_t np = factor_panel<E>(kk, P, prow, pw, L, r, panel, swaps, c, lpiv.data(), lpv.data(), cnt);
                     bool agree = np == pw;
                     for (uint32_t t = 0; agree && t < np; ++t) agree = lpiv[t] == c + t && lpv[t] == 0;
-                    if (!agree) throw Error{HX_E_RUNTIME, "ring2k smith: the odd-pivot LU disagrees with the F_2 rank profile"};
+                    if (!agree) {
+                        size_t nz = 0;
+                        for (const E& x : P) nz += x != 0;
+                        throw Error{HX_E_RUNTIME, "ring2k smith: the odd-pivot LU disagrees with the F_2 rank profile (level " +
+                                                      std::to_string(s) + ", columns " + std::to_string(c) + ".." +
+                                                      std::to_string(c1) + ", " + std::to_string(np) + " pivots, " +
+                                                      std::to_string(nz) + " non-zero panel entries of " +
+                                                      std::to_string(P.size()) + ", fetched by " +
+                                                      (have_next ? "lookahead)" : "get)")};
+                    }
                     const uint32_t nc1 = std::min(c1 + panel, climit);
                     const bool next = c1 < climit;
                     hx_ring2k_step_args a{};
@@ -486,8 +491,7 @@ void smith_run(uint32_t k, uint32_t am, uint32_t an, const E* A, size_t lda, uin
                 (void)ring2k_window(sess);              // after the last step's lookahead copy, if any
                 smith_halve_kernel<E><<<dim3((n - Rn + 255) / 256, m - Rn), 256, 0, st>>>(win, n, Rn, n - Rn, fl);
                 after_launch("smith_halve_kernel");
-                HX_HIP(hipMemcpyAsync(hf, fl, sizeof(hf), hipMemcpyDeviceToHost, st));
-                HX_HIP(hipStreamSynchronize(st));
+                ok(hx_download(hf, fl, sizeof(hf), (hx_stream)st));
                 if (hf[1]) throw Error{HX_E_RUNTIME, "ring2k smith: the Schur complement is not even"};
             }
             R = Rn;

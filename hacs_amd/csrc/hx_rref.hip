@@ -137,9 +137,8 @@ HX_API int hx_rref_u32_host(const hx_modctx* ctx, uint32_t m, uint32_t n, hx_row
                 int64_t ld = n;
                 const uint32_t* rows = src(user, r0, nr, &ld);
                 HX_CHECK(rows != nullptr && ld >= (int64_t)n, "row source returned no rows");
-                HX_HIP(hipMemcpy2DAsync(dA + (size_t)r0 * n, (size_t)n * 4, rows, (size_t)ld * 4, (size_t)n * 4, nr,
-                                        hipMemcpyHostToDevice, s));
-                HX_HIP(hipStreamSynchronize(s));   // the source view is valid until the next call
+                // through pinned staging; returns once landed (the source view is valid until the next call)
+                staged_copy_2d(dA + (size_t)r0 * n, (size_t)n * 4, rows, (size_t)ld * 4, (size_t)n * 4, nr, true, s);
                 elim_stat_add(EL_H2D, (uint64_t)nr * n * 4);
             }
             const int rc = hx_rref_u32(ctx, m, n, dA, n, dR, n, rank, pivots, pivot_rows, (hx_stream)s);
@@ -149,9 +148,7 @@ HX_API int hx_rref_u32_host(const hx_modctx* ctx, uint32_t m, uint32_t n, hx_row
                 uint32_t* out = sink(user, *rank, &ld);
                 if (out) {
                     HX_CHECK(ld >= (int64_t)n, "row sink: ld < n");
-                    HX_HIP(hipMemcpy2DAsync(out, (size_t)ld * 4, dR, (size_t)n * 4, (size_t)n * 4, *rank,
-                                            hipMemcpyDeviceToHost, s));
-                    HX_HIP(hipStreamSynchronize(s));
+                    staged_copy_2d(out, (size_t)ld * 4, dR, (size_t)n * 4, (size_t)n * 4, *rank, false, s);
                     elim_stat_add(EL_D2H, (uint64_t)*rank * n * 4);
                 }
             }

@@ -180,6 +180,56 @@ void chunk_acquire(Staging& S, int i) {
 
 }  // namespace
 
+// A 2D block between pageable host memory and the device through the pinned chunks: row chunks
+// packed into a chunk (pitch = width) and moved by one 2D DMA each, alternating chunks so that the
+// host copy of one overlaps the DMA of the other.  Returns once every byte has landed.
+void staged_copy_2d(void* dst, size_t dpitch, const void* src, size_t spitch, size_t width, size_t rows,
+                    bool to_dev, hipStream_t st) {
+    if (width == 0 || rows == 0) return;
+    HX_CHECK(width <= kStageChunk, "staged copy: one row exceeds a staging chunk");
+    Staging& S = staging();
+    std::lock_guard<std::mutex> lk(S.mu);
+    staging_init_locked(S);
+    const size_t rc = kStageChunk / width;
+    const size_t nchunks = (rows + rc - 1) / rc;
+    if (to_dev) {
+        for (size_t c = 0; c < nchunks; ++c) {
+            const int i = (int)(c & 1);
+            const size_t r0 = c * rc, nr = std::min(rc, rows - r0);
+            chunk_acquire(S, i);
+            char* ch = (char*)S.chunk[i];
+            for (size_t r = 0; r < nr; ++r) std::memcpy(ch + r * width, (const char*)src + (r0 + r) * spitch, width);
+            HX_HIP(hipMemcpy2DAsync((char*)dst + r0 * dpitch, dpitch, ch, width, width, nr, hipMemcpyHostToDevice, st));
+            HX_HIP(hipEventRecord(S.done[i], st));
+            S.busy[i] = true;
+        }
+        HX_HIP(hipStreamSynchronize(st));
+    } else {
+        // chunk c's DMA is queued before chunk c-1 is copied out of its pinned buffer
+        for (size_t c = 0; c <= nchunks; ++c) {
+            if (c < nchunks) {
+                const int i = (int)(c & 1);
+                const size_t r0 = c * rc, nr = std::min(rc, rows - r0);
+                chunk_acquire(S, i);
+                HX_HIP(hipMemcpy2DAsync(S.chunk[i], width, (const char*)src + r0 * spitch, spitch, width, nr,
+                                        hipMemcpyDeviceToHost, st));
+                HX_HIP(hipEventRecord(S.done[i], st));
+                S.busy[i] = true;
+            }
+            if (c > 0) {
+                const int j = (int)((c - 1) & 1);
+                const size_t r0 = (c - 1) * rc, nr = std::min(rc, rows - r0);
+                HX_HIP(hipEventSynchronize(S.done[j]));
+                S.busy[j] = false;
+                const char* ch = (const char*)S.chunk[j];
+                for (size_t r = 0; r < nr; ++r) std::memcpy((char*)dst + (r0 + r) * dpitch, ch + r * width, width);
+            }
+        }
+    }
+    stat_add(to_dev ? ST_H2D_BYTES : ST_D2H_BYTES, (uint64_t)rows * width);
+    stat_add(to_dev ? ST_H2D_CALLS : ST_D2H_CALLS, 1);
+}
+
 void* arena_alloc(size_t bytes) {
     if (bytes == 0) return nullptr;
     Arena& a = arena();
