@@ -6,16 +6,17 @@
 // every entry gets the one value the host code gives it, whatever the tiling or summation order.
 //
 // One step (a panel the host has factored) is, on the session's compute stream:
-//   1. the panel's factored block lands in the window (one 2D copy, rows r..m-1 of [pc0, pc1)), its
-//      L slots and the composed row permutation land in device staging;
+//   1. the panel's factored block lands in the window (one contiguous copy into device staging, then a
+//      kernel places rows r..m-1 of [pc0, pc1)), its L slots and the composed row permutation land in
+//      device staging;
 //   2. the panel's row swaps reach every other window column: the swap list is composed on the host
 //      into one permutation of the rows it touches, applied per 64-column strip through LDS;
-//   3. the unit-lower TRSM on the pivot rows r..r+np-1: one wave per 64-column strip, the strip's np
-//      x 64 block in LDS (one column per lane, no barrier), L11 read from the staged L slots;
+//   3. the unit-lower solve on the pivot rows r..r+np-1 as the product L11^-1 X (L11^-1 from the host):
+//      one 256-thread workgroup per 64-column strip with the strip and L11^-1 in LDS;
 //   4. the trailing update C[r+np.., u0..w) -= L21 U12 as a tiled wraparound product: 128 x 128
 //      output tiles, the whole K (= np <= 64) of both operands staged in LDS once, 8 x 8 outputs per
-//      thread.  The next panel's columns [la0, la1) are updated first; a second stream copies them to
-//      pinned host memory while the rest of the product runs (the lookahead fetch).
+//      thread.  The next panel's columns [la0, la1) are updated first and packed; a second stream
+//      copies them to pinned host memory while the rest of the product runs (the lookahead fetch).
 // Host staging is pinned and double-buffered by step parity (an event guards each slot's reuse), so
 // step() returns as soon as its inputs are copied into staging.
 //
@@ -56,25 +57,47 @@ __global__ __launch_bounds__(256) void r2_permute_kernel(E* win, uint32_t ld, ui
     for (uint32_t t = grp; t < nt; t += 4) win[(size_t)rows[t] * ld + c] = buf[src_slot[t] * kStrip + lane];
 }
 
-// Unit-lower forward substitution on the pivot rows r .. r+np-1 of the columns [u0, w):
-// X[t] -= sum_{s<t} L11[t][s] X[s], L11 row t = L slots of pivot row r + t (columns 0 .. t-1).
-// One wave per 64-column strip; X (np x 64) in LDS, one column per lane.
+// A rows x cols block of the window at (r0, c0) to / from a contiguous buffer (to_win: buffer -> window).
+// Narrow blocks (a panel: 64 words a row) move between host and device as one contiguous copy plus this
+// kernel, not as a strided copy of one short row after another.
 template <class E>
-__global__ __launch_bounds__(64) void r2_trsm_kernel(E* win, uint32_t ld, uint32_t r, uint32_t np, uint32_t u0,
-                                                     uint32_t w, const E* Ls, uint32_t ldl) {
-    __shared__ E X[kMaxPanel * kStrip];
-    const uint32_t lane = threadIdx.x, c = u0 + blockIdx.x * kStrip + lane;
-    const bool live = c < w;
-    for (uint32_t t = 0; t < np; ++t) X[t * kStrip + lane] = live ? win[(size_t)(r + t) * ld + c] : E(0);
-    // each lane owns its column of X: no barrier, L11's entries are the same for every lane
-    for (uint32_t t = 1; t < np; ++t) {
-        const E* lt = Ls + (size_t)t * ldl;
-        E acc = X[t * kStrip + lane];
-        for (uint32_t s = 0; s < t; ++s) acc -= lt[s] * X[s * kStrip + lane];
-        X[t * kStrip + lane] = acc;
+__global__ __launch_bounds__(256) void r2_block_kernel(E* win, uint32_t ld, uint32_t r0, uint32_t rows, uint32_t c0,
+                                                       uint32_t cols, E* buf, int to_win) {
+    const size_t total = (size_t)rows * cols;
+    for (size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (size_t)gridDim.x * 256) {
+        const uint32_t i = (uint32_t)(idx / cols), j = (uint32_t)(idx % cols);
+        E* wp = win + (size_t)(r0 + i) * ld + c0 + j;
+        if (to_win)
+            *wp = buf[idx];
+        else
+            buf[idx] = *wp;
     }
+}
+
+// The unit-lower solve on the pivot rows r .. r+np-1 of the columns [u0, w) as one product:
+// X <- L11^-1 X, with L11^-1 (np x np, unit lower, row-major) computed by the host (the forward
+// substitution's result in the ring, bit for bit).  One 256-thread workgroup per 64-column strip: the
+// strip's X and L11^-1 in LDS, lane = column, the four waves take rows t = wave, wave + 4, ...; no
+// chain of dependent steps.
+template <class E>
+__global__ __launch_bounds__(256) void r2_trsm_kernel(E* win, uint32_t ld, uint32_t r, uint32_t np, uint32_t u0,
+                                                      uint32_t w, const E* Linv) {
+    extern __shared__ __align__(16) unsigned char r2_lds[];
+    E* X = (E*)r2_lds;                                   // np x kStrip
+    E* Li = X + (size_t)np * kStrip;                     // np x np
+    const uint32_t lane = threadIdx.x % kStrip, grp = threadIdx.x / kStrip;
+    const uint32_t c = u0 + blockIdx.x * kStrip + lane;
+    const bool live = c < w;
+    for (uint32_t t = grp; t < np; t += 4) X[t * kStrip + lane] = live ? win[(size_t)(r + t) * ld + c] : E(0);
+    for (uint32_t i = threadIdx.x; i < np * np; i += 256) Li[i] = Linv[i];
+    __syncthreads();
     if (!live) return;
-    for (uint32_t t = 1; t < np; ++t) win[(size_t)(r + t) * ld + c] = X[t * kStrip + lane];
+    for (uint32_t t = grp + (grp == 0 ? 4 : 0); t < np; t += 4) {    // row 0 is unchanged
+        const E* lt = Li + (size_t)t * np;
+        E acc = X[t * kStrip + lane];                                  // L11^-1[t][t] = 1
+        for (uint32_t s = 0; s < t; ++s) acc += lt[s] * X[s * kStrip + lane];
+        win[(size_t)(r + t) * ld + c] = acc;
+    }
 }
 
 // C (rows [r0, r1) of the window, columns [c0, c1)) -= L21 U12 with L21 = Ls rows (i - lrow0), K = np,
@@ -141,6 +164,9 @@ struct Session {
     uint32_t elem = 4, m = 0, cw = 0, panel = 0;
     void* win = nullptr;          // m x cw words
     void* d_L = nullptr;          // m x panel words (L slots of the step, rows r .. m-1)
+    void* d_Linv = nullptr;       // panel x panel words: L11^-1 of the step
+    void* d_pin = nullptr;        // m x panel words: the factored panel on its way into the window
+    void* d_la = nullptr;         // m x panel words: the lookahead block on its way to the host
     uint32_t* d_perm = nullptr;   // 4 x panel: touched rows | their source slots
     void* h_la = nullptr;         // pinned lookahead block, m x panel words
     HostSlot slot[2];
@@ -167,6 +193,9 @@ void session_free(Session* S) {
     if (S->ev_la_done) (void)hipEventDestroy(S->ev_la_done);
     if (S->win) arena_free(S->win);
     if (S->d_L) arena_free(S->d_L);
+    if (S->d_Linv) arena_free(S->d_Linv);
+    if (S->d_pin) arena_free(S->d_pin);
+    if (S->d_la) arena_free(S->d_la);
     if (S->d_perm) arena_free(S->d_perm);
     if (S->s) (void)hipStreamDestroy(S->s);
     if (S->s_copy) (void)hipStreamDestroy(S->s_copy);
@@ -185,8 +214,12 @@ Session* session_open(uint32_t elem, uint32_t m, uint32_t cw, uint32_t panel) {
         S->panel = panel;
         S->win = arena_alloc(words_bytes(elem, (size_t)m * cw));
         S->d_L = arena_alloc(words_bytes(elem, (size_t)m * panel));
+        S->d_Linv = arena_alloc(words_bytes(elem, (size_t)kMaxPanel * kMaxPanel));
+        S->d_pin = arena_alloc(words_bytes(elem, (size_t)m * panel));
+        S->d_la = arena_alloc(words_bytes(elem, (size_t)m * panel));
         S->d_perm = (uint32_t*)arena_alloc(sizeof(uint32_t) * 4 * kMaxPanel);
-        S->slot_bytes = 2 * words_bytes(elem, (size_t)m * panel) + sizeof(uint32_t) * 4 * kMaxPanel;
+        S->slot_bytes = 2 * words_bytes(elem, (size_t)m * panel) + sizeof(uint32_t) * 4 * kMaxPanel +
+                        words_bytes(elem, (size_t)kMaxPanel * kMaxPanel);
         for (HostSlot& h : S->slot) {
             HX_HIP(hipHostMalloc(&h.buf, S->slot_bytes, hipHostMallocDefault));
             HX_HIP(hipEventCreateWithFlags(&h.done, hipEventDisableTiming));
@@ -244,7 +277,32 @@ void gemm_attrs() {
         (void)hipFuncSetAttribute((const void*)r2_gemm_kernel<E>, hipFuncAttributeMaxDynamicSharedMemorySize, most);
         (void)hipFuncSetAttribute((const void*)r2_permute_kernel<E>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)(sizeof(E) * 2 * kMaxPanel * kStrip));
+        (void)hipFuncSetAttribute((const void*)r2_trsm_kernel<E>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)(sizeof(E) * ((size_t)kMaxPanel * kStrip + (size_t)kMaxPanel * kMaxPanel)));
     });
+}
+
+uint32_t block_grid(size_t words) { return (uint32_t)std::min<size_t>(2048, (words + 255) / 256); }
+
+// The lookahead block (rows rb .. m-1 of the columns [la0, la1), as the stream has them now) to the
+// pinned host block: packed on the compute stream, then one contiguous copy on the copy stream while the
+// compute stream goes on.  The next step (and any window copy) waits for ev_la_done first, so the packed
+// buffer is not overwritten before its copy has left.
+template <class E>
+void lookahead_copy(Session* S, uint32_t rb, uint32_t la0, uint32_t la1) {
+    const size_t e = sizeof(E);
+    S->la_rows = S->m - rb;
+    S->la_cols = la1 - la0;
+    const size_t words = (size_t)S->la_rows * S->la_cols;
+    r2_block_kernel<E><<<block_grid(words), 256, 0, S->s>>>((E*)S->win, S->cw, rb, S->la_rows, la0, S->la_cols,
+                                                             (E*)S->d_la, 0);
+    after_launch("r2_block_kernel");
+    HX_HIP(hipEventRecord(S->ev_la_src, S->s));
+    HX_HIP(hipStreamWaitEvent(S->s_copy, S->ev_la_src, 0));
+    HX_HIP(hipMemcpyAsync(S->h_la, S->d_la, words * e, hipMemcpyDeviceToHost, S->s_copy));
+    HX_HIP(hipEventRecord(S->ev_la_done, S->s_copy));
+    stat_add(ST_D2H_BYTES, (uint64_t)words * e);
+    S->la_pending = true;
 }
 
 template <class E>
@@ -265,11 +323,25 @@ void step_impl(Session* S, const hx_ring2k_step_args& a) {
     if (hs.used) HX_HIP(hipEventSynchronize(hs.done));
     E* hP = (E*)hs.buf;
     E* hL = hP + (size_t)m * S->panel;
-    uint32_t* hperm = (uint32_t*)(hL + (size_t)m * S->panel);
+    E* hLi = hL + (size_t)m * S->panel;                         // np x np: L11^-1
+    uint32_t* hperm = (uint32_t*)(hLi + (size_t)kMaxPanel * kMaxPanel);
     if (update) {
         HX_CHECK(a.L != nullptr && a.ldl >= a.np, "ring2k step: L missing");
         const E* L = (const E*)a.L;
         for (uint32_t i = 0; i < rows; ++i) std::memcpy(hL + (size_t)i * a.np, L + (size_t)i * a.ldl, sizeof(E) * a.np);
+        // L11^-1 by columns: Li[t][j] = -sum_{j <= q < t} L11[t][q] Li[q][j], Li[j][j] = 1 (wraparound)
+        const uint32_t np = a.np;
+        for (uint32_t t = 0; t < np; ++t) {
+            const E* lt = L + (size_t)t * a.ldl;
+            E* it = hLi + (size_t)t * np;
+            for (uint32_t j = 0; j < t; ++j) {
+                E acc = lt[j];
+                for (uint32_t q = j + 1; q < t; ++q) acc += lt[q] * hLi[(size_t)q * np + j];
+                it[j] = (E)(E(0) - acc);
+            }
+            it[t] = E(1);
+            for (uint32_t j = t + 1; j < np; ++j) it[j] = E(0);
+        }
     }
     // the swaps, in order, composed into one permutation of the rows they touch: new(row) = old(src(row))
     std::vector<uint32_t> touched, srcrow;
@@ -309,12 +381,17 @@ void step_impl(Session* S, const hx_ring2k_step_args& a) {
         const size_t e = sizeof(E);
         for (uint32_t i = 0; i < rows; ++i)
             std::memcpy(hP + (size_t)i * pw, (const E*)a.panel + (size_t)i * a.ldp, e * pw);
-        HX_HIP(hipMemcpy2DAsync((char*)S->win + ((size_t)a.r * S->cw + a.pc0) * e, (size_t)S->cw * e, hP, pw * e,
-                                pw * e, rows, hipMemcpyHostToDevice, s));
+        HX_HIP(hipMemcpyAsync(S->d_pin, hP, e * pw * rows, hipMemcpyHostToDevice, s));
+        r2_block_kernel<E><<<block_grid((size_t)rows * pw), 256, 0, s>>>((E*)S->win, S->cw, a.r, rows, a.pc0, pw,
+                                                                         (E*)S->d_pin, 1);
+        after_launch("r2_block_kernel");
         stat_add(ST_H2D_BYTES, (uint64_t)rows * pw * e);
     }
     if (nt) HX_HIP(hipMemcpyAsync(S->d_perm, hperm, sizeof(uint32_t) * 4 * kMaxPanel, hipMemcpyHostToDevice, s));
-    if (update) HX_HIP(hipMemcpyAsync(S->d_L, hL, sizeof(E) * (size_t)rows * a.np, hipMemcpyHostToDevice, s));
+    if (update) {
+        HX_HIP(hipMemcpyAsync(S->d_L, hL, sizeof(E) * (size_t)rows * a.np, hipMemcpyHostToDevice, s));
+        HX_HIP(hipMemcpyAsync(S->d_Linv, hLi, sizeof(E) * (size_t)a.np * a.np, hipMemcpyHostToDevice, s));
+    }
     HX_HIP(hipEventRecord(hs.done, s));
     hs.used = true;
     // 2. the swaps on every other window column
@@ -328,8 +405,9 @@ void step_impl(Session* S, const hx_ring2k_step_args& a) {
     if (update) {
         const E* Ls = (const E*)S->d_L;          // row i of the block = global row r + i
         if (a.np > 1) {
-            r2_trsm_kernel<E><<<(w - a.u0 + kStrip - 1) / kStrip, 64, 0, s>>>((E*)S->win, S->cw, a.r, a.np, a.u0, w, Ls,
-                                                                             a.np);
+            const size_t lds = sizeof(E) * ((size_t)a.np * kStrip + (size_t)a.np * a.np);
+            r2_trsm_kernel<E><<<(w - a.u0 + kStrip - 1) / kStrip, 256, lds, s>>>((E*)S->win, S->cw, a.r, a.np, a.u0, w,
+                                                                                (const E*)S->d_Linv);
             after_launch("r2_trsm_kernel");
         }
         const uint32_t rb = a.r + a.np;
@@ -337,17 +415,7 @@ void step_impl(Session* S, const hx_ring2k_step_args& a) {
         if (la) {
             HX_CHECK(a.la0 >= a.u0, "ring2k step: lookahead columns left of the update");
             launch_gemm<E>(S, a.r, a.np, rb, m, a.la0, a.la1, Ls + (size_t)a.np * a.np, a.np, rb, s);
-            HX_HIP(hipEventRecord(S->ev_la_src, s));
-            HX_HIP(hipStreamWaitEvent(S->s_copy, S->ev_la_src, 0));
-            const size_t e = sizeof(E);
-            S->la_rows = m - rb;
-            S->la_cols = a.la1 - a.la0;
-            HX_HIP(hipMemcpy2DAsync(S->h_la, (size_t)S->la_cols * e, (char*)S->win + ((size_t)rb * S->cw + a.la0) * e,
-                                    (size_t)S->cw * e, (size_t)S->la_cols * e, S->la_rows, hipMemcpyDeviceToHost,
-                                    S->s_copy));
-            HX_HIP(hipEventRecord(S->ev_la_done, S->s_copy));
-            stat_add(ST_D2H_BYTES, (uint64_t)S->la_rows * S->la_cols * e);
-            S->la_pending = true;
+            lookahead_copy<E>(S, rb, a.la0, a.la1);
             launch_gemm<E>(S, a.r, a.np, rb, m, a.u0, a.la0, Ls + (size_t)a.np * a.np, a.np, rb, s);
             launch_gemm<E>(S, a.r, a.np, rb, m, a.la1, w, Ls + (size_t)a.np * a.np, a.np, rb, s);
         } else {
@@ -355,16 +423,7 @@ void step_impl(Session* S, const hx_ring2k_step_args& a) {
         }
     } else if (a.la1 > a.la0 && a.r + a.np < m) {
         // no update, but the caller still wants the next panel's columns back
-        HX_HIP(hipEventRecord(S->ev_la_src, s));
-        HX_HIP(hipStreamWaitEvent(S->s_copy, S->ev_la_src, 0));
-        const size_t e = sizeof(E);
-        const uint32_t rb = a.r + a.np;
-        S->la_rows = m - rb;
-        S->la_cols = a.la1 - a.la0;
-        HX_HIP(hipMemcpy2DAsync(S->h_la, (size_t)S->la_cols * e, (char*)S->win + ((size_t)rb * S->cw + a.la0) * e,
-                                (size_t)S->cw * e, (size_t)S->la_cols * e, S->la_rows, hipMemcpyDeviceToHost, S->s_copy));
-        HX_HIP(hipEventRecord(S->ev_la_done, S->s_copy));
-        S->la_pending = true;
+        lookahead_copy<E>(S, a.r + a.np, a.la0, a.la1);
     }
 }
 
@@ -394,11 +453,13 @@ using namespace hx;
 extern "C" {
 
 HX_API size_t hx_ring2k_session_bytes(uint32_t elem, uint32_t m, uint32_t cw, uint32_t panel) {
-    return (size_t)elem * ((size_t)m * cw + (size_t)m * panel) + sizeof(uint32_t) * 4 * kMaxPanel;
+    // the window, the L slots, the panel and lookahead staging, L11^-1, the permutation
+    return (size_t)elem * ((size_t)m * cw + 3 * (size_t)m * panel + (size_t)kMaxPanel * kMaxPanel) +
+           sizeof(uint32_t) * 4 * kMaxPanel;
 }
 
 HX_API size_t hx_ring2k_staging_bytes(uint32_t elem, uint32_t m, uint32_t panel) {
-    return 3 * (size_t)elem * m * panel + 2 * sizeof(uint32_t) * 4 * kMaxPanel;
+    return 3 * (size_t)elem * m * panel + 2 * (sizeof(uint32_t) * 4 * kMaxPanel + (size_t)elem * kMaxPanel * kMaxPanel);
 }
 
 HX_API int hx_ring2k_open(uint32_t elem, uint32_t m, uint32_t cw, uint32_t panel, hx_ring2k* out) {
