@@ -857,3 +857,121 @@ HXO_API int64_t hxo_smith_naive(uint32_t k, uint32_t m, uint32_t n, uint64_t *A,
 #undef SM_AT
     return cnt;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Howell form over Z/ell^k, ell prime, q = ell^k < 2^32: elim::howell      */
+/* (DenseElim.cpp:1450 -> howell_rows :1130-1300; ModCtx's valuation,       */
+/* unit_part, divide_exact, ell_pow, ModCtx.h:283-312).  The Sec. 13.5      */
+/* pass on the rows of A: columns left to right; among the active rows the  */
+/* entry of minimal ell-adic valuation, ties to the topmost (active order = */
+/* the input rows, then closure rows as created; the chosen row leaves the  */
+/* list); the pivot row scaled by the inverse of its unit part (it leads    */
+/* with ell^v); every other active row r -= (r[c] / ell^v) P (exact         */
+/* division); for v > 0 the closure row ell^(k-v) P, zero up to column c,   */
+/* joins the active list at the bottom.  Then reduce above, pivots left to  */
+/* right: H[i'] -= floor(H[i'][c_i] / ell^v_i) H[i].  Entries canonical in  */
+/* [0, q).  Returns h; H is h x n (ld n), pivots / pivot_v h entries.       */
+/* Pinned by: the reference's own elim::howell on the committed fixture     */
+/* (tests/golden/howell_ell_reference.npz); at ell = 2 the Z/2^k oracles;   */
+/* at k = 1 the RREF over F_ell (hxo_rref_rule); [[ell, 1]] over Z/ell^2 =  */
+/* [[ell, 1], [0, ell]]; on tiny matrices over Z/9 the row span of H equals */
+/* the row span of A, enumerated.                                           */
+/* ------------------------------------------------------------------------ */
+static uint32_t ell_val(uint64_t x, uint32_t ell, uint32_t k) {
+    if (x == 0) return k;
+    uint32_t v = 0;
+    while (v < k && x % ell == 0) { x /= ell; ++v; }
+    return v;
+}
+
+static uint64_t ell_inv(uint64_t a, uint64_t q) {   /* a^-1 mod q for gcd(a, q) = 1 (extended Euclid) */
+    int64_t r0 = (int64_t)q, r1 = (int64_t)(a % q), s0 = 0, s1 = 1;
+    while (r1 != 0) {
+        const int64_t t = r0 / r1, r2 = r0 - t * r1, s2 = s0 - t * s1;
+        r0 = r1; r1 = r2; s0 = s1; s1 = s2;
+    }
+    int64_t s = s0 % (int64_t)q;
+    if (s < 0) s += (int64_t)q;
+    return (uint64_t)s;
+}
+
+/* row[c0..n) -= f * src[c0..n) mod q */
+static void ell_row_sub(uint64_t *row, const uint64_t *src, uint64_t f, uint32_t c0, uint32_t n, uint64_t q) {
+    if (f == 0) return;
+    for (uint32_t j = c0; j < n; ++j)
+        if (src[j] != 0) row[j] = (row[j] + q - (f * src[j]) % q) % q;
+}
+
+HXO_API uint32_t hxo_howell_ell(uint32_t ell, uint32_t k, uint32_t m, uint32_t n, const uint32_t *A, int64_t lda,
+                                uint32_t *H, uint32_t *pivots, uint32_t *pivot_v) {
+    uint64_t pw[65];
+    pw[0] = 1;
+    for (uint32_t v = 1; v <= k && v < 65; ++v) pw[v] = pw[v - 1] * ell;
+    const uint64_t q = pw[k];
+    const uint32_t cap = m + n + 1;
+    uint64_t *rows = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)cap * (n ? n : 1));
+    uint32_t *act = (uint32_t *)malloc(sizeof(uint32_t) * cap);
+    uint32_t *prow = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+    uint32_t nact = 0, nrows = 0, h = 0;
+    for (uint32_t i = 0; i < m; ++i) {
+        for (uint32_t j = 0; j < n; ++j) rows[(size_t)nrows * n + j] = A[i * lda + j] % q;
+        act[nact++] = nrows++;
+    }
+    for (uint32_t c = 0; c < n && nact > 0; ++c) {
+        uint32_t best = nact, bv = k;
+        for (uint32_t t = 0; t < nact; ++t) {
+            const uint64_t e = rows[(size_t)act[t] * n + c];
+            if (e == 0) continue;
+            const uint32_t v = ell_val(e, ell, k);
+            if (v < bv) { bv = v; best = t; if (v == 0) break; }
+        }
+        if (best == nact) continue;
+        const uint32_t pid = act[best];
+        memmove(act + best, act + best + 1, sizeof(uint32_t) * (nact - best - 1));
+        --nact;
+        uint64_t *P = rows + (size_t)pid * n;
+        uint64_t u = P[c];
+        while (u % ell == 0) u /= ell;
+        if (u != 1) {
+            const uint64_t ui = ell_inv(u, q);
+            for (uint32_t j = c; j < n; ++j) P[j] = (P[j] * ui) % q;
+        }
+        const uint64_t lead = pw[bv];
+        for (uint32_t t = 0; t < nact; ++t) {
+            uint64_t *R = rows + (size_t)act[t] * n;
+            if (R[c] == 0) continue;
+            ell_row_sub(R, P, R[c] / lead, c, n, q);
+        }
+        if (bv > 0) {
+            const uint64_t f = pw[k - bv];
+            uint64_t *Z = rows + (size_t)nrows * n;
+            P = rows + (size_t)pid * n;
+            for (uint32_t j = 0; j < n; ++j) Z[j] = j <= c ? 0 : (P[j] * f) % q;
+            act[nact++] = nrows++;
+        }
+        prow[h] = pid;
+        pivots[h] = c;
+        pivot_v[h] = bv;
+        ++h;
+    }
+    for (uint32_t i = 0; i < h; ++i)
+        for (uint32_t j = 0; j < n; ++j) H[(size_t)i * n + j] = (uint32_t)rows[(size_t)prow[i] * n + j];
+    uint64_t *hi = (uint64_t *)malloc(sizeof(uint64_t) * (n ? n : 1));
+    uint64_t *hp = (uint64_t *)malloc(sizeof(uint64_t) * (n ? n : 1));
+    for (uint32_t i = 0; i < h; ++i) {
+        const uint32_t ji = pivots[i];
+        const uint64_t d = pw[pivot_v[i]];
+        for (uint32_t j = 0; j < n; ++j) hi[j] = H[(size_t)i * n + j];
+        for (uint32_t ip = 0; ip < i; ++ip) {
+            for (uint32_t j = 0; j < n; ++j) hp[j] = H[(size_t)ip * n + j];
+            ell_row_sub(hp, hi, hp[ji] / d, ji, n, q);
+            for (uint32_t j = 0; j < n; ++j) H[(size_t)ip * n + j] = (uint32_t)hp[j];
+        }
+    }
+    free(hi);
+    free(hp);
+    free(rows);
+    free(act);
+    free(prow);
+    return h;
+}

@@ -354,6 +354,38 @@ def _howell(fn, k, A):
     return H[:h, :n].copy(), piv[:h].copy(), pv[:h].copy()
 
 
+def howell_ell(ell, k, A):
+    """The Howell form over Z/ell^k (elim::howell's pass, DenseElim.cpp:1130-1300): (H, pivots, pivot_v)."""
+    A = _c(A)
+    m, n = A.shape
+    cap = max(1, n)
+    H = np.zeros((cap, max(n, 1)), np.uint32)
+    piv = np.zeros(cap, np.uint32)
+    pv = np.zeros(cap, np.uint32)
+    f = lib().hxo_howell_ell
+    f.restype = C.c_uint32
+    f.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, u32p, C.c_int64, u32p, u32p, u32p]
+    h = f(ell, k, m, n, A, n, H, piv, pv)
+    return H[:h, :n].copy(), piv[:h].copy(), pv[:h].copy()
+
+
+def ref_howell_ell(ell, k, A):
+    """The reference's own elim::howell over ModCtx::prime_power(ell, k) (oracle/_ref), k >= 2."""
+    A = _c(A)
+    m, n = A.shape
+    cap = max(1, n)
+    H = np.zeros((cap, max(n, 1)), np.uint32)
+    piv = np.zeros(cap, np.uint32)
+    pv = np.zeros(cap, np.uint32)
+    f = ref().hxr_elim_howell_ell
+    f.restype = C.c_int64
+    f.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, u32p, C.c_int64, u32p, u32p, u32p]
+    h = f(ell, k, m, n, A, n, H, piv, pv)
+    if h < 0:
+        raise RuntimeError("reference elim::howell raised")
+    return H[:h, :n].copy(), piv[:h].copy(), pv[:h].copy()
+
+
 def howell_naive(k, A):
     return _howell(lib().hxo_howell_naive, k, A)
 

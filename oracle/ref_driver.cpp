@@ -271,4 +271,23 @@ int64_t hxr_elim_howell(uint32_t k, uint32_t m, uint32_t n, const uint32_t* A, i
     }
 }
 
+// engine::elim::howell (DenseElim.cpp:1450) over Z/ell^k: ModCtx::prime_power(ell, k), k >= 2.
+int64_t hxr_elim_howell_ell(uint32_t ell, uint32_t k, uint32_t m, uint32_t n, const uint32_t* A, int64_t lda,
+                            uint32_t* H, uint32_t* pivots, uint32_t* pivot_v) {
+    try {
+        const ModCtx<uint32_t> ctx = ModCtx<uint32_t>::prime_power(ell, k);
+        DenseBuf<uint32_t> a = to_buf(A, m, n, lda);
+        const engine::elim::HowellResult<uint32_t> h = engine::elim::howell<uint32_t>(ctx, DenseView<const uint32_t>(a));
+        const uint32_t hr = h.rows();
+        for (uint32_t i = 0; i < hr; ++i) {
+            pivots[i] = h.pivots[i];
+            pivot_v[i] = h.pivot_v[i];
+            for (uint32_t j = 0; j < n; ++j) H[static_cast<size_t>(i) * n + j] = h.H.at(i, j);
+        }
+        return hr;
+    } catch (...) {
+        return -1;
+    }
+}
+
 }  // extern "C"

@@ -380,3 +380,87 @@ def test_smith_oracle_negative_control(O):
     assert np.array_equal(O.smith(8, D), np.zeros(6, np.uint32))
     D[2, 2] = 6
     assert not np.array_equal(O.smith(8, D), np.zeros(6, np.uint32))
+
+
+def test_howell_ell_oracle_vs_reference_fixture(O):
+    """The oracle's Howell form over Z/ell^k equals the reference's own elim::howell (prime_power contexts)
+    on the committed fixture: H, pivots and valuations, entry for entry."""
+    import os
+    G = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "howell_ell_reference.npz"),
+                allow_pickle=False)
+    for i in range(int(G["count"][0])):
+        ell, k = (int(x) for x in G[f"case{i}_meta"])
+        H, piv, pv = O.howell_ell(ell, k, G[f"case{i}_A"])
+        assert np.array_equal(H, G[f"case{i}_H"]), f"case {i}"
+        assert np.array_equal(piv, G[f"case{i}_piv"]) and np.array_equal(pv, G[f"case{i}_pv"]), f"case {i}"
+
+
+def test_howell_ell_oracle_special_cases(O):
+    """What fixes the Howell form independently: at ell = 2 it is the Z/2^k form (the oracles of
+    ring2k::howell); at k = 1 the RREF over F_ell; [[ell, 1]] over Z/ell^2 = [[ell, 1], [0, ell]]; the form
+    of a row permutation, or with a zero row or a combination of rows appended, is the same."""
+    rng = np.random.default_rng(0x4E12)
+    for k in (3, 8, 16):
+        A = rng.integers(0, 1 << k, (20, 14)).astype(np.uint32)
+        A[::3] = (A[::3] * 4) % (1 << k)
+        got = O.howell_ell(2, k, A)
+        want = O.howell_fast(k, A)
+        assert all(np.array_equal(x, y) for x, y in zip(got, want)), k
+    for ell in (3, 7, 65521):
+        A = rng.integers(0, ell, (15, 22)).astype(np.uint32)
+        A[4] = (A[1].astype(np.uint64) * 2 % ell).astype(np.uint32)
+        H, piv, pv = O.howell_ell(ell, 1, A)
+        r, rpiv, _, R = O.rref_rule(ell, A)
+        assert len(piv) == r and np.array_equal(piv, rpiv) and np.array_equal(H, R) and not pv.any()
+    for ell in (3, 5, 251):
+        H, piv, pv = O.howell_ell(ell, 2, np.array([[ell, 1]], np.uint32))
+        assert np.array_equal(H, np.array([[ell, 1], [0, ell]], np.uint32))
+        assert list(piv) == [0, 1] and list(pv) == [1, 1]
+    ell, k = 5, 6
+    q = ell ** k
+    A = rng.integers(0, q, (12, 10), dtype=np.uint64)
+    A[::2] = A[::2] * ell % q
+    A = A.astype(np.uint32)
+    base = O.howell_ell(ell, k, A)
+    comb = ((A[3].astype(np.uint64) * 7 + A[8].astype(np.uint64) * 11) % q).astype(np.uint32)
+    for B in (A[rng.permutation(12)], np.vstack([A, np.zeros((1, 10), np.uint32)]), np.vstack([A, comb[None, :]])):
+        got = O.howell_ell(ell, k, B)
+        assert all(np.array_equal(x, y) for x, y in zip(got, base))
+
+
+def test_howell_ell_oracle_row_span_tiny(O):
+    """Over Z/9 on 2 x 2 and 3 x 2 matrices: the row span of H, enumerated, equals that of A; H leads each
+    row with exactly 3^v at strictly increasing columns, and the entries above a pivot are below 3^v."""
+    import itertools
+    rng = np.random.default_rng(0x4E13)
+    ell, k, q = 3, 2, 9
+    for (m, n) in ((2, 2), (3, 2), (2, 3)):
+        for _ in range(15):
+            A = rng.integers(0, q, (m, n)).astype(np.int64)
+            if rng.random() < 0.4:
+                A[0] = A[0] * 3 % q
+            H, piv, pv = O.howell_ell(ell, k, A.astype(np.uint32))
+            H = H.astype(np.int64)
+
+            def span(M):
+                if len(M) == 0:
+                    return {tuple([0] * n)}
+                return {tuple((np.array(c) @ M) % q) for c in itertools.product(range(q), repeat=len(M))}
+            assert span(A) == span(H), A.tolist()
+            assert all(piv[i] < piv[i + 1] for i in range(len(piv) - 1))
+            for i, (c, v) in enumerate(zip(piv, pv)):
+                assert H[i, c] == 3 ** v and not H[i, :c].any()
+                assert all(H[ip, c] < 3 ** v for ip in range(i))
+
+
+def test_howell_ell_oracle_negative_control(O):
+    """A planted mistake (one entry of a non-pivot column changed by ell^(k-1)) must change the form."""
+    rng = np.random.default_rng(0x4E14)
+    ell, k = 7, 3
+    q = ell ** k
+    A = rng.integers(0, q, (3, 8)).astype(np.uint32)
+    base = O.howell_ell(ell, k, A)
+    B = A.copy()
+    B[2, 7] = (int(B[2, 7]) + ell ** (k - 1)) % q
+    got = O.howell_ell(ell, k, B)
+    assert not all(np.array_equal(x, y) for x, y in zip(got, base))
