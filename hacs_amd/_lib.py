@@ -202,6 +202,7 @@ SIGNATURES = {
     "hx_rref_batch_status": (_INT, [_V, _V]),
     "hx_howell_work_words": (_SZ, [_U32, _U32]),
     "hx_howell_batch_create": (_INT, [_U32, C.POINTER(hx_howell_job), _U32, C.POINTER(_V)]),
+    "hx_howell_batch_create_ell": (_INT, [_U32, _U32, C.POINTER(hx_howell_job), _U32, C.POINTER(_V)]),
     "hx_howell_batch_run": (_INT, [_V, _V]),
     "hx_howell_batch_destroy": (_INT, [_V]),
     "hx_mlp_forward": (_INT, [_V, _U32, _U32, _V, _V, _U32, _V, _V, _U32, _V, _V, _V]),

@@ -519,6 +519,135 @@ __global__ __launch_bounds__(HT) void howell_batch_kernel(HArgs a) {
     }
 }
 
+// ------------------------------------------------------------------------------------------
+// Z/ell^k for an odd prime ell, q = ell^k < 2^32 (elim::howell, DenseElim.cpp:1450 -> howell_rows
+// :1130-1300).  The same pass on canonical residues in [0, q): the valuation by the divisibility test
+// of an odd divisor (x is a multiple of ell iff x ell^-1 mod 2^32 <= (2^32 - 1) / ell, and then x / ell
+// is that product), the exact quotients x / ell^v as x (ell^-1)^v mod 2^32, the unit part's inverse
+// mod q by Euclid, the products mod q by Barrett (DevMod), the closure row ell^(k-v) P, and reduce
+// above with floor(H[i'][c_i] / ell^v_i).
+// ------------------------------------------------------------------------------------------
+struct EllCtx {
+    uint32_t ell, k, q;
+    uint32_t ellinv;                     // ell^-1 mod 2^32
+    uint32_t lim;                        // (2^32 - 1) / ell
+    uint32_t pw[33];                     // ell^v, v <= k
+    uint32_t ipw[33];                    // (ell^-1)^v mod 2^32
+    DevMod M;                            // Barrett constants of q
+};
+
+__device__ __forceinline__ uint32_t ell_val(uint32_t x, const EllCtx& e) {   // x != 0, canonical
+    uint32_t v = 0;
+    while (v < e.k) {
+        const uint32_t t = x * e.ellinv;
+        if (t > e.lim) break;
+        x = t;
+        ++v;
+    }
+    return v;
+}
+
+__device__ void howell_ell_one(const HJob& job, const EllCtx& e) {
+    __shared__ uint32_t s_red[2][HT / 64];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t m = job.m, n = job.n, q = e.q;
+    if (n == 0) {
+        if (tid == 0) *job.h = 0;
+        return;
+    }
+    uint32_t* rows = job.work;                               // row id i at rows + i * n
+    uint32_t* act = job.work + (size_t)(m + n) * n;          // active order: row ids, or ~0u once chosen
+    uint32_t* prow = act + (m + n);                          // pivot row ids in pivot order
+    for (uint64_t idx = tid; idx < (uint64_t)m * n; idx += HT) {
+        const uint32_t i = (uint32_t)(idx / n), j = (uint32_t)(idx % n);
+        rows[(size_t)i * n + j] = red64(job.A[(int64_t)i * job.lda + j], e.M);
+    }
+    for (uint32_t i = tid; i < m; i += HT) act[i] = i;
+    __syncthreads();
+    uint32_t na = m, h = 0, ncl = 0;
+    for (uint32_t c = 0; c < n; ++c) {
+        // 1. minimal valuation, ties to the topmost active row: key = v << 24 | position
+        uint32_t best = 0xffffffffu;
+        for (uint32_t pos = tid; pos < na; pos += HT) {
+            const uint32_t id = act[pos];
+            if (id == 0xffffffffu) continue;
+            const uint32_t x = rows[(size_t)id * n + c];
+            if (x == 0) continue;
+            best = min(best, (ell_val(x, e) << 24) | pos);
+        }
+        best = wave_min_u32(best);
+        if (lane == 0) s_red[c & 1][wave] = best;
+        __syncthreads();
+        best = min(min(s_red[c & 1][0], s_red[c & 1][1]), min(s_red[c & 1][2], s_red[c & 1][3]));
+        if (best == 0xffffffffu) continue;
+        const uint32_t bpos = best & 0xffffffu, v = best >> 24;
+        const uint32_t pid = act[bpos];
+        uint32_t* P = rows + (size_t)pid * n;
+        // 2. scale by the inverse of the unit part (P[c] == ell^v afterwards)
+        const uint32_t ui = invm(P[c] * e.ipw[v], q);
+        __syncthreads();                                     // everyone has read act[bpos] and P[c]
+        for (uint32_t j = c + tid; j < n; j += HT) P[j] = mulm(P[j], ui, e.M);
+        if (tid == 0) act[bpos] = 0xffffffffu;
+        __syncthreads();
+        // 3. eliminate every other active row exactly (one wave per row, lanes along the row)
+        for (uint32_t pos = wave; pos < na; pos += HT / 64) {
+            const uint32_t id = act[pos];
+            if (id == 0xffffffffu) continue;
+            uint32_t* R = rows + (size_t)id * n;
+            const uint32_t x = R[c];
+            if (x == 0) continue;
+            const uint32_t qf = x * e.ipw[v];                // x / ell^v, exact
+            for (uint32_t j = c + lane; j < n; j += 64) R[j] = subm(R[j], mulm(qf, P[j], e.M), q);
+        }
+        // 4. the closure row ell^(k-v) P at the bottom of the active list
+        if (v > 0) {
+            const uint32_t cid = m + ncl, f = e.pw[e.k - v];
+            uint32_t* Z = rows + (size_t)cid * n;
+            for (uint32_t j = tid; j < n; j += HT) Z[j] = j <= c ? 0u : mulm(P[j], f, e.M);
+            if (tid == 0) act[na] = cid;
+            ++na;
+            ++ncl;
+        }
+        if (tid == 0) {
+            prow[h] = pid;
+            job.pivots[h] = c;
+            job.pivot_v[h] = v;
+        }
+        ++h;
+        __syncthreads();
+    }
+    for (uint64_t idx = tid; idx < (uint64_t)h * n; idx += HT) {
+        const uint32_t i = (uint32_t)(idx / n), j = (uint32_t)(idx % n);
+        job.H[(size_t)i * n + j] = rows[(size_t)prow[i] * n + j];
+    }
+    __syncthreads();
+    for (uint32_t i = 1; i < h; ++i) {
+        const uint32_t ci = job.pivots[i], d = e.pw[job.pivot_v[i]];
+        const uint32_t* Hi = job.H + (size_t)i * n;
+        for (uint32_t ip = wave; ip < i; ip += HT / 64) {
+            uint32_t* Hp = job.H + (size_t)ip * n;
+            const uint32_t qq = Hp[ci] / d;
+            if (qq == 0) continue;
+            for (uint32_t j = ci + lane; j < n; j += 64) Hp[j] = subm(Hp[j], mulm(qq, Hi[j], e.M), q);
+        }
+        __syncthreads();
+    }
+    if (tid == 0) *job.h = h;
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(HT) void howell_ell_kernel(HArgs a, EllCtx e) {
+    __shared__ uint32_t s_job;
+    for (;;) {
+        if (threadIdx.x == 0) s_job = atomicAdd(a.next, 1u);
+        __syncthreads();
+        const uint32_t jj = s_job;
+        __syncthreads();
+        if (jj >= a.njobs) break;
+        howell_ell_one(a.jobs[a.order[jj]], e);
+    }
+}
+
 __global__ void zero_counters_kernel(uint32_t* x, uint32_t n) {
     if (threadIdx.x < n) x[threadIdx.x] = 0;
 }
@@ -536,6 +665,8 @@ struct HClass {
 
 struct HBatch {
     uint32_t k = 0, njobs = 0;
+    uint32_t ell = 2;                    // odd: the Z/ell^k kernel for every job
+    EllCtx ectx{};
     HJob* d_jobs = nullptr;
     uint32_t* d_next = nullptr;          // one counter per class
     HClass cls[NCLASS];
@@ -557,6 +688,101 @@ void destroy(HBatch* b) {
     delete b;
 }
 
+EllCtx ell_ctx(uint32_t ell, uint32_t k) {
+    EllCtx e{};
+    e.ell = ell;
+    e.k = k;
+    uint32_t inv = ell;                                          // ell^-1 mod 2^32 by Newton (ell odd)
+    for (int i = 0; i < 5; ++i) inv *= 2u - ell * inv;
+    e.ellinv = inv;
+    e.lim = 0xffffffffu / ell;
+    e.pw[0] = 1;
+    e.ipw[0] = 1;
+    for (uint32_t v = 1; v <= k; ++v) {
+        e.pw[v] = e.pw[v - 1] * ell;
+        e.ipw[v] = e.ipw[v - 1] * inv;
+    }
+    e.q = e.pw[k];
+    const unsigned __int128 two64 = (unsigned __int128)1 << 64;
+    e.M = DevMod{e.q, (uint64_t)(two64 / e.q), (uint64_t)(two64 % e.q)};
+    return e;
+}
+
+// A batch over Z/2^k (ell = 2: the blocked classes and the unblocked kernel) or Z/ell^k (ell odd).
+HBatch* batch_create(uint32_t ell, uint32_t k, const hx_howell_job* jobs, uint32_t njobs) {
+    HBatch* b = new HBatch();
+    try {
+        b->k = k;
+        b->njobs = njobs;
+        b->ell = ell;
+        if (ell != 2) b->ectx = ell_ctx(ell, k);
+        std::vector<HJob> hj(njobs);
+        for (uint32_t i = 0; i < njobs; ++i) {
+            const hx_howell_job& j = jobs[i];
+            HX_CHECK(j.n == 0 || (j.H && j.pivots && j.pivot_v && j.work), "howell job: NULL pointer");
+            HX_CHECK(j.h != nullptr, "howell job: h is NULL");
+            HX_CHECK(j.m == 0 || j.A != nullptr, "howell job: A is NULL");
+            HX_CHECK(j.lda >= (int64_t)j.n, "howell job: lda < n");
+            HX_CHECK((uint64_t)j.m + j.n < (1u << 24), "howell job: too many rows");
+            hj[i] = HJob{j.m, j.n, j.A, j.lda, j.H, j.pivots, j.pivot_v, j.h, j.work};
+        }
+        std::vector<uint32_t> order(njobs);
+        std::iota(order.begin(), order.end(), 0u);
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+            return (double)hj[x].m * hj[x].n * hj[x].n > (double)hj[y].m * hj[y].n * hj[y].n;
+        });
+        std::vector<uint32_t> per[NCLASS];
+        for (uint32_t i : order) {
+            const uint32_t rows = hj[i].m + hj[i].n;
+            int c = NCLASS - 1;                              // Z/ell^k, ell odd: every job unblocked
+            for (int t = 0; ell == 2 && t < NCLASS - 1; ++t)
+                if (rows <= kClassRows[t]) { c = t; break; }
+            per[c].push_back(i);
+        }
+        HX_HIP(hipMalloc(&b->d_jobs, std::max<size_t>(njobs, 1) * sizeof(HJob)));
+        HX_HIP(hipMalloc(&b->d_next, NCLASS * sizeof(uint32_t)));
+        if (njobs) HX_HIP(hipMemcpy(b->d_jobs, hj.data(), njobs * sizeof(HJob), hipMemcpyHostToDevice));
+        int cus = 256;
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, 0) == hipSuccess) cus = prop.multiProcessorCount;
+        static std::once_flag once;
+        std::call_once(once, [] {
+            for (uint32_t kk : {16u, 24u, 32u})
+                (void)hipFuncSetAttribute(blocked_kernel_for(kk), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)blocked_lds_bytes(kBlockedRows, blocked_elt_bytes(kk)));
+        });
+        for (int c = 0; c < NCLASS; ++c) {
+            HClass& cl = b->cls[c];
+            cl.njobs = (uint32_t)per[c].size();
+            if (!cl.njobs) continue;
+            HX_HIP(hipMalloc(&cl.d_order, cl.njobs * sizeof(uint32_t)));
+            HX_HIP(hipMemcpy(cl.d_order, per[c].data(), cl.njobs * sizeof(uint32_t), hipMemcpyHostToDevice));
+            int per_cu = 4;
+            if (c < NCLASS - 1) {
+                uint32_t rc = 0;
+                for (uint32_t i : per[c]) rc = std::max(rc, hj[i].m + hj[i].n);
+                cl.rc = rc;
+                cl.lds = blocked_lds_bytes(rc, blocked_elt_bytes(k));
+                int nb = 0;
+                const void* kern = blocked_kernel_for(k);
+                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, HT, cl.lds) == hipSuccess &&
+                    nb > 0)
+                    per_cu = nb;
+                else
+                    per_cu = 1;
+            }
+            cl.grid = std::max<uint32_t>(1, std::min<uint32_t>(cl.njobs, (uint32_t)cus * per_cu));
+        }
+        HX_HIP(hipStreamCreateWithFlags(&b->s2, hipStreamNonBlocking));
+        HX_HIP(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
+        HX_HIP(hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming));
+    } catch (...) {
+        destroy(b);
+        throw;
+    }
+    return b;
+}
+
 }  // namespace
 }  // namespace hx
 
@@ -572,75 +798,22 @@ HX_API int hx_howell_batch_create(uint32_t k, const hx_howell_job* jobs, uint32_
     return guard("hx_howell_batch_create", [&] {
         HX_CHECK(k >= 1 && k <= 32, "howell: need 1 <= k <= 32");
         HX_CHECK(out != nullptr && (jobs != nullptr || njobs == 0), "NULL argument");
-        HBatch* b = new HBatch();
-        try {
-            b->k = k;
-            b->njobs = njobs;
-            std::vector<HJob> hj(njobs);
-            for (uint32_t i = 0; i < njobs; ++i) {
-                const hx_howell_job& j = jobs[i];
-                HX_CHECK(j.n == 0 || (j.H && j.pivots && j.pivot_v && j.work), "howell job: NULL pointer");
-                HX_CHECK(j.h != nullptr, "howell job: h is NULL");
-                HX_CHECK(j.m == 0 || j.A != nullptr, "howell job: A is NULL");
-                HX_CHECK(j.lda >= (int64_t)j.n, "howell job: lda < n");
-                HX_CHECK((uint64_t)j.m + j.n < (1u << 24), "howell job: too many rows");
-                hj[i] = HJob{j.m, j.n, j.A, j.lda, j.H, j.pivots, j.pivot_v, j.h, j.work};
-            }
-            std::vector<uint32_t> order(njobs);
-            std::iota(order.begin(), order.end(), 0u);
-            std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
-                return (double)hj[x].m * hj[x].n * hj[x].n > (double)hj[y].m * hj[y].n * hj[y].n;
-            });
-            std::vector<uint32_t> per[NCLASS];
-            for (uint32_t i : order) {
-                const uint32_t rows = hj[i].m + hj[i].n;
-                int c = NCLASS - 1;
-                for (int t = 0; t < NCLASS - 1; ++t)
-                    if (rows <= kClassRows[t]) { c = t; break; }
-                per[c].push_back(i);
-            }
-            HX_HIP(hipMalloc(&b->d_jobs, std::max<size_t>(njobs, 1) * sizeof(HJob)));
-            HX_HIP(hipMalloc(&b->d_next, NCLASS * sizeof(uint32_t)));
-            if (njobs) HX_HIP(hipMemcpy(b->d_jobs, hj.data(), njobs * sizeof(HJob), hipMemcpyHostToDevice));
-            int cus = 256;
-            hipDeviceProp_t prop;
-            if (hipGetDeviceProperties(&prop, 0) == hipSuccess) cus = prop.multiProcessorCount;
-            static std::once_flag once;
-            std::call_once(once, [] {
-                for (uint32_t kk : {16u, 24u, 32u})
-                    (void)hipFuncSetAttribute(blocked_kernel_for(kk), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                              (int)blocked_lds_bytes(kBlockedRows, blocked_elt_bytes(kk)));
-            });
-            for (int c = 0; c < NCLASS; ++c) {
-                HClass& cl = b->cls[c];
-                cl.njobs = (uint32_t)per[c].size();
-                if (!cl.njobs) continue;
-                HX_HIP(hipMalloc(&cl.d_order, cl.njobs * sizeof(uint32_t)));
-                HX_HIP(hipMemcpy(cl.d_order, per[c].data(), cl.njobs * sizeof(uint32_t), hipMemcpyHostToDevice));
-                int per_cu = 4;
-                if (c < NCLASS - 1) {
-                    uint32_t rc = 0;
-                    for (uint32_t i : per[c]) rc = std::max(rc, hj[i].m + hj[i].n);
-                    cl.rc = rc;
-                    cl.lds = blocked_lds_bytes(rc, blocked_elt_bytes(k));
-                    int nb = 0;
-                    const void* kern = blocked_kernel_for(k);
-                    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, HT, cl.lds) == hipSuccess &&
-                        nb > 0)
-                        per_cu = nb;
-                    else
-                        per_cu = 1;
-                }
-                cl.grid = std::max<uint32_t>(1, std::min<uint32_t>(cl.njobs, (uint32_t)cus * per_cu));
-            }
-            HX_HIP(hipStreamCreateWithFlags(&b->s2, hipStreamNonBlocking));
-            HX_HIP(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
-            HX_HIP(hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming));
-        } catch (...) {
-            destroy(b);
-            throw;
+        *out = batch_create(2, k, jobs, njobs);
+    });
+}
+
+HX_API int hx_howell_batch_create_ell(uint32_t ell, uint32_t k, const hx_howell_job* jobs, uint32_t njobs,
+                                      hx_howell_batch* out) {
+    return guard("hx_howell_batch_create_ell", [&] {
+        HX_CHECK(out != nullptr && (jobs != nullptr || njobs == 0), "NULL argument");
+        HX_CHECK(ell >= 2 && k >= 1, "howell: need ell >= 2 and k >= 1");
+        uint64_t q = 1;
+        for (uint32_t v = 0; v < k; ++v) {
+            q *= ell;
+            HX_CHECK(q <= 0xffffffffull, "howell: ell^k must be below 2^32");
         }
-        *out = b;
+        for (uint32_t d = 2; (uint64_t)d * d <= ell; ++d) HX_CHECK(ell % d != 0, "howell: ell must be prime");
+        *out = batch_create(ell, k, jobs, njobs);
     });
 }
 
@@ -662,7 +835,10 @@ HX_API int hx_howell_batch_run(hx_howell_batch vb, hx_stream s) {
             if (!cl.njobs) continue;
             const hipStream_t sc = c == 1 ? b->s2 : s1;
             HArgs a{b->d_jobs, cl.d_order, cl.njobs, b->d_next + c, b->k};
-            if (c == NCLASS - 1) {
+            if (b->ell != 2) {
+                howell_ell_kernel<<<cl.grid, HT, 0, sc>>>(a, b->ectx);
+                after_launch("howell_ell_kernel");
+            } else if (c == NCLASS - 1) {
                 howell_batch_kernel<<<cl.grid, HT, 0, sc>>>(a);
                 after_launch("howell_batch_kernel");
             } else {

@@ -509,10 +509,11 @@ class HowellResult:
 
 
 class HowellBatch:
-    """Howell forms over Z/2^k of many uneven matrices in one persistent launch (ring2k::howell)."""
+    """Howell forms over Z/2^k (ring2k::howell), or over Z/ell^k for a prime ell (elim::howell with a
+    prime-power context), of many uneven matrices in one persistent launch."""
 
-    def __init__(self, k: int, mats: list[torch.Tensor]):
-        self.k = k
+    def __init__(self, k: int, mats: list[torch.Tensor], ell: int = 2):
+        self.k, self.ell = k, ell
         self.mats = mats
         n_jobs = len(mats)
         jobs = (_lib.hx_howell_job * max(1, n_jobs))()
@@ -533,7 +534,10 @@ class HowellBatch:
             jobs[i].work = C.c_void_p(work.data_ptr())
             self.outs.append((H, meta, work, n_))
         self.h = C.c_void_p()
-        check(lib().hx_howell_batch_create(k, jobs, n_jobs, C.byref(self.h)))
+        if ell == 2:
+            check(lib().hx_howell_batch_create(k, jobs, n_jobs, C.byref(self.h)))
+        else:
+            check(lib().hx_howell_batch_create_ell(ell, k, jobs, n_jobs, C.byref(self.h)))
 
     def run(self, stream=None) -> None:
         check(lib().hx_howell_batch_run(self.h, current_stream() if stream is None else stream))

@@ -409,6 +409,12 @@ typedef struct hx_howell_job {
 HX_API size_t hx_howell_work_words(uint32_t m, uint32_t n);
 typedef void *hx_howell_batch;
 HX_API int hx_howell_batch_create(uint32_t k, const hx_howell_job *jobs, uint32_t njobs, hx_howell_batch *out);
+/* elim::howell over a prime power (DenseElim.cpp:1450 with ModCtx::prime_power(ell, k), ModCtx.h:163-182): the
+ * same jobs over Z/ell^k for a prime ell and k >= 1 with ell^k < 2^32 (k = 1: the RREF over F_ell; ell = 2:
+ * the Z/2^k batch above).  Entries are read mod ell^k; H holds canonical residues in [0, ell^k), row i
+ * leading with exactly ell^pivot_v[i]. */
+HX_API int hx_howell_batch_create_ell(uint32_t ell, uint32_t k, const hx_howell_job *jobs, uint32_t njobs,
+                                      hx_howell_batch *out);
 HX_API int hx_howell_batch_run(hx_howell_batch b, hx_stream s);
 HX_API int hx_howell_batch_destroy(hx_howell_batch b);
 
