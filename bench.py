@@ -59,9 +59,10 @@ def parse():
     ap.add_argument("--no-f2", action="store_true", help="do not time the F_2 RREF and product")
     ap.add_argument("--no-splitk", action="store_true", help="do not time the split-K product")
     ap.add_argument("--no-smith", action="store_true", help="do not time the Smith valuations over Z/2^32")
+    ap.add_argument("--no-howell-ell", action="store_true", help="do not time the Howell batch over Z/3^10")
     ap.add_argument("--no-smith-cpu", action="store_true", help="do not time the oracle's Smith sample on the host")
     ap.add_argument("--only", choices=["batch", "lu", "mixed", "cpu", "howell", "ring2k", "level1", "u64", "f2",
-                                       "splitk", "smith"],
+                                       "splitk", "smith", "howell_ell"],
                     help="run one secondary workload alone (for profiling) and print its record")
     ap.add_argument("--pmc-mode", action="store_true",
                     help="for counter collection: one 1 GiB device copy (the traffic calibration), then one limb "
@@ -293,6 +294,43 @@ def howell_batch(E, njobs):
     out["cpu_port"] = {"jobs": len(shapes), "seconds": t_cpu, "cores": threads, "nproc": nproc, "cpu_model": model,
                        "jobs_per_s": len(shapes) / t_cpu, "device_equals_port": bool(same),
                        "routine": "hxo_howell_fast, one job per thread"}
+    return out
+
+
+def howell_ell_leg(E, njobs=1000, ell=3, k=10):
+    """Howell forms over Z/3^10 (elim::howell with a prime-power context) of the first C5(c) Howell shapes,
+    entries uniform in [0, 3^10): jobs/s of one persistent launch, and the oracle's restatement of the
+    reference's pass on the host's threads over the same jobs, compared with the device's."""
+    import torch
+    from concurrent.futures import ThreadPoolExecutor
+    from hacs_amd import workloads as W
+    from oracle import oracle as O
+    q = ell ** k
+    shapes = [n for kind, n in W.c5_stream(3 * njobs + 3, salt=0x51) if kind == "howell"][:njobs]
+    seeds = [W.seed_for(n, n, q, 0x95) ^ j for j, n in enumerate(shapes)]
+    mats = [E.random_entries(n, n, q, sd) for n, sd in zip(shapes, seeds)]
+    b = E.HowellBatch(k, mats, ell=ell)
+    b.run()
+    torch.cuda.synchronize()
+    best = 1e30
+    for _ in range(3):
+        t0 = time.perf_counter()
+        b.run()
+        torch.cuda.synchronize()
+        best = min(best, time.perf_counter() - t0)
+    out = {"jobs": len(shapes), "ell": ell, "k": k, "jobs_per_s": len(shapes) / best, "ms": best * 1e3,
+           "mean_n": float(np.mean(shapes))}
+    threads, nproc, model = host_cpu()
+    inputs = [E.from_device(M) for M in mats]
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        forms = list(ex.map(lambda A: O.howell_ell(ell, k, A), inputs))
+    t_cpu = time.perf_counter() - t0
+    same = all(np.array_equal(g.H, H) and np.array_equal(g.pivots, piv) and np.array_equal(g.pivot_v, pv)
+               for g, (H, piv, pv) in zip(b.results(), forms))
+    out["cpu_port"] = {"jobs": len(shapes), "seconds": t_cpu, "cores": threads, "nproc": nproc, "cpu_model": model,
+                       "jobs_per_s": len(shapes) / t_cpu, "device_equals_port": bool(same),
+                       "routine": "hxo_howell_ell, one job per thread"}
     return out
 
 
@@ -681,6 +719,7 @@ def main():
                "f2": lambda: f2_leg(E),
                "splitk": lambda: splitk_leg(E),
                "smith": lambda: smith_leg(E, cpu=not args.no_smith_cpu),
+               "howell_ell": lambda: howell_ell_leg(E),
                "level1": lambda: level1_kernels(E),
                "cpu": lambda: cpu_baseline(p)}[args.only]()
         print(json.dumps({"only": args.only, **leg}), flush=True)
@@ -763,6 +802,10 @@ def main():
         sm = smith_leg(E, cpu=not args.no_smith_cpu)
         log(f"ring2k smith: {sm}")
         res["ring2k_smith"] = sm
+    if not args.no_howell_ell:
+        he = howell_ell_leg(E)
+        log(f"howell over Z/3^10: {he}")
+        res["howell_ell"] = he
     if not args.no_u64:
         g64 = gemm_u64_leg(E)
         log(f"u64 GEMM: {g64}")

@@ -547,8 +547,12 @@ __device__ __forceinline__ uint32_t ell_val(uint32_t x, const EllCtx& e) {   // 
     return v;
 }
 
+// ET threads per job: the largest jobs (m + n up to 1024 rows) set the batch's makespan, and their
+// row eliminations spread over ET / 64 waves.
+constexpr int ET = 1024;
+
 __device__ void howell_ell_one(const HJob& job, const EllCtx& e) {
-    __shared__ uint32_t s_red[2][HT / 64];
+    __shared__ uint32_t s_red[2][ET / 64];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t m = job.m, n = job.n, q = e.q;
     if (n == 0) {
@@ -558,17 +562,17 @@ __device__ void howell_ell_one(const HJob& job, const EllCtx& e) {
     uint32_t* rows = job.work;                               // row id i at rows + i * n
     uint32_t* act = job.work + (size_t)(m + n) * n;          // active order: row ids, or ~0u once chosen
     uint32_t* prow = act + (m + n);                          // pivot row ids in pivot order
-    for (uint64_t idx = tid; idx < (uint64_t)m * n; idx += HT) {
+    for (uint64_t idx = tid; idx < (uint64_t)m * n; idx += ET) {
         const uint32_t i = (uint32_t)(idx / n), j = (uint32_t)(idx % n);
         rows[(size_t)i * n + j] = red64(job.A[(int64_t)i * job.lda + j], e.M);
     }
-    for (uint32_t i = tid; i < m; i += HT) act[i] = i;
+    for (uint32_t i = tid; i < m; i += ET) act[i] = i;
     __syncthreads();
     uint32_t na = m, h = 0, ncl = 0;
     for (uint32_t c = 0; c < n; ++c) {
         // 1. minimal valuation, ties to the topmost active row: key = v << 24 | position
         uint32_t best = 0xffffffffu;
-        for (uint32_t pos = tid; pos < na; pos += HT) {
+        for (uint32_t pos = tid; pos < na; pos += ET) {
             const uint32_t id = act[pos];
             if (id == 0xffffffffu) continue;
             const uint32_t x = rows[(size_t)id * n + c];
@@ -578,7 +582,9 @@ __device__ void howell_ell_one(const HJob& job, const EllCtx& e) {
         best = wave_min_u32(best);
         if (lane == 0) s_red[c & 1][wave] = best;
         __syncthreads();
-        best = min(min(s_red[c & 1][0], s_red[c & 1][1]), min(s_red[c & 1][2], s_red[c & 1][3]));
+        best = s_red[c & 1][0];
+#pragma unroll
+        for (int w2 = 1; w2 < ET / 64; ++w2) best = min(best, s_red[c & 1][w2]);
         if (best == 0xffffffffu) continue;
         const uint32_t bpos = best & 0xffffffu, v = best >> 24;
         const uint32_t pid = act[bpos];
@@ -586,11 +592,11 @@ __device__ void howell_ell_one(const HJob& job, const EllCtx& e) {
         // 2. scale by the inverse of the unit part (P[c] == ell^v afterwards)
         const uint32_t ui = invm(P[c] * e.ipw[v], q);
         __syncthreads();                                     // everyone has read act[bpos] and P[c]
-        for (uint32_t j = c + tid; j < n; j += HT) P[j] = mulm(P[j], ui, e.M);
+        for (uint32_t j = c + tid; j < n; j += ET) P[j] = mulm(P[j], ui, e.M);
         if (tid == 0) act[bpos] = 0xffffffffu;
         __syncthreads();
         // 3. eliminate every other active row exactly (one wave per row, lanes along the row)
-        for (uint32_t pos = wave; pos < na; pos += HT / 64) {
+        for (uint32_t pos = wave; pos < na; pos += ET / 64) {
             const uint32_t id = act[pos];
             if (id == 0xffffffffu) continue;
             uint32_t* R = rows + (size_t)id * n;
@@ -603,7 +609,7 @@ __device__ void howell_ell_one(const HJob& job, const EllCtx& e) {
         if (v > 0) {
             const uint32_t cid = m + ncl, f = e.pw[e.k - v];
             uint32_t* Z = rows + (size_t)cid * n;
-            for (uint32_t j = tid; j < n; j += HT) Z[j] = j <= c ? 0u : mulm(P[j], f, e.M);
+            for (uint32_t j = tid; j < n; j += ET) Z[j] = j <= c ? 0u : mulm(P[j], f, e.M);
             if (tid == 0) act[na] = cid;
             ++na;
             ++ncl;
@@ -616,7 +622,7 @@ __device__ void howell_ell_one(const HJob& job, const EllCtx& e) {
         ++h;
         __syncthreads();
     }
-    for (uint64_t idx = tid; idx < (uint64_t)h * n; idx += HT) {
+    for (uint64_t idx = tid; idx < (uint64_t)h * n; idx += ET) {
         const uint32_t i = (uint32_t)(idx / n), j = (uint32_t)(idx % n);
         job.H[(size_t)i * n + j] = rows[(size_t)prow[i] * n + j];
     }
@@ -624,7 +630,7 @@ __device__ void howell_ell_one(const HJob& job, const EllCtx& e) {
     for (uint32_t i = 1; i < h; ++i) {
         const uint32_t ci = job.pivots[i], d = e.pw[job.pivot_v[i]];
         const uint32_t* Hi = job.H + (size_t)i * n;
-        for (uint32_t ip = wave; ip < i; ip += HT / 64) {
+        for (uint32_t ip = wave; ip < i; ip += ET / 64) {
             uint32_t* Hp = job.H + (size_t)ip * n;
             const uint32_t qq = Hp[ci] / d;
             if (qq == 0) continue;
@@ -636,7 +642,7 @@ __device__ void howell_ell_one(const HJob& job, const EllCtx& e) {
     __syncthreads();
 }
 
-__global__ __launch_bounds__(HT) void howell_ell_kernel(HArgs a, EllCtx e) {
+__global__ __launch_bounds__(ET) void howell_ell_kernel(HArgs a, EllCtx e) {
     __shared__ uint32_t s_job;
     for (;;) {
         if (threadIdx.x == 0) s_job = atomicAdd(a.next, 1u);
@@ -836,7 +842,7 @@ HX_API int hx_howell_batch_run(hx_howell_batch vb, hx_stream s) {
             const hipStream_t sc = c == 1 ? b->s2 : s1;
             HArgs a{b->d_jobs, cl.d_order, cl.njobs, b->d_next + c, b->k};
             if (b->ell != 2) {
-                howell_ell_kernel<<<cl.grid, HT, 0, sc>>>(a, b->ectx);
+                howell_ell_kernel<<<cl.grid, ET, 0, sc>>>(a, b->ectx);
                 after_launch("howell_ell_kernel");
             } else if (c == NCLASS - 1) {
                 howell_batch_kernel<<<cl.grid, HT, 0, sc>>>(a);
