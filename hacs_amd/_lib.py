@@ -168,6 +168,9 @@ SIGNATURES = {
     "hx_f2_rref": (_INT, [_U32, _U32, _V, _U32, _U32P, _U32P, _V]),
     "hx_f2_rref_workspace_bytes": (_SZ, [_U32, _U32, _U32]),
     "hx_f2_gemm": (_INT, [_U32, _U32, _U32, _V, _U32, _V, _U32, _V, _U32, _INT, _V]),
+    "hx_f3_rref": (_INT, [_U32, _U32, _V, _U32, _U32P, _U32P, _V]),
+    "hx_f3_rref_workspace_bytes": (_SZ, [_U32, _U32, _U32]),
+    "hx_f3_gemm": (_INT, [_U32, _U32, _U32, _V, _U32, _V, _U32, _V, _U32, _INT, _V]),
     "hx_elim_stats_get": (_INT, [C.POINTER(hx_elim_stats)]),
     "hx_elim_stats_reset": (_INT, []),
     "hx_gemm_suggest_splits": (_U32, [_U32, _U32, _U32]),

@@ -152,6 +152,41 @@ def ring2k_smith(A: np.ndarray, k: int, panel: int = 0) -> SmithResult:
     return SmithResult(k, sigma[:ns.value].copy(), lev[:nl.value].copy())
 
 
+def pack_trits(a: np.ndarray, ldw: int | None = None) -> np.ndarray:
+    """A matrix over F_3 (entries 0 / 1 / 2, m x n) as bit-sliced rows: per row plane `one` ([x == 1]) then
+    plane `two` ([x == 2]), ldw uint64 words each (default ceil(n / 64)) -- the TritMatrix layout."""
+    a = np.asarray(a)
+    return np.hstack([pack_bits((a == 1).astype(np.uint8), ldw), pack_bits((a == 2).astype(np.uint8), ldw)])
+
+
+def unpack_trits(words: np.ndarray, n: int) -> np.ndarray:
+    """The inverse of pack_trits: an m x n uint32 matrix of 0 / 1 / 2."""
+    w = np.ascontiguousarray(words, dtype=np.uint64)
+    ldw = w.shape[1] // 2
+    return unpack_bits(w[:, :ldw], n) + 2 * unpack_bits(w[:, ldw:], n)
+
+
+def f3_rref(A_words: torch.Tensor, n: int, stream=None) -> tuple[int, np.ndarray]:
+    """In-place RREF over F_3 of a bit-sliced device matrix (int64 tensor m x 2 ldw); returns (rank, pivot
+    columns) -- rref(TritMatrix&), Binary/TritMatrix.h:96-98."""
+    m, two_ldw = A_words.shape
+    assert A_words.is_contiguous() and two_ldw % 2 == 0
+    piv = np.zeros(max(1, min(m, n)), np.uint32)
+    rank = C.c_uint32(0)
+    check(lib().hx_f3_rref(m, n, C.c_void_p(A_words.data_ptr()), two_ldw // 2, C.byref(rank),
+                           piv.ctypes.data_as(_lib._U32P), current_stream() if stream is None else stream))
+    return rank.value, piv[:rank.value].copy()
+
+
+def f3_gemm(A_words: torch.Tensor, B_words: torch.Tensor, C_words: torch.Tensor, k: int, n: int,
+            accumulate: bool = False, stream=None) -> None:
+    """C = A B (or C += A B) over F_3 on bit-sliced device matrices (multiply, TritMatrix.h:117)."""
+    m = A_words.shape[0]
+    check(lib().hx_f3_gemm(m, n, k, C.c_void_p(A_words.data_ptr()), A_words.stride(0) // 2,
+                           C.c_void_p(B_words.data_ptr()), B_words.stride(0) // 2, C.c_void_p(C_words.data_ptr()),
+                           C_words.stride(0) // 2, 1 if accumulate else 0, current_stream() if stream is None else stream))
+
+
 def current_stream() -> int:
     return torch.cuda.current_stream().cuda_stream
 

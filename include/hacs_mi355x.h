@@ -257,6 +257,17 @@ HX_API size_t hx_f2_rref_workspace_bytes(uint32_t m, uint32_t n, uint32_t ldw);
  * A m x k, B k x n, C m x n, all bit-packed with the given row strides in words. */
 HX_API int hx_f2_gemm(uint32_t m, uint32_t n, uint32_t k, const uint64_t *A, uint32_t lda, const uint64_t *B,
                       uint32_t ldb, uint64_t *C, uint32_t ldc, int accumulate, hx_stream s);
+/* ---- F_3 on bit-sliced rows (binary::TritMatrix, Binary/TritMatrix.h:42-125) ---------------------------
+ * Row i of an m x n matrix: plane `one` (ldw words: bit j = [x_j == 1]) then plane `two` (ldw words: bit j =
+ * [x_j == 2]), row stride 2 ldw words (the reference's layout with ldw = its words_per_plane()); padding
+ * bits zero.  hx_f3_rref: rref(TritMatrix&) (TritMatrix.h:96-98) in place on the device -- the RREF (leading
+ * 1s, zero above and below, zero rows last), rank and pivot columns (host, min(m, n) entries); m <= 8192;
+ * workspace from the library's arena (hx_f3_rref_workspace_bytes).  hx_f3_gemm: multiply (TritMatrix.h:117),
+ * C = A B or C += A B with A m x k, B k x n, C m x n (lda / ldb / ldc: words per plane). */
+HX_API int hx_f3_rref(uint32_t m, uint32_t n, uint64_t *A, uint32_t ldw, uint32_t *rank, uint32_t *pivots, hx_stream s);
+HX_API size_t hx_f3_rref_workspace_bytes(uint32_t m, uint32_t n, uint32_t ldw);
+HX_API int hx_f3_gemm(uint32_t m, uint32_t n, uint32_t k, const uint64_t *A, uint32_t lda, const uint64_t *B,
+                      uint32_t ldb, uint64_t *C, uint32_t ldc, int accumulate, hx_stream s);
 
 /* ---- GEMM mod p for 64-bit residues (Cuda/cuda_kernels.cu:104-164, the u64 kernel; CPU path of record
  *      kernels::gemm<uint64_t>, ModArithmetic/Kernels.h:184) ----------------------------------------- */
