@@ -60,9 +60,10 @@ def parse():
     ap.add_argument("--no-splitk", action="store_true", help="do not time the split-K product")
     ap.add_argument("--no-smith", action="store_true", help="do not time the Smith valuations over Z/2^32")
     ap.add_argument("--no-howell-ell", action="store_true", help="do not time the Howell batch over Z/3^10")
+    ap.add_argument("--no-f3", action="store_true", help="do not time the F_3 RREF and product")
     ap.add_argument("--no-smith-cpu", action="store_true", help="do not time the oracle's Smith sample on the host")
     ap.add_argument("--only", choices=["batch", "lu", "mixed", "cpu", "howell", "ring2k", "level1", "u64", "f2",
-                                       "splitk", "smith", "howell_ell"],
+                                       "splitk", "smith", "howell_ell", "f3"],
                     help="run one secondary workload alone (for profiling) and print its record")
     ap.add_argument("--pmc-mode", action="store_true",
                     help="for counter collection: one 1 GiB device copy (the traffic calibration), then one limb "
@@ -468,6 +469,43 @@ def f2_leg(E, n=8192, rank=6000, reps=3):
             "gemm_bit_macs_per_s": float(n) ** 3 / (gms * 1e-3)}
 
 
+def f3_leg(E, n=8192, rank=6000, reps=3):
+    """F_3 on bit-sliced rows: the in-place RREF of an n x n matrix of planted rank (rank random rows, the
+    others A[a] + 2 A[b] of two of them, shuffled) and the product C = A B at n^3 trits; ms and trit
+    multiply-adds per second."""
+    import torch
+    rng = np.random.default_rng(84)
+    A = np.empty((n, n), np.uint32)
+    A[:rank] = rng.integers(0, 3, (rank, n))
+    pairs = rng.integers(0, rank, (n - rank, 2))
+    A[rank:] = (A[pairs[:, 0]] + 2 * A[pairs[:, 1]]) % 3
+    A = A[rng.permutation(n)]
+    W = torch.from_numpy(E.pack_trits(A).view(np.int64).copy()).cuda()
+    work = torch.empty_like(W)
+    times = []
+    for _ in range(reps + 1):
+        work.copy_(W)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        r, _ = E.f3_rref(work, n)
+        times.append(time.perf_counter() - t)
+    Bw = torch.from_numpy(E.pack_trits(rng.integers(0, 3, (n, n)).astype(np.uint32)).view(np.int64).copy()).cuda()
+    Cw = torch.empty_like(W)
+    E.f3_gemm(W, Bw, Cw, n, n)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        E.f3_gemm(W, Bw, Cw, n, n)
+    e1.record()
+    torch.cuda.synchronize()
+    gms = e0.elapsed_time(e1) / reps
+    rref_s = min(times[1:])
+    return {"n": n, "rank": int(r), "planted_rank": rank, "rref_ms": rref_s * 1e3,
+            "rref_trit_ops_per_s": float(n) * n * rank / rref_s, "gemm_ms": gms,
+            "gemm_trit_macs_per_s": float(n) ** 3 / (gms * 1e-3)}
+
+
 def splitk_leg(E, m=256, n=256, k=65536, reps=5):
     """A thin deep product mod 2147483629 (m x n x k = 256 x 256 x 65536): the plain kernel unsplit, the
     plain kernel split K ways by the automatic policy, and the limb route; ms each, same bytes."""
@@ -720,6 +758,7 @@ def main():
                "splitk": lambda: splitk_leg(E),
                "smith": lambda: smith_leg(E, cpu=not args.no_smith_cpu),
                "howell_ell": lambda: howell_ell_leg(E),
+               "f3": lambda: f3_leg(E),
                "level1": lambda: level1_kernels(E),
                "cpu": lambda: cpu_baseline(p)}[args.only]()
         print(json.dumps({"only": args.only, **leg}), flush=True)
@@ -806,6 +845,10 @@ def main():
         he = howell_ell_leg(E)
         log(f"howell over Z/3^10: {he}")
         res["howell_ell"] = he
+    if not args.no_f3:
+        f3 = f3_leg(E)
+        log(f"F_3: {f3}")
+        res["f3"] = f3
     if not args.no_u64:
         g64 = gemm_u64_leg(E)
         log(f"u64 GEMM: {g64}")
