@@ -63,7 +63,7 @@ def parse():
     ap.add_argument("--no-f3", action="store_true", help="do not time the F_3 RREF and product")
     ap.add_argument("--no-smith-cpu", action="store_true", help="do not time the oracle's Smith sample on the host")
     ap.add_argument("--only", choices=["batch", "lu", "mixed", "cpu", "howell", "ring2k", "level1", "u64", "f2",
-                                       "splitk", "smith", "howell_ell", "f3"],
+                                       "splitk", "smith", "howell_ell", "f3", "mfma"],
                     help="run one secondary workload alone (for profiling) and print its record")
     ap.add_argument("--pmc-mode", action="store_true",
                     help="for counter collection: one 1 GiB device copy (the traffic calibration), then one limb "
@@ -674,6 +674,17 @@ def mixed_stream(njobs):
             "selector_choose_seconds": t_choose}
 
 
+def mfma_calibration(E, blocks=1024, iters=30000):
+    """The limb kernel's MFMA instruction from registers only (hx_mfma_rate, about 40 ms per run): the
+    sustained rate with uniformly random operands (what the limb planes of random residues present; the
+    ceiling of the limb GEMM on this chip) and with near-constant operands (the clock the published peak
+    assumes), best of three runs each."""
+    rnd = max(E.mfma_rate(blocks, iters, True) for _ in range(3))
+    cst = max(E.mfma_rate(blocks, iters, False) for _ in range(3))
+    return {"sustained_random_tops": rnd, "sustained_constant_tops": cst,
+            "sustained_note": f"{blocks} x 256 threads, {iters} rounds of 16 v_mfma_i32_32x32x32_i8 from registers"}
+
+
 def level1_kernels(E, n=8192, reps=20):
     """The HBM-bound level-1/2 kernels on n x n residues (256 MiB per matrix at n = 8192): average
     kernel time over `reps` launches (torch events on the launching stream) and the algorithmic bytes
@@ -790,6 +801,7 @@ def main():
                "smith": lambda: smith_leg(E, cpu=not args.no_smith_cpu),
                "howell_ell": lambda: howell_ell_leg(E),
                "f3": lambda: f3_leg(E),
+               "mfma": lambda: mfma_calibration(E),
                "level1": lambda: level1_kernels(E),
                "cpu": lambda: cpu_baseline(p)}[args.only]()
         print(json.dumps({"only": args.only, **leg}), flush=True)
@@ -852,6 +864,10 @@ def main():
                      "kernel_ms": main_l,
                      "algorithmic_ops_per_launch": 32.0 * macs},
     }
+    cal = mfma_calibration(E)
+    log(f"MFMA calibration: {cal}")
+    res["roofline"].update(cal)
+    res["roofline"]["frac_of_sustained_random"] = achieved / cal["sustained_random_tops"]
     if not args.no_batch:
         b = batched_echelon(E, args.batch_jobs)
         log(f"batched echelon: {b}")

@@ -177,6 +177,7 @@ SIGNATURES = {
     "hx_gemm_split_workspace_bytes": (_SZ, [_U32, _U32, _U32, _U32]),
     "hx_gemm_u32_noalloc": (_INT, [C.POINTER(hx_modctx), hx_view, hx_view, hx_view, _INT, _V, C.POINTER(hx_gemm_opts)]),
     "hx_gemm_noalloc_scratch_bytes": (_SZ, []),
+    "hx_mfma_rate": (_INT, [_U32, _U32, _INT, _V, C.POINTER(C.c_double)]),
     "hx_schur_update_u32": (_INT, [C.POINTER(hx_modctx), hx_view, hx_view, hx_view, _V, C.POINTER(hx_gemm_opts)]),
     "hx_gemm_batch_create": (_INT, [C.POINTER(hx_modctx), C.POINTER(hx_gemm_job), _U32, _INT, C.POINTER(_V)]),
     "hx_gemm_batch_run": (_INT, [_V, _V]),

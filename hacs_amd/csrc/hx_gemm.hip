@@ -1547,6 +1547,52 @@ GemmArgs args_of(hx_view A, hx_view B, hx_view C, int op) {
     return g;
 }
 
+// Calibration of the limb kernel's instruction: each wave runs `iters` rounds of the 16 products of a
+// 4-limb k-step (a[s] x b[t] onto acc[s + t], the accumulation pattern of limb_tile4) from registers only,
+// then folds its accumulators into one word per lane (out[blockIdx * 256 + tid]).  With `random` set the
+// operands are SplitMix64 bits of (salt, lane, slot), as uniform as the limb planes of random residues;
+// else they are small constants that barely toggle.  The ratio of the two rates is what operand data
+// costs the chip's clock.
+__global__ __launch_bounds__(256) void mfma_rate_kernel(int* out, uint32_t iters, uint64_t salt, int random) {
+    const uint64_t id = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    v4i a[4], b[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        if (random) {
+            uint64_t x = salt ^ (id * 8 + s), y = salt ^ (id * 8 + 4 + s);
+            auto mix = [](uint64_t z) {
+                z += 0x9E3779B97F4A7C15ull;
+                z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+                z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+                return z ^ (z >> 31);
+            };
+            const uint64_t x0 = mix(x), x1 = mix(x0), y0 = mix(y), y1 = mix(y0);
+            a[s] = v4i{(int)x0, (int)(x0 >> 32), (int)x1, (int)(x1 >> 32)};
+            b[s] = v4i{(int)y0, (int)(y0 >> 32), (int)y1, (int)(y1 >> 32)};
+        } else {
+            a[s] = v4i{(int)salt + s, (int)threadIdx.x, s, 1};
+            b[s] = v4i{(int)salt - s, 3, (int)threadIdx.x, 2};
+        }
+    }
+    v16i acc[7];
+#pragma unroll
+    for (int d = 0; d < 7; ++d)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[d][r] = 0;
+    for (uint32_t it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[s + t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[s], b[t], acc[s + t], 0, 0, 0);
+    }
+    int sum = 0;
+#pragma unroll
+    for (int d = 0; d < 7; ++d)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) sum += acc[d][r];
+    out[id] = sum;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1607,6 +1653,38 @@ HX_API int hx_gemm_batch_destroy(hx_gemm_batch b) {
 HX_API int hx_schur_update_u32(const hx_modctx* ctx, hx_view L, hx_view U, hx_view C, hx_stream s,
                                const hx_gemm_opts* opts) {
     return hx_gemm_u32(ctx, L, U, C, HX_OP_SUB, s, opts);
+}
+
+HX_API int hx_mfma_rate(uint32_t blocks, uint32_t iters, int random_operands, hx_stream s, double* tops) {
+    return guard("hx_mfma_rate", [&] {
+        HX_CHECK(tops != nullptr, "NULL argument");
+        HX_CHECK(blocks >= 1 && blocks <= 65536 && iters >= 1 && iters <= 1000000, "mfma rate: bad size");
+        const hipStream_t st = (hipStream_t)s;
+        struct Res {       // released on every exit, the stream drained first
+            hipStream_t st;
+            int* out = nullptr;
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            ~Res() {
+                (void)hipStreamSynchronize(st);
+                if (e0) (void)hipEventDestroy(e0);
+                if (e1) (void)hipEventDestroy(e1);
+                if (out) arena_free(out);
+            }
+        } r{st};
+        r.out = (int*)arena_alloc(sizeof(int) * 256 * (size_t)blocks);
+        HX_HIP(hipEventCreate(&r.e0));
+        HX_HIP(hipEventCreate(&r.e1));
+        mfma_rate_kernel<<<blocks, 256, 0, st>>>(r.out, 4, 1, random_operands);     // first launch, untimed
+        HX_HIP(hipGetLastError());
+        HX_HIP(hipEventRecord(r.e0, st));
+        mfma_rate_kernel<<<blocks, 256, 0, st>>>(r.out, iters, 0x6D66ull, random_operands);
+        HX_HIP(hipGetLastError());
+        HX_HIP(hipEventRecord(r.e1, st));
+        HX_HIP(hipEventSynchronize(r.e1));
+        float ms = 0.f;
+        HX_HIP(hipEventElapsedTime(&ms, r.e0, r.e1));
+        *tops = (double)blocks * 4.0 * iters * 16.0 * 65536.0 / (ms * 1e-3) / 1e12;
+    });
 }
 
 }  // extern "C"

@@ -366,6 +366,15 @@ def gemm_workspace_bytes(ctx: ModCtx, m: int, n: int, k: int, variant: int = HX_
     return lib().hx_gemm_workspace_bytes(C.byref(ctx.c), m, n, k, variant)
 
 
+def mfma_rate(blocks: int = 1024, iters: int = 20000, random_operands: bool = True, stream=None) -> float:
+    """Sustained TOP/s of the limb kernel's int8 MFMA instruction issued from registers only (calibration:
+    the ceiling the limb GEMM can reach with random operands, and the rate the published peak assumes)."""
+    out = C.c_double(0.0)
+    check(lib().hx_mfma_rate(blocks, iters, 1 if random_operands else 0,
+                             current_stream() if stream is None else stream, C.byref(out)))
+    return out.value
+
+
 def random_entries(rows: int, cols: int, ell: int, seed: int, stream=None) -> torch.Tensor:
     """RandomEntries(rows * cols, ell, seed) generated on the device (PerformanceSmokeValidation.cpp:82-89)."""
     t = torch.empty((rows, cols), dtype=torch.int32, device="cuda")

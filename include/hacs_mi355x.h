@@ -239,6 +239,13 @@ HX_API size_t hx_gemm_split_workspace_bytes(uint32_t m, uint32_t n, uint32_t k, 
 HX_API int hx_gemm_u32_noalloc(const hx_modctx *ctx, hx_view A, hx_view B, hx_view C, int op, hx_stream s,
                                const hx_gemm_opts *opts);
 HX_API size_t hx_gemm_noalloc_scratch_bytes(void);
+/* Calibration, no counterpart in the reference: the sustained rate, in TOP/s (65536 ops per instruction), of
+ * v_mfma_i32_32x32x32_i8 issued from registers only in the limb kernel's 16-product pattern, `blocks`
+ * workgroups of 256 threads each running `iters` rounds, timed with events on stream s (one untimed launch
+ * first).  random_operands = 1 feeds uniformly random bits, as the limb planes of random residues are; 0 feeds
+ * small constants.  The first is the ceiling the limb GEMM can reach on this chip; the published peak
+ * assumes the clock the second keeps.  Blocks 1..65536, iters 1..1000000. */
+HX_API int hx_mfma_rate(uint32_t blocks, uint32_t iters, int random_operands, hx_stream s, double *tops);
 /* C -= L * U (schur_update, CudaKernels.h:130-133). */
 HX_API int hx_schur_update_u32(const hx_modctx *ctx, hx_view L, hx_view U, hx_view C, hx_stream s,
                                const hx_gemm_opts *opts);

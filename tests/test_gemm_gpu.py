@@ -181,3 +181,15 @@ def test_limb_kernels_all_copy_paths(hx_gpu, O, p, tile):
         E.gemm(ctx, E.to_device(A), E.to_device(B), dC, E.GemmOp.Set, variant=E.HX_GEMM_LIMB, tile=tile)
         torch.cuda.synchronize()
         assert np.array_equal(E.from_device(dC), O.gemm_fast(p, A, B))
+
+
+def test_mfma_rate_calibration(hx_gpu):
+    """hx_mfma_rate: a finite positive rate below the published 5000 TOP/s dense INT8 figure for both operand
+    kinds, and out-of-range sizes rejected."""
+    from hacs_amd import engine as E
+    for rnd in (True, False):
+        r = E.mfma_rate(512, 200, rnd)
+        assert 0.0 < r < 5000.0 * 1.05, (rnd, r)
+    for blocks, iters in ((0, 10), (65537, 10), (8, 0)):
+        with pytest.raises(E.HxError):
+            E.mfma_rate(blocks, iters)
