@@ -2,7 +2,9 @@
 """Benchmark of the exact mod-p matrix engine on one MI355X.
 
 Headline workload (one "step"): C = A * B mod 2147483629 with A, B 8192 x 8192 uint32
-residues (the 8192^3 GEMM configuration), computed by the limb-MFMA kernel; inputs are
+residues (the 8192^3 GEMM configuration), computed by the library's default route (the residue
+route: ten int8 MFMA products per multiply-add modulo small coprime moduli and a CRT pass, in place
+of the limb kernels' sixteen; every limb kernel is timed beside it on the same inputs); inputs are
 generated on the device (SplitMix64 / SeedFor, salts 0x01 / 0x02) and stay resident in
 HBM.  The metric is exact mod-p multiply-adds per second (n^3 per step).  The same run
 also times the plain VALU kernel on the same inputs (limb vs plain), checks the two
@@ -28,9 +30,19 @@ sys.path.insert(0, ROOT)
 
 N_C3 = 8192
 PEAK_INT8_TOPS = 5000.0      # MI355X dense INT8 MFMA, tera-ops/s
-LIMB_DEFAULT = 4             # the limb kernel hx_gemm_u32 runs when opts.tile is 0
 LIMB_KERNELS = {1: "limb_kernel<%d>", 2: "limb2_kernel<%d>", 3: "limb3_kernel<%d>", 4: "limb4_kernel<%d>",
-                5: "limb5_kernel<%d>"}
+                5: "limb5_kernel<%d>", 6: "res_gemm_kernel<4, 2>"}
+RESIDUE_TILE = 6             # the residue route: one int8 product per small modulus, CRT into C
+
+
+def kernel_name(tile, limbs):
+    k = LIMB_KERNELS[tile]
+    return k % limbs if "%d" in k else k
+
+
+def int8_products(E, tile, p, k, limbs):
+    """int8 MFMA products per multiply-add: limbs^2 for the limb kernels, the moduli count for the residues."""
+    return E.residue_moduli(p, k) if tile == RESIDUE_TILE else limbs * limbs
 PEAK_HBM_GBS = 8000.0
 
 
@@ -811,12 +823,13 @@ def main():
     B = E.random_entries(n, n, p, W.seed_for(n, n, p, 0x02))
     C_limb = torch.empty((n, n), dtype=torch.int32, device="cuda")
     C_plain = torch.empty((n, n), dtype=torch.int32, device="cuda")
-    ws = torch.empty(E.gemm_workspace_bytes(ctx, n, n, n, E.HX_GEMM_LIMB), dtype=torch.uint8, device="cuda")
+    ws = torch.empty(max(E.gemm_workspace_bytes(ctx, n, n, n, E.HX_GEMM_LIMB, t) for t in LIMB_KERNELS),
+                     dtype=torch.uint8, device="cuda")
     torch.cuda.synchronize()
 
     wall_l, main_l = gemm_timed(E, ctx, A, B, C_limb, E.HX_GEMM_LIMB, args.steps, args.warmup, ws, args.tile)
     log(f"limb: {wall_l:.3f} ms/step (main kernel {main_l:.3f} ms)")
-    kver = args.tile if args.tile in LIMB_KERNELS else LIMB_DEFAULT
+    kver = E.gemm_route(ctx, n, n, n, E.HX_GEMM_LIMB, args.tile)      # the kernel the headline ran
     others, same_alt = [], True
     C_alt = torch.empty((n, n), dtype=torch.int32, device="cuda")
     for alt in LIMB_KERNELS:
@@ -825,7 +838,8 @@ def main():
         _, main_alt = gemm_timed(E, ctx, A, B, C_alt, E.HX_GEMM_LIMB, 5, 1, ws, alt)
         same = bool(torch.equal(C_limb, C_alt))
         same_alt = same_alt and same
-        others.append({"tile": alt, "kernel": LIMB_KERNELS[alt] % ctx.limbs, "kernel_ms": main_alt})
+        others.append({"tile": alt, "kernel": kernel_name(alt, ctx.limbs), "kernel_ms": main_alt,
+                       "int8_products_per_mac": int8_products(E, alt, p, n, ctx.limbs)})
         log(f"other limb kernel (tile {alt}): main kernel {main_alt:.3f} ms; same bytes: {same}")
     del C_alt
     wall_p, main_p = gemm_timed(E, ctx, A, B, C_plain, E.HX_GEMM_PLAIN, args.plain_steps, 1, None)
@@ -835,8 +849,9 @@ def main():
     log(f"limb == plain: {same}; spot entries exact: {spot}")
     macs = float(n) ** 3
     value = macs / (wall_l * 1e-3)
-    kname = LIMB_KERNELS[kver] % ctx.limbs
-    achieved = 2.0 * 16.0 * macs / (main_l * 1e-3) / 1e12
+    kname = kernel_name(kver, ctx.limbs)
+    prods = int8_products(E, kver, p, n, ctx.limbs)
+    achieved = 2.0 * prods * macs / (main_l * 1e-3) / 1e12
     res = {
         "metric": "exact mod-p GEMM mult-adds/s at n=8192 (limb-MFMA vs plain); batched LU jobs/s",
         "value": value,
@@ -851,7 +866,10 @@ def main():
         "dtype": "int8",
         "data": "synthetic",
         "config": {"workload": f"GEMM mod {p}, m=n=k={n}, uint32 residues (SplitMix64 salts 0x01/0x02), "
-                               "limb-MFMA kernel, inputs resident in HBM",
+                               + ("residue route (%d int8 MFMA products per multiply-add modulo small coprime "
+                                  "moduli, CRT into C)" % prods if kver == RESIDUE_TILE else
+                                  "limb-MFMA kernel (%d int8 products per multiply-add)" % prods)
+                               + ", inputs resident in HBM",
                    "modulus": p, "n": n, "limbs": ctx.limbs, "parallelism": "single GPU"},
         "plain_value": macs / (wall_p * 1e-3),
         "plain_ms_per_step": wall_p,
@@ -862,7 +880,8 @@ def main():
                      "frac": achieved / PEAK_INT8_TOPS, "traffic": profiled_traffic(n, ctx.limbs, kname),
                      "kernel": kname,
                      "kernel_ms": main_l,
-                     "algorithmic_ops_per_launch": 32.0 * macs},
+                     "algorithmic_ops_per_launch": 2.0 * prods * macs,
+                     "int8_products_per_mac": prods},
     }
     cal = mfma_calibration(E)
     log(f"MFMA calibration: {cal}")

@@ -159,6 +159,9 @@ SIGNATURES = {
     "hx_graph_destroy": (_INT, [_V]),
     "hx_gemm_u32": (_INT, [C.POINTER(hx_modctx), hx_view, hx_view, hx_view, _INT, _V, C.POINTER(hx_gemm_opts)]),
     "hx_gemm_workspace_bytes": (_SZ, [C.POINTER(hx_modctx), _U32, _U32, _U32, _INT]),
+    "hx_gemm_workspace_bytes_tile": (_SZ, [C.POINTER(hx_modctx), _U32, _U32, _U32, _INT, _INT]),
+    "hx_gemm_residue_moduli": (_U32, [_U32, _U32]),
+    "hx_gemm_route": (_INT, [C.POINTER(hx_modctx), _U32, _U32, _U32, _INT, _INT]),
     "hx_modctx64_init": (_INT, [_U64, C.POINTER(hx_modctx64)]),
     "hx_gemm_u64": (_INT, [C.POINTER(hx_modctx64), hx_view, hx_view, hx_view, _INT, _V]),
     "hx_gemm_limb_k_chunk": (_U32, [C.POINTER(hx_modctx), _U32, _U32, _U32, _SZ]),

@@ -1022,12 +1022,344 @@ void limb_attrs() {
     });
 }
 
+// ------------------------------------------------------------------------------------------
+// residue route: the product mod p through N small moduli (multi-modular int8 GEMM + CRT)
+// ------------------------------------------------------------------------------------------
+// The operands' balanced residues a_b in [-p/2, p/2] are reduced modulo N pairwise coprime moduli
+// q_i <= 255 into balanced bytes (|r| <= 127).  Each modulus is one plain int8 product on the MFMAs
+// whose int32 sums (|sum| <= k * 127^2 < 2^31 for k <= 133144) are reduced to bytes in [0, q_i); a
+// CRT pass then turns the N bytes into the exact sum's residue mod p.  The exact sum x satisfies
+// |x| <= k (p/2)^2, and N is the least count with prod q_i > 8 k (p/2)^2: then x / prod q_i lies in
+// [-1/8, 1/8] and the CRT's quotient estimate rounds exactly.  For p < 2^32 and k <= 8192 that is
+// N = 10 int8 products against the limb route's L^2 = 16 (the same decomposition as Ozaki, Uchino and
+// Imamura, "Ozaki Scheme II: a GEMM-oriented emulation of floating-point matrix multiplication using
+// an integer modular technique", 2025, here for exact residues).
+constexpr int RES_MAXN = 12;
+constexpr uint32_t RES_KMAX = 133144;            // k * 127^2 <= 2^31 - 1
+__host__ __device__ constexpr uint32_t res_modulus(int i) {
+    // pairwise coprime: 3*5*17, 2*127, 11*23, 251, 13*19, then primes
+    return i == 0 ? 255u : i == 1 ? 254u : i == 2 ? 253u : i == 3 ? 251u : i == 4 ? 247u : i == 5 ? 241u
+         : i == 6 ? 239u : i == 7 ? 233u : i == 8 ? 229u : i == 9 ? 227u : i == 10 ? 223u : 211u;
+}
+
+// The operand tile of prep_tile (64 x 64 of the output-row x k layout) into T; zero past R x K.
+__device__ __forceinline__ void prep_load(const PrepArgs& a, uint32_t r0, uint32_t c0, uint32_t (*T)[65]) {
+    const uint32_t R = a.r_dev ? min(*a.r_dev, a.R) : a.R;
+    const uint32_t K = a.k_dev ? min(*a.k_dev, a.K) : a.K;
+    const int tid = threadIdx.x;
+    const bool ccontig = a.s_c == 1 && a.clist == nullptr;
+#pragma unroll 4
+    for (int t = 0; t < 16; ++t) {
+        const int e = tid + 256 * t;
+        const uint32_t r = ccontig ? e / 64 : e % 64, c = ccontig ? e % 64 : e / 64;
+        uint32_t v = 0;
+        if (r0 + r < R && c0 + c < K) {
+            const int64_t sr = a.rlist ? (int64_t)a.rlist[r0 + r] : (int64_t)(r0 + r);
+            const int64_t sc = a.clist ? (int64_t)a.clist[c0 + c] : (int64_t)(c0 + c);
+            v = a.src[sr * a.s_r + sc * a.s_c];
+        }
+        T[r][c] = v;
+    }
+}
+
+// p mod q_i for the sign fold of the balanced residues (a > p/2 stands for a - p).
+struct ResPrepConsts {
+    uint32_t nmod;
+    uint32_t pmod[RES_MAXN];
+};
+
+// One 64 x 64 (rows x k) tile of the N residue planes per workgroup: every thread turns 16 k-entries
+// of one row into N 16-byte strips of balanced residues.
+__global__ __launch_bounds__(256) void res_prep_kernel(PrepArgs a, ResPrepConsts rc) {
+    __shared__ uint32_t T[64][65];
+    const uint32_t r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+    prep_load(a, r0, c0, T);
+    __syncthreads();
+    const int tid = threadIdx.x, r = tid / 4, qd = tid % 4;
+    uint32_t x[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) x[j] = T[r][16 * qd + j];
+    int8_t* out = a.out + (size_t)(r0 + r) * a.Kp + c0 + 16 * qd;
+#pragma unroll
+    for (int i = 0; i < RES_MAXN; ++i) {
+        if (i >= (int)rc.nmod) break;
+        const uint32_t Q = res_modulus(i), fold = Q - rc.pmod[i];
+        v4i vw;
+#pragma unroll
+        for (int wd = 0; wd < 4; ++wd) {
+            uint32_t w = 0;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const uint32_t v = x[4 * wd + b];
+                uint32_t u = v % Q + (v > a.half ? fold : 0u);          // a_b mod Q in [0, 2Q)
+                u = u >= Q ? u - Q : u;
+                const uint32_t sb = u > Q / 2 ? u - Q : u;               // balanced, as a byte
+                w |= (sb & 0xffu) << (8 * b);
+            }
+            vw[wd] = (int)w;
+        }
+        *(v4i*)(out + (size_t)i * a.plane) = vw;
+    }
+}
+
+// One modulus' int8 product on a 256 x 256 tile: the limb_tile4 pipeline (eight waves, three-stage
+// direct-to-LDS ring of 64-deep k-strips, fragments double-buffered against the MFMAs) with the
+// operand "planes" being four 64-row blocks of A and two 128-row blocks of B of one residue plane,
+// so each wave owns 4 x 2 separate 32 x 32 accumulator blocks (rows 64 s + 32 wm, columns
+// 128 t + 32 wn) and reads 6 fragments per 8 MFMAs.  The epilogue reduces every int32 sum to its
+// residue in [0, q) and stores it as a byte of the modulus' residue plane.
+// The instance the route runs: eight waves (two per SIMD, 4 x 2 blocks each), a ring of three stages.
+// (Measured at 8192^3, 10 moduli: a ring of 3, 4 or 5 stages 5.37 .. 5.43 ms; four waves with 4 x 4 blocks
+// each, a third less LDS reading per MFMA, 5.55 .. 5.93 ms.)
+constexpr int RES_WAVES = 8, RES_STAGES = 3;
+constexpr int RLA = 4, RLB = 2, RTM = RLA * LM, RTN = RLB * 16 * RES_WAVES;
+
+struct ResArgs {
+    const int8_t* A8;            // residue plane i of A at A8 + i * a_plane: Mp rows of Kp bytes
+    const int8_t* B8;            // residue plane i of B at B8 + i * b_plane: Np rows of Kp bytes
+    size_t a_plane, b_plane;
+    uint32_t Kp;
+    uint32_t ntm, ntn;
+    uint8_t* R;                  // product residues of modulus i at R + i * r_plane: Mp rows of ldr bytes
+    size_t r_plane;
+    uint32_t ldr;
+    uint32_t q[RES_MAXN];
+    uint32_t y[RES_MAXN];        // (M / q_i)^-1 mod q_i, the CRT factor of modulus i
+    double invq[RES_MAXN];
+};
+
+// B blocks hold 16 * WAVES rows (WAVES / 2 waves across, 32 columns each); A blocks 64 rows (two waves down).
+template <int LA, int LB, int WAVES>
+__device__ __forceinline__ void res_stage_issue(const ResArgs& a, const int8_t* Abase, const int8_t* Bbase, int wave,
+                                                uint32_t kt, uint8_t* st) {
+    constexpr int BN = 16 * WAVES, BG = BN / 16;
+    constexpr int SA = LA * LM * 64, NBLK = 4 * LA + BG * LB, PER = (NBLK + WAVES - 1) / WAVES;
+    const size_t k0 = (size_t)kt * LK;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int blk = wave + WAVES * i;
+        if (blk >= NBLK) break;
+        const int8_t* src;
+        uint8_t* dst;
+        if (blk < 4 * LA) {
+            const int s = blk >> 2, g = blk & 3;
+            src = Abase + (size_t)(s * LM + 16 * g) * a.Kp + k0;
+            dst = st + s * (LM * 64) + g * 1024;
+        } else {
+            const int s = (blk - 4 * LA) / BG, g = (blk - 4 * LA) % BG;
+            src = Bbase + (size_t)(s * BN + 16 * g) * a.Kp + k0;
+            dst = st + SA + s * (BN * 64) + g * 1024;
+        }
+        __builtin_amdgcn_global_load_lds((const void*)src, (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
+    }
+}
+
+template <int LA, int LB>
+struct RFrag {
+    v4i a[LA];
+    v4i b[LB];
+};
+
+template <int LA, int LB, int WAVES>
+__device__ __forceinline__ void res_stage_frag(const uint8_t* st, int ks, int wm, int wn, int lane, RFrag<LA, LB>& f) {
+    constexpr int SA = LA * LM * 64, BN = 16 * WAVES;
+    const int lr = lane & 31, q = ks * 2 + (lane >> 5);
+    const int ra = wm * 32 + lr, rb = wn * 32 + lr;
+#pragma unroll
+    for (int s = 0; s < LA; ++s) f.a[s] = *(const v4i*)(st + s * (LM * 64) + ra * 64 + 16 * (q ^ swz(ra)));
+#pragma unroll
+    for (int t = 0; t < LB; ++t) f.b[t] = *(const v4i*)(st + SA + t * (BN * 64) + rb * 64 + 16 * (q ^ swz(rb)));
+}
+
+template <int LA, int LB>
+__device__ __forceinline__ void res_mfma(const RFrag<LA, LB>& f, v16i (&acc)[LA][LB]) {
+#pragma unroll
+    for (int s = 0; s < LA; ++s)
+#pragma unroll
+        for (int t = 0; t < LB; ++t)
+            acc[s][t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(f.a[s], f.b[t], acc[s][t], 0, 0, 0);
+}
+
+template <int LA, int LB, int NS, int WAVES>
+__device__ __forceinline__ void res_tile(const ResArgs& a, uint32_t tm, uint32_t tn, uint32_t mi) {
+    constexpr int BN = 16 * WAVES, SA = LA * LM * 64, SB = LB * BN * 64, STAGE = SA + SB;
+    constexpr int NBLK = 4 * LA + (BN / 16) * LB;
+    constexpr int HI = (NBLK + WAVES - 1) / WAVES, LO = NBLK / WAVES, NHI = NBLK % WAVES == 0 ? WAVES : NBLK % WAVES;
+    static_assert(NS >= 3 && NS <= 5 && NS * STAGE <= 160 * 1024, "ring of 3..5 stages within the LDS");
+    __shared__ __attribute__((aligned(1024))) uint8_t s0[STAGE];
+    __shared__ __attribute__((aligned(1024))) uint8_t s1[STAGE];
+    __shared__ __attribute__((aligned(1024))) uint8_t s2[STAGE];
+    __shared__ __attribute__((aligned(1024))) uint8_t s3[NS > 3 ? STAGE : 1];
+    __shared__ __attribute__((aligned(1024))) uint8_t s4[NS > 4 ? STAGE : 1];
+    const uint32_t m0 = tm * (LA * LM), n0 = tn * (LB * BN);
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm = wave & 1, wn = wave >> 1;
+    const bool hi = wave < NHI;
+
+    v16i acc[LA][LB];
+#pragma unroll
+    for (int s = 0; s < LA; ++s)
+#pragma unroll
+        for (int t = 0; t < LB; ++t)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[s][t][r] = 0;
+
+    const int br = lane >> 2, bq = (lane & 3) ^ swz(br);
+    const int8_t* Abase = a.A8 + mi * a.a_plane + (size_t)(m0 + br) * a.Kp + 16 * bq;
+    const int8_t* Bbase = a.B8 + mi * a.b_plane + (size_t)(n0 + br) * a.Kp + 16 * bq;
+    // wait until at most `stages` stages of this wave's copies are in flight (0 .. NS - 1)
+    auto wait_stages = [&](int stages) {
+        const int per = hi ? HI : LO;
+        switch (stages * per) {
+#define HX_RES_WAIT(c) case c: __builtin_amdgcn_s_waitcnt(wait_imm(c, 15)); break;
+            HX_RES_WAIT(0) HX_RES_WAIT(1) HX_RES_WAIT(2) HX_RES_WAIT(3) HX_RES_WAIT(4) HX_RES_WAIT(5)
+            HX_RES_WAIT(6) HX_RES_WAIT(7) HX_RES_WAIT(8) HX_RES_WAIT(9) HX_RES_WAIT(10) HX_RES_WAIT(11)
+            HX_RES_WAIT(12) HX_RES_WAIT(13) HX_RES_WAIT(14) HX_RES_WAIT(15) HX_RES_WAIT(16)
+#undef HX_RES_WAIT
+            default: __builtin_amdgcn_s_waitcnt(wait_imm(0, 15)); break;
+        }
+    };
+    const uint32_t KT = a.Kp / LK;
+    if (KT > 0) {
+        // prologue: stages 0 .. NS-1 in flight, stage 0 landed everywhere
+        res_stage_issue<LA, LB, WAVES>(a, Abase, Bbase, wave, 0, s0);
+        if (KT > 1) res_stage_issue<LA, LB, WAVES>(a, Abase, Bbase, wave, 1, s1);
+        if (KT > 2) res_stage_issue<LA, LB, WAVES>(a, Abase, Bbase, wave, 2, s2);
+        if (NS > 3 && KT > 3) res_stage_issue<LA, LB, WAVES>(a, Abase, Bbase, wave, 3, s3);
+        if (NS > 4 && KT > 4) res_stage_issue<LA, LB, WAVES>(a, Abase, Bbase, wave, 4, s4);
+        wait_stages((int)min(KT - 1, (uint32_t)NS - 1));
+        wg_barrier();
+        RFrag<LA, LB> fa, fb;
+        res_stage_frag<LA, LB, WAVES>(s0, 0, wm, wn, lane, fa);
+        __builtin_amdgcn_s_waitcnt(wait_imm(63, 0));
+        // step kt: MFMAs of (kt, ks0) while (kt, ks1) is read; stage kt+1 landed and every wave done
+        // with stage kt (barrier); stage kt+NS copied into kt's slot; MFMAs of (kt, ks1) while
+        // (kt+1, ks0) is read.  Before the barrier stages kt+2 .. min(KT-1, kt+NS-1) may be in flight.
+        auto body = [&](uint32_t kt, uint8_t* cur, const uint8_t* nxt) {
+            res_stage_frag<LA, LB, WAVES>(cur, 1, wm, wn, lane, fb);
+            __builtin_amdgcn_sched_barrier(0);
+            res_mfma<LA, LB>(fa, acc);
+            __builtin_amdgcn_sched_barrier(0);
+            __builtin_amdgcn_s_waitcnt(wait_imm(63, 0));
+            wait_stages((int)min(KT - kt - 2, (uint32_t)NS - 2));
+            wg_barrier();
+            if (kt + NS < KT) res_stage_issue<LA, LB, WAVES>(a, Abase, Bbase, wave, kt + NS, cur);
+            res_stage_frag<LA, LB, WAVES>(nxt, 0, wm, wn, lane, fa);
+            __builtin_amdgcn_sched_barrier(0);
+            res_mfma<LA, LB>(fb, acc);
+            __builtin_amdgcn_sched_barrier(0);
+            __builtin_amdgcn_s_waitcnt(wait_imm(63, 0));
+        };
+        uint8_t* const ring[5] = {s0, s1, s2, s3, s4};
+        for (uint32_t kt = 0; kt + 1 < KT; kt += NS) {
+#pragma unroll
+            for (int j = 0; j < NS; ++j)
+                if (kt + j + 1 < KT) body(kt + j, ring[j], ring[(j + 1) % NS]);
+        }
+        const uint32_t ls = (KT - 1) % NS;
+        const uint8_t* cur = ls == 0 ? s0 : ls == 1 ? s1 : ls == 2 ? s2 : ls == 3 ? s3 : s4;
+        res_stage_frag<LA, LB, WAVES>(cur, 1, wm, wn, lane, fb);
+        res_mfma<LA, LB>(fa, acc);
+        res_mfma<LA, LB>(fb, acc);
+    }
+    // s = v y mod q (y = (M / q)^-1 mod q, the CRT's factor, folded in here): t = floor(v y / q) from a
+    // double (v y < 2^39 is exact there; the quotient's rounding error stays far below the 1/q spacing of
+    // the fractions, so t is exact but for a true multiple of q, where it may come out one low: then
+    // v y - t q = q, folded back)
+    const int32_t Q = (int32_t)a.q[mi], Y = (int32_t)a.y[mi];
+    const double iq = a.invq[mi];
+    uint8_t* R = a.R + mi * a.r_plane;
+    const int lr = lane & 31, lg = lane >> 5;
+#pragma unroll
+    for (int s = 0; s < LA; ++s)
+#pragma unroll
+        for (int t = 0; t < LB; ++t) {
+            const uint32_t col = n0 + t * BN + wn * 32 + lr;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const uint32_t row = m0 + s * LM + wm * 32 + (r / 4) * 8 + lg * 4 + (r % 4);
+                const int64_t vy = (int64_t)acc[s][t][r] * Y;
+                const int64_t qt = (int64_t)floor((double)vy * iq);
+                int32_t u = (int32_t)(vy - qt * Q);
+                u = u >= Q ? u - Q : u;
+                u = u < 0 ? u + Q : u;
+                R[(size_t)row * a.ldr + col] = (uint8_t)u;
+            }
+        }
+}
+
+template <int LA, int LB, int NS, int WAVES>
+__global__ __launch_bounds__(WAVES * 64, 1) void res_gemm_kernel(ResArgs a) {
+    uint32_t tm, tn;
+    tile_of(blockIdx.x, a.ntm, a.ntn, tm, tn);
+    res_tile<LA, LB, NS, WAVES>(a, tm, tn, blockIdx.y);
+}
+
+// CRT of the N product residues of four adjacent outputs per thread into x mod p, then op into C.
+// The product kernel stored s_i = v y_i mod q_i (y_i = (M / q_i)^-1 mod q_i, M = prod q_i), so
+// sum_i s_i M / q_i = x + T M with T = round(sum_i s_i / q_i) (that sum is T + x / M, |x / M| <= 1/8),
+// and x = sum_i s_i (M / q_i mod p) - T (M mod p) (mod p).  All in 24-bit products: c_i = M / q_i mod p
+// split into 16-bit halves (the two partial sums stay below 12 * 255 * 2^16 < 2^28), the fraction sum in
+// fixed point with f_i = round(2^20 / q_i) (total error <= 12 * 255 / 2 units of 2^-20 < 2^-9, far inside
+// the 1/8 margin).
+struct CrtArgs {
+    const uint8_t* R;
+    size_t r_plane;
+    uint32_t ldr;
+    uint32_t nmod;
+    uint32_t ch[RES_MAXN], cl[RES_MAXN];   // (M / q_i) mod p = ch 2^16 + cl
+    uint32_t f[RES_MAXN];                  // round(2^20 / q_i)
+    uint32_t pmm;                          // p - (M mod p)
+    uint32_t m, n;
+    const uint32_t* Cin;
+    int64_t cirs, cics;
+    const uint32_t* c_rows;
+    uint32_t* Cout;
+    int64_t cors, cocs;
+    int op;
+    DevMod M;
+};
+
+__global__ __launch_bounds__(256) void crt_kernel(CrtArgs a) {
+    const uint32_t col0 = (blockIdx.x * 64 + (threadIdx.x & 63)) * 4;
+    const uint32_t row = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (row >= a.m || col0 >= a.n) return;
+    uint32_t hi[4] = {0, 0, 0, 0}, lo[4] = {0, 0, 0, 0}, fr[4] = {0, 0, 0, 0};
+    const uint8_t* src = a.R + (size_t)row * a.ldr + col0;
+#pragma unroll
+    for (int i = 0; i < RES_MAXN; ++i) {
+        if (i >= (int)a.nmod) break;
+        const uint32_t w = *(const uint32_t*)(src + (size_t)i * a.r_plane);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const uint32_t sb = (w >> (8 * b)) & 0xffu;
+            hi[b] += __umul24(sb, a.ch[i]);
+            lo[b] += __umul24(sb, a.cl[i]);
+            fr[b] += __umul24(sb, a.f[i]);
+        }
+    }
+    const int64_t rr = a.c_rows ? (int64_t)a.c_rows[row] : (int64_t)row;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const uint32_t col = col0 + b;
+        if (col >= a.n) break;
+        const uint32_t T = (fr[b] + (1u << 19)) >> 20;
+        const uint32_t res = red64(((uint64_t)hi[b] << 16) + lo[b] + (uint64_t)T * a.pmm, a.M);
+        uint32_t c = 0;
+        if (a.op != HX_OP_SET) c = a.Cin[rr * a.cirs + (int64_t)col * a.cics];
+        a.Cout[(int64_t)row * a.cors + (int64_t)col * a.cocs] = apply_op(c, res, a.op, a.M.p);
+    }
+}
+
 // Limb kernel choice (hx_gemm_opts.tile for the limb variant): 1 = register-staged copy
 // (limb_kernel), 2 = direct-to-LDS copy ring (limb2_kernel), 3 = the ring with the LDS reads
 // pipelined against the MFMAs (limb3_kernel), 4 = limb3's pipeline on eight waves
 // (limb4_kernel), 5 = the 96 x 128 tile on twelve waves (limb5_kernel); 0 = the default.
 constexpr int kLimbDefault = 4;
 inline int limb_version(int tile) { return tile >= 1 && tile <= 5 ? tile : kLimbDefault; }
+// tile 6 of the limb variant: the residue route (res_gemm_kernel) in place of the limb kernels
+constexpr int kResidueTile = 6;
 
 template <int L>
 void launch_limb(const LimbArgs& la, hipStream_t s, int ver) {
@@ -1238,6 +1570,142 @@ void gemm_limb(const hx_modctx& ctx, const GemmArgs& g0, hipStream_t s, void* ws
     } while (k0 < g.k);
 }
 
+// ------------------------------------------------------------------------------------------
+// residue route (host side)
+// ------------------------------------------------------------------------------------------
+// The least N with prod_{i<N} q_i > 8 k (p/2)^2; 0 when k is 0 or past RES_KMAX (int32 sums) or when
+// RES_MAXN moduli do not reach the bound.
+uint32_t residue_moduli(uint32_t p, uint32_t k) {
+    if (k == 0 || k > RES_KMAX) return 0;
+    const unsigned __int128 h = p / 2, bound = (unsigned __int128)8 * k * h * h;
+    unsigned __int128 M = 1;
+    for (int i = 0; i < RES_MAXN; ++i) {
+        M *= res_modulus(i);
+        if (M > bound) return (uint32_t)i + 1;
+    }
+    return 0;
+}
+
+size_t residue_workspace_bytes(uint32_t m, uint32_t n, uint32_t k, uint32_t nmod) {
+    const size_t Mp = round_up(std::max<uint32_t>(m, 1), RTM), Np = round_up(std::max<uint32_t>(n, 1), RTN);
+    const size_t Kp = round_up(std::max<uint32_t>(k, 1), 64);
+    return (size_t)nmod * (Mp * Kp + Np * Kp + Mp * Np);
+}
+
+// The moduli count of the residue route when the limb variant takes it for this product, else 0.  It
+// needs host-side sizes, 1 <= k <= RES_KMAX and enough moduli; tile kResidueTile asks for it wherever it
+// applies; the default (tile 0) takes it where it runs at most L^2 - 4 int8 products (four-limb moduli:
+// 10 or 11 for 16) and m, n, k >= 4096, where the saved MFMA work pays for the operand splits and the CRT
+// pass (mod 2147483629, wall ms limb kernel 4 / residue route: 8192^3 7.76 / 6.26, 4096^3 1.13 / 1.01,
+// 8192^2 x 2048 2.05 / 2.11, 2048^2 x 16384 1.14 / 1.18, 2048^3 0.17 / 0.24).
+uint32_t residue_choice(const hx_modctx& ctx, uint32_t m, uint32_t n, uint32_t k, bool device_sizes, int tile) {
+    if (device_sizes || (tile != 0 && tile != kResidueTile)) return 0;
+    const uint32_t N = residue_moduli(ctx.p, k);
+    if (N == 0 || tile == kResidueTile) return N;
+    const bool large = m >= 4096 && n >= 4096 && k >= 4096;
+    return large && ctx.limbs > 0 && N + 4 <= ctx.limbs * ctx.limbs ? N : 0;
+}
+
+// C op= A B mod p by the residue route with N moduli: residue planes of A and B (res_prep_kernel),
+// one int8 product per modulus (res_gemm_kernel, grid.y = modulus), CRT into C (crt_kernel).
+// ws holds N (Mp Kp + Np Kp + Mp Np) bytes (Mp, Np: m, n rounded up to 256; Kp: k to 64).
+void gemm_residue(const hx_modctx& ctx, const GemmArgs& g, hipStream_t s, uint32_t N, void* ws, size_t ws_bytes,
+                  hipEvent_t ev0, hipEvent_t ev1) {
+    if (g.m == 0 || g.n == 0) return;
+    HX_CHECK(N >= 1 && N <= (uint32_t)RES_MAXN && g.m_dev == nullptr && g.k_dev == nullptr && g.k >= 1,
+             "residue route: no plan for this product");
+    HX_CHECK(ws != nullptr && ws_bytes >= residue_workspace_bytes(g.m, g.n, g.k, N),
+             "residue route: workspace too small");
+    const uint32_t p = ctx.p;
+    const uint32_t Mp = round_up(g.m, RTM), Np = round_up(g.n, RTN), Kp = round_up(g.k, 64);
+    int8_t* A8 = (int8_t*)ws;
+    int8_t* B8 = A8 + (size_t)N * Mp * Kp;
+    uint8_t* R = (uint8_t*)(B8 + (size_t)N * Np * Kp);
+
+    ResPrepConsts rc{};
+    rc.nmod = N;
+    for (uint32_t i = 0; i < N; ++i) rc.pmod[i] = p % res_modulus((int)i);
+    PrepArgs pa{};
+    pa.src = g.A;
+    pa.s_r = g.ars;
+    pa.s_c = g.acs;
+    pa.rlist = g.a_rows;
+    pa.clist = g.a_cols;
+    pa.R = g.m;
+    pa.K = g.k;
+    pa.Kp = Kp;
+    pa.out = A8;
+    pa.plane = (size_t)Mp * Kp;
+    pa.p = p;
+    pa.half = (p - 1) / 2;
+    res_prep_kernel<<<dim3(Kp / 64, Mp / 64), 256, 0, s>>>(pa, rc);
+    after_launch("res_prep_kernel(A)");
+    PrepArgs pb = pa;
+    pb.src = g.B;
+    pb.s_r = g.bcs;
+    pb.s_c = g.brs;
+    pb.rlist = nullptr;
+    pb.clist = nullptr;
+    pb.R = g.n;
+    pb.out = B8;
+    pb.plane = (size_t)Np * Kp;
+    res_prep_kernel<<<dim3(Kp / 64, Np / 64), 256, 0, s>>>(pb, rc);
+    after_launch("res_prep_kernel(B)");
+
+    ResArgs ra{};
+    ra.A8 = A8;
+    ra.B8 = B8;
+    ra.a_plane = (size_t)Mp * Kp;
+    ra.b_plane = (size_t)Np * Kp;
+    ra.Kp = Kp;
+    ra.ntm = Mp / RTM;
+    ra.ntn = Np / RTN;
+    ra.R = R;
+    ra.r_plane = (size_t)Mp * Np;
+    ra.ldr = Np;
+    CrtArgs ca{};
+    unsigned __int128 M = 1;
+    for (uint32_t i = 0; i < N; ++i) M *= res_modulus((int)i);
+    for (uint32_t i = 0; i < N; ++i) {
+        const uint32_t q = res_modulus((int)i);
+        ra.q[i] = q;
+        ra.invq[i] = 1.0 / (double)q;
+        const unsigned __int128 Mi = M / q;
+        const uint32_t mq = (uint32_t)(Mi % q);
+        uint32_t y = 1;
+        while (y < q && (mq * y) % q != 1) ++y;              // q <= 255 and gcd(M / q, q) = 1
+        HX_CHECK(y < q, "residue route: moduli not coprime");
+        ra.y[i] = y;
+        const uint32_t c = (uint32_t)(Mi % p);
+        ca.ch[i] = c >> 16;
+        ca.cl[i] = c & 0xffffu;
+        ca.f[i] = (uint32_t)(((1u << 20) + q / 2) / q);
+    }
+    ca.pmm = p - (uint32_t)(M % p);
+    if (ev0) HX_HIP(hipEventRecord(ev0, s));
+    res_gemm_kernel<RLA, RLB, RES_STAGES, RES_WAVES><<<dim3(ra.ntm * ra.ntn, N), RES_WAVES * 64, 0, s>>>(ra);
+    after_launch("res_gemm_kernel");
+    if (ev1) HX_HIP(hipEventRecord(ev1, s));
+
+    ca.R = R;
+    ca.r_plane = ra.r_plane;
+    ca.ldr = Np;
+    ca.nmod = N;
+    ca.m = g.m;
+    ca.n = g.n;
+    ca.Cin = g.Cin;
+    ca.cirs = g.cirs;
+    ca.cics = g.cics;
+    ca.c_rows = g.c_rows;
+    ca.Cout = g.Cout;
+    ca.cors = g.cors;
+    ca.cocs = g.cocs;
+    ca.op = g.op;
+    ca.M = devmod(ctx);
+    crt_kernel<<<dim3((g.n + 255) / 256, (g.m + 3) / 4), 256, 0, s>>>(ca);
+    after_launch("crt_kernel");
+}
+
 // The automatic choice: the limb route once the product is big enough to pay for the two
 // operand splits, and deep enough (k >= 64, one full LDS stage) for the MFMA work to outweigh the
 // recombination epilogue; thinner products (the k = 32 Schur updates of the eliminations) run
@@ -1258,7 +1726,31 @@ void gemm_dispatch(const hx_modctx& ctx, const GemmArgs& g, hipStream_t s, int v
         const uint64_t limb_tiles = (uint64_t)((g.m + LM - 1) / LM) * ((g.n + LN - 1) / LN);
         if (limb_tiles < 32 && plain_suggest_splits(g.m, g.n, g.k) > 1) variant = HX_GEMM_PLAIN;
     }
-    if (variant == HX_GEMM_LIMB && g.m != 0 && g.n != 0) {
+    uint32_t rn = 0;            // moduli of the residue route, 0 = the limb kernels
+    if (variant == HX_GEMM_LIMB && g.m != 0 && g.n != 0)
+        rn = residue_choice(ctx, g.m, g.n, g.k, g.m_dev != nullptr || g.k_dev != nullptr, tile);
+    if (rn != 0) {
+        // the default route falls back to the limb kernels when its planes do not fit (a caller workspace
+        // sized for the limb planes, or the library's own that cannot grow inside a capture)
+        const bool asked = tile == kResidueTile;
+        const size_t need = residue_workspace_bytes(g.m, g.n, g.k, rn);
+        if (ws == nullptr) {
+            void* own = stream_scratch(s, SCRATCH_LIMB, need);
+            if (own != nullptr) {
+                ws = own;
+                ws_bytes = need;
+            } else if (asked) {
+                throw Error{HX_E_CAPTURE, "the residue route needs a caller workspace inside a stream capture "
+                                          "(hx_gemm_opts.workspace; the library's own cannot grow there)"};
+            } else {
+                rn = 0;
+            }
+        } else if (ws_bytes < need) {
+            HX_CHECK(!asked, "residue route: workspace too small (hx_gemm_workspace_bytes_tile)");
+            rn = 0;
+        }
+    }
+    if (rn == 0 && variant == HX_GEMM_LIMB && g.m != 0 && g.n != 0) {
         if (ws == nullptr) {
             // library-owned planes private to this stream (never shared with another stream); inside a
             // capture they can only be used when they are already large enough
@@ -1297,7 +1789,9 @@ void gemm_dispatch(const hx_modctx& ctx, const GemmArgs& g, hipStream_t s, int v
             if (split_ws == nullptr || split_ws_bytes < need) sp = 1;      // automatic: no split
         }
     }
-    if (variant == HX_GEMM_LIMB)
+    if (rn != 0)
+        gemm_residue(ctx, g, s, rn, ws, ws_bytes, ev0, ev1);
+    else if (variant == HX_GEMM_LIMB)
         gemm_limb(ctx, g, s, ws, ws_bytes, tile, ev0, ev1);
     else
         gemm_plain(ctx, g, s, tile, ev0, ev1, sp, split_ws, split_ws_bytes);
@@ -1597,10 +2091,27 @@ __global__ __launch_bounds__(256) void mfma_rate_kernel(int* out, uint32_t iters
 
 extern "C" {
 
-HX_API size_t hx_gemm_workspace_bytes(const hx_modctx* ctx, uint32_t m, uint32_t n, uint32_t k, int variant) {
+HX_API size_t hx_gemm_workspace_bytes_tile(const hx_modctx* ctx, uint32_t m, uint32_t n, uint32_t k, int variant,
+                                           int tile) {
     if (ctx == nullptr) return 0;
     if (variant == HX_GEMM_AUTO) variant = gemm_auto_variant(*ctx, m, n, k);
-    return variant == HX_GEMM_LIMB ? limb_workspace_bytes(*ctx, m, n, k) : 0;
+    if (variant != HX_GEMM_LIMB) return 0;
+    const uint32_t rn = residue_choice(*ctx, m, n, k, false, tile);
+    return rn ? residue_workspace_bytes(m, n, k, rn) : limb_workspace_bytes(*ctx, m, n, k);
+}
+
+HX_API size_t hx_gemm_workspace_bytes(const hx_modctx* ctx, uint32_t m, uint32_t n, uint32_t k, int variant) {
+    return hx_gemm_workspace_bytes_tile(ctx, m, n, k, variant, 0);
+}
+
+HX_API uint32_t hx_gemm_residue_moduli(uint32_t p, uint32_t k) { return p >= 2 ? residue_moduli(p, k) : 0; }
+
+HX_API int hx_gemm_route(const hx_modctx* ctx, uint32_t m, uint32_t n, uint32_t k, int variant, int tile) {
+    if (ctx == nullptr) return -1;
+    if (variant == HX_GEMM_AUTO) variant = gemm_auto_variant(*ctx, m, n, k);
+    if (variant != HX_GEMM_LIMB || ctx->limbs == 0) return 0;
+    if (residue_choice(*ctx, m, n, k, false, tile) != 0) return kResidueTile;
+    return limb_version(tile);
 }
 
 HX_API int hx_gemm_u32(const hx_modctx* ctx, hx_view A, hx_view B, hx_view C, int op, hx_stream s,

@@ -362,8 +362,18 @@ def schur_update(ctx: ModCtx, L, U, C_, stream=None, variant: int = HX_GEMM_AUTO
     gemm(ctx, L, U, C_, GemmOp.Sub, stream, variant)
 
 
-def gemm_workspace_bytes(ctx: ModCtx, m: int, n: int, k: int, variant: int = HX_GEMM_LIMB) -> int:
-    return lib().hx_gemm_workspace_bytes(C.byref(ctx.c), m, n, k, variant)
+def gemm_workspace_bytes(ctx: ModCtx, m: int, n: int, k: int, variant: int = HX_GEMM_LIMB, tile: int = 0) -> int:
+    return lib().hx_gemm_workspace_bytes_tile(C.byref(ctx.c), m, n, k, variant, tile)
+
+
+def residue_moduli(p: int, k: int) -> int:
+    """Moduli count of the residue route (gemm tile 6) at modulus p and depth k; 0 where it does not apply."""
+    return int(lib().hx_gemm_residue_moduli(p, k))
+
+
+def gemm_route(ctx: ModCtx, m: int, n: int, k: int, variant: int = HX_GEMM_LIMB, tile: int = 0) -> int:
+    """The kernel gemm runs for these sizes (0 plain, 1..5 limb kernel, 6 residue route), workspace permitting."""
+    return int(lib().hx_gemm_route(C.byref(ctx.c), m, n, k, variant, tile))
 
 
 def mfma_rate(blocks: int = 1024, iters: int = 20000, random_operands: bool = True, stream=None) -> float:

@@ -82,7 +82,12 @@ typedef struct hx_gemm_opts {   /* GemmOpts, CudaKernels.h:64-79 */
                                1 = register-staged LDS copy, 2 = direct-to-LDS copy ring,
                                3 = the ring with LDS reads pipelined against the MFMAs, 4 = the ring on eight
                                waves, one 32 x 32 block each (the default), 5 = 96 x 128 workgroup tiles on
-                               twelve waves */
+                               twelve waves, 6 = the residue route: N <= 12 int8 products modulo small
+                               coprime moduli and a CRT recombination (N = 10 for p < 2^32, k <= 8192;
+                               host-side sizes and k <= 133144 only, else the limb kernels; its workspace
+                               is hx_gemm_workspace_bytes_tile's, all of it: K is not chunked).  0 takes
+                               the residue route for large four-limb products when its workspace fits
+                               (hx_gemm_route), else kernel 4 */
     void *workspace;        /* the limb route's planes (the reference's imma_workspace, CudaKernels.h:72-75):
                                hx_gemm_workspace_bytes() runs K in one piece, less (at least the bytes of one
                                64-deep K chunk, hx_gemm_limb_k_chunk) runs K in chunks, same bytes; smaller: AUTO
@@ -218,8 +223,24 @@ HX_API int hx_graph_destroy(hx_graph g);
  * variant PLAIN: any p; LIMB: p with ctx->limbs > 0 (p <= 4278124287), else HX_E_ARGUMENT. */
 HX_API int hx_gemm_u32(const hx_modctx *ctx, hx_view A, hx_view B, hx_view C, int op, hx_stream s,
                        const hx_gemm_opts *opts);
-/* Device workspace bytes the limb route needs for these shapes in one K piece (0 for PLAIN). */
+/* Device workspace bytes the variant's default route needs for these shapes in one K piece (0 for PLAIN): the
+ * limb planes, or the residue planes where the default takes the residue route (see hx_gemm_route). */
 HX_API size_t hx_gemm_workspace_bytes(const hx_modctx *ctx, uint32_t m, uint32_t n, uint32_t k, int variant);
+/* The same for a given hx_gemm_opts.tile (tile 6, the residue route: N (Mp k' + Np k' + Mp Np) bytes with Mp, Np
+ * = m, n rounded up to 256 and k' = k rounded up to 64).  No counterpart in the reference (its IMMA route has
+ * one workspace formula). */
+HX_API size_t hx_gemm_workspace_bytes_tile(const hx_modctx *ctx, uint32_t m, uint32_t n, uint32_t k, int variant,
+                                           int tile);
+/* The moduli count N of the residue route (tile 6) for modulus p and depth k: the least N with the product of
+ * the first N moduli (255, 254, 253, 251, 247, 241, 239, 233, 229, 227, 223, 211) above 8 k (p/2)^2; 0 when
+ * the route does not apply (k = 0, k > 133144, p < 2, or 12 moduli too few).  Host-only arithmetic.  No
+ * counterpart in the reference. */
+HX_API uint32_t hx_gemm_residue_moduli(uint32_t p, uint32_t k);
+/* The kernel a product with host-side sizes runs for (variant, tile), given a large enough workspace: 0 = the
+ * plain kernel, 1 .. 5 = that limb kernel, 6 = the residue route; -1 for a NULL ctx.  The default (tile 0)
+ * takes the residue route where it needs at most limbs^2 - 4 int8 products and m, n, k >= 4096, else limb
+ * kernel 4.  No counterpart in the reference. */
+HX_API int hx_gemm_route(const hx_modctx *ctx, uint32_t m, uint32_t n, uint32_t k, int variant, int tile);
 /* The K chunk a limb workspace of workspace_bytes runs (imma_k_chunk, CudaKernels.h:100-110): min(k, the
  * int32-exact chunk) when its planes fit, else the largest multiple of 64 that fits; 0 when not even a
  * 64-deep chunk fits.  Host-only arithmetic (no device needed). */

@@ -148,15 +148,16 @@ def test_gemm_batch_uneven(hx_gpu, O, variant):
             assert np.array_equal(E.from_device(jb[2]), want), f"rep {rep} {A.shape} x {B.shape} op {op}"
 
 
-@pytest.mark.parametrize("tile", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("tile", [1, 2, 3, 4, 5, 6])
 @pytest.mark.parametrize("p", [251, 65521, 16777213, 2147483629])
 def test_limb_kernels_all_copy_paths(hx_gpu, O, p, tile):
     """Every limb kernel (tile 1: register-staged LDS copy; tile 2: direct-to-LDS three-stage
     ring; tile 3: the ring with the LDS reads pipelined against the MFMAs; tile 4: tile 3's
     pipeline on eight waves, one 32 x 32 block each; tile 5: 96 x 128 tiles on twelve waves, whose
-    last row tile reads past the plane's rows when m is not a multiple of 96) for every limb count,
+    last row tile reads past the plane's rows when m is not a multiple of 96) and the residue route
+    (tile 6: one int8 product per small modulus on 256 x 256 tiles, CRT into C) for every limb count,
     with k covering 1..5 LDS stages (the rings' unroll by three), ragged m / n, every op, and a k
-    above one limb chunk (two accumulation passes)."""
+    above one limb chunk (two accumulation passes for the limb kernels, one for the residues)."""
     from hacs_amd import engine as E
     ctx = E.ModCtx(p)
     assert ctx.limbs in (1, 2, 3, 4)
@@ -193,3 +194,43 @@ def test_mfma_rate_calibration(hx_gpu):
     for blocks, iters in ((0, 10), (65537, 10), (8, 0)):
         with pytest.raises(E.HxError):
             E.mfma_rate(blocks, iters)
+
+
+def test_residue_route_bounds(hx_gpu, O):
+    """The residue route (tile 6) at its edges: operands all equal to v with v = 127 mod 255, so the first
+    modulus' int32 sums reach k * 127^2 = 2147479576 at k = 133144 (the largest k it takes; 11 moduli for
+    p = 2147483629), and all equal to -(p-1)/2, the largest |x| = k (p-1)^2/4 the CRT must place; the closed
+    form k v^2 mod p is the answer.  Past k = 133144 it runs the limb kernels (same bytes).  A caller workspace
+    of hx_gemm_workspace_bytes_tile bytes serves it; one byte less fails."""
+    from hacs_amd import engine as E
+    p = 2147483629
+    ctx = E.ModCtx(p)
+    half = (p - 1) // 2
+    v127 = 127 + 255 * 4000000
+    assert v127 % 255 == 127 and v127 < p
+    m, n = 64, 128
+    dC = torch.empty((m, n), dtype=torch.int32, device="cuda")
+    for k in (133144, 133144 + 64):
+        for v in (v127, (p - half) % p):
+            print(f"residue route k={k} v={v}", flush=True)
+            A = torch.full((m, k), v, dtype=torch.int32, device="cuda")
+            B = torch.full((k, n), v, dtype=torch.int32, device="cuda")
+            E.gemm(ctx, A, B, dC, E.GemmOp.Set, variant=E.HX_GEMM_LIMB, tile=6)
+            torch.cuda.synchronize()
+            want = (k * (v * v % p)) % p
+            assert (E.from_device(dC) == want).all(), (k, v)
+            del A, B
+    rng = np.random.default_rng(6)
+    m, n, k = 300, 260, 700
+    A = rng.integers(0, p, (m, k), dtype=np.uint64).astype(np.uint32)
+    B = rng.integers(0, p, (k, n), dtype=np.uint64).astype(np.uint32)
+    need = E.gemm_workspace_bytes(ctx, m, n, k, E.HX_GEMM_LIMB, tile=6)
+    assert need == 10 * (512 * 704 + 512 * 704 + 512 * 512)
+    ws = torch.empty(need, dtype=torch.uint8, device="cuda")
+    dC = torch.empty((m, n), dtype=torch.int32, device="cuda")
+    E.gemm(ctx, E.to_device(A), E.to_device(B), dC, E.GemmOp.Set, variant=E.HX_GEMM_LIMB, tile=6, workspace=ws)
+    torch.cuda.synchronize()
+    assert np.array_equal(E.from_device(dC), O.gemm_fast(p, A, B))
+    with pytest.raises(E.HxError):
+        E.gemm(ctx, E.to_device(A), E.to_device(B), dC, E.GemmOp.Set, variant=E.HX_GEMM_LIMB, tile=6,
+               workspace=ws[:need - 1])

@@ -142,7 +142,7 @@ def test_two_streams_null_workspace(hx_gpu, O):
         torch.cuda.synchronize()
 
 
-@pytest.mark.parametrize("tile", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("tile", [1, 2, 3, 4, 5, 6])
 def test_limb_extreme_digits_full_chunk(hx_gpu, O, tile):
     """Operands whose balanced limbs sit at the ends of the digit range, over K = 32704 (one full
     int32-exact chunk at four limbs, the bound |D_d| <= pairs(d) K 2^14 nearly met on every diagonal)

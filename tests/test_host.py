@@ -1,6 +1,7 @@
 """CPU-only checks: the C-ABI library loads and exports every symbol the header declares (no compute
 call without a GPU), the modulus context matches ModCtx.h's constants, and the synthetic workload
 generators agree with the oracle's restatement of the reference generators."""
+import math
 import ctypes as C
 
 import numpy as np
@@ -114,3 +115,52 @@ def test_split_k_policy():
     # a count above what k allows shrinks (no empty slice): k = 40 in slices of 32 is 2 slices
     assert L.hx_gemm_split_workspace_bytes(64, 64, 40, 64) == 256 + 2 * 64 * 64 * 4
     assert L.hx_gemm_split_workspace_bytes(64, 64, 5000, 1) == 0
+
+
+def test_residue_moduli_count():
+    """hx_gemm_residue_moduli (host arithmetic): the least N whose moduli product exceeds 8 k (p/2)^2, checked
+    against that product computed here in Python integers, at the workload's modulus, a small prime and the
+    largest 32-bit one, with the route's k limit (k 127^2 < 2^31) and k = 0 declined."""
+    from hacs_amd import engine as E
+    qs = [255, 254, 253, 251, 247, 241, 239, 233, 229, 227, 223, 211]
+    for a in range(len(qs)):
+        for b in range(a + 1, len(qs)):
+            assert math.gcd(qs[a], qs[b]) == 1
+
+    def least(p, k):
+        bound, M = 8 * k * (p // 2) ** 2, 1
+        for i, q in enumerate(qs):
+            M *= q
+            if M > bound:
+                return i + 1
+        return 0
+
+    for p in (2, 3, 251, 65521, 16777213, 2147483629, 4294967291):
+        for k in (1, 64, 700, 8192, 32768, 65536, 133144):
+            assert E.residue_moduli(p, k) == least(p, k), (p, k)
+    assert E.residue_moduli(2147483629, 8192) == 10          # the 8192^3 workload: 10 int8 products, not 16
+    assert E.residue_moduli(2147483629, 133144) == 11
+    assert E.residue_moduli(2147483629, 133145) == 0         # int32 sums would pass 2^31
+    assert E.residue_moduli(2147483629, 0) == 0
+    assert 133144 * 127 * 127 <= 2 ** 31 - 1 < 133145 * 127 * 127
+
+
+def test_gemm_route_policy():
+    """hx_gemm_route (host arithmetic): the default takes the residue route (6) for four-limb moduli at
+    m, n, k >= 4096 and the limb kernel 4 below that or for fewer limbs; explicit tiles are kept (6 only where
+    the route applies); PLAIN and AUTO's small products run the plain kernel (0)."""
+    from hacs_amd import engine as E
+    c4, c4s, c2 = E.ModCtx(2147483629), E.ModCtx(16777213), E.ModCtx(65521)
+    assert (c4.limbs, c4s.limbs, c2.limbs) == (4, 4, 3)
+    assert E.gemm_route(c4, 8192, 8192, 8192) == 6 and E.gemm_route(c4, 4096, 4096, 4096) == 6
+    assert E.gemm_route(c4, 4096, 4096, 2048) == 4 and E.gemm_route(c4, 2048, 8192, 8192) == 4
+    assert E.gemm_route(c4s, 8192, 8192, 8192) == 6 and E.residue_moduli(16777213, 8192) == 8
+    assert E.gemm_route(c2, 8192, 8192, 8192) == 4                       # 6 moduli against 9 limb products
+    assert E.gemm_route(c4, 8192, 8192, 200000) == 4                     # k past the int32 bound
+    for t in (1, 2, 3, 4, 5):
+        assert E.gemm_route(c4, 8192, 8192, 8192, E.HX_GEMM_LIMB, t) == t
+    assert E.gemm_route(c4, 64, 64, 64, E.HX_GEMM_LIMB, 6) == 6
+    assert E.gemm_route(c4, 64, 64, 200000, E.HX_GEMM_LIMB, 6) == 4
+    assert E.gemm_route(c4, 8192, 8192, 8192, E.HX_GEMM_PLAIN) == 0 and E.gemm_route(c4, 8, 8, 8, E.HX_GEMM_AUTO) == 0
+    assert E.gemm_workspace_bytes(c4, 8192, 8192, 8192) == 10 * 3 * 8192 * 8192
+    assert E.gemm_workspace_bytes(c4, 2048, 2048, 2048) == 4 * (2048 + 2048) * 2048

@@ -228,7 +228,7 @@ def test_c3_full_product_checksum(hx_gpu, O):
     Cm = torch.empty((n, n), dtype=torch.int32, device="cuda")
     rows = G["rows"].astype(np.int64)
     for variant, tile in ((E.HX_GEMM_LIMB, 1), (E.HX_GEMM_LIMB, 2), (E.HX_GEMM_LIMB, 3), (E.HX_GEMM_LIMB, 4),
-                          (E.HX_GEMM_LIMB, 5), (E.HX_GEMM_PLAIN, 0)):
+                          (E.HX_GEMM_LIMB, 5), (E.HX_GEMM_LIMB, 6), (E.HX_GEMM_PLAIN, 0)):
         Cm.fill_(0)
         E.gemm(ctx, A, B, Cm, E.GemmOp.Set, variant=variant, tile=tile)
         torch.cuda.synchronize()
