@@ -272,15 +272,30 @@ struct PrepArgs {
     int L;
 };
 
-// One 64 x 64 tile of the output (rows x k) per workgroup: coalesced loads along the source's
-// contiguous direction into LDS, then every thread writes 16 k-bytes of one row per plane.
-__device__ __forceinline__ void prep_tile(const PrepArgs& a, uint32_t bx, uint32_t by) {
-    __shared__ uint32_t T[64][65];
-    const uint32_t r0 = by * 64, c0 = bx * 64;
-    const uint32_t R = a.r_dev ? min(*a.r_dev, a.R) : a.R;
-    const uint32_t K = a.k_dev ? min(*a.k_dev, a.K) : a.K;
+// The operand tile of prep_tile (64 x 64 of the output-row x k layout) into T; zero past R x K.  Without
+// gathers or device-side sizes each thread walks one base pointer by a uniform stride (no per-entry
+// 64-bit index products); either way a wave reads 64 consecutive entries of the source's contiguous
+// direction.
+__device__ __forceinline__ void prep_load(const PrepArgs& a, uint32_t r0, uint32_t c0, uint32_t (*T)[65]) {
     const int tid = threadIdx.x;
     const bool ccontig = a.s_c == 1 && a.clist == nullptr;
+    if (a.rlist == nullptr && a.clist == nullptr && a.r_dev == nullptr && a.k_dev == nullptr) {
+        const uint32_t i = tid & 63, o = tid >> 6;           // i along the contiguous direction
+        const uint32_t r = ccontig ? o : i, c = ccontig ? i : o;
+        const int64_t step = 4 * (ccontig ? a.s_r : a.s_c);  // the other coordinate advances by 4 per entry
+        const uint32_t* src = a.src + (int64_t)(r0 + r) * a.s_r + (int64_t)(c0 + c) * a.s_c;
+        const bool in_i = ccontig ? c0 + c < a.K : r0 + r < a.R;
+        const uint32_t lim = ccontig ? (a.R > r0 + r ? a.R - r0 - r : 0) : (a.K > c0 + c ? a.K - c0 - c : 0);
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            const uint32_t v = in_i && 4u * t < lim ? src[t * step] : 0u;
+            if (ccontig) T[r + 4 * t][c] = v;
+            else T[r][c + 4 * t] = v;
+        }
+        return;
+    }
+    const uint32_t R = a.r_dev ? min(*a.r_dev, a.R) : a.R;
+    const uint32_t K = a.k_dev ? min(*a.k_dev, a.K) : a.K;
 #pragma unroll 4
     for (int t = 0; t < 16; ++t) {
         const int e = tid + 256 * t;
@@ -293,8 +308,16 @@ __device__ __forceinline__ void prep_tile(const PrepArgs& a, uint32_t bx, uint32
         }
         T[r][c] = v;
     }
+}
+
+// One 64 x 64 tile of the output (rows x k) per workgroup: coalesced loads along the source's
+// contiguous direction into LDS, then every thread writes 16 k-bytes of one row per plane.
+__device__ __forceinline__ void prep_tile(const PrepArgs& a, uint32_t bx, uint32_t by) {
+    __shared__ uint32_t T[64][65];
+    const uint32_t r0 = by * 64, c0 = bx * 64;
+    prep_load(a, r0, c0, T);
     __syncthreads();
-    const int r = tid / 4, q = tid % 4;
+    const int tid = threadIdx.x, r = tid / 4, q = tid % 4;
     uint32_t w[4][4];   // [plane][word]: 16 k-bytes of this row per plane
 #pragma unroll
     for (int wd = 0; wd < 4; ++wd) {
@@ -1042,63 +1065,102 @@ __host__ __device__ constexpr uint32_t res_modulus(int i) {
          : i == 6 ? 239u : i == 7 ? 233u : i == 8 ? 229u : i == 9 ? 227u : i == 10 ? 223u : 211u;
 }
 
-// The operand tile of prep_tile (64 x 64 of the output-row x k layout) into T; zero past R x K.
-__device__ __forceinline__ void prep_load(const PrepArgs& a, uint32_t r0, uint32_t c0, uint32_t (*T)[65]) {
-    const uint32_t R = a.r_dev ? min(*a.r_dev, a.R) : a.R;
-    const uint32_t K = a.k_dev ? min(*a.k_dev, a.K) : a.K;
-    const int tid = threadIdx.x;
-    const bool ccontig = a.s_c == 1 && a.clist == nullptr;
-#pragma unroll 4
-    for (int t = 0; t < 16; ++t) {
-        const int e = tid + 256 * t;
-        const uint32_t r = ccontig ? e / 64 : e % 64, c = ccontig ? e % 64 : e / 64;
-        uint32_t v = 0;
-        if (r0 + r < R && c0 + c < K) {
-            const int64_t sr = a.rlist ? (int64_t)a.rlist[r0 + r] : (int64_t)(r0 + r);
-            const int64_t sc = a.clist ? (int64_t)a.clist[c0 + c] : (int64_t)(c0 + c);
-            v = a.src[sr * a.s_r + sc * a.s_c];
-        }
-        T[r][c] = v;
-    }
-}
-
-// p mod q_i for the sign fold of the balanced residues (a > p/2 stands for a - p).
-struct ResPrepConsts {
+// Per-modulus constants of the operand split, computed on the host (the modulus loop is not unrolled,
+// so constants derived in the kernel would cost runtime divisions): q, the bytes (1, 2^8, 2^16, 2^24)
+// mod q packed into one word, q - (2^31 mod q), fl(1/q).
+struct ResSplit {
     uint32_t nmod;
-    uint32_t pmod[RES_MAXN];
+    uint32_t q[RES_MAXN], tpack[RES_MAXN], kq[RES_MAXN];
+    float iq[RES_MAXN];
 };
 
-// One 64 x 64 (rows x k) tile of the N residue planes per workgroup: every thread turns 16 k-entries
-// of one row into N 16-byte strips of balanced residues.
-__global__ __launch_bounds__(256) void res_prep_kernel(PrepArgs a, ResPrepConsts rc) {
+// The balanced residues modulo q_i of four entries given as w = a_b + 2^31 (a_b the balanced residue mod
+// p, |a_b| <= p/2 < 2^31), packed as bytes.  One byte dot product t = sum_j byte_j(w) (2^8j mod q) +
+// q - (2^31 mod q) is congruent to a_b and below 4 * 255^2 + 255 < 2^18, so fp32 holds it exactly and
+// t * fl(1/q) + 1/2 is within 2^18 / 211 * 2^-23 < 2e-4 of t / q + 1/2, whose fractional part is a
+// multiple of 1/q at least 1/(2q) > 0.0019 from the next integer when q is odd: the truncation is
+// round(t / q) and r = t - q round(t / q) lies in [-(q-1)/2, (q-1)/2].  For q = 254 a tie (t = 127 mod
+// 254) may round either way, giving +127 or -127: both are that residue and within a byte.
+__device__ __forceinline__ uint32_t res_bytes4(const uint32_t (&w)[4], const ResSplit& c, int i) {
+    const uint32_t Q = c.q[i], TP = c.tpack[i], KQ = c.kq[i];
+    const float IQ = c.iq[i];
+    uint32_t out = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const uint32_t t = __builtin_amdgcn_udot4(w[b], TP, KQ, false);
+        const uint32_t qt = (uint32_t)fmaf((float)t, IQ, 0.5f);
+        const int32_t rr = (int32_t)t - (int32_t)__umul24(qt, Q);
+        out |= ((uint32_t)rr & 0xffu) << (8 * b);
+    }
+    return out;
+}
+
+// One 64 x 64 (rows x k) tile of the N residue planes per workgroup (any strides, gathers and
+// device-side sizes; the transposing split of B): every thread turns 16 k-entries of one row, staged
+// through LDS, into N 16-byte strips of balanced residues (res_bytes4).
+__global__ __launch_bounds__(256) void res_prep_kernel(PrepArgs a, ResSplit c) {
     __shared__ uint32_t T[64][65];
     const uint32_t r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
     prep_load(a, r0, c0, T);
     __syncthreads();
     const int tid = threadIdx.x, r = tid / 4, qd = tid % 4;
-    uint32_t x[16];
+    uint32_t w[16];
+    const uint32_t p = a.p;
 #pragma unroll
-    for (int j = 0; j < 16; ++j) x[j] = T[r][16 * qd + j];
+    for (int j = 0; j < 16; ++j) {
+        const uint32_t v = T[r][16 * qd + j];
+        w[j] = (v > a.half ? v - p : v) + 0x80000000u;      // a_b + 2^31, a_b in [-p/2, p/2]
+    }
     int8_t* out = a.out + (size_t)(r0 + r) * a.Kp + c0 + 16 * qd;
 #pragma unroll
     for (int i = 0; i < RES_MAXN; ++i) {
-        if (i >= (int)rc.nmod) break;
-        const uint32_t Q = res_modulus(i), fold = Q - rc.pmod[i];
+        if (i >= (int)c.nmod) break;
         v4i vw;
 #pragma unroll
         for (int wd = 0; wd < 4; ++wd) {
-            uint32_t w = 0;
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const uint32_t v = x[4 * wd + b];
-                uint32_t u = v % Q + (v > a.half ? fold : 0u);          // a_b mod Q in [0, 2Q)
-                u = u >= Q ? u - Q : u;
-                const uint32_t sb = u > Q / 2 ? u - Q : u;               // balanced, as a byte
-                w |= (sb & 0xffu) << (8 * b);
-            }
-            vw[wd] = (int)w;
+            const uint32_t w4[4] = {w[4 * wd], w[4 * wd + 1], w[4 * wd + 2], w[4 * wd + 3]};
+            vw[wd] = (int)res_bytes4(w4, c, i);
         }
         *(v4i*)(out + (size_t)i * a.plane) = vw;
+    }
+}
+
+// Residue planes of a row-major operand whose rows are 16-byte aligned (no gathers, host-side sizes):
+// one wave per row and 1024 k, lane-contiguous 16-byte loads (four entries per lane and instruction,
+// 1 KB per wave instruction) and one dword per lane and plane for every four entries (256 contiguous
+// bytes per wave store), no LDS.  Entries past K and rows past R are written as zero up to Kp and the
+// plane's last row.  Grid: (ceil(Kp / 1024), rows / 4), 256 threads.
+__global__ __launch_bounds__(256) void res_prep_rows_kernel(PrepArgs a, ResSplit c) {
+    const uint32_t row = blockIdx.y * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const uint32_t p = a.p;
+    const uint32_t* src = a.src + (size_t)row * a.s_r;
+    int8_t* out = a.out + (size_t)row * a.Kp;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const uint32_t k4 = blockIdx.x * 1024 + (uint32_t)j * 256 + lane * 4;
+        if (k4 >= a.Kp) break;
+        uint32_t v[4] = {0, 0, 0, 0};
+        if (row < a.R) {
+            if (k4 + 3 < a.K) {
+                const uint4 q4 = *(const uint4*)(src + k4);
+                v[0] = q4.x;
+                v[1] = q4.y;
+                v[2] = q4.z;
+                v[3] = q4.w;
+            } else {
+#pragma unroll
+                for (int b = 0; b < 4; ++b)
+                    if (k4 + b < a.K) v[b] = src[k4 + b];
+            }
+        }
+        uint32_t w[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) w[b] = (v[b] > a.half ? v[b] - p : v[b]) + 0x80000000u;
+#pragma unroll
+        for (int i = 0; i < RES_MAXN; ++i) {
+            if (i >= (int)c.nmod) break;
+            *(uint32_t*)(out + (size_t)i * a.plane + k4) = res_bytes4(w, c, i);
+        }
     }
 }
 
@@ -1622,9 +1684,6 @@ void gemm_residue(const hx_modctx& ctx, const GemmArgs& g, hipStream_t s, uint32
     int8_t* B8 = A8 + (size_t)N * Mp * Kp;
     uint8_t* R = (uint8_t*)(B8 + (size_t)N * Np * Kp);
 
-    ResPrepConsts rc{};
-    rc.nmod = N;
-    for (uint32_t i = 0; i < N; ++i) rc.pmod[i] = p % res_modulus((int)i);
     PrepArgs pa{};
     pa.src = g.A;
     pa.s_r = g.ars;
@@ -1638,8 +1697,24 @@ void gemm_residue(const hx_modctx& ctx, const GemmArgs& g, hipStream_t s, uint32
     pa.plane = (size_t)Mp * Kp;
     pa.p = p;
     pa.half = (p - 1) / 2;
-    res_prep_kernel<<<dim3(Kp / 64, Mp / 64), 256, 0, s>>>(pa, rc);
-    after_launch("res_prep_kernel(A)");
+    ResSplit rs{};
+    rs.nmod = N;
+    for (uint32_t i = 0; i < N; ++i) {
+        const uint32_t q = res_modulus((int)i);
+        rs.q[i] = q;
+        rs.tpack[i] = 1u | ((256u % q) << 8) | ((65536u % q) << 16) | ((uint32_t)(16777216ull % q) << 24);
+        rs.kq[i] = q - (0x80000000u % q);
+        rs.iq[i] = 1.0f / (float)q;
+    }
+    const bool rows_ok = g.a_rows == nullptr && g.a_cols == nullptr && g.acs == 1 && g.ars % 4 == 0 &&
+                         ((uintptr_t)g.A & 15) == 0;
+    if (rows_ok) {
+        res_prep_rows_kernel<<<dim3((Kp + 1023) / 1024, Mp / 4), 256, 0, s>>>(pa, rs);
+        after_launch("res_prep_rows_kernel(A)");
+    } else {
+        res_prep_kernel<<<dim3(Kp / 64, Mp / 64), 256, 0, s>>>(pa, rs);
+        after_launch("res_prep_kernel(A)");
+    }
     PrepArgs pb = pa;
     pb.src = g.B;
     pb.s_r = g.bcs;
@@ -1649,7 +1724,7 @@ void gemm_residue(const hx_modctx& ctx, const GemmArgs& g, hipStream_t s, uint32
     pb.R = g.n;
     pb.out = B8;
     pb.plane = (size_t)Np * Kp;
-    res_prep_kernel<<<dim3(Kp / 64, Np / 64), 256, 0, s>>>(pb, rc);
+    res_prep_kernel<<<dim3(Kp / 64, Np / 64), 256, 0, s>>>(pb, rs);
     after_launch("res_prep_kernel(B)");
 
     ResArgs ra{};

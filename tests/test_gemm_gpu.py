@@ -234,3 +234,27 @@ def test_residue_route_bounds(hx_gpu, O):
     with pytest.raises(E.HxError):
         E.gemm(ctx, E.to_device(A), E.to_device(B), dC, E.GemmOp.Set, variant=E.HX_GEMM_LIMB, tile=6,
                workspace=ws[:need - 1])
+
+
+@pytest.mark.parametrize("k", [701, 1030, 2048])
+def test_residue_route_operand_layouts(hx_gpu, O, k):
+    """The residue route's operand splits on every layout: row-major A with 16-byte aligned rows (the
+    direct split; k = 1030 ends in a partial four-entry group), rows of an odd length (k = 701) and a
+    transposed A view (the tiled split), a transposed B view, and every op, against the oracle."""
+    from hacs_amd import engine as E
+    p = 2147483629
+    ctx = E.ModCtx(p)
+    rng = np.random.default_rng(k)
+    m, n = 300, 260
+    A = rng.integers(0, p, (m, k), dtype=np.uint64).astype(np.uint32)
+    B = rng.integers(0, p, (k, n), dtype=np.uint64).astype(np.uint32)
+    C0 = rng.integers(0, p, (m, n), dtype=np.uint64).astype(np.uint32)
+    dA, dB = E.to_device(A), E.to_device(B)
+    dAt = E.to_device(np.ascontiguousarray(A.T)).t()          # m x k view with unit row stride
+    dBt = E.to_device(np.ascontiguousarray(B.T)).t()
+    for (a, b) in ((dA, dB), (dAt, dB), (dA, dBt)):
+        for op in (E.GemmOp.Set, E.GemmOp.Add, E.GemmOp.Sub):
+            dC = E.to_device(C0)
+            E.gemm(ctx, a, b, dC, op, variant=E.HX_GEMM_LIMB, tile=6)
+            torch.cuda.synchronize()
+            assert np.array_equal(E.from_device(dC), O.gemm_fast(p, A, B, C0, op)), (a.stride(), b.stride(), op)
