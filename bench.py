@@ -90,16 +90,15 @@ def log(msg):
 
 
 def profiled_traffic(n, limbs, kernel):
-    """HBM bytes per main-kernel launch from the committed counter passes (profiles/), if they cover
-    this shape and kernel: (FETCH_SIZE x fetch factor + WRITE_SIZE x write factor) in bytes."""
-    path = os.path.join(ROOT, "profiles", "limb_gemm_8192_traffic.json")
-    if not os.path.exists(path):
-        return None
-    with open(path) as f:
-        t = json.load(f)
-    if t.get("n") != n or t.get("limbs") != limbs or t.get("kernel") != kernel:
-        return None
-    return t["traffic_bytes"]
+    """HBM bytes per main-kernel launch from the committed counter passes (profiles/*_traffic.json), if one
+    covers this shape and kernel: (FETCH_SIZE x fetch factor + WRITE_SIZE x write factor) in bytes."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_traffic.json"))):
+        with open(path) as f:
+            t = json.load(f)
+        if t.get("n") == n and t.get("limbs") == limbs and t.get("kernel") == kernel:
+            return t["traffic_bytes"]
+    return None
 
 
 def gemm_timed(E, ctx, A, B, Cm, variant, steps, warmup, ws, tile=0):
