@@ -1172,7 +1172,8 @@ __global__ __launch_bounds__(256) void res_prep_rows_kernel(PrepArgs a, ResSplit
 // residue in [0, q) and stores it as a byte of the modulus' residue plane.
 // The instance the route runs: eight waves (two per SIMD, 4 x 2 blocks each), a ring of three stages.
 // (Measured at 8192^3, 10 moduli: a ring of 3, 4 or 5 stages 5.37 .. 5.43 ms; four waves with 4 x 4 blocks
-// each, a third less LDS reading per MFMA, 5.55 .. 5.93 ms.)
+// each, a third less LDS reading per MFMA, 5.55 .. 5.93 ms; two workgroups per CU of four waves with 4 x 2
+// blocks each on 256 x 128 tiles, whose barriers are independent, 6.10 .. 6.18 ms.)
 constexpr int RES_WAVES = 8, RES_STAGES = 3;
 constexpr int RLA = 4, RLB = 2, RTM = RLA * LM, RTN = RLB * 16 * RES_WAVES;
 
@@ -1352,7 +1353,7 @@ __device__ __forceinline__ void res_tile(const ResArgs& a, uint32_t tm, uint32_t
 }
 
 template <int LA, int LB, int NS, int WAVES>
-__global__ __launch_bounds__(WAVES * 64, 1) void res_gemm_kernel(ResArgs a) {
+__global__ __launch_bounds__(WAVES * 64, 8 / WAVES) void res_gemm_kernel(ResArgs a) {
     uint32_t tm, tn;
     tile_of(blockIdx.x, a.ntm, a.ntn, tm, tn);
     res_tile<LA, LB, NS, WAVES>(a, tm, tn, blockIdx.y);
