@@ -75,7 +75,7 @@ def parse():
     ap.add_argument("--no-f3", action="store_true", help="do not time the F_3 RREF and product")
     ap.add_argument("--no-smith-cpu", action="store_true", help="do not time the oracle's Smith sample on the host")
     ap.add_argument("--only", choices=["batch", "lu", "mixed", "cpu", "howell", "ring2k", "level1", "u64", "f2",
-                                       "splitk", "smith", "howell_ell", "f3", "mfma"],
+                                       "splitk", "smith", "howell_ell", "f3", "mfma", "operands"],
                     help="run one secondary workload alone (for profiling) and print its record")
     ap.add_argument("--pmc-mode", action="store_true",
                     help="for counter collection: one 1 GiB device copy (the traffic calibration), then one limb "
@@ -685,6 +685,27 @@ def mixed_stream(njobs):
             "selector_choose_seconds": t_choose}
 
 
+def operand_dependence(E, n=N_C3, reps=8):
+    """The headline product on random residues and on all-equal ones (every entry 1: the MFMA inputs barely
+    toggle), on the residue route and on limb kernel 4: main-kernel ms each (HIP events, mean of `reps`).
+    The difference is what operand data costs the chip's clock."""
+    import torch
+    from hacs_amd import workloads as W
+    p = W.P_C3
+    ctx = E.ModCtx(p)
+    rnd = (E.random_entries(n, n, p, W.seed_for(n, n, p, 0x01)), E.random_entries(n, n, p, W.seed_for(n, n, p, 0x02)))
+    one = (torch.ones((n, n), dtype=torch.int32, device="cuda"), torch.ones((n, n), dtype=torch.int32, device="cuda"))
+    ws = torch.empty(max(E.gemm_workspace_bytes(ctx, n, n, n, E.HX_GEMM_LIMB, t) for t in (4, RESIDUE_TILE)),
+                     dtype=torch.uint8, device="cuda")
+    C_ = torch.empty((n, n), dtype=torch.int32, device="cuda")
+    out = {"n": n, "modulus": p}
+    for name, (A, B) in (("random", rnd), ("all_ones", one)):
+        for tile in (RESIDUE_TILE, 4):
+            _, k = gemm_timed(E, ctx, A, B, C_, E.HX_GEMM_LIMB, reps, 2, ws, tile)
+            out[f"{name}_tile{tile}_kernel_ms"] = k
+    return out
+
+
 def mfma_calibration(E, blocks=1024, iters=30000):
     """The limb kernel's MFMA instruction from registers only (hx_mfma_rate, about 40 ms per run): the
     sustained rate with uniformly random operands (what the limb planes of random residues present; the
@@ -815,6 +836,7 @@ def main():
                "howell_ell": lambda: howell_ell_leg(E),
                "f3": lambda: f3_leg(E),
                "mfma": lambda: mfma_calibration(E),
+               "operands": lambda: operand_dependence(E),
                "level1": lambda: level1_kernels(E),
                "cpu": lambda: cpu_baseline(p)}[args.only]()
         print(json.dumps({"only": args.only, **leg}), flush=True)
