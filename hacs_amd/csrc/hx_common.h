@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <functional>
+
 #include <cstdint>
 #include <cstdio>
 #include <string>
@@ -91,6 +93,16 @@ inline bool views_overlap(const hx_view& a, const hx_view& b, int eba = 4, int e
 
 // Device arena (host side): hx_malloc / hx_free and every library-owned buffer go through it.
 void* arena_alloc(size_t bytes);
+// In-place RREF of m rows (row_words 64-bit words each, contiguous) with a one-pass routine that takes at most
+// `cap` rows: the RREF of the rows so far (r <= cap nonzero rows) is stacked with the next cap - r rows and
+// reduced again until every row has been taken, then written back (zero rows below).  Exact for any field:
+// the RREF is a function of the row space alone, and stacking keeps the row space.  `one(rows, W, rank,
+// pivots)` reduces W's first rows in place (pivots only asked for on the last pass); fails with `what` when
+// the rank reaches cap while rows are left.
+void rref_stacked(uint32_t m, uint64_t* A, size_t row_words, uint32_t cap, uint32_t* rank, uint32_t* pivots,
+                  hipStream_t st, const std::function<void(uint32_t, uint64_t*, uint32_t*, uint32_t*)>& one,
+                  const char* what);
+
 void arena_free(void* p);
 // Whether s is being captured into a graph (then nothing may be allocated or freed for it).
 bool stream_capturing(hipStream_t s);

@@ -269,6 +269,71 @@ HX_API size_t hx_f2_rref_workspace_bytes(uint32_t m, uint32_t n, uint32_t ldw) {
     return (m == 0 || n == 0) ? 0 : f2_ws_bytes(m, n, std::max<uint32_t>(ldw, (n + 63) / 64));
 }
 
+}  // extern "C"
+
+namespace {
+// One pass of the panel elimination over m <= F2_PT * F2_RMAX rows (the panel kernel holds every row's
+// panel word in registers).
+void f2_rref_one(uint32_t m, uint32_t n, uint64_t* A, uint32_t ldw, uint32_t* rank, uint32_t* pivots, hx_stream s) {
+    const hipStream_t st_ = (hipStream_t)s;
+    const uint32_t mn = std::min(m, n), nwords = (n + 63) / 64;
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t bytes = f2_ws_bytes(m, n, ldw);
+    char* ws = (char*)arena_alloc(bytes);
+    try {
+        char* p = ws;
+        uint8_t* done = (uint8_t*)p;
+        p += up(m);
+        uint64_t* cmask = (uint64_t*)p;
+        p += up((size_t)m * 8);
+        uint32_t* prow = (uint32_t*)p;
+        p += up((size_t)mn * 4);
+        uint32_t* pcol = (uint32_t*)p;
+        p += up((size_t)mn * 4);
+        F2State* state = (F2State*)p;
+        p += up(sizeof(F2State));
+        uint64_t* B = (uint64_t*)p;
+        p += up((size_t)64 * ldw * 8);
+        uint64_t* T = (uint64_t*)p;
+        f2_init_kernel<<<(m + 255) / 256, 256, 0, st_>>>(done, m, state);
+        after_launch("f2_init_kernel");
+        const int R = m <= 1024 ? 1 : m <= 2048 ? 2 : m <= 4096 ? 4 : m <= 8192 ? 8 : 16;
+        const dim3 ugrid((ldw + F2_WS - 1) / F2_WS, (m + F2_ROWS - 1) / F2_ROWS);
+        for (uint32_t w = 0; w < nwords; ++w) {
+            const uint32_t bits = std::min<uint32_t>(64, n - 64 * w);
+            const uint64_t vmask = bits == 64 ? ~0ull : ((1ull << bits) - 1);
+            switch (R) {
+                case 1: launch_panel<1>(A, ldw, m, w, vmask, done, cmask, prow, pcol, state, B, st_); break;
+                case 2: launch_panel<2>(A, ldw, m, w, vmask, done, cmask, prow, pcol, state, B, st_); break;
+                case 4: launch_panel<4>(A, ldw, m, w, vmask, done, cmask, prow, pcol, state, B, st_); break;
+                case 8: launch_panel<8>(A, ldw, m, w, vmask, done, cmask, prow, pcol, state, B, st_); break;
+                default: launch_panel<16>(A, ldw, m, w, vmask, done, cmask, prow, pcol, state, B, st_); break;
+            }
+            if (ldw > 1) {
+                f2_update_kernel<<<ugrid, F2_UT, 0, st_>>>(A, ldw, m, w, cmask, state, B);
+                after_launch("f2_update_kernel");
+            }
+        }
+        const uint32_t g = (uint32_t)std::min<size_t>(4096, ((size_t)m * ldw + 255) / 256);
+        f2_gather_kernel<<<g, 256, 0, st_>>>(A, ldw, prow, state, T);
+        after_launch("f2_gather_kernel");
+        f2_place_kernel<<<g, 256, 0, st_>>>(A, ldw, m, state, T);
+        after_launch("f2_place_kernel");
+        F2State h{};
+        host_status(hx_download(&h, state, sizeof(h), s));
+        *rank = h.rank;
+        if (pivots && h.rank) host_status(hx_download(pivots, pcol, (size_t)h.rank * 4, s));
+    } catch (...) {
+        (void)hipStreamSynchronize(st_);
+        arena_free(ws);
+        throw;
+    }
+    arena_free(ws);
+}
+}  // namespace
+
+extern "C" {
+
 HX_API int hx_f2_rref(uint32_t m, uint32_t n, uint64_t* A, uint32_t ldw, uint32_t* rank, uint32_t* pivots, hx_stream s) {
     return guard("hx_f2_rref", [&] {
         HX_CHECK(rank != nullptr, "rank is NULL");
@@ -276,61 +341,15 @@ HX_API int hx_f2_rref(uint32_t m, uint32_t n, uint64_t* A, uint32_t ldw, uint32_
         if (m == 0 || n == 0) return;
         HX_CHECK(A != nullptr, "A is NULL");
         HX_CHECK(ldw >= (n + 63) / 64, "ldw < ceil(n / 64)");
-        HX_CHECK(m <= (uint32_t)(F2_PT * F2_RMAX), "F_2 RREF: m > 16384 rows is not supported");
-        const hipStream_t st_ = (hipStream_t)s;
-        const uint32_t mn = std::min(m, n), nwords = (n + 63) / 64;
-        auto up = [](size_t b) { return (b + 255) / 256 * 256; };
-        const size_t bytes = f2_ws_bytes(m, n, ldw);
-        char* ws = (char*)arena_alloc(bytes);
-        try {
-            char* p = ws;
-            uint8_t* done = (uint8_t*)p;
-            p += up(m);
-            uint64_t* cmask = (uint64_t*)p;
-            p += up((size_t)m * 8);
-            uint32_t* prow = (uint32_t*)p;
-            p += up((size_t)mn * 4);
-            uint32_t* pcol = (uint32_t*)p;
-            p += up((size_t)mn * 4);
-            F2State* state = (F2State*)p;
-            p += up(sizeof(F2State));
-            uint64_t* B = (uint64_t*)p;
-            p += up((size_t)64 * ldw * 8);
-            uint64_t* T = (uint64_t*)p;
-            f2_init_kernel<<<(m + 255) / 256, 256, 0, st_>>>(done, m, state);
-            after_launch("f2_init_kernel");
-            const int R = m <= 1024 ? 1 : m <= 2048 ? 2 : m <= 4096 ? 4 : m <= 8192 ? 8 : 16;
-            const dim3 ugrid((ldw + F2_WS - 1) / F2_WS, (m + F2_ROWS - 1) / F2_ROWS);
-            for (uint32_t w = 0; w < nwords; ++w) {
-                const uint32_t bits = std::min<uint32_t>(64, n - 64 * w);
-                const uint64_t vmask = bits == 64 ? ~0ull : ((1ull << bits) - 1);
-                switch (R) {
-                    case 1: launch_panel<1>(A, ldw, m, w, vmask, done, cmask, prow, pcol, state, B, st_); break;
-                    case 2: launch_panel<2>(A, ldw, m, w, vmask, done, cmask, prow, pcol, state, B, st_); break;
-                    case 4: launch_panel<4>(A, ldw, m, w, vmask, done, cmask, prow, pcol, state, B, st_); break;
-                    case 8: launch_panel<8>(A, ldw, m, w, vmask, done, cmask, prow, pcol, state, B, st_); break;
-                    default: launch_panel<16>(A, ldw, m, w, vmask, done, cmask, prow, pcol, state, B, st_); break;
-                }
-                if (ldw > 1) {
-                    f2_update_kernel<<<ugrid, F2_UT, 0, st_>>>(A, ldw, m, w, cmask, state, B);
-                    after_launch("f2_update_kernel");
-                }
-            }
-            const uint32_t g = (uint32_t)std::min<size_t>(4096, ((size_t)m * ldw + 255) / 256);
-            f2_gather_kernel<<<g, 256, 0, st_>>>(A, ldw, prow, state, T);
-            after_launch("f2_gather_kernel");
-            f2_place_kernel<<<g, 256, 0, st_>>>(A, ldw, m, state, T);
-            after_launch("f2_place_kernel");
-            F2State h{};
-            host_status(hx_download(&h, state, sizeof(h), s));
-            *rank = h.rank;
-            if (pivots && h.rank) host_status(hx_download(pivots, pcol, (size_t)h.rank * 4, s));
-        } catch (...) {
-            (void)hipStreamSynchronize(st_);
-            arena_free(ws);
-            throw;
+        const uint32_t cap = F2_PT * F2_RMAX;
+        if (m <= cap) {
+            f2_rref_one(m, n, A, ldw, rank, pivots, s);
+            return;
         }
-        arena_free(ws);
+        // taller: the RREF of 16384 rows at a time stacked with the next rows (rank < 16384 needed)
+        rref_stacked(m, A, ldw, cap, rank, pivots, (hipStream_t)s,
+                     [&](uint32_t rows, uint64_t* W, uint32_t* rk, uint32_t* pv) { f2_rref_one(rows, n, W, ldw, rk, pv, s); },
+                     "F_2 RREF: the rank reached 16384 with rows left (more than 16384 rows need a rank below 16384)");
     });
 }
 

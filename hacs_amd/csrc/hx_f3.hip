@@ -328,6 +328,72 @@ HX_API size_t hx_f3_rref_workspace_bytes(uint32_t m, uint32_t n, uint32_t ldw) {
     return (m == 0 || n == 0) ? 0 : f3_ws_bytes(m, n, std::max<uint32_t>(ldw, (n + 63) / 64));
 }
 
+}  // extern "C"
+
+namespace {
+// One pass of the panel elimination over m <= F3_PT * F3_RMAX rows (the panel kernel holds every row's
+// panel words in registers).
+void f3_rref_one(uint32_t m, uint32_t n, uint64_t* A, uint32_t ldw, uint32_t* rank, uint32_t* pivots, hx_stream s) {
+    const hipStream_t st_ = (hipStream_t)s;
+    const uint32_t mn = std::min(m, n), nwords = (n + 63) / 64;
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t bytes = f3_ws_bytes(m, n, ldw);
+    char* ws = (char*)arena_alloc(bytes);
+    try {
+        char* p = ws;
+        uint8_t* done = (uint8_t*)p;
+        p += up(m);
+        uint8_t* own = (uint8_t*)p;
+        p += up(m);
+        uint64_t* cmask = (uint64_t*)p;
+        p += up((size_t)m * 16);
+        uint32_t* prow = (uint32_t*)p;
+        p += up((size_t)mn * 4);
+        uint32_t* pcol = (uint32_t*)p;
+        p += up((size_t)mn * 4);
+        F3State* state = (F3State*)p;
+        p += up(sizeof(F3State));
+        uint64_t* B = (uint64_t*)p;
+        p += up((size_t)64 * 2 * ldw * 8);
+        uint64_t* T = (uint64_t*)p;
+        f3_init_kernel<<<(m + 255) / 256, 256, 0, st_>>>(done, m, state);
+        after_launch("f3_init_kernel");
+        const int R = m <= 1024 ? 1 : m <= 2048 ? 2 : m <= 4096 ? 4 : 8;
+        const dim3 ugrid((ldw + F3_WS - 1) / F3_WS, (m + F3_ROWS - 1) / F3_ROWS);
+        for (uint32_t w = 0; w < nwords; ++w) {
+            const uint32_t bits = std::min<uint32_t>(64, n - 64 * w);
+            const uint64_t vmask = bits == 64 ? ~0ull : ((1ull << bits) - 1);
+            switch (R) {
+                case 1: launch_panel<1>(A, ldw, m, w, vmask, done, cmask, own, prow, pcol, state, B, st_); break;
+                case 2: launch_panel<2>(A, ldw, m, w, vmask, done, cmask, own, prow, pcol, state, B, st_); break;
+                case 4: launch_panel<4>(A, ldw, m, w, vmask, done, cmask, own, prow, pcol, state, B, st_); break;
+                default: launch_panel<8>(A, ldw, m, w, vmask, done, cmask, own, prow, pcol, state, B, st_); break;
+            }
+            if (ldw > 1) {
+                f3_update_kernel<<<ugrid, F3_UT, 0, st_>>>(A, ldw, m, w, cmask, own, state, B);
+                after_launch("f3_update_kernel");
+            }
+        }
+        const uint32_t g = (uint32_t)std::min<size_t>(4096, ((size_t)m * 2 * ldw + 255) / 256);
+        f3_gather_kernel<<<g, 256, 0, st_>>>(A, ldw, prow, state, T);
+        after_launch("f3_gather_kernel");
+        f3_place_kernel<<<g, 256, 0, st_>>>(A, ldw, m, state, T);
+        after_launch("f3_place_kernel");
+        F3State h{};
+        host_status(hx_download(&h, state, sizeof(h), s));
+        *rank = h.rank;
+        if (pivots && h.rank) host_status(hx_download(pivots, pcol, (size_t)h.rank * 4, s));
+    } catch (...) {
+        (void)hipStreamSynchronize(st_);
+        arena_free(ws);
+        throw;
+    }
+    arena_free(ws);
+}
+}  // namespace
+
+extern "C" {
+
 HX_API int hx_f3_rref(uint32_t m, uint32_t n, uint64_t* A, uint32_t ldw, uint32_t* rank, uint32_t* pivots, hx_stream s) {
     return guard("hx_f3_rref", [&] {
         HX_CHECK(rank != nullptr, "rank is NULL");
@@ -335,62 +401,15 @@ HX_API int hx_f3_rref(uint32_t m, uint32_t n, uint64_t* A, uint32_t ldw, uint32_
         if (m == 0 || n == 0) return;
         HX_CHECK(A != nullptr, "A is NULL");
         HX_CHECK(ldw >= (n + 63) / 64, "ldw < ceil(n / 64)");
-        HX_CHECK(m <= (uint32_t)(F3_PT * F3_RMAX), "F_3 RREF: m > 8192 rows is not supported");
-        const hipStream_t st_ = (hipStream_t)s;
-        const uint32_t mn = std::min(m, n), nwords = (n + 63) / 64;
-        auto up = [](size_t b) { return (b + 255) / 256 * 256; };
-        const size_t bytes = f3_ws_bytes(m, n, ldw);
-        char* ws = (char*)arena_alloc(bytes);
-        try {
-            char* p = ws;
-            uint8_t* done = (uint8_t*)p;
-            p += up(m);
-            uint8_t* own = (uint8_t*)p;
-            p += up(m);
-            uint64_t* cmask = (uint64_t*)p;
-            p += up((size_t)m * 16);
-            uint32_t* prow = (uint32_t*)p;
-            p += up((size_t)mn * 4);
-            uint32_t* pcol = (uint32_t*)p;
-            p += up((size_t)mn * 4);
-            F3State* state = (F3State*)p;
-            p += up(sizeof(F3State));
-            uint64_t* B = (uint64_t*)p;
-            p += up((size_t)64 * 2 * ldw * 8);
-            uint64_t* T = (uint64_t*)p;
-            f3_init_kernel<<<(m + 255) / 256, 256, 0, st_>>>(done, m, state);
-            after_launch("f3_init_kernel");
-            const int R = m <= 1024 ? 1 : m <= 2048 ? 2 : m <= 4096 ? 4 : 8;
-            const dim3 ugrid((ldw + F3_WS - 1) / F3_WS, (m + F3_ROWS - 1) / F3_ROWS);
-            for (uint32_t w = 0; w < nwords; ++w) {
-                const uint32_t bits = std::min<uint32_t>(64, n - 64 * w);
-                const uint64_t vmask = bits == 64 ? ~0ull : ((1ull << bits) - 1);
-                switch (R) {
-                    case 1: launch_panel<1>(A, ldw, m, w, vmask, done, cmask, own, prow, pcol, state, B, st_); break;
-                    case 2: launch_panel<2>(A, ldw, m, w, vmask, done, cmask, own, prow, pcol, state, B, st_); break;
-                    case 4: launch_panel<4>(A, ldw, m, w, vmask, done, cmask, own, prow, pcol, state, B, st_); break;
-                    default: launch_panel<8>(A, ldw, m, w, vmask, done, cmask, own, prow, pcol, state, B, st_); break;
-                }
-                if (ldw > 1) {
-                    f3_update_kernel<<<ugrid, F3_UT, 0, st_>>>(A, ldw, m, w, cmask, own, state, B);
-                    after_launch("f3_update_kernel");
-                }
-            }
-            const uint32_t g = (uint32_t)std::min<size_t>(4096, ((size_t)m * 2 * ldw + 255) / 256);
-            f3_gather_kernel<<<g, 256, 0, st_>>>(A, ldw, prow, state, T);
-            after_launch("f3_gather_kernel");
-            f3_place_kernel<<<g, 256, 0, st_>>>(A, ldw, m, state, T);
-            after_launch("f3_place_kernel");
-            F3State h{};
-            host_status(hx_download(&h, state, sizeof(h), s));
-            *rank = h.rank;
-            if (pivots && h.rank) host_status(hx_download(pivots, pcol, (size_t)h.rank * 4, s));
-        } catch (...) {
-            (void)hipStreamSynchronize(st_);
-            arena_free(ws);
-            throw;
+        const uint32_t cap = F3_PT * F3_RMAX;
+        if (m <= cap) {
+            f3_rref_one(m, n, A, ldw, rank, pivots, s);
+            return;
         }
-        arena_free(ws);
+        // taller: the RREF of 8192 rows at a time stacked with the next rows (rank < 8192 needed)
+        rref_stacked(m, A, 2 * (size_t)ldw, cap, rank, pivots, (hipStream_t)s,
+                     [&](uint32_t rows, uint64_t* W, uint32_t* rk, uint32_t* pv) { f3_rref_one(rows, n, W, ldw, rk, pv, s); },
+                     "F_3 RREF: the rank reached 8192 with rows left (more than 8192 rows need a rank below 8192)");
     });
 }
 

@@ -311,6 +311,37 @@ void stream_scratch_release(hipStream_t s) {
     }
 }
 
+void rref_stacked(uint32_t m, uint64_t* A, size_t row_words, uint32_t cap, uint32_t* rank, uint32_t* pivots,
+                  hipStream_t st, const std::function<void(uint32_t, uint64_t*, uint32_t*, uint32_t*)>& one,
+                  const char* what) {
+    const size_t rb = row_words * 8;
+    struct Buf {                     // released on every exit, the stream drained first
+        hipStream_t st;
+        uint64_t* p = nullptr;
+        ~Buf() {
+            if (p) {
+                (void)hipStreamSynchronize(st);
+                arena_free(p);
+            }
+        }
+    } W{st};
+    W.p = (uint64_t*)arena_alloc((size_t)cap * rb);
+    uint32_t r = 0, next = 0;
+    while (next < m) {
+        const uint32_t t = std::min<uint32_t>(cap - r, m - next);
+        HX_CHECK(t > 0, what);
+        HX_HIP(hipMemcpyAsync(W.p + (size_t)r * row_words, A + (size_t)next * row_words, (size_t)t * rb,
+                              hipMemcpyDeviceToDevice, st));
+        next += t;
+        uint32_t rr = 0;
+        one(r + t, W.p, &rr, next >= m ? pivots : nullptr);
+        r = rr;
+    }
+    if (r > 0) HX_HIP(hipMemcpyAsync(A, W.p, (size_t)r * rb, hipMemcpyDeviceToDevice, st));
+    if (r < m) HX_HIP(hipMemsetAsync(A + (size_t)r * row_words, 0, (size_t)(m - r) * rb, st));
+    *rank = r;
+}
+
 }  // namespace hx
 
 using namespace hx;

@@ -278,7 +278,9 @@ HX_API int hx_schur_update_u32(const hx_modctx *ctx, hx_view L, hx_view U, hx_vi
 /* In-place reduced row echelon form (rref(BitMatrix&), BitMatrix.h:101-105: leading 1 in each pivot
  * column, zero above and below, zero rows last); rank and the pivot columns ascending (pivots: room for
  * min(m, n), or NULL).  Synchronous on s; its workspace comes from the library's arena
- * (hx_f2_rref_workspace_bytes).  m <= 16384 rows (HX_E_ARGUMENT beyond). */
+ * (hx_f2_rref_workspace_bytes, per pass).  Up to 16384 rows in one pass; taller matrices in passes of the RREF
+ * so far stacked with the next rows (16384 more rows of device memory), which needs the rank to stay below
+ * 16384 (HX_E_ARGUMENT when it reaches 16384 with rows left). */
 HX_API int hx_f2_rref(uint32_t m, uint32_t n, uint64_t *A, uint32_t ldw, uint32_t *rank, uint32_t *pivots, hx_stream s);
 HX_API size_t hx_f2_rref_workspace_bytes(uint32_t m, uint32_t n, uint32_t ldw);
 /* C = A B (accumulate = 0) or C += A B (accumulate = 1) over F_2 (multiply_m4rm, BitMatrix.h:129):
@@ -289,8 +291,9 @@ HX_API int hx_f2_gemm(uint32_t m, uint32_t n, uint32_t k, const uint64_t *A, uin
  * Row i of an m x n matrix: plane `one` (ldw words: bit j = [x_j == 1]) then plane `two` (ldw words: bit j =
  * [x_j == 2]), row stride 2 ldw words (the reference's layout with ldw = its words_per_plane()); padding
  * bits zero.  hx_f3_rref: rref(TritMatrix&) (TritMatrix.h:96-98) in place on the device -- the RREF (leading
- * 1s, zero above and below, zero rows last), rank and pivot columns (host, min(m, n) entries); m <= 8192;
- * workspace from the library's arena (hx_f3_rref_workspace_bytes).  hx_f3_gemm: multiply (TritMatrix.h:117),
+ * 1s, zero above and below, zero rows last), rank and pivot columns (host, min(m, n) entries); up to 8192 rows
+ * in one pass, taller matrices in stacked passes while the rank stays below 8192 (HX_E_ARGUMENT when it
+ * reaches 8192 with rows left); workspace from the library's arena (hx_f3_rref_workspace_bytes, per pass).  hx_f3_gemm: multiply (TritMatrix.h:117),
  * C = A B or C += A B with A m x k, B k x n, C m x n (lda / ldb / ldc: words per plane). */
 HX_API int hx_f3_rref(uint32_t m, uint32_t n, uint64_t *A, uint32_t ldw, uint32_t *rank, uint32_t *pivots, hx_stream s);
 HX_API size_t hx_f3_rref_workspace_bytes(uint32_t m, uint32_t n, uint32_t ldw);

@@ -95,11 +95,44 @@ def test_f2_rref_8192_properties(hx_gpu):
 
 
 def test_f2_rref_limits(hx_gpu):
+    """Past 16384 rows the elimination runs in stacked passes, which need the rank to stay below 16384: a
+    full-rank 16400 x 16400 matrix is refused; so is ldw < ceil(n / 64)."""
     from hacs_amd import engine as E
+    rng = np.random.default_rng(16400)
+    W = torch.from_numpy(rng.integers(np.iinfo(np.int64).min, np.iinfo(np.int64).max, (16400, 257), dtype=np.int64,
+                                      endpoint=True)).cuda()                  # every bit of every word random
+    W[:, 256] &= (1 << 16) - 1                                            # 16400 columns: 256 words + 16 bits
     with pytest.raises(E.HxError):
-        E.f2_rref(torch.zeros((16385, 1), dtype=torch.int64, device="cuda"), 64)
+        E.f2_rref(W, 16400)
     with pytest.raises(E.HxError):
         E.f2_rref(torch.zeros((8, 1), dtype=torch.int64, device="cuda"), 65)   # ldw < ceil(n / 64)
+
+
+def _planted_rref(rng, p, m, n, r):
+    """A = L R mod p with R (r x n) a random matrix in reduced row echelon form (pivots `piv`) and L (m x r)
+    random (full column rank with overwhelming probability), formed on the device in float32 (exact: every
+    sum stays below 2^24): RREF(A) = R, since the RREF is a function of the row space alone."""
+    piv = np.sort(rng.choice(n, r, replace=False))
+    R = rng.integers(0, p, (r, n)).astype(np.float32)
+    for i, c in enumerate(piv):
+        R[i, :c] = 0
+    R[:, piv] = np.eye(r, dtype=np.float32)
+    L = torch.from_numpy(rng.integers(0, p, (m, r)).astype(np.float32)).cuda()
+    A = torch.remainder(L @ torch.from_numpy(R).cuda(), p).to(torch.uint8).cpu().numpy()
+    return A, R.astype(np.uint32), piv
+
+
+@pytest.mark.parametrize("m,n,r", [(16385, 64, 64), (40000, 1000, 700), (70000, 300, 299)])
+def test_f2_rref_tall_stacked(hx_gpu, m, n, r):
+    """More than 16384 rows (stacked passes of the one-pass elimination): the planted RREF, its pivots and
+    rank, zero rows below."""
+    from hacs_amd import engine as E
+    rng = np.random.default_rng(m + n)
+    A, R, piv = _planted_rref(rng, 2, m, n, r)
+    print(f"F2 tall RREF {m}x{n} rank {r}", flush=True)
+    rank, gp, out = _device_rref(E, A)
+    assert rank == r and np.array_equal(gp, piv)
+    assert np.array_equal(E.unpack_bits(out[:r], n), R) and not out[r:].any()
 
 
 @pytest.mark.parametrize("m,n,k", [(1, 1, 1), (37, 70, 65), (300, 129, 1000), (1024, 1024, 1024), (129, 4096, 200)])

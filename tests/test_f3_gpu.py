@@ -98,11 +98,35 @@ def test_f3_rref_8192_properties(hx_gpu):
 
 
 def test_f3_rref_limits(hx_gpu):
+    """Past 8192 rows the elimination runs in stacked passes, which need the rank to stay below 8192: a
+    full-rank 8300 x 8300 matrix is refused; so is ldw < ceil(n / 64)."""
     from hacs_amd import engine as E
+    rng = np.random.default_rng(8300)
+    A = rng.integers(0, 3, (8300, 8300)).astype(np.uint32)
+    dW = torch.from_numpy(E.pack_trits(A).view(np.int64).copy()).cuda()
     with pytest.raises(E.HxError):
-        E.f3_rref(torch.zeros((8193, 2), dtype=torch.int64, device="cuda"), 64)
+        E.f3_rref(dW, 8300)
     with pytest.raises(E.HxError):
         E.f3_rref(torch.zeros((8, 2), dtype=torch.int64, device="cuda"), 65)   # ldw < ceil(n / 64)
+
+
+@pytest.mark.parametrize("m,n,r", [(8193, 64, 64), (20000, 600, 400), (30000, 200, 199)])
+def test_f3_rref_tall_stacked(hx_gpu, m, n, r):
+    """More than 8192 rows (stacked passes of the one-pass elimination): A = L R mod 3 with R a random RREF
+    (formed on the device in float32, exact), so RREF(A) = R: its pivots and rank, zero rows below."""
+    from hacs_amd import engine as E
+    rng = np.random.default_rng(m + n)
+    piv = np.sort(rng.choice(n, r, replace=False))
+    R = rng.integers(0, 3, (r, n)).astype(np.float32)
+    for i, c in enumerate(piv):
+        R[i, :c] = 0
+    R[:, piv] = np.eye(r, dtype=np.float32)
+    L = torch.from_numpy(rng.integers(0, 3, (m, r)).astype(np.float32)).cuda()
+    A = torch.remainder(L @ torch.from_numpy(R).cuda(), 3).to(torch.int64).cpu().numpy().astype(np.uint32)
+    print(f"F3 tall RREF {m}x{n} rank {r}", flush=True)
+    rank, gp, out = _device_rref(E, A)
+    assert rank == r and np.array_equal(gp, piv)
+    assert np.array_equal(E.unpack_trits(out[:r], n), R.astype(np.uint32)) and not out[r:].any()
 
 
 @pytest.mark.parametrize("m,n,k", [(1, 1, 1), (37, 70, 65), (300, 129, 1000), (1024, 1024, 1024), (129, 4096, 200)])
