@@ -1707,15 +1707,16 @@ void gemm_residue(const hx_modctx& ctx, const GemmArgs& g, hipStream_t s, uint32
         rs.kq[i] = q - (0x80000000u % q);
         rs.iq[i] = 1.0f / (float)q;
     }
-    const bool rows_ok = g.a_rows == nullptr && g.a_cols == nullptr && g.acs == 1 && g.ars % 4 == 0 &&
-                         ((uintptr_t)g.A & 15) == 0;
-    if (rows_ok) {
-        res_prep_rows_kernel<<<dim3((Kp + 1023) / 1024, Mp / 4), 256, 0, s>>>(pa, rs);
-        after_launch("res_prep_rows_kernel(A)");
-    } else {
-        res_prep_kernel<<<dim3(Kp / 64, Mp / 64), 256, 0, s>>>(pa, rs);
-        after_launch("res_prep_kernel(A)");
-    }
+    // the direct split where the operand's k runs contiguously along 16-byte aligned rows (A row-major;
+    // B given transposed, as gemm_nt does), the LDS-staged one otherwise
+    auto split = [&](const PrepArgs& pr, uint32_t rows, const char* what) {
+        const bool direct = pr.rlist == nullptr && pr.clist == nullptr && pr.s_c == 1 && pr.s_r % 4 == 0 &&
+                            ((uintptr_t)pr.src & 15) == 0;
+        if (direct) res_prep_rows_kernel<<<dim3((Kp + 1023) / 1024, rows / 4), 256, 0, s>>>(pr, rs);
+        else res_prep_kernel<<<dim3(Kp / 64, rows / 64), 256, 0, s>>>(pr, rs);
+        after_launch(what);
+    };
+    split(pa, Mp, "res_prep(A)");
     PrepArgs pb = pa;
     pb.src = g.B;
     pb.s_r = g.bcs;
@@ -1725,8 +1726,7 @@ void gemm_residue(const hx_modctx& ctx, const GemmArgs& g, hipStream_t s, uint32
     pb.R = g.n;
     pb.out = B8;
     pb.plane = (size_t)Np * Kp;
-    res_prep_kernel<<<dim3(Kp / 64, Np / 64), 256, 0, s>>>(pb, rs);
-    after_launch("res_prep_kernel(B)");
+    split(pb, Np, "res_prep(B)");
 
     ResArgs ra{};
     ra.A8 = A8;
