@@ -215,3 +215,52 @@ def test_rref_shapes_beyond_one_workgroup(hx_gpu, O, p, m, n, rank):
     assert res.rank == r and np.array_equal(res.pivots, piv) and np.array_equal(res.pivot_rows, prow)
     got = E.from_device(R)
     assert np.array_equal(got[:r], want) and not got[r:].any()
+
+
+def test_residue_route_inside_graph_capture(hx_gpu, O):
+    """The residue route (tile 6) with a caller workspace captured in a hipGraph (its splits, product and CRT
+    are kernels only, no memset): three replays each equal the oracle, with eager work and synchronisations
+    between them; without a workspace inside the capture it is refused (the library's own cannot grow there)."""
+    from hacs_amd import engine as E
+    from hacs_amd import _lib
+    p = 2147483629
+    ctx = E.ModCtx(p)
+    rng = np.random.default_rng(66)
+    m, n, k = 300, 260, 1030
+    A, B = _rand(rng, p, (m, k)), _rand(rng, p, (k, n))
+    C0 = _rand(rng, p, (m, n))
+    dA, dB = E.to_device(A), E.to_device(B)
+    ws = torch.empty(E.gemm_workspace_bytes(ctx, m, n, k, E.HX_GEMM_LIMB, tile=6), dtype=torch.uint8, device="cuda")
+    want = O.gemm_fast(p, A, B, C0, 1)
+    dC = E.to_device(C0)
+    s = torch.cuda.Stream()
+    L = _lib.lib()
+    with torch.cuda.stream(s):
+        h = s.cuda_stream
+        g = C.c_void_p()
+        _lib.check(L.hx_graph_begin(h))
+        E.gemm(ctx, dA, dB, dC, E.GemmOp.Add, stream=h, variant=E.HX_GEMM_LIMB, tile=6, workspace=ws)
+        _lib.check(L.hx_graph_end(h, C.byref(g)))
+        for rep in range(3):
+            dC.copy_(E.to_device(C0))
+            s.synchronize()
+            _lib.check(L.hx_graph_launch(g, h))
+            s.synchronize()
+            assert np.array_equal(E.from_device(dC), want), f"replay {rep}"
+            other = E.to_device(C0)                                           # eager work between replays
+            E.gemm(ctx, dA, dB, other, E.GemmOp.Add, stream=h, variant=E.HX_GEMM_PLAIN)
+            s.synchronize()
+            assert np.array_equal(E.from_device(other), want)
+        _lib.check(L.hx_graph_destroy(g))
+        fresh = torch.cuda.Stream()
+    with torch.cuda.stream(fresh):
+        h2 = fresh.cuda_stream
+        g2 = C.c_void_p()
+        _lib.check(L.hx_graph_begin(h2))
+        try:
+            with pytest.raises(E.HxError):
+                E.gemm(ctx, dA, dB, dC, E.GemmOp.Set, stream=h2, variant=E.HX_GEMM_LIMB, tile=6)
+        finally:
+            rc = L.hx_graph_end(h2, C.byref(g2))
+            if rc == 0 and g2.value:
+                L.hx_graph_destroy(g2)
