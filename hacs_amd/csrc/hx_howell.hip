@@ -595,15 +595,31 @@ __device__ void howell_ell_one(const HJob& job, const EllCtx& e) {
         for (uint32_t j = c + tid; j < n; j += ET) P[j] = mulm(P[j], ui, e.M);
         if (tid == 0) act[bpos] = 0xffffffffu;
         __syncthreads();
-        // 3. eliminate every other active row exactly (one wave per row, lanes along the row)
-        for (uint32_t pos = wave; pos < na; pos += ET / 64) {
-            const uint32_t id = act[pos];
-            if (id == 0xffffffffu) continue;
-            uint32_t* R = rows + (size_t)id * n;
-            const uint32_t x = R[c];
-            if (x == 0) continue;
-            const uint32_t qf = x * e.ipw[v];                // x / ell^v, exact
-            for (uint32_t j = c + lane; j < n; j += 64) R[j] = subm(R[j], mulm(qf, P[j], e.M), q);
+        // 3. eliminate every other active row exactly: one wave per group of four rows, lanes along the
+        //    rows, the four rows' loads in flight together (the rows are independent given P)
+        for (uint32_t pos0 = 4 * wave; pos0 < na; pos0 += 4 * (ET / 64)) {
+            uint32_t* R[4];
+            uint32_t qf[4];
+            bool live[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const uint32_t id = pos0 + u < na ? act[pos0 + u] : 0xffffffffu;
+                live[u] = id != 0xffffffffu;
+                R[u] = rows + (size_t)(live[u] ? id : pid) * n;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const uint32_t x = live[u] ? R[u][c] : 0u;
+                live[u] = x != 0;
+                qf[u] = x * e.ipw[v];                        // x / ell^v, exact
+            }
+            if (!(live[0] || live[1] || live[2] || live[3])) continue;
+            for (uint32_t j = c + lane; j < n; j += 64) {
+                const uint32_t pj = P[j];
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (live[u]) R[u][j] = subm(R[u][j], mulm(qf[u], pj, e.M), q);
+            }
         }
         // 4. the closure row ell^(k-v) P at the bottom of the active list
         if (v > 0) {
@@ -642,7 +658,7 @@ __device__ void howell_ell_one(const HJob& job, const EllCtx& e) {
     __syncthreads();
 }
 
-__global__ __launch_bounds__(ET) void howell_ell_kernel(HArgs a, EllCtx e) {
+__global__ __launch_bounds__(ET, 8) void howell_ell_kernel(HArgs a, EllCtx e) {      // two jobs per CU (64 VGPRs)
     __shared__ uint32_t s_job;
     for (;;) {
         if (threadIdx.x == 0) s_job = atomicAdd(a.next, 1u);
