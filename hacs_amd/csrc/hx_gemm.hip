@@ -1164,6 +1164,54 @@ __global__ __launch_bounds__(256) void res_prep_rows_kernel(PrepArgs a, ResSplit
     }
 }
 
+// Residue planes of an operand whose output rows are contiguous in the source and whose k is strided (a
+// row-major B: entry (row r, k c) at src[r s_r + c s_c] with s_r = 1; no gathers, host-side sizes).  One
+// 32-row x 128-k tile per workgroup: the load reads 128-byte runs of rows (two k per wave instruction) into
+// LDS as T[c][r ^ swizzle(c)]; each thread then takes 16 k of one row (8 threads per row), so a wave stores
+// 8 rows x 128 contiguous bytes per plane.  The swizzle (r XOR 4 ((c / 16) mod 8)) makes both the stores
+// into T (32 consecutive rows, one k) and the reads out of it (rows 0..3 x eight 16-k chunks, one k offset)
+// hit 32 different banks.  Chunks past Kp are not written; entries past K or R read as zero.
+// Grid: (ceil(Kp / 128), rows / 32), 256 threads.
+__global__ __launch_bounds__(256) void res_prep_cols_kernel(PrepArgs a, ResSplit c) {
+    __shared__ uint32_t T[128][33];
+    const uint32_t r0 = blockIdx.y * 32, c0 = blockIdx.x * 128;
+    const int tid = threadIdx.x;
+    {
+        const uint32_t r = tid & 31, cb = tid >> 5;              // row along the contiguous direction
+        const uint32_t* src = a.src + (int64_t)(r0 + r) * a.s_r + (int64_t)(c0 + cb) * a.s_c;
+        const bool in_r = r0 + r < a.R;
+        const uint32_t lim = a.K > c0 + cb ? a.K - c0 - cb : 0;
+        const int64_t step = 8 * a.s_c;
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            const uint32_t cc = cb + 8 * t;
+            T[cc][r ^ (((cc >> 4) & 7) << 2)] = in_r && 8u * t < lim ? src[t * step] : 0u;
+        }
+    }
+    __syncthreads();
+    const uint32_t r = tid >> 3, q = tid & 7;
+    if (c0 + 16 * q >= a.Kp) return;
+    uint32_t w[16];
+    const uint32_t p = a.p;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const uint32_t v = T[16 * q + j][r ^ (q << 2)];
+        w[j] = (v > a.half ? v - p : v) + 0x80000000u;
+    }
+    int8_t* out = a.out + (size_t)(r0 + r) * a.Kp + c0 + 16 * q;
+#pragma unroll
+    for (int i = 0; i < RES_MAXN; ++i) {
+        if (i >= (int)c.nmod) break;
+        v4i vw;
+#pragma unroll
+        for (int wd = 0; wd < 4; ++wd) {
+            const uint32_t w4[4] = {w[4 * wd], w[4 * wd + 1], w[4 * wd + 2], w[4 * wd + 3]};
+            vw[wd] = (int)res_bytes4(w4, c, i);
+        }
+        *(v4i*)(out + (size_t)i * a.plane) = vw;
+    }
+}
+
 // One modulus' int8 product on a 256 x 256 tile: the limb_tile4 pipeline (eight waves, three-stage
 // direct-to-LDS ring of 64-deep k-strips, fragments double-buffered against the MFMAs) with the
 // operand "planes" being four 64-row blocks of A and two 128-row blocks of B of one residue plane,
@@ -1712,7 +1760,9 @@ void gemm_residue(const hx_modctx& ctx, const GemmArgs& g, hipStream_t s, uint32
     auto split = [&](const PrepArgs& pr, uint32_t rows, const char* what) {
         const bool direct = pr.rlist == nullptr && pr.clist == nullptr && pr.s_c == 1 && pr.s_r % 4 == 0 &&
                             ((uintptr_t)pr.src & 15) == 0;
+        const bool cols = pr.rlist == nullptr && pr.clist == nullptr && pr.s_r == 1;
         if (direct) res_prep_rows_kernel<<<dim3((Kp + 1023) / 1024, rows / 4), 256, 0, s>>>(pr, rs);
+        else if (cols) res_prep_cols_kernel<<<dim3((Kp + 127) / 128, rows / 32), 256, 0, s>>>(pr, rs);
         else res_prep_kernel<<<dim3(Kp / 64, rows / 64), 256, 0, s>>>(pr, rs);
         after_launch(what);
     };

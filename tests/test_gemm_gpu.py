@@ -239,8 +239,10 @@ def test_residue_route_bounds(hx_gpu, O):
 @pytest.mark.parametrize("k", [701, 1030, 2048])
 def test_residue_route_operand_layouts(hx_gpu, O, k):
     """The residue route's operand splits on every layout: row-major A with 16-byte aligned rows (the
-    direct split; k = 1030 ends in a partial four-entry group), rows of an odd length (k = 701) and a
-    transposed A view (the tiled split), a transposed B view, and every op, against the oracle."""
+    direct split; k = 1030 ends in a partial four-entry group), rows of an odd length (k = 701), a row-major
+    B and a transposed A view (the column-strided split; k = 701 and 1030 end in a partial 128-k tile), a
+    transposed B view, an A view with no unit stride (the LDS-staged split), and every op, against the
+    oracle."""
     from hacs_amd import engine as E
     p = 2147483629
     ctx = E.ModCtx(p)
@@ -252,7 +254,10 @@ def test_residue_route_operand_layouts(hx_gpu, O, k):
     dA, dB = E.to_device(A), E.to_device(B)
     dAt = E.to_device(np.ascontiguousarray(A.T)).t()          # m x k view with unit row stride
     dBt = E.to_device(np.ascontiguousarray(B.T)).t()
-    for (a, b) in ((dA, dB), (dAt, dB), (dA, dBt)):
+    big = np.zeros((2 * m, 2 * k), np.uint32)
+    big[::2, ::2] = A
+    dAs = E.to_device(big)[::2, ::2]                             # neither stride unit: the LDS-staged split
+    for (a, b) in ((dA, dB), (dAt, dB), (dA, dBt), (dAs, dB)):
         for op in (E.GemmOp.Set, E.GemmOp.Add, E.GemmOp.Sub):
             dC = E.to_device(C0)
             E.gemm(ctx, a, b, dC, op, variant=E.HX_GEMM_LIMB, tile=6)
