@@ -164,3 +164,26 @@ def test_gemm_route_policy():
     assert E.gemm_route(c4, 8192, 8192, 8192, E.HX_GEMM_PLAIN) == 0 and E.gemm_route(c4, 8, 8, 8, E.HX_GEMM_AUTO) == 0
     assert E.gemm_workspace_bytes(c4, 8192, 8192, 8192) == 10 * 3 * 8192 * 8192
     assert E.gemm_workspace_bytes(c4, 2048, 2048, 2048) == 4 * (2048 + 2048) * 2048
+
+
+def test_residue_split_arithmetic_exhaustive():
+    """The residue route's operand split (res_bytes4 in hx_gemm.hip), restated in numpy for every t a byte dot
+    product can give (t < 4 * 255^2 + 256) and every modulus: qt = trunc(fma(t, fl32(1/q), 1/2)) -- the
+    fma's single rounding emulated by rounding the exact float64 value t * fl32(1/q) + 1/2 (42 significant
+    bits at most) to float32 -- leaves r = t - q qt congruent to t with |r| <= 127, so the byte is a balanced
+    residue and the products' int32 bound k 127^2 holds.  The byte dot product itself: t = sum_j byte_j(w)
+    (2^8j mod q) + q - (2^31 mod q) is congruent to w - 2^31 modulo q for random words w."""
+    qs = np.array([255, 254, 253, 251, 247, 241, 239, 233, 229, 227, 223, 211], np.int64)
+    t = np.arange(4 * 255 * 255 + 256, dtype=np.int64)
+    for q in qs:
+        iq = np.float64(np.float32(1.0) / np.float32(q))
+        f = (t.astype(np.float64) * iq + 0.5).astype(np.float32)
+        qt = np.trunc(f).astype(np.int64)
+        r = t - q * qt
+        assert np.abs(r).max() <= 127 and not ((t - r) % q).any(), int(q)
+    rng = np.random.default_rng(5)
+    w = rng.integers(0, 2**32, 100000, dtype=np.uint64).astype(np.int64)
+    for q in qs:
+        tp = [1, 256 % q, 65536 % q, (1 << 24) % q]
+        t2 = sum(((w >> (8 * j)) & 255) * tp[j] for j in range(4)) + q - (1 << 31) % q
+        assert t2.max() < 4 * 255 * 255 + 256 and not ((t2 - (w - (1 << 31))) % q).any(), int(q)
