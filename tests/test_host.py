@@ -187,3 +187,48 @@ def test_residue_split_arithmetic_exhaustive():
         tp = [1, 256 % q, 65536 % q, (1 << 24) % q]
         t2 = sum(((w >> (8 * j)) & 255) * tp[j] for j in range(4)) + q - (1 << 31) % q
         assert t2.max() < 4 * 255 * 255 + 256 and not ((t2 - (w - (1 << 31))) % q).any(), int(q)
+
+
+def test_residue_crt_reconstruction():
+    """The residue route's CRT (the product epilogue's s_i = v y_i mod q_i, then crt_kernel in hx_gemm.hip),
+    restated with Python integers: for sums x at the route's bound |x| <= k (p/2)^2 (both extremes, zero,
+    +-1, random), with N = hx_gemm_residue_moduli(p, k), the 16-bit-split sums of s_i (M / q_i mod p), the
+    fixed-point quotient T = round(sum_i s_i round(2^20 / q_i) / 2^20) and x = sum - T (M mod p) give x mod p
+    exactly; with only as many moduli as keep their product below the bound, the extreme fails (a bound is
+    needed; the route's factor 8 over the 2 exactness needs is margin for the fixed-point quotient)."""
+    from hacs_amd import engine as E
+    qs = [255, 254, 253, 251, 247, 241, 239, 233, 229, 227, 223, 211]
+
+    def crt(x, p, N):
+        M = 1
+        for q in qs[:N]:
+            M *= q
+        hi = lo = fr = 0
+        for q in qs[:N]:
+            Mi = M // q
+            y = pow(Mi % q, -1, q)
+            s = ((x % q) * y) % q                      # the epilogue's byte (v is congruent to x)
+            c = Mi % p
+            hi += s * (c >> 16)
+            lo += s * (c & 0xFFFF)
+            fr += s * ((2**20 + q // 2) // q)
+        T = (fr + 2**19) >> 20
+        return ((hi << 16) + lo + T * (p - M % p)) % p
+
+    rng = np.random.default_rng(7)
+    for p in (2147483629, 4294967291, 16777213, 65521):
+        for k in (64, 8192, 133144):
+            N = E.residue_moduli(p, k)
+            assert N >= 1
+            bound = k * (p // 2) ** 2
+            xs = [bound, -bound, 0, 1, -1] + [int(v) for v in rng.integers(-2**62, 2**62, 20)]
+            xs += [bound * int(a) // 1000 for a in rng.integers(-1000, 1001, 20)]
+            for x in xs:
+                x = max(-bound, min(bound, x))
+                assert crt(x, p, N) == x % p, (p, k, x)
+            M, nf = 1, 0                              # the most moduli whose product is still below the bound
+            while nf < len(qs) and M * qs[nf] < bound:
+                M *= qs[nf]
+                nf += 1
+            if nf >= 1:
+                assert crt(bound, p, nf) != bound % p, (p, k, nf)
