@@ -1052,8 +1052,8 @@ void limb_attrs() {
 // q_i <= 255 into balanced bytes (|r| <= 127).  Each modulus is one plain int8 product on the MFMAs
 // whose int32 sums (|sum| <= k * 127^2 < 2^31 for k <= 133144) are reduced to bytes in [0, q_i); a
 // CRT pass then turns the N bytes into the exact sum's residue mod p.  The exact sum x satisfies
-// |x| <= k (p/2)^2, and N is the least count with prod q_i > 8 k (p/2)^2: then x / prod q_i lies in
-// [-1/8, 1/8] and the CRT's quotient estimate rounds exactly.  For p < 2^32 and k <= 8192 that is
+// |x| <= k (p/2)^2, and N is the least count with 20 prod q_i > 41 k (p/2)^2: then |x / prod q_i| < 20/41
+// = 0.488, and the CRT's quotient estimate (error below 0.0015) rounds exactly.  For p < 2^32 and k <= 8192 that is
 // N = 10 int8 products against the limb route's L^2 = 16 (the same decomposition as Ozaki, Uchino and
 // Imamura, "Ozaki Scheme II: a GEMM-oriented emulation of floating-point matrix multiplication using
 // an integer modular technique", 2025, here for exact residues).
@@ -1409,11 +1409,11 @@ __global__ __launch_bounds__(WAVES * 64, 8 / WAVES) void res_gemm_kernel(ResArgs
 
 // CRT of the N product residues of four adjacent outputs per thread into x mod p, then op into C.
 // The product kernel stored s_i = v y_i mod q_i (y_i = (M / q_i)^-1 mod q_i, M = prod q_i), so
-// sum_i s_i M / q_i = x + T M with T = round(sum_i s_i / q_i) (that sum is T + x / M, |x / M| <= 1/8),
+// sum_i s_i M / q_i = x + T M with T = round(sum_i s_i / q_i) (that sum is T + x / M, |x / M| < 0.488),
 // and x = sum_i s_i (M / q_i mod p) - T (M mod p) (mod p).  All in 24-bit products: c_i = M / q_i mod p
 // split into 16-bit halves (the two partial sums stay below 12 * 255 * 2^16 < 2^28), the fraction sum in
-// fixed point with f_i = round(2^20 / q_i) (total error <= 12 * 255 / 2 units of 2^-20 < 2^-9, far inside
-// the 1/8 margin).
+// fixed point with f_i = round(2^20 / q_i) (total error <= 12 * 254 / 2 units of 2^-20 < 0.0015, inside
+// the 0.5 - 0.488 margin).
 struct CrtArgs {
     const uint8_t* R;
     size_t r_plane;
@@ -1684,15 +1684,15 @@ void gemm_limb(const hx_modctx& ctx, const GemmArgs& g0, hipStream_t s, void* ws
 // ------------------------------------------------------------------------------------------
 // residue route (host side)
 // ------------------------------------------------------------------------------------------
-// The least N with prod_{i<N} q_i > 8 k (p/2)^2; 0 when k is 0 or past RES_KMAX (int32 sums) or when
+// The least N with 20 prod_{i<N} q_i > 41 k (p/2)^2; 0 when k is 0 or past RES_KMAX (int32 sums) or when
 // RES_MAXN moduli do not reach the bound.
 uint32_t residue_moduli(uint32_t p, uint32_t k) {
     if (k == 0 || k > RES_KMAX) return 0;
-    const unsigned __int128 h = p / 2, bound = (unsigned __int128)8 * k * h * h;
+    const unsigned __int128 h = p / 2, bound = (unsigned __int128)41 * k * h * h;     // < 2^86
     unsigned __int128 M = 1;
     for (int i = 0; i < RES_MAXN; ++i) {
         M *= res_modulus(i);
-        if (M > bound) return (uint32_t)i + 1;
+        if (M * 20 > bound) return (uint32_t)i + 1;
     }
     return 0;
 }

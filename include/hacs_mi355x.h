@@ -232,7 +232,7 @@ HX_API size_t hx_gemm_workspace_bytes(const hx_modctx *ctx, uint32_t m, uint32_t
 HX_API size_t hx_gemm_workspace_bytes_tile(const hx_modctx *ctx, uint32_t m, uint32_t n, uint32_t k, int variant,
                                            int tile);
 /* The moduli count N of the residue route (tile 6) for modulus p and depth k: the least N with the product of
- * the first N moduli (255, 254, 253, 251, 247, 241, 239, 233, 229, 227, 223, 211) above 8 k (p/2)^2; 0 when
+ * the first N moduli (255, 254, 253, 251, 247, 241, 239, 233, 229, 227, 223, 211) above 2.05 k (p/2)^2; 0 when
  * the route does not apply (k = 0, k > 133144, p < 2, or 12 moduli too few).  Host-only arithmetic.  No
  * counterpart in the reference. */
 HX_API uint32_t hx_gemm_residue_moduli(uint32_t p, uint32_t k);

@@ -198,7 +198,7 @@ def test_mfma_rate_calibration(hx_gpu):
 
 def test_residue_route_bounds(hx_gpu, O):
     """The residue route (tile 6) at its edges: operands all equal to v with v = 127 mod 255, so the first
-    modulus' int32 sums reach k * 127^2 = 2147479576 at k = 133144 (the largest k it takes; 11 moduli for
+    modulus' int32 sums reach k * 127^2 = 2147479576 at k = 133144 (the largest k it takes; 10 moduli for
     p = 2147483629), and all equal to -(p-1)/2, the largest |x| = k (p-1)^2/4 the CRT must place; the closed
     form k v^2 mod p is the answer.  Past k = 133144 it runs the limb kernels (same bytes).  A caller workspace
     of hx_gemm_workspace_bytes_tile bytes serves it; one byte less fails."""
@@ -225,7 +225,8 @@ def test_residue_route_bounds(hx_gpu, O):
     A = rng.integers(0, p, (m, k), dtype=np.uint64).astype(np.uint32)
     B = rng.integers(0, p, (k, n), dtype=np.uint64).astype(np.uint32)
     need = E.gemm_workspace_bytes(ctx, m, n, k, E.HX_GEMM_LIMB, tile=6)
-    assert need == 10 * (512 * 704 + 512 * 704 + 512 * 512)
+    assert E.residue_moduli(p, k) == 9                       # 20 M_9 > 41 k (p/2)^2 at k = 700
+    assert need == 9 * (512 * 704 + 512 * 704 + 512 * 512)
     ws = torch.empty(need, dtype=torch.uint8, device="cuda")
     dC = torch.empty((m, n), dtype=torch.int32, device="cuda")
     E.gemm(ctx, E.to_device(A), E.to_device(B), dC, E.GemmOp.Set, variant=E.HX_GEMM_LIMB, tile=6, workspace=ws)
